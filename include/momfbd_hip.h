@@ -1,0 +1,162 @@
+/*
+ * momfbd_hip.h -- C-ABI of the MI355X MOMFBD patch solver (libmomfbd_hip.so).
+ *
+ * Drop-in boundary: the reference's per-patch solve, Solver::run(PatchData::Ptr)
+ * (ISP-SST/redux src/lib/momfbd/solver.cpp:330-580), called by MomfbdJob::run
+ * (src/lib/momfbd/momfbdjob.cpp:576-590) once per patch after data->initPatch().
+ * A context plays the role of Solver + thread::TmpStorage (include/redux/momfbd/
+ * solver.hpp:33-179): it owns every device buffer and one HIP stream.  The caller
+ * owns all host arrays; they are read during the call and never retained.
+ *
+ * Plain C: pointers, sizes and int status codes.  No exceptions cross the ABI.
+ * A context is single-threaded-use; separate contexts may run on separate threads.
+ */
+#ifndef MOMFBD_HIP_H
+#define MOMFBD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference throws std::logic_error or returns GSL codes) */
+#define MFBD_OK 0
+#define MFBD_EINVAL 1        /* bad argument / inconsistent sizes          */
+#define MFBD_ENOMEM 2        /* device or host allocation failed           */
+#define MFBD_EDEVICE 3       /* HIP runtime error                          */
+#define MFBD_ENONFINITE 4    /* non-finite metric or gradient              */
+#define MFBD_ESTATE 5        /* call out of order (e.g. solve before load) */
+
+/* mode bases (GlobalCfg::modeBasis, config.cpp:629) */
+#define MFBD_BASIS_ZERNIKE 0
+#define MFBD_BASIS_KL 1
+
+/* arithmetic of the hot path */
+#define MFBD_FP64 0          /* all stages in FP64 (the reference's arithmetic)              */
+#define MFBD_FP32 1          /* FP32 phase/FFT/OTF, FP64 P/Q/metric reductions and CG scalars */
+
+/* gradient forms for mfbd_eval_grad */
+#define MFBD_GRAD_FD 1       /* SubImage::gradientFiniteDifference (subimage.cpp:281-292)   */
+#define MFBD_GRAD_VOGEL 2    /* SubImage::gradientVogel (subimage.cpp:312-326, 348-390)      */
+#define MFBD_GRAD_ADJOINT 3  /* exact dL/dalpha (adjoint of the OTF chain); no reference counterpart */
+
+/* per-patch exit status, mirrors solver.cpp:470-516 */
+#define MFBD_PATCH_CONVERGED 0   /* relative-change or gradient-norm criterion met        */
+#define MFBD_PATCH_MAXITER 1     /* last stage stopped at MAX_ITER                        */
+#define MFBD_PATCH_FAILED 2      /* more than 3 failures (GSL_FAILURE path)               */
+
+typedef struct {
+    int s;              /* patch side, NUM_POINTS (Object::patchSize)                   */
+    int P;              /* pupil grid side, pupilPixels; the OTF grid is N = 2P         */
+    int K;              /* number of modes (job.nModes)                                 */
+    int nObj;           /* objects per patch                                            */
+    int nCh;            /* channels per patch (all objects)                             */
+    int nImg;           /* images per patch, ordered object -> channel -> frame         */
+    int dataY, dataX;   /* raw data block per image (patch plus shift margin)           */
+    int startY, startX; /* patch start inside the block (ChannelData::patchStart)       */
+} mfbd_geometry;
+
+typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-639 */
+    double FTOL, EPS, graddiffStep, regAlpha;
+    int minIter, maxIter, nDoneIter, modeStart, modeStep;
+    int apodSize;       /* APODIZATION, < 0: patchSize/8 (solver.cpp:117-126)          */
+    int noFilter;       /* NO_FILTER: skip the Scharmer filter in restoration           */
+    int precision;      /* MFBD_FP64 or MFBD_FP32                                      */
+    int maxBatch;       /* patches solved concurrently (1 = the reference's one-at-a-time) */
+} mfbd_params;
+
+typedef struct {
+    int status;             /* MFBD_PATCH_*                                  */
+    int iterations;         /* total CG iterations over all mode stages      */
+    int nF, nDF;            /* metric / gradient evaluations                 */
+    int failCount;
+    double initialMetric, finalMetric;
+} mfbd_result;
+
+typedef struct mfbd_ctx mfbd_ctx;
+
+/* ---- context (Solver::Solver + Solver::init, solver.cpp:64-183) ---- */
+int mfbd_ctx_create(const mfbd_geometry* geo, const mfbd_params* prm, int device, mfbd_ctx** out);
+int mfbd_ctx_destroy(mfbd_ctx* ctx);
+const char* mfbd_last_error(const mfbd_ctx* ctx);
+
+/* Patch structure: chObj[nCh] object of each channel, imgCh[nImg] channel of each
+ * image; objWeight/objWavelength/freqCutoff per object (Object::weight,
+ * wavelength, frequencyCutoff; object.cpp:206-268). */
+int mfbd_set_structure(mfbd_ctx* ctx, const int* chObj, const int* imgCh, const double* objWeight,
+                       const double* objWavelength, const double* freqCutoff);
+
+/* Pupil [P*P] (Pupil, pupil.cpp:221-277), scaled modes [nObj][K][P*P]
+ * (ModeSet via GlobalData::get(info, 1/lambda), data.cpp:658-686), tilt mode
+ * indices [nObj][2] and Jacobians [nObj][8] = J_T_xy (2x2), J_xy_T (2x2)
+ * (ModeSet::measureJacobian, modes.cpp:464-512).  Replaces Object::initProcessing's
+ * pupil/mode fetch. */
+int mfbd_set_basis(mfbd_ctx* ctx, const double* pupil, const double* modes, const int* tiltXY, const double* jac);
+
+/* Sparse nullspace N (nPar x nFree), entries sorted by (row, col): alpha = alpha0 + N beta,
+ * g_beta = N^T g_alpha (Constraints::apply/reverse, constraints.hpp:128-159). */
+int mfbd_set_constraints(mfbd_ctx* ctx, int nPar, int nFree, int nnz, const int* rows, const int* cols,
+                         const double* vals);
+
+/* Load n patches (replaces PatchData::initPatch + Channel::initPatch,
+ * channel.cpp:846-953): data [n][nImg][dataY][dataX] float32, phiFixed [nCh][P*P]
+ * diversity phases, resOffset [n][nCh][2] residual (x,y) offsets, noiseFudge [nCh].
+ * Frames are taken as already normalised (ChannelData::images after initPatch). */
+int mfbd_load_patches(mfbd_ctx* ctx, int n, const float* data, const double* phiFixed, const double* resOffset,
+                      const double* noiseFudge);
+
+/* Solver::run for every loaded patch (batched maxBatch at a time).  alpha0
+ * [n][nImg][K] warm start (Solver::loadInit) or NULL; results[n]; alphaOut
+ * [n][nImg][K]; imgOut [n][nObj][s][s] restored objects (Object::restorePatch,
+ * object.cpp:278-443), either output may be NULL. */
+int mfbd_solve(mfbd_ctx* ctx, const double* alpha0, mfbd_result* results, double* alphaOut, float* imgOut);
+
+/* ---- stage-wise entry points (parity checks, autograd harness) ---- */
+/* First SubImage::initialize (doReset) of every loaded patch at zero alpha
+ * (Solver::shiftAndInit(alpha, true), solver.cpp:583-605). */
+int mfbd_init_patches(mfbd_ctx* ctx);
+/* L(alpha) for n loaded patches: Solver::applyAlpha + Solver::metric
+ * (solver.cpp:639-657, 763-817).  alpha [n][nImg][K], L [n]. */
+int mfbd_eval_metric(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, double* L);
+/* Gradient w.r.t. alpha of the first nEnabled modes (others 0): calcPQ + gradient
+ * (solver.cpp:820-894) with method MFBD_GRAD_*.  gAlpha [n][nImg][K]. */
+int mfbd_eval_grad(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, int nEnabled, int method,
+                   double* gAlpha);
+/* Restored objects at alpha: Object::restorePatch.  img [n][nObj][s][s]. */
+int mfbd_restore(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, float* img);
+/* Diagnostics: copy a device array for patch p ("S" OTF on support, "D", "P", "Q",
+ * "ftsum", "phi") in double; returns the element count via *count. */
+int mfbd_debug_get(mfbd_ctx* ctx, const char* what, int p, double* out, int* count);
+/* Per-kernel timing of the last mfbd_solve / eval call: name[i], milliseconds,
+ * launches; returns the number of entries (<= cap). */
+int mfbd_kernel_stats(mfbd_ctx* ctx, int cap, char (*names)[48], double* ms, int* launches);
+
+/* ---- per-job setup (host; GlobalData/Object::initObject equivalents) ---- */
+const char* mfbd_setup_last_error(void);
+/* Pupil::calculatePupilSize (pupil.cpp:78-100) */
+int mfbd_pupil_size(double wavelength, int nPixels, double telescopeD, double arcSecsPerPixel,
+                    double* freqCutoff, double* radius, int* pupilPixels);
+/* makePupil + Pupil::normalize (utils.cpp:68-121, pupil.cpp:280-299) */
+int mfbd_make_pupil(int P, double radius, double coRadius, double* pupil, double* area);
+/* Pupil::generateSupport (pupil.cpp:245-277): pupil support, its OTF-grid index, OTF support */
+int mfbd_pupil_support(int P, const double* pupil, int* nPS, int* ps, int* pio, int* nOS, int* os, double* area);
+/* Zernike::getCovariance (zernike.cpp:102-148) */
+int mfbd_noll_covariance(int i, int j, double* cov);
+/* ModeSet::generate (modes.cpp:348-439): unit-RMS Zernike or KL modes [K][P*P] */
+int mfbd_make_modes(int P, double radius, double angle, int nModes, const int* modeIds, int basis, int klMin,
+                    int klMax, double klCutoff, double* modes, double* atmRms, int* tiltXY);
+/* ModeSet::getNorms + normalize(scale) (modes.cpp:443-535) */
+int mfbd_normalize_modes(int P, int K, double* modes, const double* pupil, double area, double scale);
+/* ModeSet::measureJacobian (modes.cpp:464-512) */
+int mfbd_measure_jacobian(int P, int K, const double* modes, const double* pupil, double scale, int* tiltXY,
+                          double* J, double* Ji);
+/* Constraints::init + nullspace (constraints.cpp:85-198, 429-469, 536-622) */
+int mfbd_make_constraints(int nObj, const int* nChPerObj, const int* nFramesPerCh, const int* wavefronts, int K,
+                          int* nPar, int* nFree, int* nnz, int* rows, int* cols, double* vals, int capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOMFBD_HIP_H */

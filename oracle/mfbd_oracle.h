@@ -1,0 +1,97 @@
+/*
+ * CPU FP64 oracle for the MOMFBD patch solver: TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of the reference algorithm (ISP-SST/redux,
+ * src/lib/momfbd/{solver,subimage,object,channel}.cpp, src/lib/util/gsl/
+ * conjugate_gradient.cpp, src/lib/math/helpers.cpp, src/lib/image/
+ * {fouriertransform,utils}.cpp).  It is used by tests/ as the checker, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg -- never by the
+ * product path.  Every function cites the reference file:line it follows.
+ *
+ * Conventions (FFTW): forward transform e^{-i}, backward e^{+i}, both
+ * unnormalised.  Arrays are row-major, y slow / x fast.
+ */
+#ifndef MFBD_ORACLE_H
+#define MFBD_ORACLE_H
+
+#include <complex.h>
+#include <stdint.h>
+
+typedef double complex ocplx;
+
+/* ---- Fourier helpers (fft.c) ---- */
+void o_fft2(ocplx* a, int ny, int nx, int sign);
+void o_reorder_into_c(const ocplx* in, int iny, int inx, ocplx* out, int ony, int onx, int center);
+void o_reorder_into_d(const double* in, int iny, int inx, double* out, int ony, int onx, int center);
+void o_reorder_c(ocplx* a, int ny, int nx);
+void o_reorder_d(double* a, int ny, int nx);
+void o_autocorrelate_c(ocplx* a, int ny, int nx, int center);
+void o_autocorrelate_d(double* a, int ny, int nx, int center);
+double o_noise(const ocplx* ft, int ny, int nx, int mask, double limit);
+
+/* ---- solver problem description (solver.c) ---- */
+typedef struct {
+    int s, P, N, K;               /* patch side, pupil grid, OTF grid (=2P), modes */
+    int nObj, nCh, nImg;
+    /* objects */
+    const double* weight;         /* [nObj] */
+    const double* wavelength;     /* [nObj] */
+    const double* pupil;          /* [nObj][P*P]                                   */
+    const double* modes;          /* [nObj][K][P*P]  scaled to RMS 1/lambda           */
+    const int*    tilt;           /* [nObj][2]  tiltMode.x, tiltMode.y (mode index)  */
+    const double* jac;            /* [nObj][8]  J_T_xy[2][2], J_xy_T[2][2]            */
+    const double* freqCutoff;     /* [nObj] */
+    /* channels */
+    const int*    chObj;          /* [nCh] */
+    const double* phiFixed;       /* [nCh][P*P] diversity phase */
+    const double* resOffset;      /* [nCh][2]  residual offset (x, y) in pixels */
+    const double* noiseFudge;     /* [nCh] */
+    /* images, ordered object -> channel -> frame */
+    const int*    imgCh;          /* [nImg] */
+    const float*  data;           /* [nImg][dataY][dataX] */
+    int dataY, dataX;
+    int startY, startX;           /* patch start inside the data block */
+    /* nullspace, sparse, entries sorted by (row, col) */
+    int nPar, nFree, nnz;
+    const int* nsRow; const int* nsCol; const double* nsVal;
+    /* settings (config.cpp:629-639 defaults unless changed) */
+    double FTOL, EPS, graddiffStep, regAlpha;
+    int minIter, maxIter, nDoneIter, modeStart, modeStep;
+    int apodSize;                 /* <0: patchSize/8 */
+    int noFilter;
+    const double* alphaInit;      /* [nImg][K] or NULL */
+    int maxMetrics;
+} OProblem;
+
+typedef struct {
+    double* alpha;                /* [nImg][K] */
+    double* img;                  /* [nObj][s][s] restored objects (NULL: skip) */
+    double* metrics;              /* [maxMetrics] metric history (NULL: skip) */
+    int nMetrics;
+    double initialMetric, finalMetric;
+    int totalIterations, nF, nDF, status, failCount;
+    int* shifts;                  /* [nImg][2] final integer shift (x, y), may be NULL */
+} OResult;
+
+/* Full Solver::run restatement on one patch.  Returns 0 on success. */
+int o_solve(const OProblem* pr, OResult* res);
+
+/* Stateful evaluation handle for parity checks of individual stages. */
+typedef struct OState OState;
+OState* o_state_new(const OProblem* pr);
+void o_state_free(OState* st);
+/* SubImage::initialize for every image (doReset=1 as on the first call). */
+void o_state_init(OState* st);
+/* metric L(alpha) as Solver::metric after applyAlpha (solver.cpp:639,763) */
+double o_state_metric(OState* st, const double* alpha);
+/* Vogel gradient d/dalpha (solver.cpp:860-894) at alpha, enabled = first nEnabled modes */
+void o_state_grad_vogel(OState* st, const double* alpha, int nEnabled, double* g);
+/* finite-difference gradient (subimage.cpp:281-292) at alpha */
+void o_state_grad_fd(OState* st, const double* alpha, int nEnabled, double* g);
+/* restored objects at alpha (object.cpp:278-443), img [nObj][s][s] */
+void o_state_restore(OState* st, const double* alpha, double* img);
+/* export per-image/per-object arrays after the last evaluation */
+void o_state_get(OState* st, const char* what, double* out);
+double o_state_scalar(OState* st, const char* what, int idx);
+
+#endif

@@ -1,0 +1,257 @@
+"""Oracle (test infrastructure only): ctypes wrapper around oracle/_build/libmfbd_oracle.so.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this
+module, and only as the checker.  The problem dictionary format is the one built
+by redux_amd.synth (objects / channels / images / nullspace / settings).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libmfbd_oracle.so")
+_lib = None
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+fp = C.POINTER(C.c_float)
+
+
+class OProblem(C.Structure):
+    _fields_ = [
+        ("s", C.c_int), ("P", C.c_int), ("N", C.c_int), ("K", C.c_int),
+        ("nObj", C.c_int), ("nCh", C.c_int), ("nImg", C.c_int),
+        ("weight", dp), ("wavelength", dp), ("pupil", dp), ("modes", dp), ("tilt", ip),
+        ("jac", dp), ("freqCutoff", dp),
+        ("chObj", ip), ("phiFixed", dp), ("resOffset", dp), ("noiseFudge", dp),
+        ("imgCh", ip), ("data", fp), ("dataY", C.c_int), ("dataX", C.c_int),
+        ("startY", C.c_int), ("startX", C.c_int),
+        ("nPar", C.c_int), ("nFree", C.c_int), ("nnz", C.c_int),
+        ("nsRow", ip), ("nsCol", ip), ("nsVal", dp),
+        ("FTOL", C.c_double), ("EPS", C.c_double), ("graddiffStep", C.c_double), ("regAlpha", C.c_double),
+        ("minIter", C.c_int), ("maxIter", C.c_int), ("nDoneIter", C.c_int), ("modeStart", C.c_int),
+        ("modeStep", C.c_int), ("apodSize", C.c_int), ("noFilter", C.c_int),
+        ("alphaInit", dp), ("maxMetrics", C.c_int),
+    ]
+
+
+class OResult(C.Structure):
+    _fields_ = [
+        ("alpha", dp), ("img", dp), ("metrics", dp), ("nMetrics", C.c_int),
+        ("initialMetric", C.c_double), ("finalMetric", C.c_double),
+        ("totalIterations", C.c_int), ("nF", C.c_int), ("nDF", C.c_int), ("status", C.c_int),
+        ("failCount", C.c_int), ("shifts", ip),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.o_solve.argtypes = [C.POINTER(OProblem), C.POINTER(OResult)]
+        L.o_solve_many.argtypes = [C.POINTER(OProblem), C.POINTER(OResult), C.c_int, C.c_int]
+        L.o_state_new.argtypes = [C.POINTER(OProblem)]
+        L.o_state_new.restype = C.c_void_p
+        L.o_state_free.argtypes = [C.c_void_p]
+        L.o_state_init.argtypes = [C.c_void_p]
+        L.o_state_metric.argtypes = [C.c_void_p, dp]
+        L.o_state_metric.restype = C.c_double
+        for f in (L.o_state_grad_vogel, L.o_state_grad_fd):
+            f.argtypes = [C.c_void_p, dp, C.c_int, dp]
+        L.o_state_restore.argtypes = [C.c_void_p, dp, dp]
+        L.o_state_get.argtypes = [C.c_void_p, C.c_char_p, dp]
+        L.o_state_scalar.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.o_state_scalar.restype = C.c_double
+        L.o_fft2.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.o_reorder_into_d.argtypes = [dp, C.c_int, C.c_int, dp, C.c_int, C.c_int, C.c_int]
+        L.o_autocorrelate_d.argtypes = [dp, C.c_int, C.c_int, C.c_int]
+        L.o_autocorrelate_c.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
+        L.o_noise.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double]
+        L.o_noise.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _p(a, t=dp):
+    return a.ctypes.data_as(t)
+
+
+def fft2(a, sign):
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    lib().o_fft2(a.ctypes.data, a.shape[0], a.shape[1], sign)
+    return a
+
+
+def reorder_into(a, out_shape=None, center=False):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    oy, ox = out_shape if out_shape else a.shape
+    out = np.zeros((oy, ox))
+    lib().o_reorder_into_d(_p(a), a.shape[0], a.shape[1], _p(out), oy, ox, int(center))
+    return out
+
+
+def autocorrelate(a, center=False):
+    if np.iscomplexobj(a):
+        a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+        lib().o_autocorrelate_c(a.ctypes.data, a.shape[0], a.shape[1], int(center))
+        return a
+    a = np.ascontiguousarray(a, dtype=np.float64).copy()
+    lib().o_autocorrelate_d(_p(a), a.shape[0], a.shape[1], int(center))
+    return a
+
+
+def noise(ft, mask=-1, limit=-1.0):
+    ft = np.ascontiguousarray(ft, dtype=np.complex128)
+    return lib().o_noise(ft.ctypes.data, ft.shape[0], ft.shape[1], mask, limit)
+
+
+class Problem:
+    """Holds numpy arrays alive and exposes them as an OProblem."""
+
+    def __init__(self, pb: dict, alpha_init=None):
+        objs, chans = pb["objects"], pb["channels"]
+        K, P = pb["K"], pb["P"]
+        self.keep = {}
+
+        def arr(name, a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            self.keep[name] = a
+            return a
+
+        st = pb["settings"]
+        ns = pb["nullspace"]
+        o = OProblem()
+        o.s, o.P, o.N, o.K = pb["s"], P, 2 * P, K
+        o.nObj, o.nCh, o.nImg = len(objs), len(chans), len(pb["imgCh"])
+        o.weight = _p(arr("w", [ob["weight"] for ob in objs], np.float64))
+        o.wavelength = _p(arr("wl", [ob["wavelength"] for ob in objs], np.float64))
+        o.pupil = _p(arr("pup", np.stack([ob["pupil"] for ob in objs]), np.float64))
+        o.modes = _p(arr("modes", np.stack([ob["modes"] for ob in objs]), np.float64))
+        o.tilt = _p(arr("tilt", np.array([ob["tilt"] for ob in objs]), np.int32), ip)
+        jac = np.stack([np.concatenate([np.ravel(ob["J"]), np.ravel(ob["Ji"])]) for ob in objs])
+        o.jac = _p(arr("jac", jac, np.float64))
+        o.freqCutoff = _p(arr("fc", [ob["freqCutoff"] for ob in objs], np.float64))
+        o.chObj = _p(arr("chObj", [c["obj"] for c in chans], np.int32), ip)
+        o.phiFixed = _p(arr("phiF", np.stack([c["phiFixed"] for c in chans]), np.float64))
+        o.resOffset = _p(arr("res", np.array([c["resOffset"] for c in chans]), np.float64))
+        o.noiseFudge = _p(arr("nf", [c["noiseFudge"] for c in chans], np.float64))
+        o.imgCh = _p(arr("imgCh", pb["imgCh"], np.int32), ip)
+        data = arr("data", pb["data"], np.float32)
+        o.data = _p(data, fp)
+        o.dataY, o.dataX = data.shape[1], data.shape[2]
+        o.startY, o.startX = pb["start"]
+        o.nPar, o.nFree = ns["nPar"], ns["nFree"]
+        o.nnz = len(ns["rows"])
+        o.nsRow = _p(arr("nsr", ns["rows"], np.int32), ip)
+        o.nsCol = _p(arr("nsc", ns["cols"], np.int32), ip)
+        o.nsVal = _p(arr("nsv", ns["vals"], np.float64))
+        o.FTOL, o.EPS, o.graddiffStep, o.regAlpha = st["FTOL"], st["EPS"], st["GRADDIFF_STEP"], 0.0
+        o.minIter, o.maxIter, o.nDoneIter = st["MIN_ITER"], st["MAX_ITER"], st["N_DONE_ITER"]
+        o.modeStart, o.modeStep = st["MODE_START"], st["MODE_STEP"]
+        o.apodSize, o.noFilter = st.get("APODIZATION", -1), int(st.get("NO_FILTER", 0))
+        if alpha_init is not None:
+            o.alphaInit = _p(arr("ainit", alpha_init, np.float64))
+        else:
+            o.alphaInit = dp()
+        o.maxMetrics = 100000
+        self.o = o
+        self.nObj, self.nImg, self.K, self.s = o.nObj, o.nImg, K, o.s
+        self.nPar = o.nPar
+
+
+def solve(pb: dict, alpha_init=None, restore=True):
+    prob = Problem(pb, alpha_init)
+    return _solve_many([prob], restore)[0]
+
+
+def solve_many(pbs, restore=True, threads=0):
+    probs = [Problem(pb) for pb in pbs]
+    return _solve_many(probs, restore, threads)
+
+
+def _solve_many(probs, restore=True, threads=0):
+    n = len(probs)
+    Parr = (OProblem * n)(*[p.o for p in probs])
+    Rarr = (OResult * n)()
+    bufs = []
+    for k, p in enumerate(probs):
+        alpha = np.zeros(p.nPar)
+        img = np.zeros((p.nObj, p.s, p.s)) if restore else None
+        metrics = np.zeros(100000)
+        shifts = np.zeros((p.nImg, 2), dtype=np.int32)
+        bufs.append((alpha, img, metrics, shifts))
+        Rarr[k].alpha = _p(alpha)
+        Rarr[k].img = _p(img) if restore else dp()
+        Rarr[k].metrics = _p(metrics)
+        Rarr[k].shifts = _p(shifts, ip)
+    rc = lib().o_solve_many(Parr, Rarr, n, threads)
+    if rc:
+        raise RuntimeError("oracle solve failed")
+    out = []
+    for k, p in enumerate(probs):
+        alpha, img, metrics, shifts = bufs[k]
+        r = Rarr[k]
+        out.append(dict(alpha=alpha.reshape(p.nImg, p.K), img=img, metrics=metrics[: r.nMetrics].copy(),
+                        initialMetric=r.initialMetric, finalMetric=r.finalMetric,
+                        iterations=r.totalIterations, nF=r.nF, nDF=r.nDF, status=r.status,
+                        failCount=r.failCount, shifts=shifts))
+    return out
+
+
+class State:
+    """Stage-wise evaluation (metric, gradients, restore) at arbitrary alpha."""
+
+    def __init__(self, pb: dict):
+        self.prob = Problem(pb)
+        self.h = lib().o_state_new(C.byref(self.prob.o))
+        lib().o_state_init(self.h)
+        o = self.prob.o
+        self.nImg, self.K, self.nObj, self.N, self.P, self.s = o.nImg, o.K, o.nObj, o.N, o.P, o.s
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().o_state_free(self.h)
+            self.h = None
+
+    def metric(self, alpha):
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        return lib().o_state_metric(self.h, _p(a))
+
+    def grad(self, alpha, n_enabled=None, method="vogel"):
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        g = np.zeros(self.nImg * self.K)
+        ne = self.K if n_enabled is None else n_enabled
+        f = lib().o_state_grad_vogel if method == "vogel" else lib().o_state_grad_fd
+        f(self.h, _p(a), ne, _p(g))
+        return g.reshape(self.nImg, self.K)
+
+    def restore(self, alpha):
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        img = np.zeros((self.nObj, self.s, self.s))
+        lib().o_state_restore(self.h, _p(a), _p(img))
+        return img
+
+    def get(self, what):
+        shapes = {"otf": (self.nImg, self.N, self.N, 2), "imgft": (self.nImg, self.N, self.N, 2),
+                  "ftsum": (self.nObj, self.N, self.N), "P": (self.nObj, self.N, self.N, 2),
+                  "Q": (self.nObj, self.N, self.N), "phi": (self.nImg, self.P, self.P),
+                  "vogel": (self.nImg, self.P, self.P), "phich": (len(self.prob.keep["chObj"]), self.P, self.P),
+                  "window": (self.s, self.s), "noisewindow": (self.s, self.s)}
+        out = np.zeros(shapes[what])
+        lib().o_state_get(self.h, what.encode(), _p(out))
+        if out.ndim and out.shape[-1] == 2 and what in ("otf", "imgft", "P"):
+            out = out[..., 0] + 1j * out[..., 1]
+        return out
+
+    def scalar(self, what, idx=0):
+        return lib().o_state_scalar(self.h, what.encode(), idx)

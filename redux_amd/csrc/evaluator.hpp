@@ -1,0 +1,71 @@
+// Interface between the host-side solver (CG/line search state machines, one per
+// patch) and the device pipeline that evaluates the MOMFBD metric, gradients and
+// restoration sums for a batch of patches in one set of kernel launches.
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace mfbd {
+
+struct Geometry {
+    int s, P, N, K, nObj, nCh, nImg, dataY, dataX, startY, startX;
+};
+
+struct Structure {
+    std::vector<int> chObj, imgCh, imgObj;   // per channel / per image
+    std::vector<double> weight, wavelength, freqCutoff;
+    std::vector<int> nImgObj;                 // images per object
+    double graddiffStep = 1e-2;               // GRADDIFF_STEP (times the wavelength)
+};
+
+// One evaluation request: metric at alpha for patch `slot`, optionally a gradient.
+struct EvalReq {
+    int slot = 0;
+    bool wantGrad = false;
+    int gradMethod = 2;     // 1 finite difference, 2 Vogel, 3 adjoint
+    int nEnabled = 0;       // gradient for modes [0, nEnabled)
+    const double* alpha = nullptr;   // [nImg][K], tilt offsets already added
+    double f = 0;                    // out: metric
+    double* g = nullptr;             // out: gradient [nImg][K] (when wantGrad)
+};
+
+// (Re)initialisation of one image: cutout at the current integer shift.
+struct ImgInit {
+    int slot, img, shiftX, shiftY;
+};
+
+struct KernelStat {
+    std::string name;
+    double ms = 0;
+    int launches = 0;
+};
+
+class DeviceEvaluator {
+public:
+    virtual ~DeviceEvaluator() = default;
+    virtual void set_basis(const double* pupil, const double* modes) = 0;
+    virtual void set_structure(const Structure& st, const std::vector<double>& otfNorm) = 0;
+    // raw frames of `n` patches into slots [first, first+n); phiCh [n][nCh][P*P]
+    virtual void load(int first, int n, const float* data, const double* phiCh) = 0;
+    virtual void set_phi_channel(int slot, const double* phiCh) = 0;
+    // windowing, padding and FFT of the listed images, then ftSum of the touched objects
+    virtual void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window) = 0;
+    virtual void set_gamma(int slot, const double* gamma) = 0;
+    virtual void evaluate(std::vector<EvalReq>& reqs) = 0;
+    // restoration sums at alpha: P0 = sum conj(D) S, Q0 = sum |S|^2 (no gamma),
+    // per object on the OTF support: P0 [nObj][nOS] complex, Q0 [nObj][nOS]
+    virtual void restore_sums(int slot, const double* alpha, double* P0, double* Q0) = 0;
+    virtual int nOS() const = 0;
+    virtual const std::vector<int>& otf_support() const = 0;
+    virtual void debug_get(const std::string& what, int slot, std::vector<double>& out) = 0;
+    virtual std::vector<KernelStat> kernel_stats() = 0;
+    virtual void reset_stats() = 0;
+    virtual int capacity() const = 0;
+};
+
+std::unique_ptr<DeviceEvaluator> make_device_evaluator(const Geometry& g, int precision, int maxSlots,
+                                                       int maxBatch, int device, std::string& err);
+
+}  // namespace mfbd

@@ -1,0 +1,1129 @@
+// Device pipeline of the MOMFBD patch solver for gfx950 (MI355X).
+//
+// Per evaluation of a batch of requests (patch slot + alpha):
+//   k_phase        phi = phi_channel + B alpha           (mode expansion, subimage.cpp:509-544)
+//   k_field        a = otfNorm * pupil * exp(i phi) on the OTF grid  (subimage.cpp:586-612)
+//   FFT2 / |.|^2 / IFFT2 / k_extract   S = reorder(IFFT(|FFT(a)|^2)/N^2)  (fouriertransform.hpp:280-286)
+//   k_pq           P = sum conj(D) S, Q = gamma + sum |S|^2 and metric partials
+//                  (subimage.cpp:226-234, object.cpp:459-462, 508-518, 597-615)
+//   k_metric       L = sum_o w_o L_o / patchSize^2        (solver.cpp:763-817)
+// Gradient (Vogel / adjoint, subimage.cpp:312-390):
+//   k_build_g, IFFT2, k_save_re, k_field(de-centred), IFFT2, k_mul_re, FFT2, k_extract_v, k_grad
+// Finite-difference gradient (subimage.cpp:261-292): k_phase_pert + OTF chain + k_metric_change.
+//
+// FFTs: batched 2-D transforms as a row pass and a column pass; each workgroup
+// stages a tile of lines in LDS and runs a self-sorting (Stockham) radix-16/8
+// transform with the radix butterflies held in registers.  Twiddles come from a
+// table computed on the host in long double.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "evaluator.hpp"
+
+namespace mfbd {
+
+#define HIPCHK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " #x); \
+    } while (0)
+
+template <typename T>
+struct cx {
+    T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+    return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+
+__device__ __forceinline__ void sincos_t(double a, double* s, double* c) { sincos(a, s, c); }
+__device__ __forceinline__ void sincos_t(float a, float* s, float* c) { sincosf(a, s, c); }
+
+// ---------------------------------------------------------------- FFT core ----
+
+constexpr int bitrev(int i, int R) {
+    int r = 0;
+    for (int b = 1; b < R; b <<= 1) { r = (r << 1) | (i & 1); i >>= 1; }
+    return r;
+}
+
+// 16th roots of unity exp(-2 pi i k / 16), k = 0..15
+__device__ __forceinline__ double root_c(int k) {
+    constexpr double c[16] = {1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+                              0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+                              -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+                              0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+    return c[k & 15];
+}
+__device__ __forceinline__ double root_s(int k) { return -root_c((k + 12) & 15); }   // -sin(2 pi k/16)
+
+// in-register DFT of R points (R power of two <= 16), DIR = -1 forward, +1 backward
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void dft_reg(cx<T>* v) {
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        const int j = bitrev(i, R);
+        if (i < j) { cx<T> t = v[i]; v[i] = v[j]; v[j] = t; }
+    }
+#pragma unroll
+    for (int L = 2; L <= R; L <<= 1) {
+#pragma unroll
+        for (int i = 0; i < R; i += L) {
+#pragma unroll
+            for (int j = 0; j < L / 2; ++j) {
+                const int k = j * (16 / L);
+                cx<T> w = {(T)root_c(k), (T)(DIR < 0 ? root_s(k) : -root_s(k))};
+                cx<T> t = (j == 0) ? v[i + j + L / 2] : cmul(v[i + j + L / 2], w);
+                v[i + j + L / 2] = csub(v[i + j], t);
+                v[i + j] = cadd(v[i + j], t);
+            }
+        }
+    }
+}
+
+// One Stockham pass of radix R over NL lines of length N held in LDS.
+// Element k of line l is at buf[l*ls + k*es].  NS = span already transformed.
+template <typename T, int N, int R, int NS, int DIR, int NL, int NT>
+__device__ __forceinline__ void spass(cx<T>* buf, int ls, int es, const cx<T>* __restrict__ tw) {
+    constexpr int NB = NL * (N / R);
+    constexpr int BT = (NB + NT - 1) / NT;
+    cx<T> v[BT][R];
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = b / (N / R), j = b % (N / R);
+            const cx<T>* L = buf + line * ls;
+            const int k = j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cx<T> x = L[(j + r * (N / R)) * es];
+                if (NS > 1 && r > 0) {
+                    cx<T> w = tw[r * k * (N / (NS * R))];
+                    if (DIR > 0) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[i][r] = x;
+            }
+            dft_reg<T, R, DIR>(v[i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = b / (N / R), j = b % (N / R);
+            cx<T>* L = buf + line * ls;
+            const int k = j % NS;
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) L[(base + r * NS) * es] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, int N, int DIR, int NL, int NT>
+__device__ __forceinline__ void fft_lines(cx<T>* buf, int ls, int es, const cx<T>* __restrict__ tw) {
+    if constexpr (N == 64) {
+        spass<T, N, 8, 1, DIR, NL, NT>(buf, ls, es, tw);
+        spass<T, N, 8, 8, DIR, NL, NT>(buf, ls, es, tw);
+    } else if constexpr (N == 128) {
+        spass<T, N, 16, 1, DIR, NL, NT>(buf, ls, es, tw);
+        spass<T, N, 8, 16, DIR, NL, NT>(buf, ls, es, tw);
+    } else if constexpr (N == 256) {
+        spass<T, N, 16, 1, DIR, NL, NT>(buf, ls, es, tw);
+        spass<T, N, 16, 16, DIR, NL, NT>(buf, ls, es, tw);
+    } else {
+        static_assert(N == 64 || N == 128 || N == 256, "unsupported OTF grid");
+    }
+}
+
+// Row pass: each block transforms LR consecutive rows of one N x N tile in place.
+template <typename T, int N, int DIR, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_fft_rows(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw) {
+    constexpr int NT = LR * N / 16;
+    __shared__ cx<T> lds[LR * N];
+    const int chunks = N / LR;
+    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    cx<T>* g = data + tile * N * N + chunk * LR * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) lds[e] = g[e];
+    __syncthreads();
+    fft_lines<T, N, DIR, LR, NT>(lds, N, 1, tw);
+    for (int e = threadIdx.x; e < LR * N; e += NT) g[e] = lds[e];
+}
+
+// Column pass: each block transforms LC consecutive columns of one tile in place.
+template <typename T, int N, int DIR, int LC>
+__global__ void __launch_bounds__(LC * N / 16) k_fft_cols(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw) {
+    constexpr int NT = LC * N / 16;
+    __shared__ cx<T> lds[LC * N];
+    const int chunks = N / LC;
+    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    cx<T>* g = data + tile * N * N + chunk * LC;
+    for (int e = threadIdx.x; e < LC * N; e += NT) {
+        const int r = e / LC, c = e % LC;
+        lds[e] = g[(size_t)r * N + c];
+    }
+    __syncthreads();
+    fft_lines<T, N, DIR, LC, NT>(lds, 1, LC, tw);
+    for (int e = threadIdx.x; e < LC * N; e += NT) {
+        const int r = e / LC, c = e % LC;
+        g[(size_t)r * N + c] = lds[e];
+    }
+}
+
+// ------------------------------------------------------------ elementwise ----
+
+struct QMap {             // virtual image q -> source
+    const int* slot;      // patch slot
+    const int* img;       // image index within the patch
+    const int* arow;      // alpha row
+};
+
+// phi[q][i] = phiCh[slot][ch][i] + sum_m alpha[arow][m] * B[obj][m][i]
+// Block: 16 images x 256 pixels; alpha rows staged in LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) k_phase(T* __restrict__ phi, const T* __restrict__ B, const T* __restrict__ phiCh,
+                                               const double* __restrict__ alpha, QMap qm, int nQ, int K, int nPS,
+                                               const int* __restrict__ imgObj, const int* __restrict__ imgCh, int nCh) {
+    constexpr int IMG = 16;
+    __shared__ T sa[IMG][160];
+    const int q0 = blockIdx.y * IMG;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    for (int e = threadIdx.x; e < IMG * K; e += 256) {
+        const int j = e / K, m = e % K;
+        const int q = q0 + j;
+        sa[j][m] = (q < nQ) ? (T)alpha[(size_t)qm.arow[q] * K + m] : T(0);
+    }
+    __syncthreads();
+    if (i >= nPS) return;
+    T acc[IMG];
+    const T* bq[IMG];
+#pragma unroll
+    for (int j = 0; j < IMG; ++j) {
+        const int q = min(q0 + j, nQ - 1);
+        const int img = qm.img[q];
+        acc[j] = phiCh[((size_t)qm.slot[q] * nCh + imgCh[img]) * nPS + i];
+        bq[j] = B + (size_t)imgObj[img] * K * nPS + i;
+    }
+    for (int m = 0; m < K; ++m) {
+#pragma unroll
+        for (int j = 0; j < IMG; ++j) acc[j] = acc[j] + sa[j][m] * bq[j][(size_t)m * nPS];
+    }
+#pragma unroll
+    for (int j = 0; j < IMG; ++j)
+        if (q0 + j < nQ) phi[(size_t)(q0 + j) * nPS + i] = acc[j];
+}
+
+// phiOut[q'] = phi[base(q')] + step * B[obj][mode(q')]   (finite-difference probes)
+template <typename T>
+__global__ void k_phase_pert(T* __restrict__ out, const T* __restrict__ phi, const T* __restrict__ B,
+                             const int* __restrict__ base, const int* __restrict__ mode, const int* __restrict__ obj,
+                             const T* __restrict__ step, int nQ, int K, int nPS) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * nPS) return;
+    const int q = e / nPS, i = e % nPS;
+    out[e] = phi[(size_t)base[q] * nPS + i] + step[q] * B[((size_t)obj[q] * K + mode[q]) * nPS + i];
+}
+
+// tile[q][e] = scale[q] * pupil * exp(i phi) where inv[e] names the pupil pixel (or -1)
+template <typename T>
+__global__ void k_field(cx<T>* __restrict__ tiles, const T* __restrict__ phi, const int* __restrict__ inv,
+                        const T* __restrict__ pupil, const T* __restrict__ scale, int nQ, int N2, int nPS) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * N2) return;
+    const int q = e / N2, p = e % N2;
+    const int idx = inv[p];
+    cx<T> v = {T(0), T(0)};
+    if (idx >= 0) {
+        T s, c;
+        sincos_t(phi[(size_t)q * nPS + idx], &s, &c);
+        const T mag = pupil[idx] * scale[q];
+        v = {mag * c, mag * s};
+    }
+    tiles[e] = v;
+}
+
+template <typename T>
+__global__ void k_power(cx<T>* __restrict__ tiles, size_t n) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    cx<T> v = tiles[e];
+    tiles[e] = {v.x * v.x + v.y * v.y, T(0)};
+}
+
+// S[q][u] = tile[q][osNat[u]] * scale
+template <typename T>
+__global__ void k_extract(cx<T>* __restrict__ S, const cx<T>* __restrict__ tiles, const int* __restrict__ osNat,
+                          int nQ, int nOS, int N2, T scale) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * nOS) return;
+    const int q = e / nOS, u = e % nOS;
+    cx<T> v = tiles[(size_t)q * N2 + osNat[u]];
+    S[e] = {v.x * scale, v.y * scale};
+}
+
+// Block reduction of a double (deterministic tree), blockDim = 256
+__device__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// P, Q on the OTF support for (request r, object o, chunk of 256 support points)
+// plus the chunk's metric partial sum(ftSum - |P|^2/Q).  Images are summed per
+// channel in groups of 30 (Solver::metric task split, solver.cpp:756-797).
+template <typename T>
+__global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* __restrict__ Qout,
+                                            double* __restrict__ partial, const cx<T>* __restrict__ S,
+                                            const cx<T>* __restrict__ D, const double* __restrict__ ftSum,
+                                            const double* __restrict__ gamma, const int* __restrict__ reqSlot,
+                                            const int* __restrict__ chObj, const int* __restrict__ chFirst,
+                                            const int* __restrict__ chCount, int nCh, int nObj, int nImg, int nOS,
+                                            int useGamma) {
+    __shared__ double sh[256];
+    const int r = blockIdx.z, o = blockIdx.y;
+    const int u = blockIdx.x * 256 + threadIdx.x;
+    const int slot = reqSlot[r];
+    double pr = 0, pi = 0, q = 0, part = 0;
+    if (u < nOS) {
+        q = useGamma ? gamma[(size_t)slot * nObj + o] : 0.0;
+        for (int c = 0; c < nCh; ++c) {
+            if (chObj[c] != o) continue;
+            const int f0 = chFirst[c], n = chCount[c];
+            for (int b = 0; b < n; b += 30) {
+                const int e = min(b + 30, n);
+                double tr = 0, ti = 0, tq = 0;
+                for (int j = f0 + b; j < f0 + e; ++j) {
+                    const cx<T> s = S[((size_t)r * nImg + j) * nOS + u];
+                    const cx<T> d = D[((size_t)slot * nImg + j) * nOS + u];
+                    const double sx = s.x, sy = s.y, dx = d.x, dy = d.y;
+                    tq += sx * sx + sy * sy;
+                    tr += dx * sx + dy * sy;      // conj(d) * s
+                    ti += dx * sy - dy * sx;
+                }
+                q += tq;
+                pr += tr;
+                pi += ti;
+            }
+        }
+        const size_t k = ((size_t)r * nObj + o) * nOS + u;
+        Pout[2 * k] = pr;
+        Pout[2 * k + 1] = pi;
+        Qout[k] = q;
+        part = ftSum[((size_t)slot * nObj + o) * nOS + u] - (pr * pr + pi * pi) / q;
+    }
+    const double s = block_sum(part, sh);
+    if (threadIdx.x == 0) partial[((size_t)r * nObj + o) * gridDim.x + blockIdx.x] = s;
+}
+
+// L[r] = sum_o w_o * (sum_chunks partial) / s^2
+__global__ void k_metric(double* __restrict__ L, const double* __restrict__ partial, const double* __restrict__ w,
+                         int nReq, int nObj, int nChunk, double invS2) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nReq) return;
+    double sum = 0;
+    for (int o = 0; o < nObj; ++o) {
+        double m = 0;
+        for (int c = 0; c < nChunk; ++c) m += partial[((size_t)r * nObj + o) * nChunk + c];
+        if (w[o] != 0.0) sum += w[o] * (m * invS2);
+    }
+    L[r] = sum;
+}
+
+// G' (de-centred): tile[q][e] = (PQ D - PS S)/QS on the support, else 0
+template <typename T>
+__global__ void k_build_g(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ S, const cx<T>* __restrict__ D,
+                          const double* __restrict__ P, const double* __restrict__ Q, const int* __restrict__ invOs,
+                          QMap qm, const int* __restrict__ qReq, const int* __restrict__ imgObj, int nQ, int N2,
+                          int nOS, int nImg, int nObj) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * N2) return;
+    const int q = e / N2, p = e % N2;
+    const int u = invOs[p];
+    cx<T> v = {T(0), T(0)};
+    if (u >= 0) {
+        const int img = qm.img[q], r = qReq[q], o = imgObj[img];
+        const size_t k = ((size_t)r * nObj + o) * nOS + u;
+        const double pr = P[2 * k], pi = P[2 * k + 1], qq = Q[k];
+        const double pqr = pr * qq, pqi = pi * qq, ps = pr * pr + pi * pi, qs = qq * qq;
+        const cx<T> d = D[((size_t)qm.slot[q] * nImg + img) * nOS + u];
+        const cx<T> s = S[((size_t)r * nImg + img) * nOS + u];
+        const double gr = (pqr * d.x - pqi * d.y) - ps * s.x;
+        const double gi = (pqr * d.y + pqi * d.x) - ps * s.y;
+        v = {(T)(gr / qs), (T)(gi / qs)};
+    }
+    tiles[e] = v;
+}
+
+template <typename T>
+__global__ void k_save_re(T* __restrict__ out, const cx<T>* __restrict__ tiles, size_t n, T scale) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) out[e] = tiles[e].x * scale;
+}
+
+template <typename T>
+__global__ void k_mul_re(cx<T>* __restrict__ tiles, const T* __restrict__ re, size_t n, T scale) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const T f = re[e] * scale;
+    cx<T> v = tiles[e];
+    tiles[e] = {v.x * f, v.y * f};
+}
+
+// v[q][i] = Im(conj(PF_i) t[pde_i]) * (pupil_i for the reference form, 1 for the adjoint)
+template <typename T>
+__global__ void k_extract_v(T* __restrict__ v, const cx<T>* __restrict__ tiles, const T* __restrict__ phi,
+                            const int* __restrict__ pde, const T* __restrict__ pupil, int nQ, int nPS, int N2,
+                            int extraPupil) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * nPS) return;
+    const int q = e / nPS, i = e % nPS;
+    T s, c;
+    sincos_t(phi[e], &s, &c);
+    const T pu = pupil[i];
+    const cx<T> t = tiles[(size_t)q * N2 + pde[i]];
+    const T pfr = pu * c, pfi = pu * s;
+    const T im = pfr * t.y - pfi * t.x;     // Im(conj(pf) t)
+    v[e] = extraPupil ? im * pu : im;
+}
+
+// g[arow(q)][m] = scale[q] * sum_i v[q][i] * B[obj][m][i], m < nEnabled; 0 otherwise
+template <typename T>
+__global__ void __launch_bounds__(256) k_grad(double* __restrict__ g, const T* __restrict__ v, const T* __restrict__ B,
+                                              const double* __restrict__ scale, QMap qm,
+                                              const int* __restrict__ imgObj, int K, int nPS, int nEnabled) {
+    __shared__ double sh[256];
+    const int q = blockIdx.x;
+    const int o = imgObj[qm.img[q]];
+    const T* vq = v + (size_t)q * nPS;
+    double* gq = g + (size_t)qm.arow[q] * K;
+    for (int m = 0; m < K; ++m) {
+        double acc = 0;
+        if (m < nEnabled) {
+            const T* b = B + ((size_t)o * K + m) * nPS;
+            for (int i = threadIdx.x; i < nPS; i += 256) acc += (double)(vq[i] * b[i]);
+        }
+        const double s = block_sum(acc, sh);
+        if (threadIdx.x == 0) gq[m] = (m < nEnabled) ? scale[q] * s : 0.0;
+    }
+}
+
+// dl = -sum_u (q dn - dq |p|^2) / (q (q + dq)) / N^2 for one perturbed image
+// (SubImage::metricChange, subimage.cpp:261-278)
+template <typename T>
+__global__ void __launch_bounds__(256) k_metric_change(double* __restrict__ out, const cx<T>* __restrict__ Snew,
+                                                       const cx<T>* __restrict__ Sold, const cx<T>* __restrict__ D,
+                                                       const double* __restrict__ P, const double* __restrict__ Q,
+                                                       const int* __restrict__ baseQ, QMap qm,
+                                                       const int* __restrict__ qReq, const int* __restrict__ imgObj,
+                                                       int nOS, int nImg, int nObj, double invN2) {
+    __shared__ double sh[256];
+    const int q = blockIdx.x;          // perturbed probe
+    const int b = baseQ[q];            // the unperturbed image
+    const int img = qm.img[b], r = qReq[b], o = imgObj[img];
+    double dl = 0;
+    for (int u = threadIdx.x; u < nOS; u += 256) {
+        const cx<T> sn = Snew[(size_t)q * nOS + u];
+        const cx<T> so = Sold[((size_t)r * nImg + img) * nOS + u];
+        const cx<T> d = D[((size_t)qm.slot[b] * nImg + img) * nOS + u];
+        const size_t k = ((size_t)r * nObj + o) * nOS + u;
+        const double pr = P[2 * k], pi = P[2 * k + 1], qq = Q[k];
+        const double dsr = (double)sn.x - (double)so.x, dsi = (double)sn.y - (double)so.y;
+        const double dpr = d.x * dsr + d.y * dsi, dpi = d.x * dsi - d.y * dsr;   // conj(d) * ds
+        const double dq = 2.0 * (so.x * dsr + so.y * dsi) + (dsr * dsr + dsi * dsi);
+        const double dn = 2.0 * (dpr * pr + dpi * pi) + (dpr * dpr + dpi * dpi);
+        dl -= (qq * dn - dq * (pr * pr + pi * pi)) / (qq * (qq + dq));
+    }
+    const double s = block_sum(dl, sh);
+    if (threadIdx.x == 0) out[q] = s * invN2;
+}
+
+// ------------------------------------------------------- preprocessing ----
+// SubImage::getWindowedImg + padding of SubImage::initialize (subimage.cpp:106-209):
+// cut s x s at the shifted start, (x - mean) * window + mean, pad to N x N with the
+// windowed mean.  One block per image.
+template <typename T>
+__global__ void __launch_bounds__(256) k_window(cx<T>* __restrict__ tiles, const float* __restrict__ raw,
+                                                const double* __restrict__ window, const int* __restrict__ slot,
+                                                const int* __restrict__ img, const int* __restrict__ sx,
+                                                const int* __restrict__ sy, int nImg, int dataY, int dataX,
+                                                int startY, int startX, int s, int N) {
+    __shared__ double sh[256];
+    const int q = blockIdx.x;
+    const float* blk = raw + ((size_t)slot[q] * nImg + img[q]) * dataY * dataX;
+    const int y0 = startY + sy[q], x0 = startX + sx[q];
+    double acc = 0;
+    for (int e = threadIdx.x; e < s * s; e += 256) acc += (double)blk[(size_t)(y0 + e / s) * dataX + x0 + e % s];
+    const double avg = block_sum(acc, sh) / (double)(s * s);
+    acc = 0;
+    for (int e = threadIdx.x; e < s * s; e += 256) {
+        const double v = ((double)blk[(size_t)(y0 + e / s) * dataX + x0 + e % s] - avg) * window[e] + avg;
+        acc += v;
+    }
+    const double mean = block_sum(acc, sh) / (double)(s * s);
+    const int off = (N - s) / 2;
+    cx<T>* t = tiles + (size_t)q * N * N;
+    for (int e = threadIdx.x; e < N * N; e += 256) {
+        const int y = e / N - off, x = e % N - off;
+        double v = mean;
+        if (y >= 0 && y < s && x >= 0 && x < s)
+            v = ((double)blk[(size_t)(y0 + y) * dataX + x0 + x] - avg) * window[y * s + x] + avg;
+        t[e] = {(T)v, T(0)};
+    }
+}
+
+// D[slot][img][u] = tile[q][osNat[u]]
+template <typename T>
+__global__ void k_store_d(cx<T>* __restrict__ D, const cx<T>* __restrict__ tiles, const int* __restrict__ osNat,
+                          const int* __restrict__ slot, const int* __restrict__ img, int nQ, int nOS, int nImg, int N2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * nOS) return;
+    const int q = e / nOS, u = e % nOS;
+    D[((size_t)slot[q] * nImg + img[q]) * nOS + u] = tiles[(size_t)q * N2 + osNat[u]];
+}
+
+// ftSum[slot][o][u] = sum over the object's images of |D|^2 (Object::addToFT, object.cpp:471-478)
+template <typename T>
+__global__ void k_ftsum(double* __restrict__ ftSum, const cx<T>* __restrict__ D, const int* __restrict__ slotObj,
+                        const int* __restrict__ imgObj, int nPairs, int nImg, int nObj, int nOS) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nPairs * nOS) return;
+    const int k = e / nOS, u = e % nOS;
+    const int slot = slotObj[2 * k], o = slotObj[2 * k + 1];
+    double acc = 0;
+    for (int j = 0; j < nImg; ++j) {
+        if (imgObj[j] != o) continue;
+        const cx<T> d = D[((size_t)slot * nImg + j) * nOS + u];
+        acc = acc + ((double)d.x * d.x + (double)d.y * d.y);
+    }
+    ftSum[((size_t)slot * nObj + o) * nOS + u] = acc;
+}
+
+// ------------------------------------------------------------- evaluator ----
+
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (count <= n) return;
+        if (p) HIPCHK(hipFree(p));
+        HIPCHK(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void free_() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+template <typename T>
+void upload(DBuf<T>& b, const T* h, size_t n, hipStream_t st) {
+    b.alloc(std::max<size_t>(n, 1));
+    if (n) HIPCHK(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+}
+
+template <typename T>
+void upload(DBuf<T>& b, const std::vector<T>& h, hipStream_t st) { upload(b, h.data(), h.size(), st); }
+
+inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+template <typename T, int N>
+class Pipeline final : public DeviceEvaluator {
+public:
+    using C = cx<T>;
+    static constexpr int N2 = N * N;
+    static constexpr int LINES = sizeof(T) == 8 ? 16 : 32;   // lines per FFT block (32 KB of LDS)
+
+    Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device) : g_(g), maxSlots_(maxSlots),
+                                                                         maxBatch_(maxBatch) {
+        HIPCHK(hipSetDevice(device));
+        HIPCHK(hipStreamCreate(&st_));
+        HIPCHK(hipEventCreate(&ev0_));
+        HIPCHK(hipEventCreate(&ev1_));
+        std::vector<C> tw(N);
+        for (int k = 0; k < N; ++k) {
+            long double a = -2.0L * 3.14159265358979323846264338327950288L * k / N;
+            tw[k] = {(T)cosl(a), (T)sinl(a)};
+        }
+        upload(tw_, tw, st_);
+        timing_ = getenv("MFBD_KERNEL_TIMING") != nullptr;
+    }
+
+    ~Pipeline() override {
+        for (auto* b : {&Pp_, &Qp_, &part_, &L_, &g_d_, &alpha_, &ftSum_, &gamma_, &win_, &gscale_, &fdOut_, &w_})
+            b->free_();
+        for (auto* b : {&tiles_, &D_, &S_, &Sfd_}) b->free_();
+        for (auto* b : {&phi_, &phiFd_, &B_, &phiCh_, &pupil_, &scale_, &v_, &greal_, &fdStep_}) b->free_();
+        for (auto* b : {&ints_, &invPio_, &invPde_, &osNat_, &invOs_, &pde_, &chObj_, &chFirst_, &chCount_,
+                        &imgObj_, &imgCh_})
+            b->free_();
+        rawF_.free_();
+        tw_.free_();
+        (void)hipEventDestroy(ev0_);
+        (void)hipEventDestroy(ev1_);
+        (void)hipStreamDestroy(st_);
+    }
+
+    int capacity() const override { return maxSlots_; }
+    int nOS() const override { return nOS_; }
+    const std::vector<int>& otf_support() const override { return os_; }
+
+    void set_basis(const double* pupil, const double* modes) override {
+        const int P = g_.P, P2 = P * P;
+        ps_.clear();
+        std::vector<int> pio;
+        for (int i = 0; i < P2; ++i)
+            if (pupil[i] > 1e-9) {
+                ps_.push_back(i);
+                pio.push_back((i / P) * N + (i % P) + P / 2 + (P / 2) * N);
+            }
+        nPS_ = (int)ps_.size();
+        std::vector<int> invPio(N2, -1), invPde(N2, -1), pde(nPS_);
+        std::vector<T> pup(nPS_);
+        for (int k = 0; k < nPS_; ++k) {
+            invPio[pio[k]] = k;
+            const int y = ps_[k] / P, x = ps_[k] % P;
+            const int yd = ((y - P / 2) % N + N) % N, xd = ((x - P / 2) % N + N) % N;
+            pde[k] = yd * N + xd;
+            invPde[pde[k]] = k;
+            pup[k] = (T)pupil[ps_[k]];
+        }
+        // OTF support: autocorrelation of the pupil above 1e-9 (Pupil::generateSupport)
+        os_ = otf_support_of(pupil);
+        nOS_ = (int)os_.size();
+        std::vector<int> osNat(nOS_), invOs(N2, -1);
+        for (int u = 0; u < nOS_; ++u) {
+            const int cy = os_[u] / N, cx_ = os_[u] % N;
+            osNat[u] = ((cy + N / 2) % N) * N + (cx_ + N / 2) % N;
+            invOs[osNat[u]] = u;
+        }
+        upload(invPio_, invPio, st_);
+        upload(invPde_, invPde, st_);
+        upload(pde_, pde, st_);
+        upload(pupil_, pup, st_);
+        upload(osNat_, osNat, st_);
+        upload(invOs_, invOs, st_);
+        area_ = 0;
+        for (int k = 0; k < nPS_; ++k) area_ += pupil[ps_[k]];
+        const int K = g_.K;
+        std::vector<T> Bc((size_t)g_.nObj * K * nPS_);
+        for (int o = 0; o < g_.nObj; ++o)
+            for (int m = 0; m < K; ++m)
+                for (int k = 0; k < nPS_; ++k)
+                    Bc[((size_t)o * K + m) * nPS_ + k] = (T)modes[((size_t)o * K + m) * P2 + ps_[k]];
+        upload(B_, Bc, st_);
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void set_structure(const Structure& s, const std::vector<double>& otfNorm) override {
+        st = s;
+        otfNorm_ = otfNorm;
+        std::vector<int> chFirst(g_.nCh, 0), chCount(g_.nCh, 0);
+        for (int c = 0; c < g_.nCh; ++c) {
+            chFirst[c] = -1;
+            for (int j = 0; j < g_.nImg; ++j)
+                if (s.imgCh[j] == c) {
+                    if (chFirst[c] < 0) chFirst[c] = j;
+                    chCount[c]++;
+                }
+            if (chFirst[c] < 0) chFirst[c] = 0;
+        }
+        upload(chObj_, s.chObj, st_);
+        upload(chFirst_, chFirst, st_);
+        upload(chCount_, chCount, st_);
+        upload(imgObj_, s.imgObj, st_);
+        upload(imgCh_, s.imgCh, st_);
+        upload(w_, s.weight, st_);
+        const size_t nS = maxSlots_;
+        D_.alloc(nS * g_.nImg * nOS_);
+        ftSum_.alloc(nS * g_.nObj * nOS_);
+        gamma_.alloc(nS * g_.nObj);
+        phiCh_.alloc(nS * g_.nCh * nPS_);
+        rawF_.alloc(nS * g_.nImg * (size_t)g_.dataY * g_.dataX);
+        HIPCHK(hipMemsetAsync(gamma_.p, 0, nS * g_.nObj * sizeof(double), st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void load(int first, int n, const float* data, const double* phiCh) override {
+        const size_t per = (size_t)g_.nImg * g_.dataY * g_.dataX;
+        HIPCHK(hipMemcpyAsync(rawF_.p + (size_t)first * per, data, n * per * sizeof(float), hipMemcpyHostToDevice, st_));
+        for (int k = 0; k < n; ++k) set_phi_channel(first + k, phiCh + (size_t)k * g_.nCh * g_.P * g_.P);
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void set_phi_channel(int slot, const double* phiCh) override {
+        std::vector<T> h((size_t)g_.nCh * nPS_);
+        for (int c = 0; c < g_.nCh; ++c)
+            for (int k = 0; k < nPS_; ++k) h[(size_t)c * nPS_ + k] = (T)phiCh[(size_t)c * g_.P * g_.P + ps_[k]];
+        HIPCHK(hipMemcpyAsync(phiCh_.p + (size_t)slot * g_.nCh * nPS_, h.data(), h.size() * sizeof(T),
+                              hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void set_gamma(int slot, const double* gamma) override {
+        HIPCHK(hipMemcpyAsync(gamma_.p + (size_t)slot * g_.nObj, gamma, g_.nObj * sizeof(double),
+                              hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window) override {
+        if (imgs.empty()) return;
+        upload(win_, window, st_);
+        const int nQ = (int)imgs.size();
+        const int chunk = maxTiles();
+        std::vector<int> hs(nQ), hi(nQ), hx(nQ), hy(nQ);
+        for (int q = 0; q < nQ; ++q) {
+            hs[q] = imgs[q].slot;
+            hi[q] = imgs[q].img;
+            hx[q] = imgs[q].shiftX;
+            hy[q] = imgs[q].shiftY;
+        }
+        for (int b = 0; b < nQ; b += chunk) {
+            const int n = std::min(chunk, nQ - b);
+            std::vector<int> ints;
+            ints.insert(ints.end(), hs.begin() + b, hs.begin() + b + n);
+            ints.insert(ints.end(), hi.begin() + b, hi.begin() + b + n);
+            ints.insert(ints.end(), hx.begin() + b, hx.begin() + b + n);
+            ints.insert(ints.end(), hy.begin() + b, hy.begin() + b + n);
+            upload(ints_, ints, st_);
+            const int* ds = ints_.p;
+            const int* di = ds + n;
+            const int* dx = di + n;
+            const int* dy = dx + n;
+            tiles_.alloc((size_t)chunk * N2);
+            launch("k_window", [&] {
+                hipLaunchKernelGGL(k_window<T>, dim3(n), dim3(256), 0, st_, tiles_.p, rawF_.p, win_.p, ds, di, dx, dy,
+                                   g_.nImg, g_.dataY, g_.dataX, g_.startY, g_.startX, g_.s, N);
+            });
+            fft2(tiles_.p, n, -1);
+            launch("k_store_d", [&] {
+                hipLaunchKernelGGL(k_store_d<T>, dim3(nblk((size_t)n * nOS_)), dim3(256), 0, st_, D_.p, tiles_.p,
+                                   osNat_.p, ds, di, n, nOS_, g_.nImg, N2);
+            });
+        }
+        // ftSum for every touched (slot, object)
+        std::vector<int> pairs;
+        std::map<std::pair<int, int>, int> seen;
+        for (auto& im : imgs) {
+            auto key = std::make_pair(im.slot, st.imgObj[im.img]);
+            if (!seen.count(key)) {
+                seen[key] = 1;
+                pairs.push_back(key.first);
+                pairs.push_back(key.second);
+            }
+        }
+        upload(ints_, pairs, st_);
+        const int nPairs = (int)pairs.size() / 2;
+        launch("k_ftsum", [&] {
+            hipLaunchKernelGGL(k_ftsum<T>, dim3(nblk((size_t)nPairs * nOS_)), dim3(256), 0, st_, ftSum_.p, D_.p,
+                               ints_.p, imgObj_.p, nPairs, g_.nImg, g_.nObj, nOS_);
+        });
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void evaluate(std::vector<EvalReq>& reqs) override {
+        const int nR = (int)reqs.size();
+        if (!nR) return;
+        const int K = g_.K, nImg = g_.nImg;
+        // requests are processed in chunks that fit the tile workspace
+        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / nImg));
+        for (int b = 0; b < nR; b += perChunk) {
+            const int n = std::min(perChunk, nR - b);
+            eval_chunk(&reqs[b], n, K, nImg);
+        }
+    }
+
+    void restore_sums(int slot, const double* alpha, double* P0, double* Q0) override {
+        EvalReq r;
+        r.slot = slot;
+        r.alpha = alpha;
+        forward(&r, 1, /*useGamma=*/0);
+        const size_t n = (size_t)g_.nObj * nOS_;
+        HIPCHK(hipMemcpyAsync(P0, Pp_.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipMemcpyAsync(Q0, Qp_.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void debug_get(const std::string& what, int slot, std::vector<double>& out) override {
+        auto getc = [&](const C* p, size_t n) {
+            std::vector<C> h(n);
+            HIPCHK(hipMemcpy(h.data(), p, n * sizeof(C), hipMemcpyDeviceToHost));
+            out.resize(2 * n);
+            for (size_t k = 0; k < n; ++k) { out[2 * k] = h[k].x; out[2 * k + 1] = h[k].y; }
+        };
+        auto getd = [&](const double* p, size_t n) {
+            out.resize(n);
+            HIPCHK(hipMemcpy(out.data(), p, n * sizeof(double), hipMemcpyDeviceToHost));
+        };
+        HIPCHK(hipStreamSynchronize(st_));
+        const size_t nI = (size_t)g_.nImg * nOS_, nO = (size_t)g_.nObj * nOS_;
+        if (what == "D") getc(D_.p + slot * nI, nI);
+        else if (what == "S") getc(S_.p + slot * nI, nI);           // slot = request index
+        else if (what == "P") getd(Pp_.p + slot * 2 * nO, 2 * nO);
+        else if (what == "Q") getd(Qp_.p + slot * nO, nO);
+        else if (what == "ftsum") getd(ftSum_.p + slot * nO, nO);
+        else if (what == "gamma") getd(gamma_.p + slot * g_.nObj, g_.nObj);
+        else if (what == "phi") {
+            const size_t n = (size_t)g_.nImg * nPS_;
+            std::vector<T> h(n);
+            HIPCHK(hipMemcpy(h.data(), phi_.p + slot * n, n * sizeof(T), hipMemcpyDeviceToHost));
+            out.assign(h.begin(), h.end());
+        } else out.clear();
+    }
+
+    std::vector<KernelStat> kernel_stats() override {
+        std::vector<KernelStat> v;
+        for (auto& e : stats_) v.push_back(e.second);
+        return v;
+    }
+    void reset_stats() override { stats_.clear(); }
+
+private:
+    int maxTiles() const {
+        // tile workspace bounded to ~2 GiB
+        const size_t cap = (size_t)2 << 30;
+        const size_t per = (size_t)N2 * sizeof(C);
+        return (int)std::max<size_t>(16, std::min<size_t>(cap / per, (size_t)maxBatch_ * g_.nImg));
+    }
+
+    template <typename F>
+    void launch(const char* name, F&& f) {
+        if (timing_) HIPCHK(hipEventRecord(ev0_, st_));
+        f();
+        HIPCHK(hipGetLastError());
+        if (timing_) {
+            HIPCHK(hipEventRecord(ev1_, st_));
+            HIPCHK(hipEventSynchronize(ev1_));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
+            auto& s = stats_[name];
+            s.name = name;
+            s.ms += ms;
+            s.launches++;
+        }
+    }
+
+    void fft2(C* tiles, int nTiles, int dir) {
+        const unsigned gr = nTiles * (N / LINES);
+        constexpr int NT = LINES * N / 16;
+        if (dir < 0) {
+            launch("fft_rows", [&] { hipLaunchKernelGGL((k_fft_rows<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_cols", [&] { hipLaunchKernelGGL((k_fft_cols<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+        } else {
+            launch("fft_rows", [&] { hipLaunchKernelGGL((k_fft_rows<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_cols", [&] { hipLaunchKernelGGL((k_fft_cols<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+        }
+    }
+
+    // OTF chain for nQ virtual images whose phases are in `phi`: S = reorder(IFFT(|FFT(a)|^2))/N^2
+    void otf_chain(const T* phi, const T* scale, int nQ, C* Sout) {
+        tiles_.alloc((size_t)nQ * N2);
+        launch("k_field", [&] {
+            hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, phi, invPio_.p,
+                               pupil_.p, scale, nQ, N2, nPS_);
+        });
+        fft2(tiles_.p, nQ, -1);
+        launch("k_power", [&] {
+            hipLaunchKernelGGL(k_power<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, (size_t)nQ * N2);
+        });
+        fft2(tiles_.p, nQ, +1);
+        launch("k_extract", [&] {
+            hipLaunchKernelGGL(k_extract<T>, dim3(nblk((size_t)nQ * nOS_)), dim3(256), 0, st_, Sout, tiles_.p,
+                               osNat_.p, nQ, nOS_, N2, (T)(1.0 / N2));
+        });
+    }
+
+    // phases, OTFs, P/Q and the metric for n requests (virtual images q = r*nImg + j)
+    void forward(EvalReq* rq, int n, int useGamma) {
+        const int K = g_.K, nImg = g_.nImg, nQ = n * nImg;
+        std::vector<double> ha((size_t)nQ * K);
+        std::vector<int> ints(4 * (size_t)nQ + n);
+        std::vector<T> sc(nQ);
+        for (int r = 0; r < n; ++r) {
+            std::memcpy(&ha[(size_t)r * nImg * K], rq[r].alpha, sizeof(double) * nImg * K);
+            for (int j = 0; j < nImg; ++j) {
+                const int q = r * nImg + j;
+                ints[q] = rq[r].slot;                 // slot
+                ints[nQ + q] = j;                     // img
+                ints[2 * nQ + q] = q;                 // alpha row
+                ints[3 * nQ + q] = r;                 // request
+                sc[q] = (T)otfNorm_[st.imgCh[j]];
+            }
+            ints[4 * nQ + r] = rq[r].slot;
+        }
+        upload(alpha_, ha, st_);
+        upload(ints_, ints, st_);
+        upload(scale_, sc, st_);
+        qm_ = {ints_.p, ints_.p + nQ, ints_.p + 2 * nQ};
+        qReq_ = ints_.p + 3 * nQ;
+        reqSlot_ = ints_.p + 4 * nQ;
+        phi_.alloc((size_t)nQ * nPS_);
+        S_.alloc((size_t)nQ * nOS_);
+        launch("k_phase", [&] {
+            hipLaunchKernelGGL(k_phase<T>, dim3(nblk(nPS_), (nQ + 15) / 16), dim3(256), 0, st_, phi_.p, B_.p, phiCh_.p,
+                               alpha_.p, qm_, nQ, K, nPS_, imgObj_.p, imgCh_.p, g_.nCh);
+        });
+        otf_chain(phi_.p, scale_.p, nQ, S_.p);
+        const int nChunk = (nOS_ + 255) / 256;
+        Pp_.alloc((size_t)n * g_.nObj * nOS_ * 2);
+        Qp_.alloc((size_t)n * g_.nObj * nOS_);
+        part_.alloc((size_t)n * g_.nObj * nChunk);
+        launch("k_pq", [&] {
+            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, Pp_.p, Qp_.p, part_.p, S_.p, D_.p,
+                               ftSum_.p, gamma_.p, reqSlot_, chObj_.p, chFirst_.p, chCount_.p, g_.nCh, g_.nObj, nImg,
+                               nOS_, useGamma);
+        });
+    }
+
+    void eval_chunk(EvalReq* rq, int n, int K, int nImg) {
+        forward(rq, n, 1);
+        const int nQ = n * nImg;
+        const int nChunk = (nOS_ + 255) / 256;
+        L_.alloc(n);
+        launch("k_metric", [&] {
+            hipLaunchKernelGGL(k_metric, dim3(nblk(n, 64)), dim3(64), 0, st_, L_.p, part_.p, w_.p, n, g_.nObj, nChunk,
+                               1.0 / ((double)g_.s * g_.s));
+        });
+        std::vector<double> hL(n);
+        // gradients
+        bool anyGrad = false;
+        for (int r = 0; r < n; ++r) anyGrad |= rq[r].wantGrad;
+        if (anyGrad) {
+            g_d_.alloc((size_t)nQ * K);
+            HIPCHK(hipMemsetAsync(g_d_.p, 0, (size_t)nQ * K * sizeof(double), st_));
+            gradients(rq, n, K, nImg);
+        }
+        HIPCHK(hipMemcpyAsync(hL.data(), L_.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        std::vector<double> hg;
+        if (anyGrad) {
+            hg.resize((size_t)nQ * K);
+            HIPCHK(hipMemcpyAsync(hg.data(), g_d_.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
+        }
+        HIPCHK(hipStreamSynchronize(st_));
+        for (int r = 0; r < n; ++r) {
+            rq[r].f = hL[r];
+            if (rq[r].wantGrad && rq[r].g) std::memcpy(rq[r].g, &hg[(size_t)r * nImg * K], sizeof(double) * nImg * K);
+        }
+    }
+
+    void gradients(EvalReq* rq, int n, int K, int nImg) {
+        const int nQ = n * nImg;
+        // Vogel / adjoint for requests that ask for them (all virtual images are
+        // processed; results of other requests are discarded by the zeroed scale)
+        bool anyV = false, anyFd = false;
+        for (int r = 0; r < n; ++r) {
+            if (!rq[r].wantGrad) continue;
+            if (rq[r].gradMethod == 1) anyFd = true;
+            else anyV = true;
+        }
+        if (anyV) {
+            std::vector<double> sc(nQ, 0.0);
+            int nEn = 0, extraPupil = 1;
+            for (int r = 0; r < n; ++r) {
+                if (!rq[r].wantGrad || rq[r].gradMethod == 1) continue;
+                nEn = std::max(nEn, rq[r].nEnabled);
+                for (int j = 0; j < nImg; ++j) {
+                    const int o = st.imgObj[j];
+                    const double w = st.weight[o];
+                    if (rq[r].gradMethod == 2) {
+                        // gradientVogel: -2 area/N^2 * w, times w again in Solver::gradient
+                        sc[r * nImg + j] = (w == 0.0) ? 0.0 : -2.0 * area_ / N2 * w * w;
+                    } else {
+                        // exact dL/dalpha = -(4 c^2 / s^2) w sum Im(conj(a) (G * a)), t = (G * PF)/N^2
+                        const double c = otfNorm_[st.imgCh[j]];
+                        sc[r * nImg + j] = -4.0 * c * c * N2 / ((double)g_.s * g_.s) * w;
+                        extraPupil = 0;
+                    }
+                }
+            }
+            upload(gscale_, sc, st_);
+            tiles_.alloc((size_t)nQ * N2);
+            greal_.alloc((size_t)nQ * N2);
+            v_.alloc((size_t)nQ * nPS_);
+            launch("k_build_g", [&] {
+                hipLaunchKernelGGL(k_build_g<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, S_.p, D_.p,
+                                   Pp_.p, Qp_.p, invOs_.p, qm_, qReq_, imgObj_.p, nQ, N2, nOS_, nImg, g_.nObj);
+            });
+            fft2(tiles_.p, nQ, +1);
+            launch("k_save_re", [&] {
+                hipLaunchKernelGGL(k_save_re<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, greal_.p, tiles_.p,
+                                   (size_t)nQ * N2, (T)(1.0 / N2));
+            });
+            std::vector<T> ones(nQ, T(1));
+            upload(scale_, ones, st_);
+            launch("k_field", [&] {
+                hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, phi_.p,
+                                   invPde_.p, pupil_.p, scale_.p, nQ, N2, nPS_);
+            });
+            fft2(tiles_.p, nQ, +1);
+            launch("k_mul_re", [&] {
+                hipLaunchKernelGGL(k_mul_re<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, greal_.p,
+                                   (size_t)nQ * N2, (T)(1.0 / N2));
+            });
+            fft2(tiles_.p, nQ, -1);
+            launch("k_extract_v", [&] {
+                hipLaunchKernelGGL(k_extract_v<T>, dim3(nblk((size_t)nQ * nPS_)), dim3(256), 0, st_, v_.p, tiles_.p,
+                                   phi_.p, pde_.p, pupil_.p, nQ, nPS_, N2, extraPupil);
+            });
+            launch("k_grad", [&] {
+                hipLaunchKernelGGL(k_grad<T>, dim3(nQ), dim3(256), 0, st_, g_d_.p, v_.p, B_.p, gscale_.p, qm_,
+                                   imgObj_.p, K, nPS_, nEn);
+            });
+        }
+        if (anyFd) fd_gradient(rq, n, K, nImg);
+    }
+
+    void fd_gradient(EvalReq* rq, int n, int K, int nImg) {
+        // probes (request r, image j, mode m < nEnabled): phi + step * B_m
+        std::vector<int> base, mode, obj;
+        std::vector<T> step, sc;
+        std::vector<double> wgt;
+        std::vector<std::pair<int, int>> dst;   // (alpha row, mode)
+        for (int r = 0; r < n; ++r) {
+            if (!rq[r].wantGrad || rq[r].gradMethod != 1) continue;
+            for (int j = 0; j < nImg; ++j) {
+                const int o = st.imgObj[j];
+                for (int m = 0; m < rq[r].nEnabled; ++m) {
+                    base.push_back(r * nImg + j);
+                    mode.push_back(m);
+                    obj.push_back(o);
+                    step.push_back((T)(st.graddiffStep * st.wavelength[o]));
+                    sc.push_back((T)otfNorm_[st.imgCh[j]]);
+                    wgt.push_back(st.weight[o]);
+                    dst.push_back({r * nImg + j, m});
+                }
+            }
+        }
+        const int nP = (int)base.size();
+        std::vector<double> res(nP);
+        const int chunk = maxTiles();
+        // probe tables live after the request tables in a second integer buffer
+        for (int b = 0; b < nP; b += chunk) {
+            const int m = std::min(chunk, nP - b);
+            std::vector<int> ints;
+            ints.insert(ints.end(), base.begin() + b, base.begin() + b + m);
+            ints.insert(ints.end(), mode.begin() + b, mode.begin() + b + m);
+            ints.insert(ints.end(), obj.begin() + b, obj.begin() + b + m);
+            upload(fdInts_, ints, st_);
+            upload(fdStep_, step.data() + b, m, st_);
+            upload(fdScale_, sc.data() + b, m, st_);
+            phiFd_.alloc((size_t)m * nPS_);
+            Sfd_.alloc((size_t)m * nOS_);
+            fdOut_.alloc(m);
+            launch("k_phase_pert", [&] {
+                hipLaunchKernelGGL(k_phase_pert<T>, dim3(nblk((size_t)m * nPS_)), dim3(256), 0, st_, phiFd_.p, phi_.p,
+                                   B_.p, fdInts_.p, fdInts_.p + m, fdInts_.p + 2 * m, fdStep_.p, m, K, nPS_);
+            });
+            otf_chain(phiFd_.p, fdScale_.p, m, Sfd_.p);
+            launch("k_metric_change", [&] {
+                hipLaunchKernelGGL(k_metric_change<T>, dim3(m), dim3(256), 0, st_, fdOut_.p, Sfd_.p, S_.p, D_.p, Pp_.p,
+                                   Qp_.p, fdInts_.p, qm_, qReq_, imgObj_.p, nOS_, nImg, g_.nObj, 1.0 / N2);
+            });
+            HIPCHK(hipMemcpyAsync(res.data() + b, fdOut_.p, m * sizeof(double), hipMemcpyDeviceToHost, st_));
+        }
+        HIPCHK(hipStreamSynchronize(st_));
+        // g = w * (dl / step * w); merge into the device gradient buffer
+        std::vector<double> hg((size_t)n * nImg * K);
+        HIPCHK(hipMemcpy(hg.data(), g_d_.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost));
+        for (int k = 0; k < nP; ++k) {
+            const double stp = (double)step[k];
+            const double w = wgt[k];
+            hg[(size_t)dst[k].first * K + dst[k].second] = (w == 0.0) ? 0.0 : w * (res[k] / stp * w);
+        }
+        HIPCHK(hipMemcpy(g_d_.p, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
+
+    std::vector<int> otf_support_of(const double* pupil) {
+        // autocorrelation of the pupil placed in the lower-left P x P block, centred,
+        // computed on the host in double (same threshold as the reference)
+        const int P = g_.P;
+        std::vector<double> ac((size_t)N2, 0.0);
+        for (int dy = -(P - 1); dy <= P - 1; ++dy)
+            for (int dx = -(P - 1); dx <= P - 1; ++dx) {
+                double acc = 0;
+                for (int y = std::max(0, -dy); y < std::min(P, P - dy); ++y)
+                    for (int x = std::max(0, -dx); x < std::min(P, P - dx); ++x)
+                        acc += pupil[(y + dy) * P + x + dx] * pupil[y * P + x];
+                ac[(size_t)(dy + N / 2) * N + dx + N / 2] = acc;
+            }
+        std::vector<int> os;
+        for (int i = 0; i < N2; ++i)
+            if (std::fabs(ac[i]) > 1e-9) os.push_back(i);
+        return os;
+    }
+
+public:
+    Structure st;
+
+private:
+    Geometry g_;
+    int maxSlots_, maxBatch_;
+    hipStream_t st_;
+    hipEvent_t ev0_, ev1_;
+    bool timing_ = false;
+    std::map<std::string, KernelStat> stats_;
+    std::vector<int> ps_, os_;
+    int nPS_ = 0, nOS_ = 0;
+    double area_ = 0;
+    std::vector<double> otfNorm_;
+    QMap qm_{};
+    const int* qReq_ = nullptr;
+    const int* reqSlot_ = nullptr;
+    DBuf<C> tw_, tiles_, D_, S_, Sfd_;
+    DBuf<T> phi_, phiFd_, B_, phiCh_, pupil_, scale_, v_, greal_, fdStep_, fdScale_;
+    DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_;
+    DBuf<int> ints_, fdInts_, invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_;
+    DBuf<float> rawF_;
+};
+
+std::unique_ptr<DeviceEvaluator> make_device_evaluator(const Geometry& g, int precision, int maxSlots, int maxBatch,
+                                                       int device, std::string& err) {
+    try {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device) {
+            err = "no HIP device available";
+            return nullptr;
+        }
+        if (precision == 0) {
+            if (g.N == 64) return std::make_unique<Pipeline<double, 64>>(g, maxSlots, maxBatch, device);
+            if (g.N == 128) return std::make_unique<Pipeline<double, 128>>(g, maxSlots, maxBatch, device);
+            if (g.N == 256) return std::make_unique<Pipeline<double, 256>>(g, maxSlots, maxBatch, device);
+        } else {
+            if (g.N == 64) return std::make_unique<Pipeline<float, 64>>(g, maxSlots, maxBatch, device);
+            if (g.N == 128) return std::make_unique<Pipeline<float, 128>>(g, maxSlots, maxBatch, device);
+            if (g.N == 256) return std::make_unique<Pipeline<float, 256>>(g, maxSlots, maxBatch, device);
+        }
+        err = "unsupported OTF grid (N must be 64, 128 or 256)";
+    } catch (const std::exception& e) {
+        err = e.what();
+    }
+    return nullptr;
+}
+
+}  // namespace mfbd

@@ -1,0 +1,1102 @@
+// Host side of the MI355X MOMFBD patch solver: the C-ABI context, the per-patch
+// Solver::run state machine and the batch driver.
+//
+// Each patch runs the reference's control flow (src/lib/momfbd/solver.cpp:330-580:
+// mode stepping, conjugate_rdx iterations with bracket + Brent line search,
+// convergence tests, shift re-initialisation between stages) as a C++20 coroutine.
+// Every metric or gradient evaluation is a co_await: the driver resumes all
+// patches until each waits for an evaluation, then evaluates all pending requests
+// in one batched device pass (many patches x all images per launch) and resumes
+// them.  With maxBatch = 1 this is the reference's one-patch-at-a-time drop-in.
+#include "../../include/momfbd_hip.h"
+#include "evaluator.hpp"
+#include "hostfft.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <coroutine>
+#include <cstring>
+#include <exception>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+using namespace mfbd;
+
+namespace {
+
+constexpr double GOLDEN = 1.6180339887498949025;   // constants.hpp:26
+constexpr double TINY = 1.0E-32;                   // helpers.cpp:56
+constexpr int ST_SUCCESS = 0, ST_FAILURE = -1, ST_ENOPROG = 27;
+
+// ------------------------------------------------------------ coroutines ----
+template <typename T>
+struct Task {
+    struct promise_type {
+        T value{};
+        std::coroutine_handle<> cont;
+        std::exception_ptr exc;
+        Task get_return_object() { return Task{std::coroutine_handle<promise_type>::from_promise(*this)}; }
+        std::suspend_always initial_suspend() noexcept { return {}; }
+        struct Final {
+            bool await_ready() noexcept { return false; }
+            std::coroutine_handle<> await_suspend(std::coroutine_handle<promise_type> h) noexcept {
+                auto c = h.promise().cont;
+                return c ? c : std::noop_coroutine();
+            }
+            void await_resume() noexcept {}
+        };
+        Final final_suspend() noexcept { return {}; }
+        void return_value(T v) { value = v; }
+        void unhandled_exception() { exc = std::current_exception(); }
+    };
+    std::coroutine_handle<promise_type> h;
+    explicit Task(std::coroutine_handle<promise_type> hh) : h(hh) {}
+    Task(Task&& o) noexcept : h(o.h) { o.h = {}; }
+    Task(const Task&) = delete;
+    ~Task() { if (h) h.destroy(); }
+    bool await_ready() { return false; }
+    std::coroutine_handle<> await_suspend(std::coroutine_handle<> c) {
+        h.promise().cont = c;
+        return h;
+    }
+    T await_resume() {
+        if (h.promise().exc) std::rethrow_exception(h.promise().exc);
+        return h.promise().value;
+    }
+};
+
+// ------------------------------------------------------------ parallel for ----
+void parallel_for(int n, const std::function<void(int)>& f) {
+    int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+    nt = std::min(nt, n);
+    if (nt <= 1) {
+        for (int i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<int> next{0};
+    std::vector<std::thread> th;
+    std::exception_ptr exc;
+    std::atomic<bool> bad{false};
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&] {
+            for (int i; (i = next.fetch_add(1)) < n;) {
+                try { f(i); } catch (...) { if (!bad.exchange(true)) exc = std::current_exception(); }
+            }
+        });
+    for (auto& t : th) t.join();
+    if (exc) std::rethrow_exception(exc);
+}
+
+// ----------------------------------------------- host statistics & filters ----
+double mean_rev(const double* d, long n) {       // ArrayStats::getMinMaxMean (arraystats.cpp:17-45)
+    double sum = 0;
+    long cnt = 0;
+    for (const double* p = d + n; p-- > d;)
+        if (std::isfinite(*p)) { sum += *p; cnt++; }
+    return cnt ? sum / cnt : sum;
+}
+
+double stddev_rev(const double* d, long n, double mean) {   // getRmsStddev (arraystats.cpp:59-81)
+    double rms = 0;
+    long cnt = 0;
+    for (const double* p = d + n; p-- > d;)
+        if (*p == *p) { rms += *p * *p; cnt++; }
+    if (!cnt) return 0;
+    rms /= cnt;
+    return std::sqrt(rms - mean * mean);
+}
+
+void apodize1(double* data, int n, double target) {         // math::apodize (functions.cpp:39-50)
+    double mean = 0.5 * (*data + target), amp = 0.5 * (*data - target), step = M_PI / (n - 1);
+    for (int i = 1; i < n; ++i) data[i] = mean + amp * std::cos(i * step);
+}
+
+void apodize_in_place(double* d, int nR, int nC, int blend) {   // utils.cpp:583-613
+    int rb = std::min(blend, nC / 2), cb = std::min(blend, nR / 2);
+    std::vector<double> t(std::max(rb, cb) + 2, 0.0);
+    apodize1(t.data(), rb + 2, 1.0);
+    for (int c = 0; c < nC; ++c)
+        for (int r = 0; r < rb; ++r) { d[r * nC + c] *= t[r + 1]; d[(nR - r - 1) * nC + c] *= t[r + 1]; }
+    std::fill(t.begin(), t.end(), 0.0);
+    apodize1(t.data(), cb + 2, 1.0);
+    for (int r = 0; r < nR; ++r)
+        for (int c = 0; c < cb; ++c) { d[r * nC + c] *= t[c + 1]; d[r * nC + nC - c - 1] *= t[c + 1]; }
+}
+
+// FourierTransform::noise (fouriertransform.cpp:448-495), FULLCOMPLEX, mask = -1, limit = -1
+double noise_level(const cd* ft, int ny, int nx) {
+    int mask = std::max(ny, nx) / 6 + 1;
+    double limits = (double)ny * nx / 4.0;
+    double np = 0;
+    long cnt = 0;
+    for (int x = mask; x < nx - mask + 1; ++x) {
+        double xx = std::min(x, nx - x);
+        xx *= xx;
+        for (int y = mask; y < ny - mask + 1; ++y) {
+            double yy = std::min(y, ny - y);
+            if (xx + yy * yy < limits) continue;
+            np += std::norm(ft[(long)y * nx + x]);
+            cnt++;
+        }
+    }
+    return cnt ? std::sqrt(np / ((double)cnt * ny * nx)) : np;
+}
+
+void smooth2(double* d, int ny, int nx, int n) {             // image::smooth (utils.cpp:304-328)
+    std::vector<double> t((size_t)ny * nx);
+    for (int y = 0; y < ny; ++y) {
+        int yl = std::max(y - n, 0), yh = std::min(y + n, ny);
+        for (int x = 0; x < nx; ++x) {
+            int xl = std::max(x - n, 0), xh = std::min(x + n, nx);
+            double acc = 0;
+            int c = 0;
+            for (int yy = yl; yy < yh; ++yy)
+                for (int xx = xl; xx < xh; ++xx) { acc += d[yy * nx + xx]; c++; }
+            t[(size_t)y * nx + x] = acc / c;
+        }
+    }
+    std::copy(t.begin(), t.end(), d);
+}
+
+// ScharmerFilter (utils.cpp:334-392) with connectedRegion (utils.cpp:49-55, 294-300)
+void scharmer(cd* data, const double* q2, int ny, int nx, double noise_power, double cutoff) {
+    const double nf = noise_power * ny * nx;
+    const size_t n = (size_t)ny * nx;
+    std::vector<double> pt(n), rr(n), col(ny);
+    for (size_t k = 0; k < n; ++k) pt[k] = std::norm(data[k]) * q2[k];
+    smooth2(pt.data(), ny, nx, 1);
+    for (auto& v : pt) v = nf / v;
+    for (int y = 0; y < ny; ++y) {
+        rr[(size_t)y * nx] = pt[(size_t)y * nx];
+        for (int x = 1; x < nx; ++x) rr[(size_t)y * nx + x] = pt[(size_t)y * nx + nx - x];
+    }
+    for (int x = 0; x < nx; ++x) {
+        double v = 1.0 - 0.5 * (rr[x] + pt[x]);
+        rr[x] = v < 0.0 ? 0.0 : v;
+        for (int y = 1; y < ny; ++y) col[y] = rr[(size_t)y * nx + x];
+        for (int y = 1; y < ny; ++y) {
+            double w = 1.0 - 0.5 * (col[ny - y] + pt[(size_t)y * nx + x]);
+            rr[(size_t)y * nx + x] = w < 0.0 ? 0.0 : w;
+        }
+    }
+    for (auto& v : rr) { if (v < 0.2) v = 0.0; if (v > 1.0) v = 1.0; }
+    const int yH = ny / 2, xH = nx / 2;
+    rr[(size_t)yH * nx + xH] = 1.0;
+    std::vector<unsigned char> mask(n, 0);
+    std::vector<int> stack{yH * nx + xH};
+    while (!stack.empty()) {
+        int idx = stack.back();
+        stack.pop_back();
+        int y = idx / nx, x = idx % nx;
+        if (mask[idx]) continue;
+        if (!((y == yH && x == xH) || rr[idx] > 0.0)) continue;
+        mask[idx] = 1;
+        if (y + 1 < ny) stack.push_back(idx + nx);
+        if (y - 1 >= 0) stack.push_back(idx - nx);
+        if (x + 1 < nx) stack.push_back(idx + 1);
+        if (x - 1 >= 0) stack.push_back(idx - 1);
+    }
+    for (size_t k = 0; k < n; ++k) rr[k] *= mask[k];
+    smooth2(rr.data(), ny, nx, 4);
+    const double c2 = cutoff * cutoff;
+    for (int y = 0; y < ny; ++y)
+        for (int x = 0; x < nx; ++x) {
+            size_t k = (size_t)y * nx + x;
+            if ((double)((y - yH) * (y - yH) + (x - xH) * (x - xH)) > c2) data[k] = 0.0;
+            else data[k] *= rr[k];
+        }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context ----
+
+
+struct mfbd_ctx {
+    Geometry g{};
+    mfbd_params prm{};
+    int device = 0;
+    std::string err;
+    std::unique_ptr<DeviceEvaluator> dev;
+    Structure st;
+    bool haveStructure = false, haveBasis = false, haveCons = false;
+    // basis
+    std::vector<double> pupil, modes, jac, otfNorm;
+    std::vector<int> tilt;
+    // constraints
+    int nPar = 0, nFree = 0;
+    std::vector<int> nsRow, nsCol;
+    std::vector<double> nsVal;
+    // patches
+    int nPatch = 0;
+    std::vector<float> data;          // host copy of raw frames [n][nImg][dataY][dataX]
+    std::vector<double> phiFixed, resOffset, noiseFudge;
+    std::vector<double> window, noiseWindow;
+    // per-patch host state (stage-wise API)
+    std::vector<std::vector<int>> shifts;       // [patch][nImg*2]
+    std::vector<std::vector<double>> tiltOff;   // [patch][nImg*2]
+    std::vector<std::vector<double>> noise;     // [patch][nImg]
+    std::vector<std::vector<double>> gamma;     // [patch][nObj]
+    bool initDone = false;
+};
+
+namespace {
+
+int set_err(mfbd_ctx* c, int code, const std::string& m) {
+    if (c) c->err = m;
+    return code;
+}
+
+// phi_channel per patch: phi_fixed - residual-tilt (Channel::initPatch, channel.cpp:908-939)
+void phi_channels(const mfbd_ctx* c, int p, std::vector<double>& out) {
+    const int P2 = c->g.P * c->g.P, K = c->g.K;
+    out.assign((size_t)c->g.nCh * P2, 0.0);
+    for (int ch = 0; ch < c->g.nCh; ++ch) {
+        const int o = c->st.chObj[ch];
+        double* phi = &out[(size_t)ch * P2];
+        std::memcpy(phi, &c->phiFixed[(size_t)ch * P2], sizeof(double) * P2);
+        const double* J = &c->jac[8 * o];
+        const double rx = c->resOffset[((size_t)p * c->g.nCh + ch) * 2], ry = c->resOffset[((size_t)p * c->g.nCh + ch) * 2 + 1];
+        const double ax = rx * J[4] + ry * J[5], ay = rx * J[6] + ry * J[7];
+        const int tx = c->tilt[2 * o], ty = c->tilt[2 * o + 1];
+        if (tx >= 0 && std::fabs(rx) > 0) {
+            const double* m = &c->modes[((size_t)o * K + tx) * P2];
+            for (int k = 0; k < P2; ++k) phi[k] = phi[k] + (-ax) * m[k];
+        }
+        if (ty >= 0 && std::fabs(ry) > 0) {
+            const double* m = &c->modes[((size_t)o * K + ty) * P2];
+            for (int k = 0; k < P2; ++k) phi[k] = phi[k] + (-ay) * m[k];
+        }
+    }
+}
+
+// otfNormalization = 1/sqrt(max |S|) of the aberration-free OTF (channel.cpp:941-952);
+// max |S| = S(0) = sum pupil^2 for the autocorrelation of the pupil field
+double otf_normalization(const mfbd_ctx* c) {
+    const int P = c->g.P, N = 2 * P;
+    std::vector<cd> a((size_t)N * N, 0.0);
+    for (int i = 0; i < P * P; ++i)
+        if (c->pupil[i] > 1e-9) a[(size_t)(i / P + P / 2) * N + i % P + P / 2] = c->pupil[i];
+    fft2d(a.data(), N, N, -1);
+    for (auto& v : a) v = std::norm(v);
+    fft2d(a.data(), N, N, +1);
+    double mx = 0;
+    for (auto& v : a) mx = std::max(mx, std::abs(v) / ((double)N * N));
+    return 1.0 / std::sqrt(mx);
+}
+
+// windowed image of one frame at a shift (SubImage::getWindowedImg), its stats and
+// noise level (SubImage::initialize with doReset, subimage.cpp:155-187)
+void frame_noise(const mfbd_ctx* c, int p, int j, int sx, int sy, double& noise, double& sd) {
+    const int s = c->g.s;
+    std::vector<double> img((size_t)s * s);
+    const float* blk = &c->data[((size_t)p * c->g.nImg + j) * c->g.dataY * c->g.dataX];
+    const int y0 = c->g.startY + sy, x0 = c->g.startX + sx;
+    for (int y = 0; y < s; ++y)
+        for (int x = 0; x < s; ++x) img[(size_t)y * s + x] = blk[(size_t)(y0 + y) * c->g.dataX + x0 + x];
+    const double avg = mean_rev(img.data(), (long)s * s);
+    for (size_t k = 0; k < img.size(); ++k) img[k] = (img[k] - avg) * c->window[k] + avg;
+    const double mean = mean_rev(img.data(), (long)s * s);
+    sd = stddev_rev(img.data(), (long)s * s, mean);
+    std::vector<cd> d2((size_t)s * s);
+    for (size_t k = 0; k < img.size(); ++k) d2[k] = (img[k] - mean) * c->noiseWindow[k];
+    fft2d(d2.data(), s, s, -1);
+    noise = noise_level(d2.data(), s, s) * c->noiseFudge[c->st.imgCh[j]];
+}
+
+// reset initialisation of patch p at the given shifts: noise, gamma, device D/ftSum
+void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
+    const int nImg = c->g.nImg, nObj = c->g.nObj;
+    std::vector<double> noiseAll((size_t)patches.size() * nImg), sdAll((size_t)patches.size() * nImg);
+    parallel_for((int)(patches.size() * nImg), [&](int k) {
+        const int p = patches[k / nImg], j = k % nImg;
+        frame_noise(c, p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1], noiseAll[k], sdAll[k]);
+    });
+    std::vector<ImgInit> imgs;
+    for (size_t a = 0; a < patches.size(); ++a) {
+        const int p = patches[a];
+        c->noise[p].assign(noiseAll.begin() + a * nImg, noiseAll.begin() + (a + 1) * nImg);
+        std::vector<double> gam(nObj, 0.0);
+        for (int j = 0; j < nImg; ++j) {                      // Object::addRegGamma
+            const int o = c->st.imgObj[j];
+            gam[o] += 0.10 * (noiseAll[a * nImg + j] / sdAll[a * nImg + j]) / c->st.nImgObj[o];
+        }
+        c->gamma[p] = gam;
+        c->dev->set_gamma(p, gam.data());
+        for (int j = 0; j < nImg; ++j) imgs.push_back({p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1]});
+    }
+    c->dev->init_images(imgs, c->window);
+}
+
+// SubImage::adjustShifts (subimage.cpp:444-480) for one image; returns true if shifted
+bool adjust_shift(mfbd_ctx* c, int p, int j, const double* a) {
+    const int o = c->st.imgObj[j];
+    const double* J = &c->jac[8 * o];
+    const int tx = c->tilt[2 * o], ty = c->tilt[2 * o + 1];
+    if (tx < 0 || ty < 0) return false;
+    double* off = &c->tiltOff[p][2 * j];
+    int* sh = &c->shifts[p][2 * j];
+    const double ax = a[tx] + off[0], ay = a[ty] + off[1];
+    const double sx = ax * J[0] + ay * J[1], sy = ax * J[2] + ay * J[3];
+    const int thx = -(int)std::lround(sx), thy = -(int)std::lround(sy);
+    auto shift1 = [&](int first, int size, int n) {           // Array::shift (array.hpp:1119-1140)
+        if (n == 0) return 0;
+        const int last = first + c->g.s - 1;
+        if (n + first < 0) n = -first;
+        if (std::abs(n + last) >= size) n = size - last - 1;
+        return n;
+    };
+    const int actX = thx ? shift1(c->g.startX + sh[0], c->g.dataX, thx) : 0;
+    const int actY = thy ? shift1(c->g.startY + sh[1], c->g.dataY, thy) : 0;
+    if (actX || actY) {
+        sh[0] += actX;
+        sh[1] += actY;
+        off[0] += actX * J[4] + actY * J[5];
+        off[1] += actX * J[6] + actY * J[7];
+        return true;
+    }
+    return false;
+}
+
+// alpha with the tilt offsets folded in (SubImage::addToPhi, subimage.cpp:509-532)
+void effective_alpha(const mfbd_ctx* c, int p, const double* alpha, double* out) {
+    const int K = c->g.K;
+    std::memcpy(out, alpha, sizeof(double) * c->g.nImg * K);
+    for (int j = 0; j < c->g.nImg; ++j) {
+        const int o = c->st.imgObj[j];
+        const int tx = c->tilt[2 * o], ty = c->tilt[2 * o + 1];
+        if (tx >= 0) out[(size_t)j * K + tx] += c->tiltOff[p][2 * j];
+        if (ty >= 0 && ty != tx) out[(size_t)j * K + ty] += c->tiltOff[p][2 * j + 1];
+    }
+}
+
+// Object::restorePatch after the sums (object.cpp:278-443): conj, safeDivide, Scharmer
+// filter, inverse transform, centred crop
+void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const double* P0, const double* Q0,
+                    float* imgOut) {
+    const int N = 2 * c->g.P, s = c->g.s, nOS = (int)os.size();
+    for (int o = 0; o < c->g.nObj; ++o) {
+        std::vector<cd> ao((size_t)N * N, 0.0);
+        std::vector<double> dn((size_t)N * N, 0.0);
+        for (int u = 0; u < nOS; ++u) {
+            const size_t k = (size_t)o * nOS + u;
+            ao[os[u]] = std::conj(cd(P0[2 * k], P0[2 * k + 1]));
+            dn[os[u]] = Q0[k];
+        }
+        for (size_t k = 0; k < ao.size(); ++k) ao[k] = (std::fabs(dn[k]) > 0) ? ao[k] / dn[k] : cd(0.0);
+        double nv = 0;
+        int cnt = 0;
+        for (int j = 0; j < c->g.nImg; ++j)
+            if (c->st.imgObj[j] == o) { nv += c->noise[p][j] * c->noise[p][j]; cnt++; }
+        nv /= cnt;
+        if (!c->prm.noFilter) scharmer(ao.data(), dn.data(), N, N, nv, 0.90 * c->st.freqCutoff[o]);
+        std::vector<cd> t((size_t)N * N);
+        reorder_into(ao.data(), N, N, t.data());
+        fft2d(t.data(), N, N, +1);
+        const int off = (N - s) / 2;
+        float* out = imgOut + (size_t)o * s * s;
+        for (int y = 0; y < s; ++y)
+            for (int x = 0; x < s; ++x)
+                out[(size_t)y * s + x] = (float)(t[(size_t)(y + off) * N + x + off].real() / ((double)N * N));
+    }
+}
+
+// --------------------------------------------------------- patch solver ----
+struct EvalSlot {
+    bool pending = false;
+    bool wantGrad = false;
+    int method = 2, nEnabled = 0;
+    std::vector<double> alphaEff;
+    double f = 0;
+    std::vector<double> gAlpha;
+    std::coroutine_handle<> h;
+};
+
+struct PatchRun {
+    mfbd_ctx* c = nullptr;
+    int p = 0;
+    EvalSlot ev;
+    bool wantShiftInit = false;     // pending re-initialisation after a stage
+    std::vector<int> shiftedObjs;
+    std::coroutine_handle<> initH;
+    // solver state (solver.cpp:134-144, conjugate_gradient.cpp:29-38)
+    std::vector<double> alpha, alphaOffset, x, g, dx, x1, p_, prevG, beta0, ga;
+    int iterCG = 0;
+    double pnorm = 0, g0norm = 0, step = 0;
+    int nEnabled = 0, gradMethod = 1;
+    int nF = 0, nDF = 0;
+    mfbd_result res{};
+    std::vector<double> metrics;
+};
+
+struct EvalAwait {
+    PatchRun* pr;
+    bool await_ready() { return false; }
+    void await_suspend(std::coroutine_handle<> h) { pr->ev.h = h; pr->ev.pending = true; }
+    void await_resume() {}
+};
+
+struct InitAwait {
+    PatchRun* pr;
+    bool await_ready() { return false; }
+    void await_suspend(std::coroutine_handle<> h) { pr->initH = h; pr->wantShiftInit = true; }
+    void await_resume() {}
+};
+
+void ns_reverse_add(const mfbd_ctx* c, const double* in, const double* offset, double* out) {
+    std::memcpy(out, offset, sizeof(double) * c->nPar);
+    for (size_t e = 0; e < c->nsRow.size(); ++e) out[c->nsRow[e]] += c->nsVal[e] * in[c->nsCol[e]];
+}
+
+void ns_apply(const mfbd_ctx* c, const double* in, double* out) {
+    std::fill(out, out + c->nFree, 0.0);
+    for (size_t e = 0; e < c->nsRow.size(); ++e) out[c->nsCol[e]] += c->nsVal[e] * in[c->nsRow[e]];
+}
+
+Task<double> my_f(PatchRun& r, const double* beta) {   // Solver::my_f (solver.cpp:219-225)
+    ns_reverse_add(r.c, beta, r.alphaOffset.data(), r.alpha.data());
+    r.ev.wantGrad = false;
+    effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff.data());
+    co_await EvalAwait{&r};
+    r.nF++;
+    co_return r.ev.f;
+}
+
+Task<int> my_df(PatchRun& r, const double* beta, double* gb) {   // Solver::my_df (solver.cpp:228-234)
+    ns_reverse_add(r.c, beta, r.alphaOffset.data(), r.alpha.data());
+    r.ev.wantGrad = true;
+    r.ev.method = r.gradMethod;
+    r.ev.nEnabled = r.nEnabled;
+    effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff.data());
+    co_await EvalAwait{&r};
+    r.nDF++;
+    ns_apply(r.c, r.ev.gAlpha.data(), gb);
+    co_return 0;
+}
+
+// value_at / take_step (conjugate_gradient.cpp:110-125)
+Task<double> value_at(PatchRun& r, double stp, double lambda) {
+    const int n = r.c->nFree;
+    for (int k = 0; k < n; ++k) r.dx[k] = 0.0 + (-stp * lambda) * r.p_[k];
+    for (int k = 0; k < n; ++k) r.x1[k] = r.x[k] + 1.0 * r.dx[k];
+    double f = co_await my_f(r, r.x1.data());
+    co_return f;
+}
+
+#define SHFT(a, b, c, d) (a) = (b); (b) = (c); (c) = (d);
+#define SIGN(a, b) ((b) >= 0.0 ? std::fabs(a) : -std::fabs(a))
+
+struct Bracket { double a, b, mid, fa, fb, fmid; };
+
+// math::bracket (helpers.cpp:58-129)
+Task<Bracket> bracket(PatchRun& r, double lambda, Bracket B, int limit) {
+    double u, fu, ulim;
+    if (B.fb > B.fa) { std::swap(B.a, B.b); std::swap(B.fa, B.fb); }
+    B.mid = B.b + GOLDEN * (B.b - B.a);
+    B.fmid = co_await value_at(r, B.mid, lambda);
+    while (B.fb > B.fmid) {
+        double rr = (B.b - B.a) * (B.fb - B.fmid);
+        double q = (B.b - B.mid) * (B.fb - B.fa);
+        u = B.b - ((B.b - B.mid) * q - (B.b - B.a) * rr) / (2.0 * SIGN(std::max(std::fabs(q - rr), TINY), q - rr));
+        ulim = B.b + limit * (B.mid - B.b);
+        if ((B.b - u) * (u - B.mid) > 0.0) {
+            fu = co_await value_at(r, u, lambda);
+            if (fu < B.fmid) { B.a = B.b; B.b = u; B.fa = B.fb; B.fb = fu; break; }
+            else if (fu > B.fb) { B.mid = u; B.fmid = fu; break; }
+            u = B.mid + GOLDEN * (B.mid - B.b);
+            fu = co_await value_at(r, u, lambda);
+        } else if ((B.mid - u) * (u - ulim) > 0.0) {
+            fu = co_await value_at(r, u, lambda);
+            if (fu < B.fmid) {
+                SHFT(B.b, B.mid, u, B.mid + GOLDEN * (B.mid - B.b))
+                double fnew = co_await value_at(r, u, lambda);
+                SHFT(B.fb, B.fmid, fu, fnew)
+            }
+        } else if ((B.mid - ulim) * (ulim - u) >= 0.0) {
+            u = ulim;
+            fu = co_await value_at(r, u, lambda);
+        } else {
+            u = B.mid + GOLDEN * (B.mid - B.b);
+            fu = co_await value_at(r, u, lambda);
+        }
+        SHFT(B.a, B.b, B.mid, u)
+        SHFT(B.fa, B.fb, B.fmid, fu)
+    }
+    if (B.a > B.b) { std::swap(B.a, B.b); std::swap(B.fa, B.fb); }
+    if (B.mid > B.b) { std::swap(B.mid, B.b); std::swap(B.fmid, B.fb); }
+    if (B.mid < B.a) { std::swap(B.mid, B.a); std::swap(B.fmid, B.fa); }
+    co_return B;
+}
+
+struct BrentOut { double x, fx; };
+
+// math::brent (helpers.cpp:133-216)
+Task<BrentOut> brent(PatchRun& r, double lambda, double ax, double bx, double x, double tol, int limit) {
+    const double CGOLD = 2.0 - GOLDEN;
+    double a = std::min(ax, bx), b = std::max(ax, bx);
+    double u, v, w, fu, fv, fw, fx;
+    u = v = w = x;
+    fx = co_await value_at(r, x, lambda);
+    fu = fv = fw = fx;
+    double d = 0.0, e = 0.0;
+    for (int iter = 1; iter <= limit; ++iter) {
+        double xm = 0.5 * (a + b);
+        double tol1 = tol * std::fabs(x) + TINY, tol2 = 2.0 * tol1;
+        if (std::fabs(x - xm) <= (tol2 - 0.5 * (b - a))) co_return BrentOut{x, fx};
+        if (std::fabs(e) > tol1) {
+            double rr = (x - w) * (fx - fv);
+            double q = (x - v) * (fx - fw);
+            double pp = (x - v) * q - (x - w) * rr;
+            q = 2.0 * (q - rr);
+            if (q > 0.0) pp = -pp;
+            q = std::fabs(q);
+            double etemp = e;
+            e = d;
+            if (std::fabs(pp) >= std::fabs(0.5 * q * etemp) || pp <= q * (a - x) || pp >= q * (b - x)) {
+                e = (x >= xm ? a - x : b - x);
+                d = CGOLD * e;
+            } else {
+                d = pp / q;
+                u = x + d;
+                if (u - a < tol2 || b - u < tol2) d = SIGN(tol1, xm - x);
+            }
+        } else {
+            e = (x >= xm ? a - x : b - x);
+            d = CGOLD * e;
+        }
+        if (std::fabs(d) < tol1) d *= (tol1 / std::fabs(d));
+        u = ((std::fabs(d) >= tol1) ? x + d : x + SIGN(tol1, d));
+        fu = co_await value_at(r, u, lambda);
+        if (fu <= fx) {
+            if (u >= x) a = x; else b = x;
+            SHFT(v, w, x, u)
+            SHFT(fv, fw, fx, fu)
+        } else {
+            if (u < x) a = u; else b = u;
+            if (fu <= fw || w == x) { v = w; w = u; fv = fw; fw = fu; }
+            else if (fu <= fv || v == x || v == w) { v = u; fv = fu; }
+        }
+    }
+    co_return BrentOut{x, fx};
+}
+
+double dnrm2(const std::vector<double>& v) {
+    double s = 0;
+    for (double a : v) s += a * a;
+    return std::sqrt(s);
+}
+
+// conjugate_rdx_iterate (conjugate_gradient.cpp:128-256)
+Task<int> cg_iterate(PatchRun& r, double& f) {
+    const int n = r.c->nFree;
+    co_await my_df(r, r.x.data(), r.g.data());
+    double pnorm = r.pnorm, g0norm = r.g0norm;
+    const double g1norm = dnrm2(r.g);
+    double pg;
+    if (r.iterCG++ == 0) {
+        r.p_ = r.g;
+        pnorm = r.pnorm = g1norm;
+        pg = g1norm * g1norm;
+    } else {
+        if (g0norm == 0.0) { std::fill(r.dx.begin(), r.dx.end(), 0.0); co_return ST_ENOPROG; }
+        double g0g1 = 0;
+        for (int k = 0; k < n; ++k) {
+            double t = r.prevG[k] - r.g[k];
+            g0g1 += t * r.g[k];
+            r.prevG[k] = t;
+        }
+        const double beta = g0g1 / (g0norm * g0norm);
+        for (int k = 0; k < n; ++k) r.p_[k] = r.g[k] - beta * r.p_[k];
+        pnorm = r.pnorm = dnrm2(r.p_);
+        pg = 0;
+        for (int k = 0; k < n; ++k) pg += r.p_[k] * r.g[k];
+    }
+    const double lambda = (pg >= 0.0) ? +1.0 / pnorm : -1.0 / pnorm;
+    if (pnorm == 0.0) { std::fill(r.dx.begin(), r.dx.end(), 0.0); co_return ST_ENOPROG; }
+    const double fa = f;
+    const double stepc = 1E-21 / lambda;
+    const double fc = co_await value_at(r, stepc, lambda);
+    Bracket B = co_await bracket(r, lambda, Bracket{0.0, stepc, 0.0, fa, fc, 0.0}, 100);
+    BrentOut bo = co_await brent(r, lambda, B.a, B.b, B.mid, 0.1, 100);
+    for (int k = 0; k < n; ++k) r.dx[k] = 0.0 + (-bo.x * lambda) * r.p_[k];
+    for (int k = 0; k < n; ++k) r.x1[k] = r.x[k] + 1.0 * r.dx[k];
+    r.x = r.x1;
+    f = bo.fx;
+    r.step = 0.5 * std::max(std::fabs(bo.x), 2.0E-9);
+    r.g0norm = g1norm;
+    r.prevG = r.g;
+    r.iterCG = r.iterCG % n;
+    co_return ST_SUCCESS;
+}
+
+// Solver::run (solver.cpp:330-580), starting after the reset initialisation
+Task<int> run_patch(PatchRun& r) {
+    mfbd_ctx* c = r.c;
+    const mfbd_params& pr = c->prm;
+    const int K = c->g.K, nFree = c->nFree, nPar = c->nPar;
+    r.gradMethod = 1;                                      // GM_DIFF for the first gradient
+    double prev = 0, thisMetric = 0, gradNorm = 0, tol = 1.0E-1;
+    int failCount = 0, totalIter = 0, maxIterations = 10, status = 0;
+    const int maxFails = 3;
+    bool lastDone = false;
+    double initial = co_await my_f(r, r.beta0.data());
+    r.metrics.push_back(initial);
+    const double nTotalPixels = (double)c->g.nImg * c->g.s * c->g.s;
+    const double gradScale = 1.0 / (nTotalPixels * nTotalPixels);
+    bool exitLoop = false;
+    for (int modeCount = pr.modeStart; !exitLoop; modeCount += pr.modeStep) {
+        modeCount = std::min(modeCount, K);
+        r.nEnabled = std::max(r.nEnabled, modeCount);
+        if (modeCount == K || pr.maxIter == 0) {
+            exitLoop = true;
+            tol = pr.FTOL;
+            maxIterations = pr.maxIter;
+        }
+        // gsl_multimin_fdfminimizer_set + conjugate_rdx_set: x = 0, f(x); the gradient
+        // the reference also computes here is overwritten before use.
+        std::fill(r.x.begin(), r.x.end(), 0.0);
+        std::fill(r.dx.begin(), r.dx.end(), 0.0);
+        r.iterCG = 0;
+        double f = co_await my_f(r, r.x.data());
+        int iter = 0, successCount = 0;
+        bool done = false;
+        while ((!done || iter < pr.minIter) && iter < maxIterations) {
+            iter++;
+            status = co_await cg_iterate(r, f);
+            if (failCount > maxFails) { status = ST_FAILURE; exitLoop = true; done = true; break; }
+            r.gradMethod = 2;                              // Vogel from here on (solver.cpp:478)
+            if (status == ST_ENOPROG) { failCount++; break; }
+            thisMetric = f;
+            if (!std::isfinite(thisMetric)) { failCount++; break; }
+            r.metrics.push_back(thisMetric);
+            gradNorm = dnrm2(r.g) * gradScale;
+            if (!std::isfinite(gradNorm)) { failCount++; break; }
+            if (gradNorm < pr.FTOL) { exitLoop = true; done = true; break; }
+            if (iter > 1) {
+                double rel = 2.0 * std::fabs(thisMetric - prev) / (std::fabs(thisMetric) + std::fabs(prev) + pr.EPS);
+                if (rel < tol) {
+                    if (++successCount >= pr.nDoneIter) { successCount = 0; done = true; }
+                } else successCount = 0;
+            }
+            prev = thisMetric;
+        }
+        thisMetric = co_await my_f(r, r.x.data());
+        totalIter += iter;
+        lastDone = done;
+        if (!exitLoop) {
+            // Solver::shiftAndInit() + alpha_offset <- alpha
+            r.shiftedObjs.assign(c->g.nObj, 0);
+            for (int j = 0; j < c->g.nImg; ++j)
+                if (adjust_shift(c, r.p, j, &r.alpha[(size_t)j * K])) r.shiftedObjs[c->st.imgObj[j]] = 1;
+            bool any = false;
+            for (int v : r.shiftedObjs) any |= v != 0;
+            if (any) co_await InitAwait{&r};
+            r.alphaOffset = r.alpha;
+        }
+    }
+    (void)nFree;
+    (void)nPar;
+    r.res.initialMetric = initial;
+    r.res.finalMetric = thisMetric;
+    r.res.iterations = totalIter;
+    r.res.failCount = failCount;
+    r.res.status = (status == ST_FAILURE) ? MFBD_PATCH_FAILED
+                   : (lastDone || maxIterations == 0) ? MFBD_PATCH_CONVERGED : MFBD_PATCH_MAXITER;
+    co_return 0;
+}
+
+// re-initialisation of the shifted objects' images (Object::reInitialize, doReset = false)
+void shift_inits(mfbd_ctx* c, std::vector<PatchRun*>& rs) {
+    std::vector<ImgInit> imgs;
+    for (auto* r : rs)
+        for (int j = 0; j < c->g.nImg; ++j)
+            if (r->shiftedObjs[c->st.imgObj[j]])
+                imgs.push_back({r->p, j, c->shifts[r->p][2 * j], c->shifts[r->p][2 * j + 1]});
+    c->dev->init_images(imgs, c->window);
+}
+
+int check_ready(mfbd_ctx* c) {
+    if (!c) return MFBD_EINVAL;
+    if (!c->haveStructure || !c->haveBasis || !c->haveCons || c->nPatch == 0)
+        return set_err(c, MFBD_ESTATE, "structure, basis, constraints and patches must be set first");
+    return MFBD_OK;
+}
+
+void ensure_init(mfbd_ctx* c) {
+    if (c->initDone) return;
+    std::vector<int> all(c->nPatch);
+    for (int p = 0; p < c->nPatch; ++p) {
+        all[p] = p;
+        c->shifts[p].assign(2 * c->g.nImg, 0);
+        c->tiltOff[p].assign(2 * c->g.nImg, 0.0);
+    }
+    reset_init(c, all);
+    c->initDone = true;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- C-ABI ----
+extern "C" {
+
+int mfbd_ctx_create(const mfbd_geometry* geo, const mfbd_params* prm, int device, mfbd_ctx** out) {
+    if (!geo || !prm || !out) return MFBD_EINVAL;
+    *out = nullptr;
+    const int P = geo->P;
+    if (geo->s < 4 || P < 8 || geo->K < 1 || geo->K > 160 || geo->nObj < 1 || geo->nCh < geo->nObj ||
+        geo->nImg < geo->nCh || geo->dataY < geo->s || geo->dataX < geo->s || geo->startY < 0 || geo->startX < 0 ||
+        geo->startY + geo->s > geo->dataY || geo->startX + geo->s > geo->dataX || 2 * P < geo->s)
+        return MFBD_EINVAL;
+    auto* c = new mfbd_ctx();
+    c->g = {geo->s, P, 2 * P, geo->K, geo->nObj, geo->nCh, geo->nImg, geo->dataY, geo->dataX, geo->startY, geo->startX};
+    c->prm = *prm;
+    if (c->prm.maxBatch < 1) c->prm.maxBatch = 1;
+    c->device = device;
+    *out = c;
+    return MFBD_OK;
+}
+
+int mfbd_ctx_destroy(mfbd_ctx* c) {
+    delete c;
+    return MFBD_OK;
+}
+
+const char* mfbd_last_error(const mfbd_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mfbd_set_structure(mfbd_ctx* c, const int* chObj, const int* imgCh, const double* objWeight,
+                       const double* objWavelength, const double* freqCutoff) {
+    if (!c || !chObj || !imgCh || !objWeight || !objWavelength || !freqCutoff) return set_err(c, MFBD_EINVAL, "null argument");
+    Structure s;
+    s.chObj.assign(chObj, chObj + c->g.nCh);
+    s.imgCh.assign(imgCh, imgCh + c->g.nImg);
+    s.weight.assign(objWeight, objWeight + c->g.nObj);
+    s.wavelength.assign(objWavelength, objWavelength + c->g.nObj);
+    s.freqCutoff.assign(freqCutoff, freqCutoff + c->g.nObj);
+    s.nImgObj.assign(c->g.nObj, 0);
+    for (int j = 0; j < c->g.nImg; ++j) {
+        if (s.imgCh[j] < 0 || s.imgCh[j] >= c->g.nCh) return set_err(c, MFBD_EINVAL, "imgCh out of range");
+        if (j && s.imgCh[j] < s.imgCh[j - 1]) return set_err(c, MFBD_EINVAL, "images must be ordered by channel");
+        const int o = s.chObj[s.imgCh[j]];
+        if (o < 0 || o >= c->g.nObj) return set_err(c, MFBD_EINVAL, "chObj out of range");
+        s.imgObj.push_back(o);
+        s.nImgObj[o]++;
+    }
+    c->st = s;
+    c->haveStructure = true;
+    return MFBD_OK;
+}
+
+int mfbd_set_basis(mfbd_ctx* c, const double* pupil, const double* modes, const int* tiltXY, const double* jac) {
+    if (!c || !pupil || !modes || !tiltXY || !jac) return set_err(c, MFBD_EINVAL, "null argument");
+    if (!c->haveStructure) return set_err(c, MFBD_ESTATE, "mfbd_set_structure must precede mfbd_set_basis");
+    const size_t P2 = (size_t)c->g.P * c->g.P;
+    c->pupil.assign(pupil, pupil + P2);
+    c->modes.assign(modes, modes + (size_t)c->g.nObj * c->g.K * P2);
+    c->tilt.assign(tiltXY, tiltXY + 2 * c->g.nObj);
+    c->jac.assign(jac, jac + 8 * c->g.nObj);
+    for (int o = 0; o < c->g.nObj; ++o)
+        for (int t = 0; t < 2; ++t)
+            if (c->tilt[2 * o + t] >= c->g.K) return set_err(c, MFBD_EINVAL, "tilt mode index out of range");
+    const double nrm = otf_normalization(c);
+    c->otfNorm.assign(c->g.nCh, nrm);
+    c->st.graddiffStep = c->prm.graddiffStep;
+    // Solver::init windows (solver.cpp:112-126)
+    const int s = c->g.s;
+    c->window.assign((size_t)s * s, 1.0);
+    c->noiseWindow.assign((size_t)s * s, 1.0);
+    const int apod = c->prm.apodSize < 0 ? s / 8 : c->prm.apodSize;
+    if (apod) {
+        apodize_in_place(c->window.data(), s, s, apod);
+        apodize_in_place(c->noiseWindow.data(), s, s, apod / 2);
+    }
+    c->haveBasis = true;
+    return MFBD_OK;
+}
+
+int mfbd_set_constraints(mfbd_ctx* c, int nPar, int nFree, int nnz, const int* rows, const int* cols,
+                         const double* vals) {
+    if (!c || !rows || !cols || !vals) return set_err(c, MFBD_EINVAL, "null argument");
+    if (nPar != c->g.nImg * c->g.K || nFree < 1 || nFree > nPar || nnz < 1)
+        return set_err(c, MFBD_EINVAL, "nullspace has the wrong number of entries");   // solver.cpp:102-110
+    for (int e = 0; e < nnz; ++e)
+        if (rows[e] < 0 || rows[e] >= nPar || cols[e] < 0 || cols[e] >= nFree)
+            return set_err(c, MFBD_EINVAL, "nullspace entry out of range");
+    c->nPar = nPar;
+    c->nFree = nFree;
+    c->nsRow.assign(rows, rows + nnz);
+    c->nsCol.assign(cols, cols + nnz);
+    c->nsVal.assign(vals, vals + nnz);
+    c->haveCons = true;
+    return MFBD_OK;
+}
+
+int mfbd_load_patches(mfbd_ctx* c, int n, const float* data, const double* phiFixed, const double* resOffset,
+                      const double* noiseFudge) {
+    if (!c || n < 1 || !data || !phiFixed || !noiseFudge) return set_err(c, MFBD_EINVAL, "bad arguments");
+    if (!c->haveBasis) return set_err(c, MFBD_ESTATE, "mfbd_set_basis must precede mfbd_load_patches");
+    const size_t per = (size_t)c->g.nImg * c->g.dataY * c->g.dataX;
+    c->nPatch = n;
+    c->data.assign(data, data + per * n);
+    c->phiFixed.assign(phiFixed, phiFixed + (size_t)c->g.nCh * c->g.P * c->g.P);
+    if (resOffset) c->resOffset.assign(resOffset, resOffset + (size_t)n * c->g.nCh * 2);
+    else c->resOffset.assign((size_t)n * c->g.nCh * 2, 0.0);
+    c->noiseFudge.assign(noiseFudge, noiseFudge + c->g.nCh);
+    c->shifts.assign(n, {});
+    c->tiltOff.assign(n, {});
+    c->noise.assign(n, {});
+    c->gamma.assign(n, {});
+    c->initDone = false;
+    try {
+        // device slots: the evaluator keeps every loaded patch resident
+        std::string err;
+        c->dev = make_device_evaluator(c->g, c->prm.precision, n, c->prm.maxBatch, c->device, err);
+        if (!c->dev) return set_err(c, MFBD_EDEVICE, "device pipeline unavailable: " + err);
+        c->dev->set_basis(c->pupil.data(), c->modes.data());
+        Structure s = c->st;
+        c->dev->set_structure(s, c->otfNorm);
+        std::vector<double> phi;
+        const size_t perPhi = (size_t)c->g.nCh * c->g.P * c->g.P;
+        std::vector<double> all(perPhi * n);
+        for (int p = 0; p < n; ++p) {
+            phi_channels(c, p, phi);
+            std::copy(phi.begin(), phi.end(), all.begin() + perPhi * p);
+        }
+        c->dev->load(0, n, data, all.data());
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_init_patches(mfbd_ctx* c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    try {
+        c->initDone = false;
+        ensure_init(c);
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_eval_metric(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, double* L) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (n < 1 || !patchIdx || !alpha || !L) return set_err(c, MFBD_EINVAL, "bad arguments");
+    try {
+        ensure_init(c);
+        const size_t per = (size_t)c->g.nImg * c->g.K;
+        std::vector<std::vector<double>> ae(n, std::vector<double>(per));
+        std::vector<EvalReq> rq(n);
+        for (int k = 0; k < n; ++k) {
+            if (patchIdx[k] < 0 || patchIdx[k] >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
+            effective_alpha(c, patchIdx[k], alpha + per * k, ae[k].data());
+            rq[k].slot = patchIdx[k];
+            rq[k].alpha = ae[k].data();
+        }
+        c->dev->evaluate(rq);
+        for (int k = 0; k < n; ++k) L[k] = rq[k].f;
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_eval_grad(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, int nEnabled, int method,
+                   double* gAlpha) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (n < 1 || !patchIdx || !alpha || !gAlpha || method < 1 || method > 3 || nEnabled < 0 || nEnabled > c->g.K)
+        return set_err(c, MFBD_EINVAL, "bad arguments");
+    try {
+        ensure_init(c);
+        const size_t per = (size_t)c->g.nImg * c->g.K;
+        std::vector<std::vector<double>> ae(n, std::vector<double>(per));
+        std::vector<EvalReq> rq(n);
+        for (int k = 0; k < n; ++k) {
+            if (patchIdx[k] < 0 || patchIdx[k] >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
+            effective_alpha(c, patchIdx[k], alpha + per * k, ae[k].data());
+            rq[k].slot = patchIdx[k];
+            rq[k].alpha = ae[k].data();
+            rq[k].wantGrad = true;
+            rq[k].gradMethod = method;
+            rq[k].nEnabled = nEnabled;
+            rq[k].g = gAlpha + per * k;
+        }
+        c->dev->evaluate(rq);
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_restore(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, float* img) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (n < 1 || !patchIdx || !alpha || !img) return set_err(c, MFBD_EINVAL, "bad arguments");
+    try {
+        ensure_init(c);
+        const size_t per = (size_t)c->g.nImg * c->g.K;
+        const int nOS = c->dev->nOS();
+        for (int k = 0; k < n; ++k) {
+            const int p = patchIdx[k];
+            if (p < 0 || p >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
+            std::vector<double> ae(per), P0(2 * (size_t)c->g.nObj * nOS), Q0((size_t)c->g.nObj * nOS);
+            effective_alpha(c, p, alpha + per * k, ae.data());
+            c->dev->restore_sums(p, ae.data(), P0.data(), Q0.data());
+            finish_restore(c, p, c->dev->otf_support(), P0.data(), Q0.data(),
+                           img + (size_t)k * c->g.nObj * c->g.s * c->g.s);
+        }
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* alphaOut, float* imgOut) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    try {
+        const int K = c->g.K, nImg = c->g.nImg, nPar = c->nPar, nFree = c->nFree;
+        const int batch = std::max(1, std::min(c->prm.maxBatch, c->nPatch));
+        for (int b0 = 0; b0 < c->nPatch; b0 += batch) {
+            const int nb = std::min(batch, c->nPatch - b0);
+            std::vector<std::unique_ptr<PatchRun>> runs;
+            std::vector<int> ids;
+            for (int k = 0; k < nb; ++k) {
+                auto r = std::make_unique<PatchRun>();
+                r->c = c;
+                r->p = b0 + k;
+                r->alpha.assign(nPar, 0.0);
+                r->alphaOffset.assign(nPar, 0.0);
+                if (alpha0) std::copy(alpha0 + (size_t)r->p * nPar, alpha0 + (size_t)(r->p + 1) * nPar, r->alphaOffset.begin());
+                r->x.assign(nFree, 0.0);
+                r->g.assign(nFree, 0.0);
+                r->dx.assign(nFree, 0.0);
+                r->x1.assign(nFree, 0.0);
+                r->p_.assign(nFree, 0.0);
+                r->prevG.assign(nFree, 0.0);
+                r->beta0.assign(nFree, 0.0);
+                r->ev.alphaEff.assign(nPar, 0.0);
+                r->ev.gAlpha.assign(nPar, 0.0);
+                c->shifts[r->p].assign(2 * nImg, 0);
+                c->tiltOff[r->p].assign(2 * nImg, 0.0);
+                ids.push_back(r->p);
+                runs.push_back(std::move(r));
+            }
+            // shiftAndInit(alpha_offset, true): shifts from the warm start, then reset init
+            for (auto& r : runs)
+                for (int j = 0; j < nImg; ++j) adjust_shift(c, r->p, j, &r->alphaOffset[(size_t)j * K]);
+            reset_init(c, ids);
+            std::vector<Task<int>> tasks;
+            for (auto& r : runs) tasks.push_back(run_patch(*r));
+            for (auto& t : tasks) t.h.resume();
+            while (true) {
+                // batched re-initialisations after a stage
+                std::vector<PatchRun*> inits;
+                for (auto& r : runs)
+                    if (r->wantShiftInit) inits.push_back(r.get());
+                if (!inits.empty()) {
+                    shift_inits(c, inits);
+                    for (auto* r : inits) {
+                        r->wantShiftInit = false;
+                        auto h = r->initH;
+                        h.resume();
+                    }
+                    continue;
+                }
+                std::vector<PatchRun*> pend;
+                for (auto& r : runs)
+                    if (r->ev.pending) pend.push_back(r.get());
+                if (pend.empty()) break;
+                std::vector<EvalReq> rq(pend.size());
+                for (size_t k = 0; k < pend.size(); ++k) {
+                    rq[k].slot = pend[k]->p;
+                    rq[k].alpha = pend[k]->ev.alphaEff.data();
+                    rq[k].wantGrad = pend[k]->ev.wantGrad;
+                    rq[k].gradMethod = pend[k]->ev.method;
+                    rq[k].nEnabled = pend[k]->ev.nEnabled;
+                    rq[k].g = pend[k]->ev.gAlpha.data();
+                }
+                c->dev->evaluate(rq);
+                for (size_t k = 0; k < pend.size(); ++k) {
+                    pend[k]->ev.f = rq[k].f;
+                    pend[k]->ev.pending = false;
+                }
+                for (auto* r : pend) r->ev.h.resume();
+            }
+            for (size_t k = 0; k < tasks.size(); ++k) {
+                if (!tasks[k].h.done()) throw std::runtime_error("patch state machine stalled");
+                if (tasks[k].h.promise().exc) std::rethrow_exception(tasks[k].h.promise().exc);
+            }
+            // results + restoration (Solver::run tail, solver.cpp:549-566)
+            const int nOS = c->dev->nOS();
+            std::vector<std::vector<double>> P0(nb), Q0(nb);
+            for (int k = 0; k < nb; ++k) {
+                auto& r = *runs[k];
+                r.res.nF = r.nF;
+                r.res.nDF = r.nDF;
+                if (results) results[r.p] = r.res;
+                if (alphaOut) std::copy(r.alpha.begin(), r.alpha.end(), alphaOut + (size_t)r.p * nPar);
+                if (imgOut) {
+                    std::vector<double> ae(nPar);
+                    effective_alpha(c, r.p, r.alpha.data(), ae.data());
+                    P0[k].resize(2 * (size_t)c->g.nObj * nOS);
+                    Q0[k].resize((size_t)c->g.nObj * nOS);
+                    c->dev->restore_sums(r.p, ae.data(), P0[k].data(), Q0[k].data());
+                }
+            }
+            if (imgOut) {
+                const auto& os = c->dev->otf_support();
+                parallel_for(nb, [&](int k) {
+                    const int p = runs[k]->p;
+                    finish_restore(c, p, os, P0[k].data(), Q0[k].data(),
+                                   imgOut + (size_t)p * c->g.nObj * c->g.s * c->g.s);
+                });
+            }
+        }
+        c->initDone = false;    // shifts moved during the solve
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_debug_get(mfbd_ctx* c, const char* what, int p, double* out, int* count) {
+    if (!c || !what || !count) return MFBD_EINVAL;
+    try {
+        std::vector<double> v;
+        if (!strcmp(what, "gamma_host")) v = c->gamma.at(p);
+        else if (!strcmp(what, "noise_host")) v = c->noise.at(p);
+        else if (!strcmp(what, "otfnorm")) v = c->otfNorm;
+        else if (!strcmp(what, "window")) v = c->window;
+        else if (c->dev) c->dev->debug_get(what, p, v);
+        *count = (int)v.size();
+        if (out) std::copy(v.begin(), v.end(), out);
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_kernel_stats(mfbd_ctx* c, int cap, char (*names)[48], double* ms, int* launches) {
+    if (!c || !c->dev) return 0;
+    auto st = c->dev->kernel_stats();
+    int n = 0;
+    for (auto& s : st) {
+        if (n >= cap) break;
+        std::snprintf(names[n], 48, "%s", s.name.c_str());
+        ms[n] = s.ms;
+        launches[n] = s.launches;
+        n++;
+    }
+    return n;
+}
+
+}  // extern "C"

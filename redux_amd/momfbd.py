@@ -1,0 +1,158 @@
+"""Host-side mirror of the reference's per-patch solver interface.
+
+`Solver` plays redux::momfbd::Solver (include/redux/momfbd/solver.hpp:86-179):
+construct it once per job geometry, load patches, then `run()` solves them
+(Solver::run, solver.cpp:330-580) and returns per-patch alphas, metrics and the
+restored objects (Object::restorePatch, object.cpp:278-443).  The work happens in
+libmomfbd_hip.so on the GPU; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import Geometry, Params, Result, dp, fp, ip, ptr
+
+
+class SolverError(RuntimeError):
+    pass
+
+
+def default_params(**kw):
+    """GlobalCfg defaults (config.cpp:629-639) with GETSTEP=conjugate."""
+    p = dict(FTOL=1e-3, EPS=1e-10, GRADDIFF_STEP=1e-2, REG_ALPHA=0.0, MIN_ITER=5, MAX_ITER=500, N_DONE_ITER=3,
+             MODE_START=5, MODE_STEP=5, APODIZATION=-1, NO_FILTER=0)
+    p.update(kw)
+    return p
+
+
+class Solver:
+    """One context: geometry + structure + basis + constraints; patches are loaded in bulk."""
+
+    def __init__(self, pb: dict, precision: str = "fp64", max_batch: int = 1, device: int = 0, settings=None):
+        self.pb = pb
+        st = default_params(**pb.get("settings", {}))
+        if settings:
+            st.update(settings)
+        self.settings = st
+        objs, chans = pb["objects"], pb["channels"]
+        self.nObj, self.nCh, self.nImg = len(objs), len(chans), len(pb["imgCh"])
+        self.K, self.s, self.P = pb["K"], pb["s"], pb["P"]
+        data = pb["data"]
+        g = Geometry(self.s, self.P, self.K, self.nObj, self.nCh, self.nImg, data.shape[-2], data.shape[-1],
+                     pb["start"][0], pb["start"][1])
+        prm = Params(st["FTOL"], st["EPS"], st["GRADDIFF_STEP"], st["REG_ALPHA"], st["MIN_ITER"], st["MAX_ITER"],
+                     st["N_DONE_ITER"], st["MODE_START"], st["MODE_STEP"], st["APODIZATION"], int(st["NO_FILTER"]),
+                     L.FP64 if precision == "fp64" else L.FP32, max_batch)
+        h = C.c_void_p()
+        self._check(L.lib().mfbd_ctx_create(C.byref(g), C.byref(prm), device, C.byref(h)), "ctx_create", None)
+        self.h = h
+        chObj = np.array([c["obj"] for c in chans], dtype=np.int32)
+        imgCh = np.ascontiguousarray(pb["imgCh"], dtype=np.int32)
+        w = np.array([o["weight"] for o in objs], dtype=np.float64)
+        wl = np.array([o["wavelength"] for o in objs], dtype=np.float64)
+        fc = np.array([o["freqCutoff"] for o in objs], dtype=np.float64)
+        self._check(L.lib().mfbd_set_structure(h, ptr(chObj, ip), ptr(imgCh, ip), ptr(w), ptr(wl), ptr(fc)),
+                    "set_structure")
+        pupil = np.ascontiguousarray(objs[0]["pupil"], dtype=np.float64)
+        modes = np.ascontiguousarray(np.stack([o["modes"] for o in objs]), dtype=np.float64)
+        tilt = np.ascontiguousarray([o["tilt"] for o in objs], dtype=np.int32)
+        jac = np.ascontiguousarray([np.concatenate([np.ravel(o["J"]), np.ravel(o["Ji"])]) for o in objs],
+                                   dtype=np.float64)
+        self._check(L.lib().mfbd_set_basis(h, ptr(pupil), ptr(modes), ptr(tilt, ip), ptr(jac)), "set_basis")
+        ns = pb["nullspace"]
+        rows = np.ascontiguousarray(ns["rows"], dtype=np.int32)
+        cols = np.ascontiguousarray(ns["cols"], dtype=np.int32)
+        vals = np.ascontiguousarray(ns["vals"], dtype=np.float64)
+        self.nPar, self.nFree = ns["nPar"], ns["nFree"]
+        self._check(L.lib().mfbd_set_constraints(h, self.nPar, self.nFree, len(rows), ptr(rows, ip), ptr(cols, ip),
+                                                 ptr(vals)), "set_constraints")
+        self.nPatch = 0
+
+    def _check(self, rc, what, h="self"):
+        if rc != 0:
+            msg = ""
+            hh = self.h if h == "self" else None
+            if hh is not None:
+                msg = L.lib().mfbd_last_error(hh).decode()
+            raise SolverError(f"{what} failed ({rc}): {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().mfbd_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def load(self, patches):
+        """patches: list of problem dicts sharing this geometry (their data/resOffset differ)."""
+        data = np.ascontiguousarray(np.stack([p["data"] for p in patches]), dtype=np.float32)
+        chans = self.pb["channels"]
+        phiF = np.ascontiguousarray(np.stack([c["phiFixed"] for c in chans]), dtype=np.float64)
+        res = np.ascontiguousarray([[c["resOffset"] for c in p["channels"]] for p in patches], dtype=np.float64)
+        nf = np.ascontiguousarray([c["noiseFudge"] for c in chans], dtype=np.float64)
+        self._check(L.lib().mfbd_load_patches(self.h, len(patches), ptr(data, fp), ptr(phiF), ptr(res), ptr(nf)),
+                    "load_patches")
+        self.nPatch = len(patches)
+        self._data = data
+
+    def run(self, alpha0=None, restore=True):
+        n = self.nPatch
+        res = (Result * n)()
+        alpha = np.zeros((n, self.nImg, self.K))
+        img = np.zeros((n, self.nObj, self.s, self.s), dtype=np.float32) if restore else None
+        a0 = None if alpha0 is None else np.ascontiguousarray(alpha0, dtype=np.float64)
+        self._check(L.lib().mfbd_solve(self.h, ptr(a0) if a0 is not None else dp(), res, ptr(alpha),
+                                       ptr(img, fp) if restore else fp()), "solve")
+        out = []
+        for k in range(n):
+            r = res[k]
+            out.append(dict(alpha=alpha[k], img=None if img is None else img[k], status=r.status,
+                            iterations=r.iterations, nF=r.nF, nDF=r.nDF, failCount=r.failCount,
+                            initialMetric=r.initialMetric, finalMetric=r.finalMetric))
+        return out
+
+    # ---- stage-wise ----
+    def init(self):
+        self._check(L.lib().mfbd_init_patches(self.h), "init_patches")
+
+    def metric(self, alpha, patches=None):
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, self.nImg, self.K)
+        idx = np.ascontiguousarray(patches if patches is not None else np.arange(len(a)), dtype=np.int32)
+        out = np.zeros(len(idx))
+        self._check(L.lib().mfbd_eval_metric(self.h, len(idx), ptr(idx, ip), ptr(a), ptr(out)), "eval_metric")
+        return out
+
+    def grad(self, alpha, n_enabled=None, method="vogel", patches=None):
+        m = {"fd": L.GRAD_FD, "vogel": L.GRAD_VOGEL, "adjoint": L.GRAD_ADJOINT}[method]
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, self.nImg, self.K)
+        idx = np.ascontiguousarray(patches if patches is not None else np.arange(len(a)), dtype=np.int32)
+        g = np.zeros((len(idx), self.nImg, self.K))
+        ne = self.K if n_enabled is None else n_enabled
+        self._check(L.lib().mfbd_eval_grad(self.h, len(idx), ptr(idx, ip), ptr(a), ne, m, ptr(g)), "eval_grad")
+        return g
+
+    def restore(self, alpha, patches=None):
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(-1, self.nImg, self.K)
+        idx = np.ascontiguousarray(patches if patches is not None else np.arange(len(a)), dtype=np.int32)
+        img = np.zeros((len(idx), self.nObj, self.s, self.s), dtype=np.float32)
+        self._check(L.lib().mfbd_restore(self.h, len(idx), ptr(idx, ip), ptr(a), ptr(img, fp)), "restore")
+        return img
+
+    def debug(self, what, p=0):
+        cnt = np.zeros(1, dtype=np.int32)
+        self._check(L.lib().mfbd_debug_get(self.h, what.encode(), p, dp(), ptr(cnt, ip)), "debug_get")
+        out = np.zeros(int(cnt[0]))
+        self._check(L.lib().mfbd_debug_get(self.h, what.encode(), p, ptr(out), ptr(cnt, ip)), "debug_get")
+        return out
+
+    def kernel_stats(self):
+        cap = 64
+        names = (C.c_char * 48 * cap)()
+        ms = np.zeros(cap)
+        launches = np.zeros(cap, dtype=np.int32)
+        n = L.lib().mfbd_kernel_stats(self.h, cap, names, ptr(ms), ptr(launches, ip))
+        return {names[i].value.decode(): (float(ms[i]), int(launches[i])) for i in range(n)}

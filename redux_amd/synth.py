@@ -1,0 +1,161 @@
+"""Seeded synthetic MOMFBD patches (objects, wavefronts, frames).
+
+Builds problem dictionaries of the shape the solver consumes (the same layout the
+reference's PatchData/Object/Channel carry into Solver::run):
+  objects : weight, wavelength, pupil[P,P], modes[K,P,P] (scaled to RMS 1/lambda),
+            tilt (x,y) mode indices, J/Ji tilt Jacobians, freqCutoff
+  channels: obj, phiFixed[P,P] (diversity), resOffset (x,y), noiseFudge
+  imgCh   : channel of each image (images ordered object -> channel -> frame)
+  data    : float32 [nImg, s+2m, s+2m] data blocks, patch starts at (m, m)
+  nullspace: nPar, nFree, rows, cols, vals (Constraints::init layout)
+  settings: solver knobs with the reference's names
+
+Truth:
+  * object: Gaussian random field with a k^-3 power spectrum, mean `mean_counts`;
+  * wavefronts: Kolmogorov mode coefficients drawn with the Noll covariance
+    (Zernike) or the KL variances, times (D/r0)^(5/3);
+  * frames: object convolved with the model PSF of each wavefront (plus the
+    channel's diversity), then Poisson noise.
+
+The setup provider (pupil, modes, Jacobian, nullspace, covariance) is passed in;
+redux_amd.setup (the C++ implementation behind the C-ABI) is the default.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+SST_D = 0.97
+WAVELENGTH = 6.302e-7
+
+
+def _setup_default():
+    from . import setup as S
+    return S
+
+
+def grf_field(n, seed, mean=1000.0, contrast=0.1, slope=-3.0):
+    """Gaussian random field with power spectrum ~ k^slope on an n x n grid."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((n, n))
+    ky = np.fft.fftfreq(n)[:, None]
+    kx = np.fft.fftfreq(n)[None, :]
+    k = np.sqrt(kx * kx + ky * ky)
+    amp = np.zeros_like(k)
+    amp[k > 0] = k[k > 0] ** (slope / 2.0)
+    f = np.real(np.fft.ifft2(np.fft.fft2(w) * amp))
+    f = (f - f.mean()) / f.std()
+    return mean * (1.0 + contrast * f)
+
+
+def geometry(s, pupil_diameter=None):
+    """Optics giving pupil radius = s/4 on P = s/2 (critical sampling, N = s) by default,
+    or a chosen pupil diameter; returns dict(arcsec, fc, radius, P)."""
+    lam, D = WAVELENGTH, SST_D
+    r = (pupil_diameter if pupil_diameter else s / 2.0) / 2.0
+    fc = 2.0 * r
+    rad_per_pix = fc * lam / (s * D)
+    arcsec = rad_per_pix / (math.pi / (180.0 * 3600.0))
+    return dict(arcsec=arcsec, lam=lam, D=D)
+
+
+def build_problem(cfg: dict, seed: int = 1, setup=None, field=None, field_offset=(0, 0)):
+    """cfg keys: s, n_frames (per object), n_objects, modes (list of Noll/KL ids), basis
+    ('zernike'|'kl'), d_r0, mean_counts, diversity (list per object, RMS rad of Z4),
+    margin, pupil_diameter (optional), settings overrides."""
+    S = setup or _setup_default()
+    s = cfg["s"]
+    geo = geometry(s, cfg.get("pupil_diameter"))
+    fc, radius, P = S.calculate_pupil_size(geo["lam"], s, geo["D"], geo["arcsec"])
+    N = 2 * P
+    pup = S.generate_pupil(P, radius)
+    pupil, area = pup["pupil"], pup["area"]
+    modes_ids = list(cfg["modes"])
+    K = len(modes_ids)
+    basis = cfg.get("basis", "zernike")
+    B, atm_rms, tilt = S.mode_set(P, radius, modes_ids, basis=basis, kl_min=2,
+                                  kl_max=max(modes_ids) if basis == "kl" else 2000)
+    lam = geo["lam"]
+    Bs = S.normalize_modes(B, pupil, area, scale=1.0 / lam)
+    jscale = lam * (0.5 * fc) / S.pix2cf(geo["arcsec"], geo["D"])
+    tilt_m, J, Ji = S.measure_jacobian(Bs, pupil, jscale)
+    nO = cfg.get("n_objects", 1)
+    nF = cfg["n_frames"]
+    margin = cfg.get("margin", 5)
+    rng = np.random.default_rng(seed)
+
+    # wavefront coefficients (radians, unit-RMS modes), shared by all objects
+    d_r0 = cfg["d_r0"]
+    cov = np.array([[S.noll_cov(i, j) if basis == "zernike" else 0.0 for j in modes_ids] for i in modes_ids])
+    if basis != "zernike":
+        for k, j in enumerate(modes_ids):
+            cov[k, k] = atm_rms[k] ** 2
+            if j in (2, 3):
+                for k2, j2 in enumerate(modes_ids):
+                    if j2 in (2, 3):
+                        cov[k, k2] = S.noll_cov(j, j2)
+    cov *= d_r0 ** (5.0 / 3.0)
+    L = np.linalg.cholesky(cov + 1e-12 * np.eye(K))
+    a_rad = (L @ rng.standard_normal((K, nF))).T          # [nF, K]
+    alpha_true = a_rad * lam                               # reference alpha units
+
+    div = cfg.get("diversity", [0.0] * nO)
+    z4 = S.mode_set(P, radius, [4], basis="zernike")[0][0] if any(div) else None
+    objects, channels, imgCh, data = [], [], [], []
+    M = s + 2 * margin
+    G = N if N >= M else int(2 ** math.ceil(math.log2(M)))
+    ps, pio = pup["ps"], pup["pio"]
+    for o in range(nO):
+        objects.append(dict(weight=1.0, wavelength=lam, pupil=pupil, modes=Bs, tilt=tilt_m, J=J, Ji=Ji,
+                            freqCutoff=fc))
+        phi_fixed = div[o] * z4 if div[o] else np.zeros((P, P))
+        channels.append(dict(obj=o, phiFixed=phi_fixed, resOffset=(0.0, 0.0), noiseFudge=1.0))
+        if field is None:
+            obj_img = grf_field(G, seed * 7919 + 31 * o + 1, mean=cfg.get("mean_counts", 1000.0))
+        else:
+            fy, fx = field_offset
+            obj_img = field[o][fy:fy + G, fx:fx + G]
+        F_obj = np.fft.fft2(obj_img)
+        for f in range(nF):
+            # model PSF on the N grid (same sampling as the solver's OTF), embedded
+            # centred in the G x G convolution grid
+            phi = phi_fixed + np.tensordot(alpha_true[f], Bs, axes=(0, 0))
+            a = np.zeros((N, N), dtype=complex)
+            py, px = ps // P, ps % P
+            a[(py - P // 2) % N, (px - P // 2) % N] = pupil.ravel()[ps] * np.exp(1j * phi.ravel()[ps])
+            # FFT(psf) must equal the model OTF S = IFFT(|FFT(a)|^2)/N^2, i.e.
+            # psf(n) = |FFT(a)(-n)|^2 / N^2 = N^2 |IFFT(a)(n)|^2
+            psfN = np.fft.fftshift(np.abs(np.fft.ifft2(a)) ** 2)
+            psfN /= psfN.sum()
+            psf = np.zeros((G, G))
+            o0 = (G - N) // 2
+            psf[o0:o0 + N, o0:o0 + N] = psfN
+            psf = np.fft.ifftshift(psf)
+            img = np.real(np.fft.ifft2(F_obj * np.fft.fft2(psf)))
+            off = (G - M) // 2
+            blk = img[off:off + M, off:off + M]
+            blk = rng.poisson(np.clip(blk, 0, None)).astype(np.float32)
+            data.append(blk)
+            imgCh.append(o)
+    wfs = [[list(range(nF))] for _ in range(nO)]
+    nPar, nFree, rows, cols, vals = S.constraints([[nF] for _ in range(nO)], wfs, K)
+    settings = dict(FTOL=1e-3, EPS=1e-10, GRADDIFF_STEP=1e-2, MIN_ITER=5, MAX_ITER=500, N_DONE_ITER=3,
+                    MODE_START=5, MODE_STEP=5, APODIZATION=-1, NO_FILTER=0)
+    settings.update(cfg.get("settings", {}))
+    return dict(s=s, P=P, N=N, K=K, objects=objects, channels=channels, imgCh=np.array(imgCh, dtype=np.int32),
+                data=np.stack(data), start=(margin, margin),
+                nullspace=dict(nPar=nPar, nFree=nFree, rows=rows, cols=cols, vals=vals),
+                settings=settings, truth=dict(alpha=np.tile(alpha_true, (nO, 1)), a_rad=a_rad),
+                meta=dict(radius=radius, area=area, arcsec=geo["arcsec"], D=geo["D"], basis=basis,
+                          modes=modes_ids, atm_rms=atm_rms))
+
+
+# Configurations (sizes follow the benchmark definitions; cfg2 keeps N = 128 with a
+# 64-pixel pupil because the reference ties N = 2P and a 128 grid cannot hold a 100-px
+# pupil without aliasing the OTF).
+CFG1 = dict(s=64, n_frames=8, n_objects=1, modes=list(range(2, 22)), basis="zernike", d_r0=10.0,
+            mean_counts=1000.0, margin=5, settings=dict(FTOL=1e-6))
+CFG2_PATCH = dict(s=88, n_frames=24, n_objects=2, modes=list(range(2, 52)), basis="kl", d_r0=20.0,
+                  mean_counts=2000.0, diversity=[0.0, math.pi / math.sqrt(3.0)], margin=5,
+                  pupil_diameter=64.0, settings=dict(FTOL=1e-3))

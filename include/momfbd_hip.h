@@ -64,7 +64,12 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
     int noFilter;       /* NO_FILTER: skip the Scharmer filter in restoration           */
     int precision;      /* MFBD_FP64 or MFBD_FP32                                      */
     int maxBatch;       /* patches solved concurrently (1 = the reference's one-at-a-time) */
+    int gradient;       /* MFBD_GRAD_SCHEDULE_REFERENCE: finite differences for the first
+                           gradient, Vogel after (solver.cpp:385,478); or MFBD_GRAD_ADJOINT
+                           for the exact dL/dalpha throughout                              */
 } mfbd_params;
+
+#define MFBD_GRAD_SCHEDULE_REFERENCE 0
 
 typedef struct {
     int status;             /* MFBD_PATCH_*                                  */
@@ -128,9 +133,10 @@ int mfbd_restore(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha,
 /* Diagnostics: copy a device array for patch p ("S" OTF on support, "D", "P", "Q",
  * "ftsum", "phi") in double; returns the element count via *count. */
 int mfbd_debug_get(mfbd_ctx* ctx, const char* what, int p, double* out, int* count);
-/* Per-kernel timing of the last mfbd_solve / eval call: name[i], milliseconds,
- * launches; returns the number of entries (<= cap). */
-int mfbd_kernel_stats(mfbd_ctx* ctx, int cap, char (*names)[48], double* ms, int* launches);
+/* Per-kernel timing (environment MFBD_KERNEL_TIMING set when the patches were
+ * loaded): name[i], summed milliseconds, launches and images processed, accumulated
+ * over all calls; returns the number of entries (<= cap). */
+int mfbd_kernel_stats(mfbd_ctx* ctx, int cap, char (*names)[48], double* ms, int* launches, long long* units);
 
 /* ---- per-job setup (host; GlobalData/Object::initObject equivalents) ---- */
 const char* mfbd_setup_last_error(void);

@@ -42,7 +42,9 @@ def make_pupil(n: int, radius: float, inner: float = 0.0):
     D, ang = grid(n, n / 2.0 + 0.5)
     absang = np.abs(ang)
     pi = LD(math.pi)
-    d = np.where((absang > pi / 4) & (absang < 3 * pi / 4), np.abs(1 / np.sin(absang)), np.abs(1 / np.cos(absang)))
+    with np.errstate(divide="ignore"):
+        d = np.where((absang > pi / 4) & (absang < 3 * pi / 4), np.abs(1 / np.sin(absang)),
+                     np.abs(1 / np.cos(absang)))
     out = np.zeros((n, n), dtype=LD)
     area = LD(0)
     has_inner = inner > 0

@@ -14,6 +14,7 @@ fp = C.POINTER(C.c_float)
 MFBD_OK, MFBD_EINVAL, MFBD_ENOMEM, MFBD_EDEVICE, MFBD_ENONFINITE, MFBD_ESTATE = 0, 1, 2, 3, 4, 5
 FP64, FP32 = 0, 1
 GRAD_FD, GRAD_VOGEL, GRAD_ADJOINT = 1, 2, 3
+GRAD_SCHEDULE_REFERENCE = 0
 BASIS_ZERNIKE, BASIS_KL = 0, 1
 
 
@@ -27,7 +28,7 @@ class Params(C.Structure):
     _fields_ = [("FTOL", C.c_double), ("EPS", C.c_double), ("graddiffStep", C.c_double),
                 ("regAlpha", C.c_double), ("minIter", C.c_int), ("maxIter", C.c_int), ("nDoneIter", C.c_int),
                 ("modeStart", C.c_int), ("modeStep", C.c_int), ("apodSize", C.c_int), ("noFilter", C.c_int),
-                ("precision", C.c_int), ("maxBatch", C.c_int)]
+                ("precision", C.c_int), ("maxBatch", C.c_int), ("gradient", C.c_int)]
 
 
 class Result(C.Structure):
@@ -52,7 +53,7 @@ SIGNATURES = {
     "mfbd_eval_grad": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_int, C.c_int, dp]),
     "mfbd_restore": (C.c_int, [C.c_void_p, C.c_int, ip, dp, fp]),
     "mfbd_debug_get": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, dp, ip]),
-    "mfbd_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, dp, ip]),
+    "mfbd_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, dp, ip, C.POINTER(C.c_longlong)]),
     "mfbd_setup_last_error": (C.c_char_p, []),
     "mfbd_pupil_size": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double, dp, dp, ip]),
     "mfbd_make_pupil": (C.c_int, [C.c_int, C.c_double, C.c_double, dp, dp]),

@@ -40,6 +40,7 @@ struct KernelStat {
     std::string name;
     double ms = 0;
     int launches = 0;
+    long long units = 0;   // images (tiles) processed, summed over launches
 };
 
 class DeviceEvaluator {

@@ -531,7 +531,10 @@ struct DBuf {
     size_t n = 0;
     void alloc(size_t count) {
         if (count <= n) return;
-        if (p) HIPCHK(hipFree(p));
+        if (p) {
+            HIPCHK(hipDeviceSynchronize());   // in-flight kernels may still read the old buffer
+            HIPCHK(hipFree(p));
+        }
         HIPCHK(hipMalloc(&p, count * sizeof(T)));
         n = count;
     }
@@ -542,10 +545,15 @@ struct DBuf {
     }
 };
 
+// Host staging arrays are ordinary vectors, so each upload completes before return.
 template <typename T>
 void upload(DBuf<T>& b, const T* h, size_t n, hipStream_t st) {
+    if (n > b.n) HIPCHK(hipStreamSynchronize(st));
     b.alloc(std::max<size_t>(n, 1));
-    if (n) HIPCHK(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+    if (n) {
+        HIPCHK(hipMemcpyAsync(b.p, h, n * sizeof(T), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
 }
 
 template <typename T>
@@ -587,6 +595,8 @@ public:
         tw_.free_();
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
+        for (auto& e : evPool_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
+        for (auto& e : evPending_) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
         (void)hipStreamDestroy(st_);
     }
 
@@ -745,6 +755,7 @@ public:
                                ints_.p, imgObj_.p, nPairs, g_.nImg, g_.nObj, nOS_);
         });
         HIPCHK(hipStreamSynchronize(st_));
+        collect();
     }
 
     void evaluate(std::vector<EvalReq>& reqs) override {
@@ -768,6 +779,7 @@ public:
         HIPCHK(hipMemcpyAsync(P0, Pp_.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipMemcpyAsync(Q0, Qp_.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipStreamSynchronize(st_));
+        collect();
     }
 
     void debug_get(const std::string& what, int slot, std::vector<double>& out) override {
@@ -812,48 +824,77 @@ private:
         return (int)std::max<size_t>(16, std::min<size_t>(cap / per, (size_t)maxBatch_ * g_.nImg));
     }
 
+    // Per-kernel timing (MFBD_KERNEL_TIMING set): an event pair around each launch on
+    // the pipeline's stream, read back after the stream synchronises.
     template <typename F>
-    void launch(const char* name, F&& f) {
-        if (timing_) HIPCHK(hipEventRecord(ev0_, st_));
+    void launch(const char* name, F&& f) { launch(name, 0, f); }
+
+    template <typename F>
+    void launch(const char* name, long long units, F&& f) {
+        if (!timing_) {
+            f();
+            HIPCHK(hipGetLastError());
+            return;
+        }
+        if (evPool_.empty()) {
+            hipEvent_t a, b;
+            HIPCHK(hipEventCreate(&a));
+            HIPCHK(hipEventCreate(&b));
+            evPool_.push_back({a, b});
+        }
+        auto pr = evPool_.back();
+        evPool_.pop_back();
+        HIPCHK(hipEventRecord(pr.first, st_));
         f();
         HIPCHK(hipGetLastError());
-        if (timing_) {
-            HIPCHK(hipEventRecord(ev1_, st_));
-            HIPCHK(hipEventSynchronize(ev1_));
+        HIPCHK(hipEventRecord(pr.second, st_));
+        evPending_.push_back({name, pr});
+        evUnits_.push_back(units);
+    }
+
+    void collect() {
+        if (evPending_.empty()) return;
+        HIPCHK(hipStreamSynchronize(st_));
+        for (size_t k = 0; k < evPending_.size(); ++k) {
+            auto& e = evPending_[k];
             float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, ev0_, ev1_));
-            auto& s = stats_[name];
-            s.name = name;
+            HIPCHK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+            auto& s = stats_[e.first];
+            s.name = e.first;
             s.ms += ms;
             s.launches++;
+            s.units += evUnits_[k];
+            evPool_.push_back(e.second);
         }
+        evPending_.clear();
+        evUnits_.clear();
     }
 
     void fft2(C* tiles, int nTiles, int dir) {
         const unsigned gr = nTiles * (N / LINES);
         constexpr int NT = LINES * N / 16;
         if (dir < 0) {
-            launch("fft_rows", [&] { hipLaunchKernelGGL((k_fft_rows<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
-            launch("fft_cols", [&] { hipLaunchKernelGGL((k_fft_cols<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_rows", nTiles, [&] { hipLaunchKernelGGL((k_fft_rows<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_cols", nTiles, [&] { hipLaunchKernelGGL((k_fft_cols<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
         } else {
-            launch("fft_rows", [&] { hipLaunchKernelGGL((k_fft_rows<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
-            launch("fft_cols", [&] { hipLaunchKernelGGL((k_fft_cols<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_rows", nTiles, [&] { hipLaunchKernelGGL((k_fft_rows<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_cols", nTiles, [&] { hipLaunchKernelGGL((k_fft_cols<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
         }
     }
 
     // OTF chain for nQ virtual images whose phases are in `phi`: S = reorder(IFFT(|FFT(a)|^2))/N^2
     void otf_chain(const T* phi, const T* scale, int nQ, C* Sout) {
         tiles_.alloc((size_t)nQ * N2);
-        launch("k_field", [&] {
+        launch("k_field", nQ, [&] {
             hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, phi, invPio_.p,
                                pupil_.p, scale, nQ, N2, nPS_);
         });
         fft2(tiles_.p, nQ, -1);
-        launch("k_power", [&] {
+        launch("k_power", nQ, [&] {
             hipLaunchKernelGGL(k_power<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, (size_t)nQ * N2);
         });
         fft2(tiles_.p, nQ, +1);
-        launch("k_extract", [&] {
+        launch("k_extract", nQ, [&] {
             hipLaunchKernelGGL(k_extract<T>, dim3(nblk((size_t)nQ * nOS_)), dim3(256), 0, st_, Sout, tiles_.p,
                                osNat_.p, nQ, nOS_, N2, (T)(1.0 / N2));
         });
@@ -885,7 +926,7 @@ private:
         reqSlot_ = ints_.p + 4 * nQ;
         phi_.alloc((size_t)nQ * nPS_);
         S_.alloc((size_t)nQ * nOS_);
-        launch("k_phase", [&] {
+        launch("k_phase", nQ, [&] {
             hipLaunchKernelGGL(k_phase<T>, dim3(nblk(nPS_), (nQ + 15) / 16), dim3(256), 0, st_, phi_.p, B_.p, phiCh_.p,
                                alpha_.p, qm_, nQ, K, nPS_, imgObj_.p, imgCh_.p, g_.nCh);
         });
@@ -894,7 +935,7 @@ private:
         Pp_.alloc((size_t)n * g_.nObj * nOS_ * 2);
         Qp_.alloc((size_t)n * g_.nObj * nOS_);
         part_.alloc((size_t)n * g_.nObj * nChunk);
-        launch("k_pq", [&] {
+        launch("k_pq", nQ, [&] {
             hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, Pp_.p, Qp_.p, part_.p, S_.p, D_.p,
                                ftSum_.p, gamma_.p, reqSlot_, chObj_.p, chFirst_.p, chCount_.p, g_.nCh, g_.nObj, nImg,
                                nOS_, useGamma);
@@ -926,6 +967,7 @@ private:
             HIPCHK(hipMemcpyAsync(hg.data(), g_d_.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
         }
         HIPCHK(hipStreamSynchronize(st_));
+        collect();
         for (int r = 0; r < n; ++r) {
             rq[r].f = hL[r];
             if (rq[r].wantGrad && rq[r].g) std::memcpy(rq[r].g, &hg[(size_t)r * nImg * K], sizeof(double) * nImg * K);
@@ -977,7 +1019,7 @@ private:
             });
             std::vector<T> ones(nQ, T(1));
             upload(scale_, ones, st_);
-            launch("k_field", [&] {
+            launch("k_field", nQ, [&] {
                 hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, phi_.p,
                                    invPde_.p, pupil_.p, scale_.p, nQ, N2, nPS_);
             });
@@ -1088,6 +1130,9 @@ private:
     hipEvent_t ev0_, ev1_;
     bool timing_ = false;
     std::map<std::string, KernelStat> stats_;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> evPool_;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> evPending_;
+    std::vector<long long> evUnits_;
     std::vector<int> ps_, os_;
     int nPS_ = 0, nOS_ = 0;
     double area_ = 0;

@@ -639,7 +639,8 @@ Task<int> run_patch(PatchRun& r) {
     mfbd_ctx* c = r.c;
     const mfbd_params& pr = c->prm;
     const int K = c->g.K, nFree = c->nFree, nPar = c->nPar;
-    r.gradMethod = 1;                                      // GM_DIFF for the first gradient
+    const bool adjoint = pr.gradient == MFBD_GRAD_ADJOINT;
+    r.gradMethod = adjoint ? MFBD_GRAD_ADJOINT : MFBD_GRAD_FD;   // GM_DIFF for the first gradient
     double prev = 0, thisMetric = 0, gradNorm = 0, tol = 1.0E-1;
     int failCount = 0, totalIter = 0, maxIterations = 10, status = 0;
     const int maxFails = 3;
@@ -669,7 +670,7 @@ Task<int> run_patch(PatchRun& r) {
             iter++;
             status = co_await cg_iterate(r, f);
             if (failCount > maxFails) { status = ST_FAILURE; exitLoop = true; done = true; break; }
-            r.gradMethod = 2;                              // Vogel from here on (solver.cpp:478)
+            if (!adjoint) r.gradMethod = MFBD_GRAD_VOGEL;   // Vogel from here on (solver.cpp:478)
             if (status == ST_ENOPROG) { failCount++; break; }
             thisMetric = f;
             if (!std::isfinite(thisMetric)) { failCount++; break; }
@@ -756,6 +757,10 @@ int mfbd_ctx_create(const mfbd_geometry* geo, const mfbd_params* prm, int device
     c->g = {geo->s, P, 2 * P, geo->K, geo->nObj, geo->nCh, geo->nImg, geo->dataY, geo->dataX, geo->startY, geo->startX};
     c->prm = *prm;
     if (c->prm.maxBatch < 1) c->prm.maxBatch = 1;
+    if (c->prm.gradient != MFBD_GRAD_SCHEDULE_REFERENCE && c->prm.gradient != MFBD_GRAD_ADJOINT) {
+        delete c;
+        return MFBD_EINVAL;
+    }
     c->device = device;
     *out = c;
     return MFBD_OK;
@@ -1085,7 +1090,7 @@ int mfbd_debug_get(mfbd_ctx* c, const char* what, int p, double* out, int* count
     return MFBD_OK;
 }
 
-int mfbd_kernel_stats(mfbd_ctx* c, int cap, char (*names)[48], double* ms, int* launches) {
+int mfbd_kernel_stats(mfbd_ctx* c, int cap, char (*names)[48], double* ms, int* launches, long long* units) {
     if (!c || !c->dev) return 0;
     auto st = c->dev->kernel_stats();
     int n = 0;
@@ -1094,6 +1099,7 @@ int mfbd_kernel_stats(mfbd_ctx* c, int cap, char (*names)[48], double* ms, int* 
         std::snprintf(names[n], 48, "%s", s.name.c_str());
         ms[n] = s.ms;
         launches[n] = s.launches;
+        if (units) units[n] = s.units;
         n++;
     }
     return n;

@@ -31,7 +31,8 @@ def default_params(**kw):
 class Solver:
     """One context: geometry + structure + basis + constraints; patches are loaded in bulk."""
 
-    def __init__(self, pb: dict, precision: str = "fp64", max_batch: int = 1, device: int = 0, settings=None):
+    def __init__(self, pb: dict, precision: str = "fp64", max_batch: int = 1, device: int = 0, settings=None,
+                 gradient: str = "reference"):
         self.pb = pb
         st = default_params(**pb.get("settings", {}))
         if settings:
@@ -45,7 +46,8 @@ class Solver:
                      pb["start"][0], pb["start"][1])
         prm = Params(st["FTOL"], st["EPS"], st["GRADDIFF_STEP"], st["REG_ALPHA"], st["MIN_ITER"], st["MAX_ITER"],
                      st["N_DONE_ITER"], st["MODE_START"], st["MODE_STEP"], st["APODIZATION"], int(st["NO_FILTER"]),
-                     L.FP64 if precision == "fp64" else L.FP32, max_batch)
+                     L.FP64 if precision == "fp64" else L.FP32, max_batch,
+                     L.GRAD_ADJOINT if gradient == "adjoint" else L.GRAD_SCHEDULE_REFERENCE)
         h = C.c_void_p()
         self._check(L.lib().mfbd_ctx_create(C.byref(g), C.byref(prm), device, C.byref(h)), "ctx_create", None)
         self.h = h
@@ -154,5 +156,8 @@ class Solver:
         names = (C.c_char * 48 * cap)()
         ms = np.zeros(cap)
         launches = np.zeros(cap, dtype=np.int32)
-        n = L.lib().mfbd_kernel_stats(self.h, cap, names, ptr(ms), ptr(launches, ip))
-        return {names[i].value.decode(): (float(ms[i]), int(launches[i])) for i in range(n)}
+        units = np.zeros(cap, dtype=np.int64)
+        n = L.lib().mfbd_kernel_stats(self.h, cap, names, ptr(ms), ptr(launches, ip),
+                                      units.ctypes.data_as(C.POINTER(C.c_longlong)))
+        return {names[i].value.decode(): dict(ms=float(ms[i]), launches=int(launches[i]), units=int(units[i]))
+                for i in range(n)}

@@ -60,33 +60,24 @@ def geometry(s, pupil_diameter=None):
     return dict(arcsec=arcsec, lam=lam, D=D)
 
 
-def build_problem(cfg: dict, seed: int = 1, setup=None, field=None, field_offset=(0, 0)):
-    """cfg keys: s, n_frames (per object), n_objects, modes (list of Noll/KL ids), basis
-    ('zernike'|'kl'), d_r0, mean_counts, diversity (list per object, RMS rad of Z4),
-    margin, pupil_diameter (optional), settings overrides."""
+def build_setup(cfg: dict, setup=None):
+    """Geometry, pupil, scaled modes, Jacobian, nullspace and wavefront covariance
+    shared by every patch of a configuration."""
     S = setup or _setup_default()
     s = cfg["s"]
     geo = geometry(s, cfg.get("pupil_diameter"))
     fc, radius, P = S.calculate_pupil_size(geo["lam"], s, geo["D"], geo["arcsec"])
-    N = 2 * P
     pup = S.generate_pupil(P, radius)
-    pupil, area = pup["pupil"], pup["area"]
     modes_ids = list(cfg["modes"])
     K = len(modes_ids)
     basis = cfg.get("basis", "zernike")
-    B, atm_rms, tilt = S.mode_set(P, radius, modes_ids, basis=basis, kl_min=2,
-                                  kl_max=max(modes_ids) if basis == "kl" else 2000)
+    kl_max = cfg.get("kl_max", max(modes_ids)) if basis == "kl" else 2000
+    B, atm_rms, tilt = S.mode_set(P, radius, modes_ids, basis=basis, kl_min=2, kl_max=kl_max)
     lam = geo["lam"]
-    Bs = S.normalize_modes(B, pupil, area, scale=1.0 / lam)
+    Bs = S.normalize_modes(B, pup["pupil"], pup["area"], scale=1.0 / lam)
     jscale = lam * (0.5 * fc) / S.pix2cf(geo["arcsec"], geo["D"])
-    tilt_m, J, Ji = S.measure_jacobian(Bs, pupil, jscale)
-    nO = cfg.get("n_objects", 1)
-    nF = cfg["n_frames"]
-    margin = cfg.get("margin", 5)
-    rng = np.random.default_rng(seed)
-
-    # wavefront coefficients (radians, unit-RMS modes), shared by all objects
-    d_r0 = cfg["d_r0"]
+    tilt_m, J, Ji = S.measure_jacobian(Bs, pup["pupil"], jscale)
+    nO, nF = cfg.get("n_objects", 1), cfg["n_frames"]
     cov = np.array([[S.noll_cov(i, j) if basis == "zernike" else 0.0 for j in modes_ids] for i in modes_ids])
     if basis != "zernike":
         for k, j in enumerate(modes_ids):
@@ -95,21 +86,41 @@ def build_problem(cfg: dict, seed: int = 1, setup=None, field=None, field_offset
                 for k2, j2 in enumerate(modes_ids):
                     if j2 in (2, 3):
                         cov[k, k2] = S.noll_cov(j, j2)
-    cov *= d_r0 ** (5.0 / 3.0)
-    L = np.linalg.cholesky(cov + 1e-12 * np.eye(K))
-    a_rad = (L @ rng.standard_normal((K, nF))).T          # [nF, K]
-    alpha_true = a_rad * lam                               # reference alpha units
-
+    cov *= cfg["d_r0"] ** (5.0 / 3.0)
     div = cfg.get("diversity", [0.0] * nO)
     z4 = S.mode_set(P, radius, [4], basis="zernike")[0][0] if any(div) else None
-    objects, channels, imgCh, data = [], [], [], []
+    wfs = [[list(range(nF))] for _ in range(nO)]
+    nPar, nFree, rows, cols, vals = S.constraints([[nF] for _ in range(nO)], wfs, K)
+    settings = dict(FTOL=1e-3, EPS=1e-10, GRADDIFF_STEP=1e-2, MIN_ITER=5, MAX_ITER=500, N_DONE_ITER=3,
+                    MODE_START=5, MODE_STEP=5, APODIZATION=-1, NO_FILTER=0)
+    settings.update(cfg.get("settings", {}))
+    return dict(cfg=cfg, s=s, P=P, N=2 * P, K=K, fc=fc, radius=radius, pup=pup, Bs=Bs, tilt=tilt_m, J=J, Ji=Ji,
+                lam=lam, geo=geo, chol=np.linalg.cholesky(cov + 1e-12 * np.eye(K)), div=div, z4=z4,
+                nullspace=dict(nPar=nPar, nFree=nFree, rows=rows, cols=cols, vals=vals), settings=settings,
+                atm_rms=atm_rms, modes=modes_ids, basis=basis)
+
+
+def build_patch(su: dict, seed: int, field=None, field_offset=(0, 0)):
+    """One patch: wavefronts drawn from `seed`, frames from `field` (a list of object
+    images, one per object) or from a per-patch random field."""
+    cfg = su["cfg"]
+    s, P, N, K, lam = su["s"], su["P"], su["N"], su["K"], su["lam"]
+    pupil, ps = su["pup"]["pupil"], su["pup"]["ps"]
+    Bs = su["Bs"]
+    nO, nF = cfg.get("n_objects", 1), cfg["n_frames"]
+    margin = cfg.get("margin", 5)
+    rng = np.random.default_rng(seed)
+    a_rad = (su["chol"] @ rng.standard_normal((K, nF))).T          # [nF, K] radians, unit-RMS modes
+    alpha_true = a_rad * lam                                        # reference alpha units
     M = s + 2 * margin
     G = N if N >= M else int(2 ** math.ceil(math.log2(M)))
-    ps, pio = pup["ps"], pup["pio"]
+    objects, channels, imgCh, data = [], [], [], []
+    py, px = ps // P, ps % P
     for o in range(nO):
-        objects.append(dict(weight=1.0, wavelength=lam, pupil=pupil, modes=Bs, tilt=tilt_m, J=J, Ji=Ji,
-                            freqCutoff=fc))
-        phi_fixed = div[o] * z4 if div[o] else np.zeros((P, P))
+        objects.append(dict(weight=1.0, wavelength=lam, pupil=pupil, modes=Bs, tilt=su["tilt"], J=su["J"],
+                            Ji=su["Ji"], freqCutoff=su["fc"]))
+        d = su["div"][o]
+        phi_fixed = d * su["z4"] if d else np.zeros((P, P))
         channels.append(dict(obj=o, phiFixed=phi_fixed, resOffset=(0.0, 0.0), noiseFudge=1.0))
         if field is None:
             obj_img = grf_field(G, seed * 7919 + 31 * o + 1, mean=cfg.get("mean_counts", 1000.0))
@@ -119,36 +130,50 @@ def build_problem(cfg: dict, seed: int = 1, setup=None, field=None, field_offset
         F_obj = np.fft.fft2(obj_img)
         for f in range(nF):
             # model PSF on the N grid (same sampling as the solver's OTF), embedded
-            # centred in the G x G convolution grid
+            # centred in the G x G convolution grid.  FFT(psf) must equal the model
+            # OTF S = IFFT(|FFT(a)|^2)/N^2, i.e. psf(n) = N^2 |IFFT(a)(n)|^2.
             phi = phi_fixed + np.tensordot(alpha_true[f], Bs, axes=(0, 0))
             a = np.zeros((N, N), dtype=complex)
-            py, px = ps // P, ps % P
             a[(py - P // 2) % N, (px - P // 2) % N] = pupil.ravel()[ps] * np.exp(1j * phi.ravel()[ps])
-            # FFT(psf) must equal the model OTF S = IFFT(|FFT(a)|^2)/N^2, i.e.
-            # psf(n) = |FFT(a)(-n)|^2 / N^2 = N^2 |IFFT(a)(n)|^2
             psfN = np.fft.fftshift(np.abs(np.fft.ifft2(a)) ** 2)
             psfN /= psfN.sum()
             psf = np.zeros((G, G))
             o0 = (G - N) // 2
             psf[o0:o0 + N, o0:o0 + N] = psfN
-            psf = np.fft.ifftshift(psf)
-            img = np.real(np.fft.ifft2(F_obj * np.fft.fft2(psf)))
+            img = np.real(np.fft.ifft2(F_obj * np.fft.fft2(np.fft.ifftshift(psf))))
             off = (G - M) // 2
             blk = img[off:off + M, off:off + M]
-            blk = rng.poisson(np.clip(blk, 0, None)).astype(np.float32)
-            data.append(blk)
+            data.append(rng.poisson(np.clip(blk, 0, None)).astype(np.float32))
             imgCh.append(o)
-    wfs = [[list(range(nF))] for _ in range(nO)]
-    nPar, nFree, rows, cols, vals = S.constraints([[nF] for _ in range(nO)], wfs, K)
-    settings = dict(FTOL=1e-3, EPS=1e-10, GRADDIFF_STEP=1e-2, MIN_ITER=5, MAX_ITER=500, N_DONE_ITER=3,
-                    MODE_START=5, MODE_STEP=5, APODIZATION=-1, NO_FILTER=0)
-    settings.update(cfg.get("settings", {}))
     return dict(s=s, P=P, N=N, K=K, objects=objects, channels=channels, imgCh=np.array(imgCh, dtype=np.int32),
-                data=np.stack(data), start=(margin, margin),
-                nullspace=dict(nPar=nPar, nFree=nFree, rows=rows, cols=cols, vals=vals),
-                settings=settings, truth=dict(alpha=np.tile(alpha_true, (nO, 1)), a_rad=a_rad),
-                meta=dict(radius=radius, area=area, arcsec=geo["arcsec"], D=geo["D"], basis=basis,
-                          modes=modes_ids, atm_rms=atm_rms))
+                data=np.stack(data), start=(margin, margin), nullspace=su["nullspace"], settings=su["settings"],
+                truth=dict(alpha=np.tile(alpha_true, (nO, 1)), a_rad=a_rad),
+                meta=dict(radius=su["radius"], area=su["pup"]["area"], arcsec=su["geo"]["arcsec"], D=su["geo"]["D"],
+                          basis=su["basis"], modes=su["modes"], atm_rms=su["atm_rms"]))
+
+
+def build_problem(cfg: dict, seed: int = 1, setup=None, field=None, field_offset=(0, 0)):
+    """cfg keys: s, n_frames (per object), n_objects, modes (list of Noll/KL ids), basis
+    ('zernike'|'kl'), d_r0, mean_counts, diversity (list per object, RMS rad of Z4),
+    margin, pupil_diameter (optional), kl_max (optional), settings overrides."""
+    return build_patch(build_setup(cfg, setup), seed, field, field_offset)
+
+
+def build_field_batch(cfg: dict, n_patches: int, seed: int = 2024, field_size: int = 1024, setup=None):
+    """n_patches overlapping patches on a square grid over one seeded k^-3 field per
+    object (the benchmark's config-2 layout: 256 patches from a 1024 x 1024 field)."""
+    su = build_setup(cfg, setup)
+    nO = cfg.get("n_objects", 1)
+    fields = [grf_field(field_size, seed + 17 * o, mean=cfg.get("mean_counts", 1000.0)) for o in range(nO)]
+    side = int(math.ceil(math.sqrt(n_patches)))
+    N, s, margin = su["N"], su["s"], cfg.get("margin", 5)
+    G = N if N >= s + 2 * margin else int(2 ** math.ceil(math.log2(s + 2 * margin)))
+    stride = (field_size - G) // max(side - 1, 1)
+    out = []
+    for k in range(n_patches):
+        iy, ix = divmod(k, side)
+        out.append(build_patch(su, seed * 1000 + k, field=fields, field_offset=(iy * stride, ix * stride)))
+    return su, out
 
 
 # Configurations (sizes follow the benchmark definitions; cfg2 keeps N = 128 with a

@@ -110,7 +110,11 @@ def roofline(stats, N, nPS, nOS):
     k_phase writes phi (4 nPS) and reads alpha (K doubles, negligible); k_pq reads S
     and D (16 nOS) per image."""
     per_unit = {"fft_rows": 16 * N * N, "fft_cols": 16 * N * N, "k_field": 4 * nPS + 8 * N * N,
-                "k_power": 16 * N * N, "k_extract": 16 * nOS, "k_phase": 4 * nPS, "k_pq": 16 * nOS}
+                "k_power": 16 * N * N, "k_extract": 16 * nOS, "k_phase": 4 * nPS, "k_pq": 16 * nOS,
+                # fused: read phi on the pupil support, write S on the OTF support
+                "k_otf_fused": 4 * nPS + 8 * nOS,
+                # fused Vogel: read S, D, P, Q on the support and phi; write v
+                "k_vogel_fused": 16 * nOS + 24 * nOS + 8 * nPS}
     dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
     name, s = dom
     out = dict(kernel=name)

@@ -64,6 +64,9 @@ public:
     virtual std::vector<KernelStat> kernel_stats() = 0;
     virtual void reset_stats() = 0;
     virtual int capacity() const = 0;
+    // pinned [slots][nImg][K] buffer: a request whose alpha points at its slot's row is
+    // uploaded from here without a host-side copy
+    virtual double* alpha_staging() = 0;
 };
 
 std::unique_ptr<DeviceEvaluator> make_device_evaluator(const Geometry& g, int precision, int maxSlots,

@@ -38,6 +38,29 @@ public:
     // in-place 1-D transform of x[0], x[stride], ..., sign -1 forward / +1 backward
     void run(cd* x, int stride, int sign) {
         if (n_ <= 1) return;
+        if ((n_ & (n_ - 1)) == 0) {      // iterative radix-2, no allocation
+            cd* a = scratch_.data();
+            for (int k = 0; k < n_; ++k) a[k] = x[(long)k * stride];
+            for (int i = 1, j = 0; i < n_; ++i) {
+                int bit = n_ >> 1;
+                for (; j & bit; bit >>= 1) j ^= bit;
+                j ^= bit;
+                if (i < j) std::swap(a[i], a[j]);
+            }
+            for (int len = 2; len <= n_; len <<= 1) {
+                const int half = len >> 1, step = n_ / len;
+                for (int i = 0; i < n_; i += len)
+                    for (int j = 0; j < half; ++j) {
+                        cd w = tw_[j * step];
+                        if (sign > 0) w = std::conj(w);
+                        const cd u = a[i + j], v = a[i + j + half] * w;
+                        a[i + j] = u + v;
+                        a[i + j + half] = u - v;
+                    }
+            }
+            for (int k = 0; k < n_; ++k) x[(long)k * stride] = a[k];
+            return;
+        }
         for (int k = 0; k < n_; ++k) scratch_[k] = x[(long)k * stride];
         std::vector<cd> out(n_);
         rec(scratch_.data(), 1, out.data(), n_, 0, sign);

@@ -24,6 +24,7 @@
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "evaluator.hpp"
@@ -98,7 +99,10 @@ __device__ __forceinline__ void dft_reg(cx<T>* v) {
 
 // One Stockham pass of radix R over NL lines of length N held in LDS.
 // Element k of line l is at buf[l*ls + k*es].  NS = span already transformed.
-template <typename T, int N, int R, int NS, int DIR, int NL, int NT>
+// MAP 0: consecutive threads take consecutive butterflies of one line (lines
+// contiguous in LDS); MAP 1: consecutive threads take consecutive lines (lines
+// interleaved in LDS) -- each keeps neighbouring lanes on neighbouring banks.
+template <typename T, int N, int R, int NS, int DIR, int NL, int NT, int MAP = 0>
 __device__ __forceinline__ void spass(cx<T>* buf, int ls, int es, const cx<T>* __restrict__ tw) {
     constexpr int NB = NL * (N / R);
     constexpr int BT = (NB + NT - 1) / NT;
@@ -107,7 +111,7 @@ __device__ __forceinline__ void spass(cx<T>* buf, int ls, int es, const cx<T>* _
     for (int i = 0; i < BT; ++i) {
         const int b = threadIdx.x + i * NT;
         if (NB % NT == 0 || b < NB) {
-            const int line = b / (N / R), j = b % (N / R);
+            const int line = MAP ? b % NL : b / (N / R), j = MAP ? b / NL : b % (N / R);
             const cx<T>* L = buf + line * ls;
             const int k = j % NS;
 #pragma unroll
@@ -128,7 +132,7 @@ __device__ __forceinline__ void spass(cx<T>* buf, int ls, int es, const cx<T>* _
     for (int i = 0; i < BT; ++i) {
         const int b = threadIdx.x + i * NT;
         if (NB % NT == 0 || b < NB) {
-            const int line = b / (N / R), j = b % (N / R);
+            const int line = MAP ? b % NL : b / (N / R), j = MAP ? b / NL : b % (N / R);
             cx<T>* L = buf + line * ls;
             const int k = j % NS;
             const int base = (j / NS) * NS * R + k;
@@ -139,17 +143,17 @@ __device__ __forceinline__ void spass(cx<T>* buf, int ls, int es, const cx<T>* _
     __syncthreads();
 }
 
-template <typename T, int N, int DIR, int NL, int NT>
+template <typename T, int N, int DIR, int NL, int NT, int MAP = 0>
 __device__ __forceinline__ void fft_lines(cx<T>* buf, int ls, int es, const cx<T>* __restrict__ tw) {
     if constexpr (N == 64) {
-        spass<T, N, 8, 1, DIR, NL, NT>(buf, ls, es, tw);
-        spass<T, N, 8, 8, DIR, NL, NT>(buf, ls, es, tw);
+        spass<T, N, 8, 1, DIR, NL, NT, MAP>(buf, ls, es, tw);
+        spass<T, N, 8, 8, DIR, NL, NT, MAP>(buf, ls, es, tw);
     } else if constexpr (N == 128) {
-        spass<T, N, 16, 1, DIR, NL, NT>(buf, ls, es, tw);
-        spass<T, N, 8, 16, DIR, NL, NT>(buf, ls, es, tw);
+        spass<T, N, 16, 1, DIR, NL, NT, MAP>(buf, ls, es, tw);
+        spass<T, N, 8, 16, DIR, NL, NT, MAP>(buf, ls, es, tw);
     } else if constexpr (N == 256) {
-        spass<T, N, 16, 1, DIR, NL, NT>(buf, ls, es, tw);
-        spass<T, N, 16, 16, DIR, NL, NT>(buf, ls, es, tw);
+        spass<T, N, 16, 1, DIR, NL, NT, MAP>(buf, ls, es, tw);
+        spass<T, N, 16, 16, DIR, NL, NT, MAP>(buf, ls, es, tw);
     } else {
         static_assert(N == 64 || N == 128 || N == 256, "unsupported OTF grid");
     }
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(LC * N / 16) k_fft_cols(cx<T>* __restrict__ da
         lds[e] = g[(size_t)r * N + c];
     }
     __syncthreads();
-    fft_lines<T, N, DIR, LC, NT>(lds, 1, LC, tw);
+    fft_lines<T, N, DIR, LC, NT, 1>(lds, 1, LC, tw);
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
         g[(size_t)r * N + c] = lds[e];
@@ -246,16 +250,18 @@ __global__ void k_phase_pert(T* __restrict__ out, const T* __restrict__ phi, con
 // tile[q][e] = scale[q] * pupil * exp(i phi) where inv[e] names the pupil pixel (or -1)
 template <typename T>
 __global__ void k_field(cx<T>* __restrict__ tiles, const T* __restrict__ phi, const int* __restrict__ inv,
-                        const T* __restrict__ pupil, const T* __restrict__ scale, int nQ, int N2, int nPS) {
+                        const T* __restrict__ pupil, const int* __restrict__ qImg, const T* __restrict__ imgScale,
+                        const int* __restrict__ qMap, int nQ, int N2, int nPS) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nQ * N2) return;
-    const int q = e / N2, p = e % N2;
+    const int qt = e / N2, p = e % N2;
+    const int q = qMap ? qMap[qt] : qt;      // tile index -> virtual image
     const int idx = inv[p];
     cx<T> v = {T(0), T(0)};
     if (idx >= 0) {
         T s, c;
         sincos_t(phi[(size_t)q * nPS + idx], &s, &c);
-        const T mag = pupil[idx] * scale[q];
+        const T mag = pupil[idx] * (imgScale ? imgScale[qImg[q]] : T(1));
         v = {mag * c, mag * s};
     }
     tiles[e] = v;
@@ -358,11 +364,11 @@ __global__ void k_metric(double* __restrict__ L, const double* __restrict__ part
 template <typename T>
 __global__ void k_build_g(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ S, const cx<T>* __restrict__ D,
                           const double* __restrict__ P, const double* __restrict__ Q, const int* __restrict__ invOs,
-                          QMap qm, const int* __restrict__ qReq, const int* __restrict__ imgObj, int nQ, int N2,
-                          int nOS, int nImg, int nObj) {
+                          QMap qm, const int* __restrict__ qReq, const int* __restrict__ imgObj,
+                          const int* __restrict__ gIdx, int nQ, int N2, int nOS, int nImg, int nObj) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nQ * N2) return;
-    const int q = e / N2, p = e % N2;
+    const int q = gIdx[e / N2], p = e % N2;
     const int u = invOs[p];
     cx<T> v = {T(0), T(0)};
     if (u >= 0) {
@@ -397,13 +403,13 @@ __global__ void k_mul_re(cx<T>* __restrict__ tiles, const T* __restrict__ re, si
 // v[q][i] = Im(conj(PF_i) t[pde_i]) * (pupil_i for the reference form, 1 for the adjoint)
 template <typename T>
 __global__ void k_extract_v(T* __restrict__ v, const cx<T>* __restrict__ tiles, const T* __restrict__ phi,
-                            const int* __restrict__ pde, const T* __restrict__ pupil, int nQ, int nPS, int N2,
-                            int extraPupil) {
+                            const int* __restrict__ pde, const T* __restrict__ pupil, const int* __restrict__ gIdx,
+                            int nQ, int nPS, int N2, int extraPupil) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nQ * nPS) return;
     const int q = e / nPS, i = e % nPS;
     T s, c;
-    sincos_t(phi[e], &s, &c);
+    sincos_t(phi[(size_t)gIdx[q] * nPS + i], &s, &c);
     const T pu = pupil[i];
     const cx<T> t = tiles[(size_t)q * N2 + pde[i]];
     const T pfr = pu * c, pfi = pu * s;
@@ -415,12 +421,13 @@ __global__ void k_extract_v(T* __restrict__ v, const cx<T>* __restrict__ tiles, 
 template <typename T>
 __global__ void __launch_bounds__(256) k_grad(double* __restrict__ g, const T* __restrict__ v, const T* __restrict__ B,
                                               const double* __restrict__ scale, QMap qm,
-                                              const int* __restrict__ imgObj, int K, int nPS, int nEnabled) {
+                                              const int* __restrict__ imgObj, const int* __restrict__ gIdx, int K,
+                                              int nPS, int nEnabled) {
     __shared__ double sh[256];
-    const int q = blockIdx.x;
+    const int qg = blockIdx.x, q = gIdx[qg];
     const int o = imgObj[qm.img[q]];
-    const T* vq = v + (size_t)q * nPS;
-    double* gq = g + (size_t)qm.arow[q] * K;
+    const T* vq = v + (size_t)qg * nPS;
+    double* gq = g + (size_t)q * K;
     for (int m = 0; m < K; ++m) {
         double acc = 0;
         if (m < nEnabled) {
@@ -428,7 +435,7 @@ __global__ void __launch_bounds__(256) k_grad(double* __restrict__ g, const T* _
             for (int i = threadIdx.x; i < nPS; i += 256) acc += (double)(vq[i] * b[i]);
         }
         const double s = block_sum(acc, sh);
-        if (threadIdx.x == 0) gq[m] = (m < nEnabled) ? scale[q] * s : 0.0;
+        if (threadIdx.x == 0) gq[m] = (m < nEnabled) ? scale[qg] * s : 0.0;
     }
 }
 
@@ -523,6 +530,231 @@ __global__ void k_ftsum(double* __restrict__ ftSum, const cx<T>* __restrict__ D,
     ftSum[((size_t)slot * nObj + o) * nOS + u] = acc;
 }
 
+// Per-frame statistics and noise level of SubImage::initialize with doReset
+// (subimage.cpp:155-187): windowed image stats, then FourierTransform::noise
+// (fouriertransform.cpp:448-495, mask = s/6+1, limit = s^2/4) of
+// (img - mean) * noiseWindow.  The s x s DFT is evaluated directly (any s <= 128,
+// the patch side need not be a power of two), one workgroup per frame, FP64.
+__global__ void __launch_bounds__(256) k_noise(double* __restrict__ noise, double* __restrict__ stddev,
+                                               const float* __restrict__ raw, const double* __restrict__ window,
+                                               const double* __restrict__ nwindow, const int* __restrict__ slot,
+                                               const int* __restrict__ img, const int* __restrict__ sx,
+                                               const int* __restrict__ sy, int nImg, int dataY, int dataX,
+                                               int startY, int startX, int s) {
+    extern __shared__ double nsh[];
+    double* X = nsh;                      // s*s real
+    double* twr = X + s * s;              // cos(2 pi k/s)
+    double* twi = twr + s;                // -sin(2 pi k/s)
+    double* Zr = twi + s;                 // two columns of Z = X W
+    double* Zi = Zr + 2 * s;
+    __shared__ double red[256];
+    const int q = blockIdx.x;
+    const float* blk = raw + ((size_t)slot[q] * nImg + img[q]) * dataY * dataX;
+    const int y0 = startY + sy[q], x0 = startX + sx[q];
+    const int s2 = s * s;
+    double acc = 0;
+    for (int e = threadIdx.x; e < s2; e += 256) {
+        const double v = blk[(size_t)(y0 + e / s) * dataX + x0 + e % s];
+        X[e] = v;
+        acc += v;
+    }
+    for (int k = threadIdx.x; k < s; k += 256) {
+        double sn, cs;
+        sincospi(2.0 * k / s, &sn, &cs);
+        twr[k] = cs;
+        twi[k] = -sn;
+    }
+    const double avg = block_sum(acc, red) / s2;
+    acc = 0;
+    for (int e = threadIdx.x; e < s2; e += 256) {
+        const double v = (X[e] - avg) * window[e] + avg;
+        X[e] = v;
+        acc += v;
+    }
+    const double mean = block_sum(acc, red) / s2;
+    acc = 0;
+    for (int e = threadIdx.x; e < s2; e += 256) acc += X[e] * X[e];
+    const double rms = block_sum(acc, red) / s2;
+    for (int e = threadIdx.x; e < s2; e += 256) X[e] = (X[e] - mean) * nwindow[e];
+    __syncthreads();
+    const int mask = s / 6 + 1;
+    const double limits = s2 / 4.0;
+    double np = 0;
+    long cnt = 0;
+    for (int v0 = 0; v0 < s; v0 += 2) {               // column frequencies v0, v0+1
+        for (int t = threadIdx.x; t < 2 * s; t += 256) {
+            const int c = t / s, y = t % s, v = v0 + c;
+            double zr = 0, zi = 0;
+            if (v < s)
+                for (int x = 0; x < s; ++x) {
+                    const int k = (x * v) % s;
+                    zr += X[y * s + x] * twr[k];
+                    zi += X[y * s + x] * twi[k];
+                }
+            Zr[c * s + y] = zr;
+            Zi[c * s + y] = zi;
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < 2 * s; t += 256) {
+            const int c = t / s, u = t % s, v = v0 + c;
+            if (v >= s || v < mask || v >= s - mask + 1 || u < mask || u >= s - mask + 1) continue;
+            const double xx = (double)min(v, s - v) * min(v, s - v);
+            const double yy = min(u, s - u);
+            if (xx + yy * yy < limits) continue;
+            double fr = 0, fi = 0;
+            for (int y = 0; y < s; ++y) {
+                const int k = (u * y) % s;
+                const double ar = twr[k], ai = twi[k], br = Zr[c * s + y], bi = Zi[c * s + y];
+                fr += ar * br - ai * bi;
+                fi += ar * bi + ai * br;
+            }
+            np += fr * fr + fi * fi;
+            cnt++;
+        }
+        __syncthreads();
+    }
+    const double tot = block_sum(np, red);
+    const double n = block_sum((double)cnt, red);
+    if (threadIdx.x == 0) {
+        noise[q] = n > 0 ? sqrt(tot / (n * s2)) : tot;
+        stddev[q] = sqrt(rms - mean * mean);
+    }
+}
+
+// --------------------------------------------------------- fused (LDS) ----
+// One workgroup per image with the whole N x N tile resident in LDS (FP32, N <= 128:
+// 128 x 129 complex floats = 132 KB of gfx950's 160 KB).  Rows are padded by one
+// element so row and column passes both walk consecutive banks (MAP 1).
+template <int N>
+struct Tile {
+    static constexpr int LD = N + 1;                 // padded row stride (complex elements)
+    static constexpr int SIZE = N * LD;
+    __device__ static int at(int y, int x) { return y * LD + x; }
+};
+
+template <typename T, int N, int DIR, int NT>
+__device__ __forceinline__ void tile_fft2(cx<T>* t, const cx<T>* __restrict__ tw, int rowFirst, int nRows) {
+    // rows [rowFirst, rowFirst + nRows) (others known zero), then all columns
+    if (nRows == N) fft_lines<T, N, DIR, N, NT, 1>(t, Tile<N>::LD, 1, tw);
+    else fft_lines<T, N, DIR, N / 2, NT, 1>(t + rowFirst * Tile<N>::LD, Tile<N>::LD, 1, tw);
+    fft_lines<T, N, DIR, N, NT, 1>(t, 1, Tile<N>::LD, tw);
+}
+
+// S[q][u] = reorder(IFFT(|FFT(a)|^2))/N^2 on the OTF support, a = scale * pupil * exp(i phi)
+// placed with the pupil centred at (N/2, N/2) (pupilInOTF, pupil.cpp:261).
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, const float* __restrict__ phi,
+                                                  const int* __restrict__ pioY, const int* __restrict__ pioX,
+                                                  const float* __restrict__ pupil, const int* __restrict__ qImg,
+                                                  const float* __restrict__ imgScale,
+                                                  const int* __restrict__ osNatY, const int* __restrict__ osNatX,
+                                                  const cx<float>* __restrict__ tw, int nPS, int nOS) {
+    extern __shared__ cx<float> lds_dyn[];
+    cx<float>* t = lds_dyn;
+    const int q = blockIdx.x;
+    for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) t[e] = {0.f, 0.f};
+    __syncthreads();
+    const float sc = imgScale[qImg[q]];
+    const float* ph = phi + (size_t)q * nPS;
+    for (int i = threadIdx.x; i < nPS; i += NT) {
+        float s, c;
+        sincosf(ph[i], &s, &c);
+        const float m = pupil[i] * sc;
+        t[Tile<N>::at(pioY[i], pioX[i])] = {m * c, m * s};
+    }
+    __syncthreads();
+    tile_fft2<float, N, -1, NT>(t, tw, N / 4, N / 2);        // pupil rows [P/2, 3P/2)
+    for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) {
+        const cx<float> v = t[e];
+        t[e] = {v.x * v.x + v.y * v.y, 0.f};
+    }
+    __syncthreads();
+    tile_fft2<float, N, +1, NT>(t, tw, 0, N);
+    constexpr float inv = 1.0f / (N * N);
+    cx<float>* Sq = S + (size_t)q * nOS;
+    for (int u = threadIdx.x; u < nOS; u += NT) {
+        const cx<float> v = t[Tile<N>::at(osNatY[u], osNatX[u])];
+        Sq[u] = {v.x * inv, v.y * inv};
+    }
+}
+
+// Vogel / adjoint weights (subimage.cpp:348-390) for one image:
+//   gl = IFFT(G')/N^2 with G' = (PQ D - PS S)/QS de-centred, hj = IFFT(PF')/N^2,
+//   t = FFT(hj * Re gl), v_i = Im(conj(PF_i) t(pde_i)) [* pupil_i]
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const cx<float>* __restrict__ S,
+                                                    const cx<float>* __restrict__ D, const double* __restrict__ P,
+                                                    const double* __restrict__ Q, const float* __restrict__ phi,
+                                                    const int* __restrict__ pdeY, const int* __restrict__ pdeX,
+                                                    const float* __restrict__ pupil, const int* __restrict__ osNatY,
+                                                    const int* __restrict__ osNatX, QMap qm, const int* __restrict__ qReq,
+                                                    const int* __restrict__ imgObj, const int* __restrict__ gIdx,
+                                                    const cx<float>* __restrict__ tw,
+                                                    int nPS, int nOS, int nImg, int nObj, int extraPupil) {
+    extern __shared__ cx<float> lds_dyn[];
+    cx<float>* t = lds_dyn;
+    constexpr int PER = (N * N + NT - 1) / NT;
+    const int q = gIdx[blockIdx.x];
+    const int img = qm.img[q], r = qReq[q], o = imgObj[img];
+    for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) t[e] = {0.f, 0.f};
+    __syncthreads();
+    const cx<float>* Dq = D + ((size_t)qm.slot[q] * nImg + img) * nOS;
+    const cx<float>* Sq = S + ((size_t)r * nImg + img) * nOS;
+    const double* Po = P + 2 * ((size_t)r * nObj + o) * nOS;
+    const double* Qo = Q + ((size_t)r * nObj + o) * nOS;
+    for (int u = threadIdx.x; u < nOS; u += NT) {
+        const double pr = Po[2 * u], pi = Po[2 * u + 1], qq = Qo[u];
+        const double pqr = pr * qq, pqi = pi * qq, ps = pr * pr + pi * pi, qs = qq * qq;
+        const cx<float> d = Dq[u], s = Sq[u];
+        const double gr = (pqr * d.x - pqi * d.y) - ps * s.x;
+        const double gi = (pqr * d.y + pqi * d.x) - ps * s.y;
+        // de-centred position = natural index of the centred support point
+        t[Tile<N>::at(osNatY[u], osNatX[u])] = {(float)(gr / qs), (float)(gi / qs)};
+    }
+    __syncthreads();
+    tile_fft2<float, N, +1, NT>(t, tw, 0, N);
+    float greal[PER];
+    constexpr float inv = 1.0f / (N * N);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * NT;
+        greal[k] = (e < N * N) ? t[Tile<N>::at(e / N, e % N)].x * inv : 0.f;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) t[e] = {0.f, 0.f};
+    __syncthreads();
+    const float* ph = phi + (size_t)q * nPS;
+    for (int i = threadIdx.x; i < nPS; i += NT) {
+        float s, c;
+        sincosf(ph[i], &s, &c);
+        const float m = pupil[i];
+        t[Tile<N>::at(pdeY[i], pdeX[i])] = {m * c, m * s};
+    }
+    __syncthreads();
+    tile_fft2<float, N, +1, NT>(t, tw, 0, N);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int e = threadIdx.x + k * NT;
+        if (e < N * N) {
+            const int a = Tile<N>::at(e / N, e % N);
+            const float f = greal[k] * inv;
+            const cx<float> h = t[a];
+            t[a] = {h.x * f, h.y * f};
+        }
+    }
+    __syncthreads();
+    tile_fft2<float, N, -1, NT>(t, tw, 0, N);
+    float* vq = v + (size_t)blockIdx.x * nPS;
+    for (int i = threadIdx.x; i < nPS; i += NT) {
+        float s, c;
+        sincosf(ph[i], &s, &c);
+        const float pu = pupil[i];
+        const cx<float> tv = t[Tile<N>::at(pdeY[i], pdeX[i])];
+        const float im = (pu * c) * tv.y - (pu * s) * tv.x;
+        vq[i] = extraPupil ? im * pu : im;
+    }
+}
+
 // ------------------------------------------------------------- evaluator ----
 
 template <typename T>
@@ -559,6 +791,27 @@ void upload(DBuf<T>& b, const T* h, size_t n, hipStream_t st) {
 template <typename T>
 void upload(DBuf<T>& b, const std::vector<T>& h, hipStream_t st) { upload(b, h.data(), h.size(), st); }
 
+// Pinned host staging: asynchronous copies without a per-copy synchronisation.
+template <typename T>
+struct HBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (count <= n) return;
+        if (p) {
+            HIPCHK(hipDeviceSynchronize());
+            HIPCHK(hipHostFree(p));
+        }
+        HIPCHK(hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+    void free_() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 template <typename T, int N>
@@ -567,6 +820,10 @@ public:
     using C = cx<T>;
     static constexpr int N2 = N * N;
     static constexpr int LINES = sizeof(T) == 8 ? 16 : 32;   // lines per FFT block (32 KB of LDS)
+    // whole-tile LDS kernels: FP32 tiles up to 128 x 129 complex (132 KB)
+    static constexpr bool FUSED = std::is_same_v<T, float> && N <= 128;
+    static constexpr int FNT = N == 128 ? 1024 : 256;
+    static constexpr size_t FLDS = (size_t)Tile<N>::SIZE * sizeof(cx<float>);
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device) : g_(g), maxSlots_(maxSlots),
                                                                          maxBatch_(maxBatch) {
@@ -587,8 +844,10 @@ public:
         for (auto* b : {&Pp_, &Qp_, &part_, &L_, &g_d_, &alpha_, &ftSum_, &gamma_, &win_, &gscale_, &fdOut_, &w_})
             b->free_();
         for (auto* b : {&tiles_, &D_, &S_, &Sfd_}) b->free_();
-        for (auto* b : {&phi_, &phiFd_, &B_, &phiCh_, &pupil_, &scale_, &v_, &greal_, &fdStep_}) b->free_();
-        for (auto* b : {&ints_, &invPio_, &invPde_, &osNat_, &invOs_, &pde_, &chObj_, &chFirst_, &chCount_,
+        for (auto* b : {&phi_, &phiFd_, &B_, &phiCh_, &pupil_, &imgScale_, &v_, &greal_, &fdStep_}) b->free_();
+        for (auto* b : {&hStage_, &hAlpha_, &hL_, &hG_}) b->free_();
+        hInts_.free_();
+        for (auto* b : {&ints_, &fdInts_, &gIdx_, &pyx_, &osyx_, &invPio_, &invPde_, &osNat_, &invOs_, &pde_, &chObj_, &chFirst_, &chCount_,
                         &imgObj_, &imgCh_})
             b->free_();
         rawF_.free_();
@@ -601,6 +860,7 @@ public:
     }
 
     int capacity() const override { return maxSlots_; }
+    double* alpha_staging() override { return hStage_.p; }
     int nOS() const override { return nOS_; }
     const std::vector<int>& otf_support() const override { return os_; }
 
@@ -635,10 +895,31 @@ public:
         }
         upload(invPio_, invPio, st_);
         upload(invPde_, invPde, st_);
+        {
+            std::vector<int> yx(4 * (size_t)nPS_);
+            for (int k = 0; k < nPS_; ++k) {
+                yx[k] = pio[k] / N;
+                yx[nPS_ + k] = pio[k] % N;
+                yx[2 * nPS_ + k] = pde[k] / N;
+                yx[3 * nPS_ + k] = pde[k] % N;
+            }
+            upload(pyx_, yx, st_);
+        }
         upload(pde_, pde, st_);
         upload(pupil_, pup, st_);
         upload(osNat_, osNat, st_);
         upload(invOs_, invOs, st_);
+        {
+            std::vector<int> yx(2 * (size_t)nOS_);
+            for (int u = 0; u < nOS_; ++u) { yx[u] = osNat[u] / N; yx[nOS_ + u] = osNat[u] % N; }
+            upload(osyx_, yx, st_);
+        }
+        if constexpr (FUSED) {
+            HIPCHK(hipFuncSetAttribute((const void*)k_otf_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)FLDS));
+            HIPCHK(hipFuncSetAttribute((const void*)k_vogel_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)FLDS));
+        }
         area_ = 0;
         for (int k = 0; k < nPS_; ++k) area_ += pupil[ps_[k]];
         const int K = g_.K;
@@ -670,6 +951,13 @@ public:
         upload(imgObj_, s.imgObj, st_);
         upload(imgCh_, s.imgCh, st_);
         upload(w_, s.weight, st_);
+        {
+            std::vector<T> isc(g_.nImg);
+            for (int j = 0; j < g_.nImg; ++j) isc[j] = (T)otfNorm[s.imgCh[j]];
+            upload(imgScale_, isc, st_);
+        }
+        hStage_.alloc((size_t)maxSlots_ * g_.nImg * g_.K);
+        std::fill(hStage_.p, hStage_.p + hStage_.n, 0.0);
         const size_t nS = maxSlots_;
         D_.alloc(nS * g_.nImg * nOS_);
         ftSum_.alloc(nS * g_.nObj * nOS_);
@@ -883,11 +1171,19 @@ private:
     }
 
     // OTF chain for nQ virtual images whose phases are in `phi`: S = reorder(IFFT(|FFT(a)|^2))/N^2
-    void otf_chain(const T* phi, const T* scale, int nQ, C* Sout) {
+    void otf_chain(const T* phi, const int* qImg, int nQ, C* Sout) {
+        if constexpr (FUSED) {
+            launch("k_otf_fused", nQ, [&] {
+                hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(nQ), dim3(FNT), FLDS, st_, Sout, phi, pyx_.p,
+                                   pyx_.p + nPS_, pupil_.p, qImg, imgScale_.p, osyx_.p, osyx_.p + nOS_, tw_.p, nPS_,
+                                   nOS_);
+            });
+            return;
+        }
         tiles_.alloc((size_t)nQ * N2);
         launch("k_field", nQ, [&] {
             hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, phi, invPio_.p,
-                               pupil_.p, scale, nQ, N2, nPS_);
+                               pupil_.p, qImg, imgScale_.p, (const int*)nullptr, nQ, N2, nPS_);
         });
         fft2(tiles_.p, nQ, -1);
         launch("k_power", nQ, [&] {
@@ -900,27 +1196,44 @@ private:
         });
     }
 
-    // phases, OTFs, P/Q and the metric for n requests (virtual images q = r*nImg + j)
+    // phases, OTFs, P/Q and the metric for n requests (virtual images q = r*nImg + j).
+    // Alpha rows come from the slot-indexed pinned staging when every request is
+    // staged there (the solver's case), else they are packed per request.
     void forward(EvalReq* rq, int n, int useGamma) {
         const int K = g_.K, nImg = g_.nImg, nQ = n * nImg;
-        std::vector<double> ha((size_t)nQ * K);
-        std::vector<int> ints(4 * (size_t)nQ + n);
-        std::vector<T> sc(nQ);
+        const size_t row = (size_t)nImg * K;
+        bool staged = true;
+        int lo = maxSlots_, hi = -1;
         for (int r = 0; r < n; ++r) {
-            std::memcpy(&ha[(size_t)r * nImg * K], rq[r].alpha, sizeof(double) * nImg * K);
+            staged &= rq[r].alpha == hStage_.p + (size_t)rq[r].slot * row;
+            lo = std::min(lo, rq[r].slot);
+            hi = std::max(hi, rq[r].slot);
+        }
+        hInts_.alloc(4 * (size_t)nQ + n);
+        int* ints = hInts_.p;
+        for (int r = 0; r < n; ++r) {
             for (int j = 0; j < nImg; ++j) {
                 const int q = r * nImg + j;
-                ints[q] = rq[r].slot;                 // slot
-                ints[nQ + q] = j;                     // img
-                ints[2 * nQ + q] = q;                 // alpha row
-                ints[3 * nQ + q] = r;                 // request
-                sc[q] = (T)otfNorm_[st.imgCh[j]];
+                ints[q] = rq[r].slot;                                          // slot
+                ints[nQ + q] = j;                                              // image
+                ints[2 * nQ + q] = staged ? (rq[r].slot - lo) * nImg + j : q;  // alpha row
+                ints[3 * nQ + q] = r;                                          // request
             }
             ints[4 * nQ + r] = rq[r].slot;
         }
-        upload(alpha_, ha, st_);
-        upload(ints_, ints, st_);
-        upload(scale_, sc, st_);
+        if (staged) {
+            const size_t cnt = (size_t)(hi - lo + 1) * row;
+            alpha_.alloc(std::max(cnt, alpha_.n));
+            HIPCHK(hipMemcpyAsync(alpha_.p, hStage_.p + (size_t)lo * row, cnt * sizeof(double),
+                                  hipMemcpyHostToDevice, st_));
+        } else {
+            hAlpha_.alloc((size_t)nQ * K);
+            for (int r = 0; r < n; ++r) std::memcpy(hAlpha_.p + (size_t)r * row, rq[r].alpha, sizeof(double) * row);
+            alpha_.alloc((size_t)nQ * K);
+            HIPCHK(hipMemcpyAsync(alpha_.p, hAlpha_.p, (size_t)nQ * K * sizeof(double), hipMemcpyHostToDevice, st_));
+        }
+        ints_.alloc(4 * (size_t)nQ + n);
+        HIPCHK(hipMemcpyAsync(ints_.p, ints, (4 * (size_t)nQ + n) * sizeof(int), hipMemcpyHostToDevice, st_));
         qm_ = {ints_.p, ints_.p + nQ, ints_.p + 2 * nQ};
         qReq_ = ints_.p + 3 * nQ;
         reqSlot_ = ints_.p + 4 * nQ;
@@ -930,7 +1243,7 @@ private:
             hipLaunchKernelGGL(k_phase<T>, dim3(nblk(nPS_), (nQ + 15) / 16), dim3(256), 0, st_, phi_.p, B_.p, phiCh_.p,
                                alpha_.p, qm_, nQ, K, nPS_, imgObj_.p, imgCh_.p, g_.nCh);
         });
-        otf_chain(phi_.p, scale_.p, nQ, S_.p);
+        otf_chain(phi_.p, qm_.img, nQ, S_.p);
         const int nChunk = (nOS_ + 255) / 256;
         Pp_.alloc((size_t)n * g_.nObj * nOS_ * 2);
         Qp_.alloc((size_t)n * g_.nObj * nOS_);
@@ -951,8 +1264,6 @@ private:
             hipLaunchKernelGGL(k_metric, dim3(nblk(n, 64)), dim3(64), 0, st_, L_.p, part_.p, w_.p, n, g_.nObj, nChunk,
                                1.0 / ((double)g_.s * g_.s));
         });
-        std::vector<double> hL(n);
-        // gradients
         bool anyGrad = false;
         for (int r = 0; r < n; ++r) anyGrad |= rq[r].wantGrad;
         if (anyGrad) {
@@ -960,17 +1271,18 @@ private:
             HIPCHK(hipMemsetAsync(g_d_.p, 0, (size_t)nQ * K * sizeof(double), st_));
             gradients(rq, n, K, nImg);
         }
-        HIPCHK(hipMemcpyAsync(hL.data(), L_.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
-        std::vector<double> hg;
+        hL_.alloc(n);
+        HIPCHK(hipMemcpyAsync(hL_.p, L_.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
         if (anyGrad) {
-            hg.resize((size_t)nQ * K);
-            HIPCHK(hipMemcpyAsync(hg.data(), g_d_.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
+            hG_.alloc((size_t)nQ * K);
+            HIPCHK(hipMemcpyAsync(hG_.p, g_d_.p, (size_t)nQ * K * sizeof(double), hipMemcpyDeviceToHost, st_));
         }
         HIPCHK(hipStreamSynchronize(st_));
         collect();
         for (int r = 0; r < n; ++r) {
-            rq[r].f = hL[r];
-            if (rq[r].wantGrad && rq[r].g) std::memcpy(rq[r].g, &hg[(size_t)r * nImg * K], sizeof(double) * nImg * K);
+            rq[r].f = hL_.p[r];
+            if (rq[r].wantGrad && rq[r].g)
+                std::memcpy(rq[r].g, hG_.p + (size_t)r * nImg * K, sizeof(double) * nImg * K);
         }
     }
 
@@ -985,7 +1297,9 @@ private:
             else anyV = true;
         }
         if (anyV) {
-            std::vector<double> sc(nQ, 0.0);
+            // compact list of the virtual images whose requests want a Vogel/adjoint gradient
+            std::vector<int> gIdx;
+            std::vector<double> sc;
             int nEn = 0, extraPupil = 1;
             for (int r = 0; r < n; ++r) {
                 if (!rq[r].wantGrad || rq[r].gradMethod == 1) continue;
@@ -993,49 +1307,60 @@ private:
                 for (int j = 0; j < nImg; ++j) {
                     const int o = st.imgObj[j];
                     const double w = st.weight[o];
+                    gIdx.push_back(r * nImg + j);
                     if (rq[r].gradMethod == 2) {
                         // gradientVogel: -2 area/N^2 * w, times w again in Solver::gradient
-                        sc[r * nImg + j] = (w == 0.0) ? 0.0 : -2.0 * area_ / N2 * w * w;
+                        sc.push_back((w == 0.0) ? 0.0 : -2.0 * area_ / N2 * w * w);
                     } else {
                         // exact dL/dalpha = -(4 c^2 / s^2) w sum Im(conj(a) (G * a)), t = (G * PF)/N^2
                         const double c = otfNorm_[st.imgCh[j]];
-                        sc[r * nImg + j] = -4.0 * c * c * N2 / ((double)g_.s * g_.s) * w;
+                        sc.push_back(-4.0 * c * c * N2 / ((double)g_.s * g_.s) * w);
                         extraPupil = 0;
                     }
                 }
             }
+            const int nG = (int)gIdx.size();
             upload(gscale_, sc, st_);
-            tiles_.alloc((size_t)nQ * N2);
-            greal_.alloc((size_t)nQ * N2);
-            v_.alloc((size_t)nQ * nPS_);
-            launch("k_build_g", [&] {
-                hipLaunchKernelGGL(k_build_g<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, S_.p, D_.p,
-                                   Pp_.p, Qp_.p, invOs_.p, qm_, qReq_, imgObj_.p, nQ, N2, nOS_, nImg, g_.nObj);
-            });
-            fft2(tiles_.p, nQ, +1);
-            launch("k_save_re", [&] {
-                hipLaunchKernelGGL(k_save_re<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, greal_.p, tiles_.p,
-                                   (size_t)nQ * N2, (T)(1.0 / N2));
-            });
-            std::vector<T> ones(nQ, T(1));
-            upload(scale_, ones, st_);
-            launch("k_field", nQ, [&] {
-                hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, phi_.p,
-                                   invPde_.p, pupil_.p, scale_.p, nQ, N2, nPS_);
-            });
-            fft2(tiles_.p, nQ, +1);
-            launch("k_mul_re", [&] {
-                hipLaunchKernelGGL(k_mul_re<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, greal_.p,
-                                   (size_t)nQ * N2, (T)(1.0 / N2));
-            });
-            fft2(tiles_.p, nQ, -1);
-            launch("k_extract_v", [&] {
-                hipLaunchKernelGGL(k_extract_v<T>, dim3(nblk((size_t)nQ * nPS_)), dim3(256), 0, st_, v_.p, tiles_.p,
-                                   phi_.p, pde_.p, pupil_.p, nQ, nPS_, N2, extraPupil);
-            });
-            launch("k_grad", [&] {
-                hipLaunchKernelGGL(k_grad<T>, dim3(nQ), dim3(256), 0, st_, g_d_.p, v_.p, B_.p, gscale_.p, qm_,
-                                   imgObj_.p, K, nPS_, nEn);
+            upload(gIdx_, gIdx, st_);
+            v_.alloc((size_t)nG * nPS_);
+            if constexpr (FUSED) {
+                launch("k_vogel_fused", nG, [&] {
+                    hipLaunchKernelGGL((k_vogel_fused<N, FNT>), dim3(nG), dim3(FNT), FLDS, st_, v_.p, S_.p, D_.p, Pp_.p,
+                                       Qp_.p, phi_.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, osyx_.p,
+                                       osyx_.p + nOS_, qm_, qReq_, imgObj_.p, gIdx_.p, tw_.p, nPS_, nOS_, nImg,
+                                       g_.nObj, extraPupil);
+                });
+            } else {
+                tiles_.alloc((size_t)nG * N2);
+                greal_.alloc((size_t)nG * N2);
+                launch("k_build_g", nG, [&] {
+                    hipLaunchKernelGGL(k_build_g<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, tiles_.p, S_.p,
+                                       D_.p, Pp_.p, Qp_.p, invOs_.p, qm_, qReq_, imgObj_.p, gIdx_.p, nG, N2, nOS_, nImg,
+                                       g_.nObj);
+                });
+                fft2(tiles_.p, nG, +1);
+                launch("k_save_re", nG, [&] {
+                    hipLaunchKernelGGL(k_save_re<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, greal_.p,
+                                       tiles_.p, (size_t)nG * N2, (T)(1.0 / N2));
+                });
+                launch("k_field", nG, [&] {
+                    hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, tiles_.p, phi_.p,
+                                       invPde_.p, pupil_.p, qm_.img, (const T*)nullptr, gIdx_.p, nG, N2, nPS_);
+                });
+                fft2(tiles_.p, nG, +1);
+                launch("k_mul_re", nG, [&] {
+                    hipLaunchKernelGGL(k_mul_re<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, tiles_.p,
+                                       greal_.p, (size_t)nG * N2, (T)(1.0 / N2));
+                });
+                fft2(tiles_.p, nG, -1);
+                launch("k_extract_v", nG, [&] {
+                    hipLaunchKernelGGL(k_extract_v<T>, dim3(nblk((size_t)nG * nPS_)), dim3(256), 0, st_, v_.p,
+                                       tiles_.p, phi_.p, pde_.p, pupil_.p, gIdx_.p, nG, nPS_, N2, extraPupil);
+                });
+            }
+            launch("k_grad", nG, [&] {
+                hipLaunchKernelGGL(k_grad<T>, dim3(nG), dim3(256), 0, st_, g_d_.p, v_.p, B_.p, gscale_.p, qm_,
+                                   imgObj_.p, gIdx_.p, K, nPS_, nEn);
             });
         }
         if (anyFd) fd_gradient(rq, n, K, nImg);
@@ -1043,8 +1368,8 @@ private:
 
     void fd_gradient(EvalReq* rq, int n, int K, int nImg) {
         // probes (request r, image j, mode m < nEnabled): phi + step * B_m
-        std::vector<int> base, mode, obj;
-        std::vector<T> step, sc;
+        std::vector<int> base, mode, obj, pimg;
+        std::vector<T> step;
         std::vector<double> wgt;
         std::vector<std::pair<int, int>> dst;   // (alpha row, mode)
         for (int r = 0; r < n; ++r) {
@@ -1056,7 +1381,7 @@ private:
                     mode.push_back(m);
                     obj.push_back(o);
                     step.push_back((T)(st.graddiffStep * st.wavelength[o]));
-                    sc.push_back((T)otfNorm_[st.imgCh[j]]);
+                    pimg.push_back(j);
                     wgt.push_back(st.weight[o]);
                     dst.push_back({r * nImg + j, m});
                 }
@@ -1072,9 +1397,9 @@ private:
             ints.insert(ints.end(), base.begin() + b, base.begin() + b + m);
             ints.insert(ints.end(), mode.begin() + b, mode.begin() + b + m);
             ints.insert(ints.end(), obj.begin() + b, obj.begin() + b + m);
+            ints.insert(ints.end(), pimg.begin() + b, pimg.begin() + b + m);
             upload(fdInts_, ints, st_);
             upload(fdStep_, step.data() + b, m, st_);
-            upload(fdScale_, sc.data() + b, m, st_);
             phiFd_.alloc((size_t)m * nPS_);
             Sfd_.alloc((size_t)m * nOS_);
             fdOut_.alloc(m);
@@ -1082,7 +1407,7 @@ private:
                 hipLaunchKernelGGL(k_phase_pert<T>, dim3(nblk((size_t)m * nPS_)), dim3(256), 0, st_, phiFd_.p, phi_.p,
                                    B_.p, fdInts_.p, fdInts_.p + m, fdInts_.p + 2 * m, fdStep_.p, m, K, nPS_);
             });
-            otf_chain(phiFd_.p, fdScale_.p, m, Sfd_.p);
+            otf_chain(phiFd_.p, fdInts_.p + 3 * m, m, Sfd_.p);
             launch("k_metric_change", [&] {
                 hipLaunchKernelGGL(k_metric_change<T>, dim3(m), dim3(256), 0, st_, fdOut_.p, Sfd_.p, S_.p, D_.p, Pp_.p,
                                    Qp_.p, fdInts_.p, qm_, qReq_, imgObj_.p, nOS_, nImg, g_.nObj, 1.0 / N2);
@@ -1141,9 +1466,12 @@ private:
     const int* qReq_ = nullptr;
     const int* reqSlot_ = nullptr;
     DBuf<C> tw_, tiles_, D_, S_, Sfd_;
-    DBuf<T> phi_, phiFd_, B_, phiCh_, pupil_, scale_, v_, greal_, fdStep_, fdScale_;
+    DBuf<T> phi_, phiFd_, B_, phiCh_, pupil_, imgScale_, v_, greal_, fdStep_;
+    HBuf<double> hStage_, hAlpha_, hL_, hG_;
+    HBuf<int> hInts_;
     DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_;
-    DBuf<int> ints_, fdInts_, invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_;
+    DBuf<int> ints_, fdInts_, gIdx_, invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_,
+        pyx_, osyx_;
     DBuf<float> rawF_;
 };
 

@@ -14,12 +14,15 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <coroutine>
 #include <cstring>
 #include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -91,6 +94,82 @@ void parallel_for(int n, const std::function<void(int)>& f) {
     for (auto& t : th) t.join();
     if (exc) std::rethrow_exception(exc);
 }
+
+// Persistent worker pool: run(n, f) calls f(0..n-1) on the workers and the caller.
+class Pool {
+public:
+    explicit Pool(int nt) {
+        for (int t = 0; t < nt; ++t) th_.emplace_back([this] { loop(); });
+    }
+    ~Pool() {
+        {
+            std::lock_guard<std::mutex> l(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void run(int count, std::function<void(int)> fn) {
+        if (count <= 0) return;
+        if (th_.empty() || count == 1) {
+            for (int i = 0; i < count; ++i) fn(i);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> l(m_);
+            f_ = std::move(fn);
+            n_ = count;
+            next_ = 0;
+            pending_ = (int)th_.size();
+            exc_ = nullptr;
+            ++gen_;
+        }
+        cv_.notify_all();
+        work();
+        {
+            std::unique_lock<std::mutex> l(m_);
+            done_.wait(l, [&] { return pending_ == 0; });
+        }
+        if (exc_) std::rethrow_exception(exc_);
+    }
+
+private:
+    void loop() {
+        long seen = 0;
+        while (true) {
+            {
+                std::unique_lock<std::mutex> l(m_);
+                cv_.wait(l, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+            }
+            work();
+            {
+                std::lock_guard<std::mutex> l(m_);
+                if (--pending_ == 0) done_.notify_one();
+            }
+        }
+    }
+    void work() {
+        for (int i; (i = next_.fetch_add(1)) < n_;) {
+            try {
+                f_(i);
+            } catch (...) {
+                std::lock_guard<std::mutex> l(m_);
+                if (!exc_) exc_ = std::current_exception();
+            }
+        }
+    }
+    std::vector<std::thread> th_;
+    std::mutex m_;
+    std::condition_variable cv_, done_;
+    std::function<void(int)> f_;
+    int n_ = 0, pending_ = 0;
+    std::atomic<int> next_{0};
+    long gen_ = 0;
+    bool stop_ = false;
+    std::exception_ptr exc_;
+};
 
 // ----------------------------------------------- host statistics & filters ----
 double mean_rev(const double* d, long n) {       // ArrayStats::getMinMaxMean (arraystats.cpp:17-45)
@@ -243,6 +322,9 @@ struct mfbd_ctx {
     std::vector<std::vector<double>> noise;     // [patch][nImg]
     std::vector<std::vector<double>> gamma;     // [patch][nObj]
     bool initDone = false;
+    std::unique_ptr<Pool> pool;
+    // host-side timing of the batch driver: evaluate / resume / re-init phases
+    std::vector<KernelStat> hostStats = std::vector<KernelStat>(3);
 };
 
 namespace {
@@ -411,7 +493,7 @@ struct EvalSlot {
     bool pending = false;
     bool wantGrad = false;
     int method = 2, nEnabled = 0;
-    std::vector<double> alphaEff;
+    double* alphaEff = nullptr;        // this patch's row of the device's pinned alpha staging
     double f = 0;
     std::vector<double> gAlpha;
     std::coroutine_handle<> h;
@@ -461,7 +543,7 @@ void ns_apply(const mfbd_ctx* c, const double* in, double* out) {
 Task<double> my_f(PatchRun& r, const double* beta) {   // Solver::my_f (solver.cpp:219-225)
     ns_reverse_add(r.c, beta, r.alphaOffset.data(), r.alpha.data());
     r.ev.wantGrad = false;
-    effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff.data());
+    effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
     co_await EvalAwait{&r};
     r.nF++;
     co_return r.ev.f;
@@ -472,7 +554,7 @@ Task<int> my_df(PatchRun& r, const double* beta, double* gb) {   // Solver::my_d
     r.ev.wantGrad = true;
     r.ev.method = r.gradMethod;
     r.ev.nEnabled = r.nEnabled;
-    effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff.data());
+    effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
     co_await EvalAwait{&r};
     r.nDF++;
     ns_apply(r.c, r.ev.gAlpha.data(), gb);
@@ -988,7 +1070,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 r->p_.assign(nFree, 0.0);
                 r->prevG.assign(nFree, 0.0);
                 r->beta0.assign(nFree, 0.0);
-                r->ev.alphaEff.assign(nPar, 0.0);
+                r->ev.alphaEff = c->dev->alpha_staging() + (size_t)r->p * nPar;
                 r->ev.gAlpha.assign(nPar, 0.0);
                 c->shifts[r->p].assign(2 * nImg, 0);
                 c->tiltOff[r->p].assign(2 * nImg, 0.0);
@@ -1001,14 +1083,27 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             reset_init(c, ids);
             std::vector<Task<int>> tasks;
             for (auto& r : runs) tasks.push_back(run_patch(*r));
-            for (auto& t : tasks) t.h.resume();
+            if (!c->pool) c->pool = std::make_unique<Pool>(
+                (int)std::min<unsigned>(std::max(2u, std::thread::hardware_concurrency()), 16u) - 1);
+            c->pool->run((int)tasks.size(), [&](int k) { tasks[k].h.resume(); });
+            using clk = std::chrono::steady_clock;
+            auto since = [](clk::time_point t0) {
+                return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+            };
+            c->hostStats[0].name = "host_evaluate";
+            c->hostStats[1].name = "host_resume";
+            c->hostStats[2].name = "host_reinit";
             while (true) {
                 // batched re-initialisations after a stage
                 std::vector<PatchRun*> inits;
                 for (auto& r : runs)
                     if (r->wantShiftInit) inits.push_back(r.get());
                 if (!inits.empty()) {
+                    auto t0 = clk::now();
                     shift_inits(c, inits);
+                    c->hostStats[2].ms += since(t0);
+                    c->hostStats[2].launches++;
+                    c->hostStats[2].units += (long long)inits.size();
                     for (auto* r : inits) {
                         r->wantShiftInit = false;
                         auto h = r->initH;
@@ -1023,18 +1118,25 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 std::vector<EvalReq> rq(pend.size());
                 for (size_t k = 0; k < pend.size(); ++k) {
                     rq[k].slot = pend[k]->p;
-                    rq[k].alpha = pend[k]->ev.alphaEff.data();
+                    rq[k].alpha = pend[k]->ev.alphaEff;
                     rq[k].wantGrad = pend[k]->ev.wantGrad;
                     rq[k].gradMethod = pend[k]->ev.method;
                     rq[k].nEnabled = pend[k]->ev.nEnabled;
                     rq[k].g = pend[k]->ev.gAlpha.data();
                 }
+                auto t0 = clk::now();
                 c->dev->evaluate(rq);
+                c->hostStats[0].ms += since(t0);
+                c->hostStats[0].launches++;
+                c->hostStats[0].units += (long long)pend.size();
                 for (size_t k = 0; k < pend.size(); ++k) {
                     pend[k]->ev.f = rq[k].f;
                     pend[k]->ev.pending = false;
                 }
-                for (auto* r : pend) r->ev.h.resume();
+                t0 = clk::now();
+                c->pool->run((int)pend.size(), [&](int k) { pend[k]->ev.h.resume(); });
+                c->hostStats[1].ms += since(t0);
+                c->hostStats[1].launches++;
             }
             for (size_t k = 0; k < tasks.size(); ++k) {
                 if (!tasks[k].h.done()) throw std::runtime_error("patch state machine stalled");
@@ -1093,6 +1195,8 @@ int mfbd_debug_get(mfbd_ctx* c, const char* what, int p, double* out, int* count
 int mfbd_kernel_stats(mfbd_ctx* c, int cap, char (*names)[48], double* ms, int* launches, long long* units) {
     if (!c || !c->dev) return 0;
     auto st = c->dev->kernel_stats();
+    for (auto& h : c->hostStats)
+        if (h.launches) st.push_back(h);
     int n = 0;
     for (auto& s : st) {
         if (n >= cap) break;

@@ -920,6 +920,8 @@ public:
             HIPCHK(hipFuncSetAttribute((const void*)k_vogel_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)FLDS));
         }
+        HIPCHK(hipFuncSetAttribute((const void*)k_noise, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)((128 * 128 + 6 * 128) * sizeof(double))));
         area_ = 0;
         for (int k = 0; k < nPS_; ++k) area_ += pupil[ps_[k]];
         const int K = g_.K;
@@ -982,6 +984,36 @@ public:
         HIPCHK(hipMemcpyAsync(phiCh_.p + (size_t)slot * g_.nCh * nPS_, h.data(), h.size() * sizeof(T),
                               hipMemcpyHostToDevice, st_));
         HIPCHK(hipStreamSynchronize(st_));
+    }
+
+    void frame_stats(const std::vector<ImgInit>& imgs, const std::vector<double>& window,
+                     const std::vector<double>& noiseWindow, double* noise, double* stddev) override {
+        const int nQ = (int)imgs.size();
+        if (!nQ) return;
+        const int s = g_.s;
+        if (s > 128) throw std::runtime_error("frame_stats: patch side above 128");
+        std::vector<double> wins(window);
+        wins.insert(wins.end(), noiseWindow.begin(), noiseWindow.end());
+        upload(win_, wins, st_);
+        std::vector<int> ints(4 * (size_t)nQ);
+        for (int q = 0; q < nQ; ++q) {
+            ints[q] = imgs[q].slot;
+            ints[nQ + q] = imgs[q].img;
+            ints[2 * nQ + q] = imgs[q].shiftX;
+            ints[3 * nQ + q] = imgs[q].shiftY;
+        }
+        upload(ints_, ints, st_);
+        statOut_.alloc(2 * (size_t)nQ);
+        const size_t lds = ((size_t)s * s + 6 * (size_t)s) * sizeof(double);
+        launch("k_noise", nQ, [&] {
+            hipLaunchKernelGGL(k_noise, dim3(nQ), dim3(256), lds, st_, statOut_.p, statOut_.p + nQ, rawF_.p, win_.p,
+                               win_.p + (size_t)s * s, ints_.p, ints_.p + nQ, ints_.p + 2 * nQ, ints_.p + 3 * nQ,
+                               g_.nImg, g_.dataY, g_.dataX, g_.startY, g_.startX, s);
+        });
+        HIPCHK(hipMemcpyAsync(noise, statOut_.p, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipMemcpyAsync(stddev, statOut_.p + nQ, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+        collect();
     }
 
     void set_gamma(int slot, const double* gamma) override {
@@ -1058,15 +1090,21 @@ public:
         }
     }
 
-    void restore_sums(int slot, const double* alpha, double* P0, double* Q0) override {
-        EvalReq r;
-        r.slot = slot;
-        r.alpha = alpha;
-        forward(&r, 1, /*useGamma=*/0);
-        const size_t n = (size_t)g_.nObj * nOS_;
-        HIPCHK(hipMemcpyAsync(P0, Pp_.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIPCHK(hipMemcpyAsync(Q0, Qp_.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIPCHK(hipStreamSynchronize(st_));
+    void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) override {
+        const size_t per = (size_t)g_.nObj * nOS_;
+        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
+        std::vector<EvalReq> rq(n);
+        for (int k = 0; k < n; ++k) {
+            rq[k].slot = slots[k];
+            rq[k].alpha = alpha[k];
+        }
+        for (int b = 0; b < n; b += perChunk) {
+            const int m = std::min(perChunk, n - b);
+            forward(&rq[b], m, /*useGamma=*/0);
+            HIPCHK(hipMemcpyAsync(P0 + 2 * b * per, Pp_.p, 2 * m * per * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipMemcpyAsync(Q0 + b * per, Qp_.p, m * per * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+        }
         collect();
     }
 
@@ -1469,7 +1507,7 @@ private:
     DBuf<T> phi_, phiFd_, B_, phiCh_, pupil_, imgScale_, v_, greal_, fdStep_;
     HBuf<double> hStage_, hAlpha_, hL_, hG_;
     HBuf<int> hInts_;
-    DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_;
+    DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_, statOut_;
     DBuf<int> ints_, fdInts_, gIdx_, invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_,
         pyx_, osyx_;
     DBuf<float> rawF_;

@@ -324,7 +324,7 @@ struct mfbd_ctx {
     bool initDone = false;
     std::unique_ptr<Pool> pool;
     // host-side timing of the batch driver: evaluate / resume / re-init phases
-    std::vector<KernelStat> hostStats = std::vector<KernelStat>(3);
+    std::vector<KernelStat> hostStats = std::vector<KernelStat>(5);
 };
 
 namespace {
@@ -395,11 +395,18 @@ void frame_noise(const mfbd_ctx* c, int p, int j, int sx, int sy, double& noise,
 void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
     const int nImg = c->g.nImg, nObj = c->g.nObj;
     std::vector<double> noiseAll((size_t)patches.size() * nImg), sdAll((size_t)patches.size() * nImg);
-    parallel_for((int)(patches.size() * nImg), [&](int k) {
-        const int p = patches[k / nImg], j = k % nImg;
-        frame_noise(c, p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1], noiseAll[k], sdAll[k]);
-    });
     std::vector<ImgInit> imgs;
+    for (int p : patches)
+        for (int j = 0; j < nImg; ++j) imgs.push_back({p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1]});
+    if (c->g.s <= 128) {
+        c->dev->frame_stats(imgs, c->window, c->noiseWindow, noiseAll.data(), sdAll.data());
+        for (size_t k = 0; k < noiseAll.size(); ++k) noiseAll[k] *= c->noiseFudge[c->st.imgCh[k % nImg]];
+    } else {
+        parallel_for((int)(patches.size() * nImg), [&](int k) {
+            const int p = patches[k / nImg], j = k % nImg;
+            frame_noise(c, p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1], noiseAll[k], sdAll[k]);
+        });
+    }
     for (size_t a = 0; a < patches.size(); ++a) {
         const int p = patches[a];
         c->noise[p].assign(noiseAll.begin() + a * nImg, noiseAll.begin() + (a + 1) * nImg);
@@ -410,7 +417,6 @@ void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
         }
         c->gamma[p] = gam;
         c->dev->set_gamma(p, gam.data());
-        for (int j = 0; j < nImg; ++j) imgs.push_back({p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1]});
     }
     c->dev->init_images(imgs, c->window);
 }
@@ -1030,16 +1036,20 @@ int mfbd_restore(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, f
     try {
         ensure_init(c);
         const size_t per = (size_t)c->g.nImg * c->g.K;
-        const int nOS = c->dev->nOS();
+        const size_t po = (size_t)c->g.nObj * c->dev->nOS();
+        for (int k = 0; k < n; ++k)
+            if (patchIdx[k] < 0 || patchIdx[k] >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
+        std::vector<double> ae(per * n), P0(2 * po * n), Q0(po * n);
+        std::vector<const double*> ap(n);
         for (int k = 0; k < n; ++k) {
-            const int p = patchIdx[k];
-            if (p < 0 || p >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
-            std::vector<double> ae(per), P0(2 * (size_t)c->g.nObj * nOS), Q0((size_t)c->g.nObj * nOS);
-            effective_alpha(c, p, alpha + per * k, ae.data());
-            c->dev->restore_sums(p, ae.data(), P0.data(), Q0.data());
-            finish_restore(c, p, c->dev->otf_support(), P0.data(), Q0.data(),
-                           img + (size_t)k * c->g.nObj * c->g.s * c->g.s);
+            effective_alpha(c, patchIdx[k], alpha + per * k, &ae[per * k]);
+            ap[k] = &ae[per * k];
         }
+        c->dev->restore_sums(n, patchIdx, ap.data(), P0.data(), Q0.data());
+        parallel_for(n, [&](int k) {
+            finish_restore(c, patchIdx[k], c->dev->otf_support(), &P0[2 * po * k], &Q0[po * k],
+                           img + (size_t)k * c->g.nObj * c->g.s * c->g.s);
+        });
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
     }
@@ -1080,12 +1090,6 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             // shiftAndInit(alpha_offset, true): shifts from the warm start, then reset init
             for (auto& r : runs)
                 for (int j = 0; j < nImg; ++j) adjust_shift(c, r->p, j, &r->alphaOffset[(size_t)j * K]);
-            reset_init(c, ids);
-            std::vector<Task<int>> tasks;
-            for (auto& r : runs) tasks.push_back(run_patch(*r));
-            if (!c->pool) c->pool = std::make_unique<Pool>(
-                (int)std::min<unsigned>(std::max(2u, std::thread::hardware_concurrency()), 16u) - 1);
-            c->pool->run((int)tasks.size(), [&](int k) { tasks[k].h.resume(); });
             using clk = std::chrono::steady_clock;
             auto since = [](clk::time_point t0) {
                 return std::chrono::duration<double, std::milli>(clk::now() - t0).count();
@@ -1093,6 +1097,20 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             c->hostStats[0].name = "host_evaluate";
             c->hostStats[1].name = "host_resume";
             c->hostStats[2].name = "host_reinit";
+            c->hostStats[3].name = "host_reset_init";
+            c->hostStats[4].name = "host_restore";
+            {
+                auto t0 = clk::now();
+                reset_init(c, ids);
+                c->hostStats[3].ms += since(t0);
+                c->hostStats[3].launches++;
+                c->hostStats[3].units += nb;
+            }
+            std::vector<Task<int>> tasks;
+            for (auto& r : runs) tasks.push_back(run_patch(*r));
+            if (!c->pool) c->pool = std::make_unique<Pool>(
+                (int)std::min<unsigned>(std::max(2u, std::thread::hardware_concurrency()), 16u) - 1);
+            c->pool->run((int)tasks.size(), [&](int k) { tasks[k].h.resume(); });
             while (true) {
                 // batched re-initialisations after a stage
                 std::vector<PatchRun*> inits;
@@ -1143,29 +1161,32 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 if (tasks[k].h.promise().exc) std::rethrow_exception(tasks[k].h.promise().exc);
             }
             // results + restoration (Solver::run tail, solver.cpp:549-566)
-            const int nOS = c->dev->nOS();
-            std::vector<std::vector<double>> P0(nb), Q0(nb);
+            const size_t per = (size_t)c->g.nObj * c->dev->nOS();
             for (int k = 0; k < nb; ++k) {
                 auto& r = *runs[k];
                 r.res.nF = r.nF;
                 r.res.nDF = r.nDF;
                 if (results) results[r.p] = r.res;
                 if (alphaOut) std::copy(r.alpha.begin(), r.alpha.end(), alphaOut + (size_t)r.p * nPar);
-                if (imgOut) {
-                    std::vector<double> ae(nPar);
-                    effective_alpha(c, r.p, r.alpha.data(), ae.data());
-                    P0[k].resize(2 * (size_t)c->g.nObj * nOS);
-                    Q0[k].resize((size_t)c->g.nObj * nOS);
-                    c->dev->restore_sums(r.p, ae.data(), P0[k].data(), Q0[k].data());
-                }
             }
             if (imgOut) {
+                auto t0 = clk::now();
+                std::vector<double> ae((size_t)nb * nPar), P0(2 * nb * per), Q0(nb * per);
+                std::vector<const double*> ap(nb);
+                for (int k = 0; k < nb; ++k) {
+                    effective_alpha(c, runs[k]->p, runs[k]->alpha.data(), &ae[(size_t)k * nPar]);
+                    ap[k] = &ae[(size_t)k * nPar];
+                }
+                c->dev->restore_sums(nb, ids.data(), ap.data(), P0.data(), Q0.data());
                 const auto& os = c->dev->otf_support();
                 parallel_for(nb, [&](int k) {
                     const int p = runs[k]->p;
-                    finish_restore(c, p, os, P0[k].data(), Q0[k].data(),
+                    finish_restore(c, p, os, &P0[2 * k * per], &Q0[k * per],
                                    imgOut + (size_t)p * c->g.nObj * c->g.s * c->g.s);
                 });
+                c->hostStats[4].ms += since(t0);
+                c->hostStats[4].launches++;
+                c->hostStats[4].units += nb;
             }
         }
         c->initDone = false;    // shifts moved during the solve

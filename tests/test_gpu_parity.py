@@ -31,7 +31,8 @@ def _solver(pb, prec="fp64", batch=1):
 def test_metric_and_vogel_gradient(golden, prec, tol_m, tol_g):
     pb, z = golden
     sv = _solver(pb, prec)
-    assert abs(sv.debug("gamma_host")[0] - z["gamma"][0]) < 1e-15
+    # gamma comes from the frame noise (a direct DFT on the GPU, FFTW in the oracle)
+    assert abs(sv.debug("gamma_host")[0] - z["gamma"][0]) <= 1e-12 * z["gamma"][0]
     for k, a in enumerate(z["alphas"]):
         L = sv.metric(a[None])[0]
         assert abs(L - z["metric"][k]) <= tol_m * abs(z["metric"][k]), (prec, k, L, z["metric"][k])

@@ -1,0 +1,95 @@
+"""HIP path against the FP64 oracle on one patch of the benchmark geometry:
+s = 88 (not a power of two), N = 128 OTF grid with a 64-pixel pupil, two objects
+(the second with a phase diversity) of 24 frames each, 50 KL modes.  This is the
+shape the batched benchmark runs, so it exercises the N = 128 fused FP32 kernels
+and the direct-DFT frame noise estimate.
+
+Tolerances as for the golden problem: FP64 metric 1e-12 relative, gradients 1e-9;
+FP32 metric 1e-6, gradients 1e-4 (relative 2-norm); frame noise and regularisation
+gamma 1e-12 relative (FP64 in both precisions).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def patch():
+    from redux_amd import synth
+    from oracle import setup_np
+    return synth.build_problem(synth.CFG2_PATCH, seed=11, setup=setup_np)
+
+
+@pytest.fixture(scope="module")
+def ref(patch, oracle_lib):
+    st = oracle_lib.State(patch)
+    at = patch["truth"]["alpha"]
+    rng = np.random.default_rng(3)
+    alphas = [np.zeros_like(at), 0.5 * at, at * rng.standard_normal(at.shape) * 0.3]
+    return dict(state=st, alphas=alphas, metric=[st.metric(a) for a in alphas],
+                vogel=[st.grad(a, method="vogel") for a in alphas])
+
+
+def _solver(pb, prec):
+    from redux_amd.momfbd import Solver
+    sv = Solver(pb, precision=prec, max_batch=1)
+    sv.load([pb])
+    sv.init()
+    return sv
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_frame_noise_and_gamma(patch, ref, prec):
+    sv = _solver(patch, prec)
+    st = ref["state"]
+    nImg = len(patch["imgCh"])
+    noise = sv.debug("noise_host")
+    want = np.array([st.scalar("noise", i) for i in range(nImg)])
+    assert np.abs(noise - want).max() <= 1e-12 * np.abs(want).max(), np.abs(noise - want).max()
+    gam = sv.debug("gamma_host")
+    for o in range(len(patch["objects"])):
+        assert abs(gam[o] - st.scalar("gamma", o)) <= 1e-12 * abs(st.scalar("gamma", o))
+    sv.close()
+
+
+@pytest.mark.parametrize("prec,tol_m,tol_g", [("fp64", 1e-12, 1e-9), ("fp32", 1e-6, 1e-4)])
+def test_metric_and_vogel_gradient(patch, ref, prec, tol_m, tol_g):
+    sv = _solver(patch, prec)
+    for k, a in enumerate(ref["alphas"]):
+        L = sv.metric(a[None])[0]
+        assert abs(L - ref["metric"][k]) <= tol_m * abs(ref["metric"][k]), (prec, k, L, ref["metric"][k])
+        g = sv.grad(a[None], method="vogel")[0]
+        err = np.linalg.norm(g - ref["vogel"][k]) / np.linalg.norm(ref["vogel"][k])
+        assert err <= tol_g, (prec, k, err)
+    sv.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-6), ("fp32", 1e-3)])
+def test_adjoint_gradient(patch, ref, prec, tol):
+    """dL/dalpha from the adjoint kernels against central differences of the
+    oracle's FP64 metric (h = 1e-4 lambda)."""
+    sv = _solver(patch, prec)
+    st = ref["state"]
+    a = ref["alphas"][2]
+    g = sv.grad(a[None], method="adjoint")[0]
+    h = 1e-4 * patch["objects"][0]["wavelength"]
+    for i, m in [(0, 0), (5, 1), (23, 9), (24, 4), (40, 30), (47, 49)]:
+        ap, am = a.copy(), a.copy()
+        ap[i, m] += h
+        am[i, m] -= h
+        gc = (st.metric(ap) - st.metric(am)) / (2 * h)
+        assert abs(g[i, m] - gc) <= tol * np.abs(g).max(), (i, m, g[i, m], gc)
+    sv.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-6), ("fp32", 1e-3)])
+def test_restore(patch, ref, prec, tol):
+    st = ref["state"]
+    sv = _solver(patch, prec)
+    for a in ref["alphas"][1:]:
+        img = sv.restore(a[None])[0]
+        want = st.restore(a)
+        err = np.linalg.norm(img - want) / np.linalg.norm(want)
+        assert err < tol, (prec, err)
+    sv.close()

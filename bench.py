@@ -70,53 +70,39 @@ def make_batch(n, seed, workers):
     return su, pbs
 
 
-def solve(pbs, precision, batch, gradient, restore=True):
+def solve(pbs, precision, batch, gradient, restore=True, timing=False):
+    """One step: load the patches, solve all of them, restore the objects.  With
+    `timing`, every kernel launch is bracketed by HIP events on the pipeline's stream
+    and the per-kernel totals are returned as well."""
     from redux_amd.momfbd import Solver
-    sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient)
-    sv.load(pbs)
+    if timing:
+        os.environ["MFBD_KERNEL_TIMING"] = "1"
+    try:
+        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient)
+        sv.load(pbs)
+    finally:
+        os.environ.pop("MFBD_KERNEL_TIMING", None)
     t0 = time.perf_counter()
     res = sv.run(restore=restore)
     dt = time.perf_counter() - t0
+    stats = sv.kernel_stats() if timing else None
     sv.close()
-    return res, dt
-
-
-def kernel_profile(pbs, alphas, gradient):
-    """Per-kernel HIP-event timing (on the pipeline's stream) over a fixed set of
-    batched evaluations at the converged alphas: 8 metric passes and 2 gradient passes."""
-    from redux_amd.momfbd import Solver
-    os.environ["MFBD_KERNEL_TIMING"] = "1"
-    try:
-        sv = Solver(pbs[0], precision="fp32", max_batch=len(pbs), gradient=gradient)
-        sv.load(pbs)
-        sv.init()
-        a = np.stack(alphas)
-        for _ in range(8):
-            sv.metric(a)
-        for _ in range(2):
-            sv.grad(a, method="adjoint" if gradient == "adjoint" else "vogel")
-        st = sv.kernel_stats()
-        sv.close()
-    finally:
-        os.environ.pop("MFBD_KERNEL_TIMING", None)
-    return st
+    return res, dt, stats
 
 
 def roofline(stats, N, nPS, nOS):
-    """Dominant kernel by total time; achieved = algorithmic bytes / mean launch time.
-    Algorithmic bytes per image (FP32): an FFT pass reads and writes the N x N complex
-    tile (2 * 8 N^2); k_field reads phi on the pupil support and writes the tile
-    (4 nPS + 8 N^2); k_extract reads the support (8 nOS) and writes S (8 nOS);
-    k_phase writes phi (4 nPS) and reads alpha (K doubles, negligible); k_pq reads S
-    and D (16 nOS) per image."""
+    """Dominant kernel by total time; achieved = algorithmic bytes per launch / mean
+    launch time, over every launch of the timed solves.  Algorithmic bytes per image
+    (FP32): k_otf_fused reads phi on the pupil support and writes S on the OTF support
+    (4 nPS + 8 nOS); k_pq reads S and D (16 nOS); k_phase writes phi (4 nPS); the fused
+    Vogel kernel reads S, D (8 nOS each), P, Q (24 nOS, FP64) and phi (4 nPS) and
+    writes v (4 nPS); an unfused FFT pass reads and writes the N x N tile (16 N^2)."""
     per_unit = {"fft_rows": 16 * N * N, "fft_cols": 16 * N * N, "k_field": 4 * nPS + 8 * N * N,
                 "k_power": 16 * N * N, "k_extract": 16 * nOS, "k_phase": 4 * nPS, "k_pq": 16 * nOS,
-                # fused: read phi on the pupil support, write S on the OTF support
                 "k_otf_fused": 4 * nPS + 8 * nOS,
-                # fused Vogel: read S, D, P, Q on the support and phi; write v
                 "k_vogel_fused": 16 * nOS + 24 * nOS + 8 * nPS}
-    dom = max(stats.items(), key=lambda kv: kv[1]["ms"])
-    name, s = dom
+    kern = {k: v for k, v in stats.items() if not k.startswith("host")}
+    name, s = max(kern.items(), key=lambda kv: kv[1]["ms"])
     out = dict(kernel=name)
     if name in per_unit and s["units"] > 0:
         mean_ms = s["ms"] / s["launches"]
@@ -124,8 +110,9 @@ def roofline(stats, N, nPS, nOS):
         bytes_per_launch = per_unit[name] * units_per_launch
         ach = bytes_per_launch / (mean_ms * 1e-3) / 1e9
         out.update(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                   frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, mean_launch_ms=round(mean_ms, 4),
-                   bytes_per_launch=int(bytes_per_launch))
+                   frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, launches=s["launches"],
+                   mean_launch_ms=round(mean_ms, 4), images_per_launch=round(units_per_launch, 1),
+                   bytes_per_image=per_unit[name], bytes_per_launch=int(bytes_per_launch))
     return out
 
 
@@ -196,24 +183,28 @@ def main():
     log(f"generated {len(pbs)} patches in {time.perf_counter() - t0:.1f}s")
 
     # FP64 batched solve: the reference objects (identical to the plain drop-in, batch-invariant)
-    ref, t64b = solve(pbs, "fp64", len(pbs), args.gradient)
+    ref, t64b, _ = solve(pbs, "fp64", len(pbs), args.gradient)
     log(f"fp64 batched: {t64b:.2f}s")
     ev_f = float(np.mean([r["nF"] for r in ref]))
     ev_g = float(np.mean([r["nDF"] for r in ref]))
 
     # plain FP64 drop-in: one patch at a time
     plain = pbs[: args.plain_patches]
-    _, t64p = solve(plain, "fp64", 1, args.gradient)
+    _, t64p, _ = solve(plain, "fp64", 1, args.gradient)
     plain_rate = len(plain) / t64p
     log(f"fp64 plain: {len(plain)} patches in {t64p:.2f}s")
 
     # warmup + timed batched FP32 steps
     for _ in range(args.warmup):
         solve(pbs, "fp32", len(pbs), args.gradient)
-    times, res32 = [], None
+    times, res32, stats = [], None, {}
     for _ in range(args.steps):
-        res32, dt = solve(pbs, "fp32", len(pbs), args.gradient)
+        res32, dt, st = solve(pbs, "fp32", len(pbs), args.gradient, timing=True)
         times.append(dt)
+        for k, v in st.items():
+            acc = stats.setdefault(k, dict(ms=0.0, launches=0, units=0))
+            for f in acc:
+                acc[f] += v[f]
         log(f"fp32 batched step: {dt:.2f}s")
     t32 = max(times)
     errs = []
@@ -224,8 +215,6 @@ def main():
     conv = int(np.sum(errs <= 1e-4))
     value = conv / t32
 
-    stats = kernel_profile(pbs, [r["alpha"] for r in res32], args.gradient)
-    from redux_amd import synth  # noqa: F401
     nPS = len(su["pup"]["ps"])
     nOS = len(su["pup"]["os"])
     roof = roofline(stats, su["N"], nPS, nOS)
@@ -254,7 +243,8 @@ def main():
         "fp64_plain_patches_per_s": round(plain_rate, 4),
         "speedup_fp32_batched_vs_fp64_plain": round((len(pbs) / t32) / plain_rate, 2),
         "evals_per_patch": {"f": ev_f, "grad": ev_g},
-        "kernels_ms": {k: round(v["ms"], 3) for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])},
+        "kernels_ms_per_step": {k: round(v["ms"] / args.steps, 3)
+                                for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

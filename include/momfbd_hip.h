@@ -131,7 +131,9 @@ int mfbd_eval_grad(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alph
 /* Restored objects at alpha: Object::restorePatch.  img [n][nObj][s][s]. */
 int mfbd_restore(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, float* img);
 /* Diagnostics: copy a device array for patch p ("S" OTF on support, "D", "P", "Q",
- * "ftsum", "phi") in double; returns the element count via *count. */
+ * "ftsum", "phi") or host state ("gamma_host", "noise_host", "otfnorm", "window",
+ * "shifts" [nImg][2] integer cut-out shifts (x, y), "tiltoff" [nImg][2]) in double;
+ * returns the element count via *count. */
 int mfbd_debug_get(mfbd_ctx* ctx, const char* what, int p, double* out, int* count);
 /* Per-kernel timing (environment MFBD_KERNEL_TIMING set when the patches were
  * loaded): name[i], summed milliseconds, launches and images processed, accumulated

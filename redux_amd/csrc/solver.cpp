@@ -1204,6 +1204,8 @@ int mfbd_debug_get(mfbd_ctx* c, const char* what, int p, double* out, int* count
         else if (!strcmp(what, "noise_host")) v = c->noise.at(p);
         else if (!strcmp(what, "otfnorm")) v = c->otfNorm;
         else if (!strcmp(what, "window")) v = c->window;
+        else if (!strcmp(what, "shifts")) v.assign(c->shifts.at(p).begin(), c->shifts.at(p).end());
+        else if (!strcmp(what, "tiltoff")) v = c->tiltOff.at(p);
         else if (c->dev) c->dev->debug_get(what, p, v);
         *count = (int)v.size();
         if (out) std::copy(v.begin(), v.end(), out);

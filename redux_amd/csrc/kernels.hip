@@ -1,7 +1,7 @@
 // Device pipeline of the MOMFBD patch solver for gfx950 (MI355X).
 //
 // Per evaluation of a batch of requests (patch slot + alpha):
-//   k_phase        phi = phi_channel + B alpha           (mode expansion, subimage.cpp:509-544)
+//   k_phase_mfma   phi = phi_channel + B alpha           (mode expansion, subimage.cpp:509-544)
 //   k_field        a = otfNorm * pupil * exp(i phi) on the OTF grid  (subimage.cpp:586-612)
 //   FFT2 / |.|^2 / IFFT2 / k_extract   S = reorder(IFFT(|FFT(a)|^2)/N^2)  (fouriertransform.hpp:280-286)
 //   k_pq           P = sum conj(D) S, Q = gamma + sum |S|^2 and metric partials
@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -201,39 +202,103 @@ struct QMap {             // virtual image q -> source
     const int* arow;      // alpha row
 };
 
-// phi[q][i] = phiCh[slot][ch][i] + sum_m alpha[arow][m] * B[obj][m][i]
-// Block: 16 images x 256 pixels; alpha rows staged in LDS.
+// Mode expansion as a GEMM on the matrix cores: phi = phiCh + A B with A = alpha rows
+// (FP64 in memory, converted on load) and B = modes [K][nPS] of the image's object,
+// K stepped by 4 through v_mfma_{f32,f64}_16x16x4.  Every wave is independent (no LDS,
+// no barrier): it owns 16 images x 64 pupil points (4 accumulator tiles), issues all
+// alpha and mode fragment loads of a chunk of k-steps at once and then runs the MFMAs.
+// Lane layout of the 16x16x4 MFMAs (measured on gfx950): A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k; C[i][j] is in lane j + 16(i/4), register i%4 for f32, and
+// in lane j + 16(i%4), register i/4 for f64.  Rows of different objects in one wave
+// take one pass per object with the other rows' A zeroed, so every element is the same
+// k-ordered sum whatever shares its tile.
 template <typename T>
-__global__ void __launch_bounds__(256) k_phase(T* __restrict__ phi, const T* __restrict__ B, const T* __restrict__ phiCh,
-                                               const double* __restrict__ alpha, QMap qm, int nQ, int K, int nPS,
-                                               const int* __restrict__ imgObj, const int* __restrict__ imgCh, int nCh) {
-    constexpr int IMG = 16;
-    __shared__ T sa[IMG][160];
-    const int q0 = blockIdx.y * IMG;
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    for (int e = threadIdx.x; e < IMG * K; e += 256) {
-        const int j = e / K, m = e % K;
-        const int q = q0 + j;
-        sa[j][m] = (q < nQ) ? (T)alpha[(size_t)qm.arow[q] * K + m] : T(0);
-    }
-    __syncthreads();
-    if (i >= nPS) return;
-    T acc[IMG];
-    const T* bq[IMG];
+struct MfmaVec;
+template <>
+struct MfmaVec<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 mac(float a, float b, v4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lk, int v) { return 4 * lk + v; }     // accumulator row of (lane/16, register)
+};
+template <>
+struct MfmaVec<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    __device__ static v4 mac(double a, double b, v4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+    __device__ static int row(int lk, int v) { return lk + 4 * v; }
+};
+
+template <typename T>
+struct PhaseChunk { static constexpr int value = 13; };        // k-steps whose fragments are loaded together
+template <>
+struct PhaseChunk<double> { static constexpr int value = 5; };
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T* __restrict__ B,
+                                                    const T* __restrict__ phiCh, const double* __restrict__ alpha,
+                                                    QMap qm, int nQ, int K, int nPS,
+                                                    const int* __restrict__ imgObj, const int* __restrict__ imgCh,
+                                                    int nCh) {
+    using V = typename MfmaVec<T>::v4;
+    constexpr int KS = PhaseChunk<T>::value;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int q0 = blockIdx.y * 64 + 16 * w, c0 = blockIdx.x * 64;
+    if (q0 >= nQ) return;                                    // whole wave exits together
+    // this lane's A row and the 4 output rows it holds
+    const int qa = min(q0 + li, nQ - 1);
+    const int imgA = qm.img[qa];
+    const int objA = (q0 + li < nQ) ? imgObj[imgA] : INT_MAX;
+    const double* arow = alpha + (size_t)qm.arow[qa] * K;
+    V acc[4];
 #pragma unroll
-    for (int j = 0; j < IMG; ++j) {
-        const int q = min(q0 + j, nQ - 1);
+    for (int v = 0; v < 4; ++v) {
+        const int q = min(q0 + MfmaVec<T>::row(lk, v), nQ - 1);
         const int img = qm.img[q];
-        acc[j] = phiCh[((size_t)qm.slot[q] * nCh + imgCh[img]) * nPS + i];
-        bq[j] = B + (size_t)imgObj[img] * K * nPS + i;
-    }
-    for (int m = 0; m < K; ++m) {
+        const T* pc = phiCh + (size_t)(qm.slot[q] * nCh + imgCh[img]) * nPS;
 #pragma unroll
-        for (int j = 0; j < IMG; ++j) acc[j] = acc[j] + sa[j][m] * bq[j][(size_t)m * nPS];
+        for (int c = 0; c < 4; ++c) {
+            const int col = c0 + 16 * c + li;
+            acc[c][v] = col < nPS ? pc[col] : T(0);
+        }
+    }
+    // distinct objects among the wave's 16 rows, ascending (wave-uniform)
+    int o = INT_MAX;
+    for (int r = 0; r < 16; ++r) o = min(o, __shfl(objA, r));
+    while (o != INT_MAX) {
+        const T* Bo = B + (size_t)o * K * nPS;
+        const bool mine = objA == o;
+        for (int s0 = 0; 4 * s0 < K; s0 += KS) {
+            T a[KS], b[4][KS];
+#pragma unroll
+            for (int t = 0; t < KS; ++t) {
+                const int k = 4 * (s0 + t) + lk;
+                a[t] = (mine && k < K) ? (T)arow[k] : T(0);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int col = c0 + 16 * c + li;
+                    b[c][t] = (k < K && col < nPS) ? Bo[(size_t)k * nPS + col] : T(0);
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < KS; ++t)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[c] = MfmaVec<T>::mac(a[t], b[c][t], acc[c]);
+        }
+        int next = INT_MAX;
+        for (int r = 0; r < 16; ++r) {
+            const int x = __shfl(objA, r);
+            if (x > o) next = min(next, x);
+        }
+        o = next;
     }
 #pragma unroll
-    for (int j = 0; j < IMG; ++j)
-        if (q0 + j < nQ) phi[(size_t)(q0 + j) * nPS + i] = acc[j];
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int q = q0 + MfmaVec<T>::row(lk, v), col = c0 + 16 * c + li;
+            if (q < nQ && col < nPS) phi[(size_t)q * nPS + col] = acc[c][v];
+        }
+    }
 }
 
 // phiOut[q'] = phi[base(q')] + step * B[obj][mode(q')]   (finite-difference probes)
@@ -299,8 +364,9 @@ __device__ double block_sum(double v, double* sh) {
     return r;
 }
 
-// P, Q on the OTF support for (request r, object o, chunk of 256 support points)
-// plus the chunk's metric partial sum(ftSum - |P|^2/Q).  Images are summed per
+// P, Q on the stored half of the OTF support for (request r, object o, chunk of 256
+// points) plus the chunk's metric partial sum(wt (ftSum - |P|^2/Q)), wt = 2 for a point
+// standing for itself and its mirror.  Images are summed per
 // channel in groups of 30 (Solver::metric task split, solver.cpp:756-797).
 template <typename T>
 __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* __restrict__ Qout,
@@ -308,8 +374,8 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
                                             const cx<T>* __restrict__ D, const double* __restrict__ ftSum,
                                             const double* __restrict__ gamma, const int* __restrict__ reqSlot,
                                             const int* __restrict__ chObj, const int* __restrict__ chFirst,
-                                            const int* __restrict__ chCount, int nCh, int nObj, int nImg, int nOS,
-                                            int useGamma) {
+                                            const int* __restrict__ chCount, const double* __restrict__ wt, int nCh,
+                                            int nObj, int nImg, int nOS, int useGamma) {
     __shared__ double sh[256];
     const int r = blockIdx.z, o = blockIdx.y;
     const int u = blockIdx.x * 256 + threadIdx.x;
@@ -340,7 +406,7 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
         Pout[2 * k] = pr;
         Pout[2 * k + 1] = pi;
         Qout[k] = q;
-        part = ftSum[((size_t)slot * nObj + o) * nOS + u] - (pr * pr + pi * pi) / q;
+        part = wt[u] * (ftSum[((size_t)slot * nObj + o) * nOS + u] - (pr * pr + pi * pi) / q);
     }
     const double s = block_sum(part, sh);
     if (threadIdx.x == 0) partial[((size_t)r * nObj + o) * gridDim.x + blockIdx.x] = s;
@@ -360,7 +426,8 @@ __global__ void k_metric(double* __restrict__ L, const double* __restrict__ part
     L[r] = sum;
 }
 
-// G' (de-centred): tile[q][e] = (PQ D - PS S)/QS on the support, else 0
+// G' (de-centred): tile[q][e] = (PQ D - PS S)/QS on the support (the mirror half as
+// the conjugate of the stored half), else 0
 template <typename T>
 __global__ void k_build_g(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ S, const cx<T>* __restrict__ D,
                           const double* __restrict__ P, const double* __restrict__ Q, const int* __restrict__ invOs,
@@ -369,9 +436,10 @@ __global__ void k_build_g(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ S
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nQ * N2) return;
     const int q = gIdx[e / N2], p = e % N2;
-    const int u = invOs[p];
+    const int code = invOs[p];                 // h: stored point, -2-h: mirror of h, -1: off the support
+    const int u = code >= 0 ? code : -2 - code;
     cx<T> v = {T(0), T(0)};
-    if (u >= 0) {
+    if (code != -1) {
         const int img = qm.img[q], r = qReq[q], o = imgObj[img];
         const size_t k = ((size_t)r * nObj + o) * nOS + u;
         const double pr = P[2 * k], pi = P[2 * k + 1], qq = Q[k];
@@ -380,7 +448,7 @@ __global__ void k_build_g(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ S
         const cx<T> s = S[((size_t)r * nImg + img) * nOS + u];
         const double gr = (pqr * d.x - pqi * d.y) - ps * s.x;
         const double gi = (pqr * d.y + pqi * d.x) - ps * s.y;
-        v = {(T)(gr / qs), (T)(gi / qs)};
+        v = {(T)(gr / qs), (T)((code >= 0 ? gi : -gi) / qs)};   // G(-u) = G(u)*
     }
     tiles[e] = v;
 }
@@ -417,26 +485,108 @@ __global__ void k_extract_v(T* __restrict__ v, const cx<T>* __restrict__ tiles, 
     v[e] = extraPupil ? im * pu : im;
 }
 
-// g[arow(q)][m] = scale[q] * sum_i v[q][i] * B[obj][m][i], m < nEnabled; 0 otherwise
+// g[gIdx[r]][m] = scale[r] * sum_i v[r][i] * B[obj][m][i] for m < nEnabled, 0 otherwise
+// (Solver::gradient's mode projection, solver.cpp:860-894), as a GEMM on the matrix
+// cores in two passes.  k_grad_mfma: workgroup (16 gradient images, pixel slice s)
+// computes all modes in tiles of 64; the pixels are cut into runs of GR_RUN, run c
+// belongs to slice c % GR_SLICES and, inside the workgroup, to wave (c / GR_SLICES) % 4.
+// Lane (i, k) of the 16x16x4 MFMA holds pixels [run + k*GR_RUN/4, +GR_RUN/4) of row /
+// mode i, so each k-step feeds 4 pixels and each row is read in contiguous runs.
+// Partial sums leave the MFMA accumulators for FP64 after every run; the 4 waves'
+// partials are added in a fixed order and written to part[s][r][m].  k_grad_sum adds
+// the slices in a fixed order and applies the scale.
+constexpr int GR_SLICES = 8;
 template <typename T>
-__global__ void __launch_bounds__(256) k_grad(double* __restrict__ g, const T* __restrict__ v, const T* __restrict__ B,
-                                              const double* __restrict__ scale, QMap qm,
-                                              const int* __restrict__ imgObj, const int* __restrict__ gIdx, int K,
-                                              int nPS, int nEnabled) {
-    __shared__ double sh[256];
-    const int qg = blockIdx.x, q = gIdx[qg];
-    const int o = imgObj[qm.img[q]];
-    const T* vq = v + (size_t)qg * nPS;
-    double* gq = g + (size_t)q * K;
-    for (int m = 0; m < K; ++m) {
-        double acc = 0;
-        if (m < nEnabled) {
-            const T* b = B + ((size_t)o * K + m) * nPS;
-            for (int i = threadIdx.x; i < nPS; i += 256) acc += (double)(vq[i] * b[i]);
+struct GradRun { static constexpr int value = 32; };
+template <>
+struct GradRun<double> { static constexpr int value = 16; };
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_grad_mfma(double* __restrict__ part, const T* __restrict__ v,
+                                                   const T* __restrict__ B, QMap qm,
+                                                   const int* __restrict__ imgObj, const int* __restrict__ gIdx, int nG,
+                                                   int K, int Kt, int nPS, int nEnabled) {
+    using V = typename MfmaVec<T>::v4;
+    constexpr int RUN = GradRun<T>::value, PL = RUN / 4;      // pixels per run / per lane
+    __shared__ double red[4][16][65];
+    __shared__ int sobj[16];
+    const int g0 = blockIdx.x * 16, slice = blockIdx.y;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    if (threadIdx.x < 16)
+        sobj[threadIdx.x] = (g0 + (int)threadIdx.x < nG) ? imgObj[qm.img[gIdx[g0 + threadIdx.x]]] : INT_MAX;
+    __syncthreads();
+    const T* vr = v + (size_t)min(g0 + li, nG - 1) * nPS;
+    const int nRun = (nPS + RUN - 1) / RUN;
+    for (int mt = 0; mt < Kt; mt += 64) {
+        double accd[4][4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) accd[c][e] = 0.0;
+        int o = INT_MAX;
+        for (int r = 0; r < 16; ++r) o = min(o, sobj[r]);
+        while (o != INT_MAX) {
+            const T* Bo = B + (size_t)o * K * nPS;
+            const bool mine = sobj[li] == o;
+            for (int run = slice + GR_SLICES * w; run < nRun; run += 4 * GR_SLICES) {
+                const int px0 = run * RUN + lk * PL;
+                T a[PL], b[4][PL];
+#pragma unroll
+                for (int t = 0; t < PL; ++t) {
+                    const int px = px0 + t;
+                    a[t] = (mine && px < nPS) ? vr[px] : T(0);
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const int m = mt + 16 * c + li;
+#pragma unroll
+                    for (int t = 0; t < PL; ++t) {
+                        const int px = px0 + t;
+                        b[c][t] = (m < nEnabled && px < nPS) ? Bo[(size_t)m * nPS + px] : T(0);
+                    }
+                }
+                V acc[4];
+#pragma unroll
+                for (int c = 0; c < 4; ++c) acc[c] = V{0, 0, 0, 0};
+#pragma unroll
+                for (int t = 0; t < PL; ++t)
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) acc[c] = MfmaVec<T>::mac(a[t], b[c][t], acc[c]);
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) accd[c][e] += (double)acc[c][e];
+            }
+            int next = INT_MAX;
+            for (int r = 0; r < 16; ++r)
+                if (sobj[r] > o) next = min(next, sobj[r]);
+            o = next;
         }
-        const double s = block_sum(acc, sh);
-        if (threadIdx.x == 0) gq[m] = (m < nEnabled) ? scale[qg] * s : 0.0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) red[w][MfmaVec<T>::row(lk, e)][16 * c + li] = accd[c][e];
+        __syncthreads();
+        for (int t = threadIdx.x; t < 16 * 64; t += 256) {
+            const int r = t / 64, mm = t % 64, m = mt + mm;
+            if (g0 + r < nG && m < Kt)
+                part[((size_t)slice * nG + g0 + r) * Kt + m] =
+                    ((red[0][r][mm] + red[1][r][mm]) + red[2][r][mm]) + red[3][r][mm];
+        }
+        __syncthreads();
     }
+}
+
+__global__ void k_grad_sum(double* __restrict__ g, const double* __restrict__ part, const double* __restrict__ scale,
+                           const int* __restrict__ gIdx, int nG, int K, int Kt, int nEnabled) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nG * K) return;
+    const int r = e / K, m = e % K;
+    double sum = 0.0;
+    if (m < nEnabled)
+        for (int sl = 0; sl < GR_SLICES; ++sl) sum += part[((size_t)sl * nG + r) * Kt + m];
+    g[(size_t)gIdx[r] * K + m] = (m < nEnabled) ? scale[r] * sum : 0.0;
 }
 
 // dl = -sum_u (q dn - dq |p|^2) / (q (q + dq)) / N^2 for one perturbed image
@@ -447,7 +597,8 @@ __global__ void __launch_bounds__(256) k_metric_change(double* __restrict__ out,
                                                        const double* __restrict__ P, const double* __restrict__ Q,
                                                        const int* __restrict__ baseQ, QMap qm,
                                                        const int* __restrict__ qReq, const int* __restrict__ imgObj,
-                                                       int nOS, int nImg, int nObj, double invN2) {
+                                                       const double* __restrict__ wt, int nOS, int nImg, int nObj,
+                                                       double invN2) {
     __shared__ double sh[256];
     const int q = blockIdx.x;          // perturbed probe
     const int b = baseQ[q];            // the unperturbed image
@@ -463,7 +614,7 @@ __global__ void __launch_bounds__(256) k_metric_change(double* __restrict__ out,
         const double dpr = d.x * dsr + d.y * dsi, dpi = d.x * dsi - d.y * dsr;   // conj(d) * ds
         const double dq = 2.0 * (so.x * dsr + so.y * dsi) + (dsr * dsr + dsi * dsi);
         const double dn = 2.0 * (dpr * pr + dpi * pi) + (dpr * dpr + dpi * dpi);
-        dl -= (qq * dn - dq * (pr * pr + pi * pi)) / (qq * (qq + dq));
+        dl -= wt[u] * ((qq * dn - dq * (pr * pr + pi * pi)) / (qq * (qq + dq)));
     }
     const double s = block_sum(dl, sh);
     if (threadIdx.x == 0) out[q] = s * invN2;
@@ -687,7 +838,8 @@ __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const
                                                     const double* __restrict__ Q, const float* __restrict__ phi,
                                                     const int* __restrict__ pdeY, const int* __restrict__ pdeX,
                                                     const float* __restrict__ pupil, const int* __restrict__ osNatY,
-                                                    const int* __restrict__ osNatX, QMap qm, const int* __restrict__ qReq,
+                                                    const int* __restrict__ osNatX, const int* __restrict__ osMirY,
+                                                    const int* __restrict__ osMirX, QMap qm, const int* __restrict__ qReq,
                                                     const int* __restrict__ imgObj, const int* __restrict__ gIdx,
                                                     const cx<float>* __restrict__ tw,
                                                     int nPS, int nOS, int nImg, int nObj, int extraPupil) {
@@ -708,8 +860,10 @@ __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const
         const cx<float> d = Dq[u], s = Sq[u];
         const double gr = (pqr * d.x - pqi * d.y) - ps * s.x;
         const double gi = (pqr * d.y + pqi * d.x) - ps * s.y;
-        // de-centred position = natural index of the centred support point
-        t[Tile<N>::at(osNatY[u], osNatX[u])] = {(float)(gr / qs), (float)(gi / qs)};
+        // de-centred position = natural index of the centred support point; G(-u) = G(u)*
+        const float g_r = (float)(gr / qs), g_i = (float)(gi / qs);
+        t[Tile<N>::at(osNatY[u], osNatX[u])] = {g_r, g_i};
+        if (osMirY[u] >= 0) t[Tile<N>::at(osMirY[u], osMirX[u])] = {g_r, -g_i};
     }
     __syncthreads();
     tile_fft2<float, N, +1, NT>(t, tw, 0, N);
@@ -775,6 +929,10 @@ struct DBuf {
         p = nullptr;
         n = 0;
     }
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { free_(); }
 };
 
 // Host staging arrays are ordinary vectors, so each upload completes before return.
@@ -810,6 +968,10 @@ struct HBuf {
         p = nullptr;
         n = 0;
     }
+    HBuf() = default;
+    HBuf(const HBuf&) = delete;
+    HBuf& operator=(const HBuf&) = delete;
+    ~HBuf() { free_(); }
 };
 
 inline unsigned nblk(size_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
@@ -861,7 +1023,7 @@ public:
 
     int capacity() const override { return maxSlots_; }
     double* alpha_staging() override { return hStage_.p; }
-    int nOS() const override { return nOS_; }
+    int nOS() const override { return nOSfull_; }
     const std::vector<int>& otf_support() const override { return os_; }
 
     void set_basis(const double* pupil, const double* modes) override {
@@ -886,13 +1048,37 @@ public:
         }
         // OTF support: autocorrelation of the pupil above 1e-9 (Pupil::generateSupport)
         os_ = otf_support_of(pupil);
-        nOS_ = (int)os_.size();
-        std::vector<int> osNat(nOS_), invOs(N2, -1);
-        for (int u = 0; u < nOS_; ++u) {
-            const int cy = os_[u] / N, cx_ = os_[u] % N;
-            osNat[u] = ((cy + N / 2) % N) * N + (cx_ + N / 2) % N;
-            invOs[osNat[u]] = u;
+        nOSfull_ = (int)os_.size();
+        // Hermitian half of the support: S(-u) = S(u)*, D(-u) = D(u)* (real frames), so
+        // P, Q, ftSum and the metric terms of u and -u agree.  Keep one representative
+        // of each pair (the smaller centred index) with weight 2, unpaired points (-u
+        // outside the support, or u = -u) with weight 1.
+        auto natOf = [&](int c) { return ((c / N + N / 2) % N) * N + (c % N + N / 2) % N; };
+        auto mirOf = [&](int c) { return ((N - c / N) % N) * N + (N - c % N) % N; };
+        std::vector<int> where(N2, -1);
+        for (int u = 0; u < nOSfull_; ++u) where[os_[u]] = u;
+        std::vector<int> rep;
+        std::vector<double> wt;
+        std::vector<int> mirNat;
+        fullMap_.assign(nOSfull_, 0);
+        for (int u = 0; u < nOSfull_; ++u) {
+            const int c = os_[u], m = mirOf(c);
+            const bool paired = where[m] >= 0 && m != c;
+            if (paired && m < c) continue;                   // the mirror of an earlier representative
+            fullMap_[u] = (int)rep.size();
+            if (paired) fullMap_[where[m]] = -1 - (int)rep.size();
+            rep.push_back(c);
+            wt.push_back(paired ? 2.0 : 1.0);
+            mirNat.push_back(paired ? natOf(m) : -1);
         }
+        nOS_ = (int)rep.size();
+        std::vector<int> osNat(nOS_), invOs(N2, -1);
+        for (int h = 0; h < nOS_; ++h) {
+            osNat[h] = natOf(rep[h]);
+            invOs[osNat[h]] = h;
+            if (mirNat[h] >= 0) invOs[mirNat[h]] = -2 - h;
+        }
+        upload(wt_, wt, st_);
         upload(invPio_, invPio, st_);
         upload(invPde_, invPde, st_);
         {
@@ -909,9 +1095,14 @@ public:
         upload(pupil_, pup, st_);
         upload(osNat_, osNat, st_);
         upload(invOs_, invOs, st_);
-        {
-            std::vector<int> yx(2 * (size_t)nOS_);
-            for (int u = 0; u < nOS_; ++u) { yx[u] = osNat[u] / N; yx[nOS_ + u] = osNat[u] % N; }
+        {   // [Y, X of the representative, Y, X of its mirror or -1] in the natural layout
+            std::vector<int> yx(4 * (size_t)nOS_);
+            for (int u = 0; u < nOS_; ++u) {
+                yx[u] = osNat[u] / N;
+                yx[nOS_ + u] = osNat[u] % N;
+                yx[2 * nOS_ + u] = mirNat[u] >= 0 ? mirNat[u] / N : -1;
+                yx[3 * nOS_ + u] = mirNat[u] >= 0 ? mirNat[u] % N : -1;
+            }
             upload(osyx_, yx, st_);
         }
         if constexpr (FUSED) {
@@ -1098,12 +1289,24 @@ public:
             rq[k].slot = slots[k];
             rq[k].alpha = alpha[k];
         }
+        std::vector<double> hP, hQ;
         for (int b = 0; b < n; b += perChunk) {
             const int m = std::min(perChunk, n - b);
             forward(&rq[b], m, /*useGamma=*/0);
-            HIPCHK(hipMemcpyAsync(P0 + 2 * b * per, Pp_.p, 2 * m * per * sizeof(double), hipMemcpyDeviceToHost, st_));
-            HIPCHK(hipMemcpyAsync(Q0 + b * per, Qp_.p, m * per * sizeof(double), hipMemcpyDeviceToHost, st_));
+            hP.resize(2 * (size_t)m * per);
+            hQ.resize((size_t)m * per);
+            HIPCHK(hipMemcpyAsync(hP.data(), Pp_.p, hP.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipMemcpyAsync(hQ.data(), Qp_.p, hQ.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipStreamSynchronize(st_));
+            // expand the stored half to the full support: P(-u) = P(u)*, Q(-u) = Q(u)
+            for (int k = 0; k < m * g_.nObj; ++k)
+                for (int u = 0; u < nOSfull_; ++u) {
+                    const int code = fullMap_[u], h = code >= 0 ? code : -1 - code;
+                    const size_t src = (size_t)k * nOS_ + h, dst = (size_t)(b * g_.nObj + k) * nOSfull_ + u;
+                    P0[2 * dst] = hP[2 * src];
+                    P0[2 * dst + 1] = code >= 0 ? hP[2 * src + 1] : -hP[2 * src + 1];
+                    Q0[dst] = hQ[src];
+                }
         }
         collect();
     }
@@ -1119,13 +1322,28 @@ public:
             out.resize(n);
             HIPCHK(hipMemcpy(out.data(), p, n * sizeof(double), hipMemcpyDeviceToHost));
         };
+        // support arrays are stored on the Hermitian half; expand rows of nOS_ to the full
+        // support (complex: conjugate for mirrors)
+        auto expand = [&](int rows, bool cplx) {
+            std::vector<double> h;
+            h.swap(out);
+            const int w = cplx ? 2 : 1;
+            out.assign((size_t)rows * nOSfull_ * w, 0.0);
+            for (int k = 0; k < rows; ++k)
+                for (int u = 0; u < nOSfull_; ++u) {
+                    const int code = fullMap_[u], hh = code >= 0 ? code : -1 - code;
+                    const size_t src = ((size_t)k * nOS_ + hh) * w, dst = ((size_t)k * nOSfull_ + u) * w;
+                    out[dst] = h[src];
+                    if (cplx) out[dst + 1] = code >= 0 ? h[src + 1] : -h[src + 1];
+                }
+        };
         HIPCHK(hipStreamSynchronize(st_));
         const size_t nI = (size_t)g_.nImg * nOS_, nO = (size_t)g_.nObj * nOS_;
-        if (what == "D") getc(D_.p + slot * nI, nI);
-        else if (what == "S") getc(S_.p + slot * nI, nI);           // slot = request index
-        else if (what == "P") getd(Pp_.p + slot * 2 * nO, 2 * nO);
-        else if (what == "Q") getd(Qp_.p + slot * nO, nO);
-        else if (what == "ftsum") getd(ftSum_.p + slot * nO, nO);
+        if (what == "D") { getc(D_.p + slot * nI, nI); expand(g_.nImg, true); }
+        else if (what == "S") { getc(S_.p + slot * nI, nI); expand(g_.nImg, true); }   // slot = request index
+        else if (what == "P") { getd(Pp_.p + slot * 2 * nO, 2 * nO); expand(g_.nObj, true); }
+        else if (what == "Q") { getd(Qp_.p + slot * nO, nO); expand(g_.nObj, false); }
+        else if (what == "ftsum") { getd(ftSum_.p + slot * nO, nO); expand(g_.nObj, false); }
         else if (what == "gamma") getd(gamma_.p + slot * g_.nObj, g_.nObj);
         else if (what == "phi") {
             const size_t n = (size_t)g_.nImg * nPS_;
@@ -1277,9 +1495,9 @@ private:
         reqSlot_ = ints_.p + 4 * nQ;
         phi_.alloc((size_t)nQ * nPS_);
         S_.alloc((size_t)nQ * nOS_);
-        launch("k_phase", nQ, [&] {
-            hipLaunchKernelGGL(k_phase<T>, dim3(nblk(nPS_), (nQ + 15) / 16), dim3(256), 0, st_, phi_.p, B_.p, phiCh_.p,
-                               alpha_.p, qm_, nQ, K, nPS_, imgObj_.p, imgCh_.p, g_.nCh);
+        launch("k_phase_mfma", nQ, [&] {
+            hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_, phi_.p, B_.p,
+                               phiCh_.p, alpha_.p, qm_, nQ, K, nPS_, imgObj_.p, imgCh_.p, g_.nCh);
         });
         otf_chain(phi_.p, qm_.img, nQ, S_.p);
         const int nChunk = (nOS_ + 255) / 256;
@@ -1288,8 +1506,8 @@ private:
         part_.alloc((size_t)n * g_.nObj * nChunk);
         launch("k_pq", nQ, [&] {
             hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, Pp_.p, Qp_.p, part_.p, S_.p, D_.p,
-                               ftSum_.p, gamma_.p, reqSlot_, chObj_.p, chFirst_.p, chCount_.p, g_.nCh, g_.nObj, nImg,
-                               nOS_, useGamma);
+                               ftSum_.p, gamma_.p, reqSlot_, chObj_.p, chFirst_.p, chCount_.p, wt_.p, g_.nCh, g_.nObj,
+                               nImg, nOS_, useGamma);
         });
     }
 
@@ -1365,8 +1583,8 @@ private:
                 launch("k_vogel_fused", nG, [&] {
                     hipLaunchKernelGGL((k_vogel_fused<N, FNT>), dim3(nG), dim3(FNT), FLDS, st_, v_.p, S_.p, D_.p, Pp_.p,
                                        Qp_.p, phi_.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, osyx_.p,
-                                       osyx_.p + nOS_, qm_, qReq_, imgObj_.p, gIdx_.p, tw_.p, nPS_, nOS_, nImg,
-                                       g_.nObj, extraPupil);
+                                       osyx_.p + nOS_, osyx_.p + 2 * nOS_, osyx_.p + 3 * nOS_, qm_, qReq_, imgObj_.p,
+                                       gIdx_.p, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
                 });
             } else {
                 tiles_.alloc((size_t)nG * N2);
@@ -1396,10 +1614,18 @@ private:
                                        tiles_.p, phi_.p, pde_.p, pupil_.p, gIdx_.p, nG, nPS_, N2, extraPupil);
                 });
             }
-            launch("k_grad", nG, [&] {
-                hipLaunchKernelGGL(k_grad<T>, dim3(nG), dim3(256), 0, st_, g_d_.p, v_.p, B_.p, gscale_.p, qm_,
-                                   imgObj_.p, gIdx_.p, K, nPS_, nEn);
-            });
+            {
+                const int Kt = std::min(K, (nEn + 15) & ~15);      // mode columns actually computed
+                gpart_.alloc((size_t)GR_SLICES * nG * std::max(Kt, 1));
+                launch("k_grad_mfma", nG, [&] {
+                    hipLaunchKernelGGL(k_grad_mfma<T>, dim3((nG + 15) / 16, GR_SLICES), dim3(256), 0, st_, gpart_.p,
+                                       v_.p, B_.p, qm_, imgObj_.p, gIdx_.p, nG, K, Kt, nPS_, nEn);
+                });
+                launch("k_grad_sum", [&] {
+                    hipLaunchKernelGGL(k_grad_sum, dim3(nblk((size_t)nG * K)), dim3(256), 0, st_, g_d_.p, gpart_.p,
+                                       gscale_.p, gIdx_.p, nG, K, Kt, nEn);
+                });
+            }
         }
         if (anyFd) fd_gradient(rq, n, K, nImg);
     }
@@ -1448,7 +1674,7 @@ private:
             otf_chain(phiFd_.p, fdInts_.p + 3 * m, m, Sfd_.p);
             launch("k_metric_change", [&] {
                 hipLaunchKernelGGL(k_metric_change<T>, dim3(m), dim3(256), 0, st_, fdOut_.p, Sfd_.p, S_.p, D_.p, Pp_.p,
-                                   Qp_.p, fdInts_.p, qm_, qReq_, imgObj_.p, nOS_, nImg, g_.nObj, 1.0 / N2);
+                                   Qp_.p, fdInts_.p, qm_, qReq_, imgObj_.p, wt_.p, nOS_, nImg, g_.nObj, 1.0 / N2);
             });
             HIPCHK(hipMemcpyAsync(res.data() + b, fdOut_.p, m * sizeof(double), hipMemcpyDeviceToHost, st_));
         }
@@ -1497,7 +1723,8 @@ private:
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> evPending_;
     std::vector<long long> evUnits_;
     std::vector<int> ps_, os_;
-    int nPS_ = 0, nOS_ = 0;
+    int nPS_ = 0, nOS_ = 0, nOSfull_ = 0;   // nOS_: stored (half) support, nOSfull_: full support
+    std::vector<int> fullMap_;               // full support point -> half index h, or -1-h for a mirror
     double area_ = 0;
     std::vector<double> otfNorm_;
     QMap qm_{};
@@ -1507,7 +1734,7 @@ private:
     DBuf<T> phi_, phiFd_, B_, phiCh_, pupil_, imgScale_, v_, greal_, fdStep_;
     HBuf<double> hStage_, hAlpha_, hL_, hG_;
     HBuf<int> hInts_;
-    DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_, statOut_;
+    DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_, statOut_, gpart_, wt_;
     DBuf<int> ints_, fdInts_, gIdx_, invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_,
         pyx_, osyx_;
     DBuf<float> rawF_;

@@ -776,6 +776,82 @@ __global__ void __launch_bounds__(256) k_noise(double* __restrict__ noise, doubl
 // One workgroup per image with the whole N x N tile resident in LDS (FP32, N <= 128:
 // 128 x 129 complex floats = 132 KB of gfx950's 160 KB).  Rows are padded by one
 // element so row and column passes both walk consecutive banks (MAP 1).
+// First Stockham pass (NS = 1, forward) of lines whose only non-zero elements are
+// [N/4, 3N/4): element j + r N/R is non-zero only for r in [R/4, 3R/4), so only those
+// are read and the rest of the butterfly input is the constant zero.
+template <typename T, int N, int R, int NL, int NT, int MAP>
+__device__ __forceinline__ void spass_first_mid(cx<T>* buf, int ls, int es) {
+    constexpr int NB = NL * (N / R);
+    constexpr int BT = (NB + NT - 1) / NT;
+    cx<T> v[BT][R];
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = MAP ? b % NL : b / (N / R), j = MAP ? b / NL : b % (N / R);
+            const cx<T>* L = buf + line * ls;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                v[i][r] = (r >= R / 4 && r < 3 * R / 4) ? L[(j + r * (N / R)) * es] : cx<T>{T(0), T(0)};
+            dft_reg<T, R, -1>(v[i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = MAP ? b % NL : b / (N / R), j = MAP ? b / NL : b % (N / R);
+            cx<T>* L = buf + line * ls;
+#pragma unroll
+            for (int r = 0; r < R; ++r) L[(j * R + r) * es] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Last forward Stockham pass (NS = N/R) of NL lines that writes, instead of the
+// transform F, the packed power spectrum pk[line][y] = |F(y)|^2 + i |F(y + N/2)|^2
+// for y < N/2 (both outputs of one butterfly: y = j + r NS and y + N/2 = j + (r + R/2) NS).
+template <typename T, int N, int R, int NL, int NT, int MAP>
+__device__ __forceinline__ void spass_last_power(cx<T>* buf, int ls, int es, const cx<T>* __restrict__ tw,
+                                                 cx<T>* pk, int pls, int pes) {
+    constexpr int NS = N / R;
+    constexpr int NB = NL * (N / R);
+    constexpr int BT = (NB + NT - 1) / NT;
+    cx<T> v[BT][R];
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = MAP ? b % NL : b / (N / R), j = MAP ? b / NL : b % (N / R);
+            const cx<T>* L = buf + line * ls;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cx<T> x = L[(j + r * (N / R)) * es];
+                if (r > 0) x = cmul(x, tw[r * j * (N / (NS * R))]);
+                v[i][r] = x;
+            }
+            dft_reg<T, R, -1>(v[i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = MAP ? b % NL : b / (N / R), j = MAP ? b / NL : b % (N / R);
+            cx<T>* L = pk + line * pls;
+#pragma unroll
+            for (int r = 0; r < R / 2; ++r) {
+                const cx<T> a = v[i][r], c = v[i][r + R / 2];
+                L[(j + r * NS) * pes] = {a.x * a.x + a.y * a.y, c.x * c.x + c.y * c.y};
+            }
+        }
+    }
+    __syncthreads();
+}
+
 template <int N>
 struct Tile {
     static constexpr int LD = N + 1;                 // padded row stride (complex elements)
@@ -791,41 +867,68 @@ __device__ __forceinline__ void tile_fft2(cx<T>* t, const cx<T>* __restrict__ tw
     fft_lines<T, N, DIR, N, NT, 1>(t, 1, Tile<N>::LD, tw);
 }
 
-// S[q][u] = reorder(IFFT(|FFT(a)|^2))/N^2 on the OTF support, a = scale * pupil * exp(i phi)
-// placed with the pupil centred at (N/2, N/2) (pupilInOTF, pupil.cpp:261).
+// S[q][h] = reorder(IFFT(|FFT(a)|^2))/N^2 on the stored half of the OTF support,
+// a = scale * pupil * exp(i phi) with the pupil centred at (N/2, N/2) (pupilInOTF,
+// pupil.cpp:261), so a is non-zero only in rows and columns [N/4, 3N/4).
+//   1. write the P x P pupil region (zeros off the pupil support); nothing else is read
+//   2. forward row FFTs of the N/2 pupil rows, first pass pruned to the pupil columns
+//   3. forward column FFTs, first pass pruned to the pupil rows; the last pass writes
+//      the real |F|^2 packed two rows per complex row: pk[y] = |F|^2(y) + i |F|^2(y+N/2)
+//   4. inverse FFTs of the N/2 packed rows; each splits into the half spectra of its
+//      two real rows: A(k) = (Z(k) + Z(-k)*)/2, B(k) = (Z(k) - Z(-k)*)/2i, k <= N/2
+//   5. inverse column FFTs of the N/2+1 half-spectrum columns; the 2-D result is
+//      Hermitian, so these columns hold every stored support point (hread: index into
+//      the half array, -1-index for a point read as the conjugate of its mirror).
 template <int N, int NT>
 __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, const float* __restrict__ phi,
-                                                  const int* __restrict__ pioY, const int* __restrict__ pioX,
-                                                  const float* __restrict__ pupil, const int* __restrict__ qImg,
-                                                  const float* __restrict__ imgScale,
-                                                  const int* __restrict__ osNatY, const int* __restrict__ osNatX,
-                                                  const cx<float>* __restrict__ tw, int nPS, int nOS) {
+                                                  const int* __restrict__ invPs, const float* __restrict__ pupil,
+                                                  const int* __restrict__ qImg, const float* __restrict__ imgScale,
+                                                  const int* __restrict__ hread, const cx<float>* __restrict__ tw,
+                                                  int nPS, int nOS) {
+    constexpr int P = N / 2, LD = Tile<N>::LD, HC = N / 2 + 1;
+    constexpr int RF = N == 64 ? 8 : 16;                     // radix of the first pass (fft_lines)
+    constexpr int RL = N == 256 ? 16 : 8;                    // radix of the last pass
     extern __shared__ cx<float> lds_dyn[];
-    cx<float>* t = lds_dyn;
+    cx<float>* t = lds_dyn;                                  // N x LD tile (natural layout)
+    cx<float>* hs = lds_dyn + (N / 2) * LD;                  // N x HC half spectra, element (y, k) at y*HC + k
     const int q = blockIdx.x;
-    for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) t[e] = {0.f, 0.f};
-    __syncthreads();
     const float sc = imgScale[qImg[q]];
     const float* ph = phi + (size_t)q * nPS;
-    for (int i = threadIdx.x; i < nPS; i += NT) {
-        float s, c;
-        sincosf(ph[i], &s, &c);
-        const float m = pupil[i] * sc;
-        t[Tile<N>::at(pioY[i], pioX[i])] = {m * c, m * s};
+    for (int e = threadIdx.x; e < P * P; e += NT) {
+        const int k = invPs[e];
+        cx<float> val = {0.f, 0.f};
+        if (k >= 0) {
+            float sn, cs;
+            sincosf(ph[k], &sn, &cs);
+            const float m = pupil[k] * sc;
+            val = {m * cs, m * sn};
+        }
+        t[(e / P + N / 4) * LD + e % P + N / 4] = val;
     }
     __syncthreads();
-    tile_fft2<float, N, -1, NT>(t, tw, N / 4, N / 2);        // pupil rows [P/2, 3P/2)
-    for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) {
-        const cx<float> v = t[e];
-        t[e] = {v.x * v.x + v.y * v.y, 0.f};
+    // 2. rows [N/4, 3N/4)
+    spass_first_mid<float, N, RF, N / 2, NT, 1>(t + (N / 4) * LD, LD, 1);
+    spass<float, N, RL, RF, -1, N / 2, NT, 1>(t + (N / 4) * LD, LD, 1, tw);
+    // 3. all columns; the packed power spectrum goes to rows [0, N/2) of the tile
+    spass_first_mid<float, N, RF, N, NT, 1>(t, 1, LD);
+    spass_last_power<float, N, RL, N, NT, 1>(t, 1, LD, tw, t, 1, LD);
+    // 4. inverse row FFTs of the packed rows, then split into half spectra
+    fft_lines<float, N, +1, N / 2, NT, 1>(t, LD, 1, tw);
+    for (int e = threadIdx.x; e < (N / 2) * HC; e += NT) {
+        const int y = e / HC, k = e % HC;
+        const cx<float> z = t[y * LD + k], zm = t[y * LD + (N - k) % N];
+        hs[y * HC + k] = {0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)};
+        hs[(y + N / 2) * HC + k] = {0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)};
     }
     __syncthreads();
-    tile_fft2<float, N, +1, NT>(t, tw, 0, N);
+    // 5. inverse column FFTs of the N/2+1 half-spectrum columns
+    fft_lines<float, N, +1, HC, NT, 1>(hs, 1, HC, tw);
     constexpr float inv = 1.0f / (N * N);
     cx<float>* Sq = S + (size_t)q * nOS;
     for (int u = threadIdx.x; u < nOS; u += NT) {
-        const cx<float> v = t[Tile<N>::at(osNatY[u], osNatX[u])];
-        Sq[u] = {v.x * inv, v.y * inv};
+        const int code = hread[u];
+        const cx<float> v = hs[code >= 0 ? code : -1 - code];
+        Sq[u] = {v.x * inv, (code >= 0 ? v.y : -v.y) * inv};
     }
 }
 
@@ -986,6 +1089,7 @@ public:
     static constexpr bool FUSED = std::is_same_v<T, float> && N <= 128;
     static constexpr int FNT = N == 128 ? 1024 : 256;
     static constexpr size_t FLDS = (size_t)Tile<N>::SIZE * sizeof(cx<float>);
+    static constexpr size_t OLDS = ((size_t)(N / 2) * Tile<N>::LD + (size_t)N * (N / 2 + 1)) * sizeof(cx<float>);
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device) : g_(g), maxSlots_(maxSlots),
                                                                          maxBatch_(maxBatch) {
@@ -1064,7 +1168,11 @@ public:
         for (int u = 0; u < nOSfull_; ++u) {
             const int c = os_[u], m = mirOf(c);
             const bool paired = where[m] >= 0 && m != c;
-            if (paired && m < c) continue;                   // the mirror of an earlier representative
+            // representatives sit in the natural columns [0, N/2] (the half the fused OTF
+            // kernel computes); on columns 0 and N/2 the smaller natural row
+            const int ny = natOf(c) / N, nx = natOf(c) % N;
+            const bool isRep = (nx >= 1 && nx < N / 2) || ((nx == 0 || nx == N / 2) && ny <= (N - ny) % N);
+            if (paired && !isRep) continue;                  // the mirror of a representative
             fullMap_[u] = (int)rep.size();
             if (paired) fullMap_[where[m]] = -1 - (int)rep.size();
             rep.push_back(c);
@@ -1072,6 +1180,17 @@ public:
             mirNat.push_back(paired ? natOf(m) : -1);
         }
         nOS_ = (int)rep.size();
+        {   // where each stored point sits in the fused kernel's (N) x (N/2+1) half array
+            std::vector<int> hread(nOS_);
+            for (int h = 0; h < nOS_; ++h) {
+                const int nat = natOf(rep[h]), ny = nat / N, nx = nat % N;
+                hread[h] = nx <= N / 2 ? ny * (N / 2 + 1) + nx : -1 - (((N - ny) % N) * (N / 2 + 1) + (N - nx));
+            }
+            upload(hread_, hread, st_);
+            std::vector<int> invPs(P2, -1);
+            for (int k = 0; k < nPS_; ++k) invPs[ps_[k]] = k;
+            upload(invPs_, invPs, st_);
+        }
         std::vector<int> osNat(nOS_), invOs(N2, -1);
         for (int h = 0; h < nOS_; ++h) {
             osNat[h] = natOf(rep[h]);
@@ -1107,7 +1226,7 @@ public:
         }
         if constexpr (FUSED) {
             HIPCHK(hipFuncSetAttribute((const void*)k_otf_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)FLDS));
+                                       (int)OLDS));
             HIPCHK(hipFuncSetAttribute((const void*)k_vogel_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)FLDS));
         }
@@ -1430,9 +1549,8 @@ private:
     void otf_chain(const T* phi, const int* qImg, int nQ, C* Sout) {
         if constexpr (FUSED) {
             launch("k_otf_fused", nQ, [&] {
-                hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(nQ), dim3(FNT), FLDS, st_, Sout, phi, pyx_.p,
-                                   pyx_.p + nPS_, pupil_.p, qImg, imgScale_.p, osyx_.p, osyx_.p + nOS_, tw_.p, nPS_,
-                                   nOS_);
+                hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(nQ), dim3(FNT), OLDS, st_, Sout, phi, invPs_.p,
+                                   pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_);
             });
             return;
         }
@@ -1736,7 +1854,7 @@ private:
     HBuf<int> hInts_;
     DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_, statOut_, gpart_, wt_;
     DBuf<int> ints_, fdInts_, gIdx_, invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_,
-        pyx_, osyx_;
+        pyx_, osyx_, hread_, invPs_;
     DBuf<float> rawF_;
 };
 

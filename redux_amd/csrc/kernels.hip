@@ -40,7 +40,7 @@ namespace mfbd {
     } while (0)
 
 template <typename T>
-struct cx {
+struct alignas(2 * sizeof(T)) cx {   // aligned as a unit: one 64/128-bit LDS or global access
     T x, y;
 };
 
@@ -74,6 +74,24 @@ __device__ __forceinline__ double root_c(int k) {
 }
 __device__ __forceinline__ double root_s(int k) { return -root_c((k + 12) & 15); }   // -sin(2 pi k/16)
 
+// a * exp(DIR 2 pi i k / 16) for a compile-time k in [0, 8): the multiplications by 0
+// and 1 of k = 0, 4 and the common factor of k = 2, 6 are written out, since IEEE
+// arithmetic does not let the compiler fold them
+template <typename T, int DIR>
+__device__ __forceinline__ cx<T> rot16(cx<T> a, int k) {
+    constexpr T h = (T)0.70710678118654752440;
+    switch (k) {
+        case 0: return a;
+        case 4: return DIR < 0 ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x};
+        case 2: return DIR < 0 ? cx<T>{(a.x + a.y) * h, (a.y - a.x) * h} : cx<T>{(a.x - a.y) * h, (a.y + a.x) * h};
+        case 6: return DIR < 0 ? cx<T>{(a.y - a.x) * h, -(a.x + a.y) * h} : cx<T>{-(a.x + a.y) * h, (a.x - a.y) * h};
+        default: {
+            const cx<T> w = {(T)root_c(k), (T)(DIR < 0 ? root_s(k) : -root_s(k))};
+            return cmul(a, w);
+        }
+    }
+}
+
 // in-register DFT of R points (R power of two <= 16), DIR = -1 forward, +1 backward
 template <typename T, int R, int DIR>
 __device__ __forceinline__ void dft_reg(cx<T>* v) {
@@ -88,9 +106,7 @@ __device__ __forceinline__ void dft_reg(cx<T>* v) {
         for (int i = 0; i < R; i += L) {
 #pragma unroll
             for (int j = 0; j < L / 2; ++j) {
-                const int k = j * (16 / L);
-                cx<T> w = {(T)root_c(k), (T)(DIR < 0 ? root_s(k) : -root_s(k))};
-                cx<T> t = (j == 0) ? v[i + j + L / 2] : cmul(v[i + j + L / 2], w);
+                const cx<T> t = rot16<T, DIR>(v[i + j + L / 2], j * (16 / L));
                 v[i + j + L / 2] = csub(v[i + j], t);
                 v[i + j] = cadd(v[i + j], t);
             }
@@ -235,9 +251,8 @@ struct PhaseChunk<double> { static constexpr int value = 5; };
 template <typename T>
 __global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T* __restrict__ B,
                                                     const T* __restrict__ phiCh, const double* __restrict__ alpha,
-                                                    QMap qm, int nQ, int K, int nPS,
-                                                    const int* __restrict__ imgObj, const int* __restrict__ imgCh,
-                                                    int nCh) {
+                                                    const int* __restrict__ qArow, const int* __restrict__ qPhRow,
+                                                    const int* __restrict__ qObj, int nQ, int K, int nPS) {
     using V = typename MfmaVec<T>::v4;
     constexpr int KS = PhaseChunk<T>::value;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -246,15 +261,13 @@ __global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T
     if (q0 >= nQ) return;                                    // whole wave exits together
     // this lane's A row and the 4 output rows it holds
     const int qa = min(q0 + li, nQ - 1);
-    const int imgA = qm.img[qa];
-    const int objA = (q0 + li < nQ) ? imgObj[imgA] : INT_MAX;
-    const double* arow = alpha + (size_t)qm.arow[qa] * K;
+    const int objA = (q0 + li < nQ) ? qObj[qa] : INT_MAX;
+    const double* arow = alpha + (size_t)qArow[qa] * K;
     V acc[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int q = min(q0 + MfmaVec<T>::row(lk, v), nQ - 1);
-        const int img = qm.img[q];
-        const T* pc = phiCh + (size_t)(qm.slot[q] * nCh + imgCh[img]) * nPS;
+        const T* pc = phiCh + (size_t)qPhRow[q] * nPS;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int col = c0 + 16 * c + li;
@@ -389,6 +402,7 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
             for (int b = 0; b < n; b += 30) {
                 const int e = min(b + 30, n);
                 double tr = 0, ti = 0, tq = 0;
+#pragma unroll 8
                 for (int j = f0 + b; j < f0 + e; ++j) {
                     const cx<T> s = S[((size_t)r * nImg + j) * nOS + u];
                     const cx<T> d = D[((size_t)slot * nImg + j) * nOS + u];
@@ -1583,7 +1597,8 @@ private:
             lo = std::min(lo, rq[r].slot);
             hi = std::max(hi, rq[r].slot);
         }
-        hInts_.alloc(4 * (size_t)nQ + n);
+        const size_t nInts = 6 * (size_t)nQ + n;
+        hInts_.alloc(nInts);
         int* ints = hInts_.p;
         for (int r = 0; r < n; ++r) {
             for (int j = 0; j < nImg; ++j) {
@@ -1592,6 +1607,8 @@ private:
                 ints[nQ + q] = j;                                              // image
                 ints[2 * nQ + q] = staged ? (rq[r].slot - lo) * nImg + j : q;  // alpha row
                 ints[3 * nQ + q] = r;                                          // request
+                ints[4 * nQ + n + q] = rq[r].slot * g_.nCh + st.imgCh[j];     // phiCh row
+                ints[5 * nQ + n + q] = st.imgObj[j];                           // object
             }
             ints[4 * nQ + r] = rq[r].slot;
         }
@@ -1606,16 +1623,18 @@ private:
             alpha_.alloc((size_t)nQ * K);
             HIPCHK(hipMemcpyAsync(alpha_.p, hAlpha_.p, (size_t)nQ * K * sizeof(double), hipMemcpyHostToDevice, st_));
         }
-        ints_.alloc(4 * (size_t)nQ + n);
-        HIPCHK(hipMemcpyAsync(ints_.p, ints, (4 * (size_t)nQ + n) * sizeof(int), hipMemcpyHostToDevice, st_));
+        ints_.alloc(nInts);
+        HIPCHK(hipMemcpyAsync(ints_.p, ints, nInts * sizeof(int), hipMemcpyHostToDevice, st_));
         qm_ = {ints_.p, ints_.p + nQ, ints_.p + 2 * nQ};
         qReq_ = ints_.p + 3 * nQ;
         reqSlot_ = ints_.p + 4 * nQ;
+        const int* qPhRow = ints_.p + 4 * nQ + n;
+        const int* qObj = ints_.p + 5 * nQ + n;
         phi_.alloc((size_t)nQ * nPS_);
         S_.alloc((size_t)nQ * nOS_);
         launch("k_phase_mfma", nQ, [&] {
             hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_, phi_.p, B_.p,
-                               phiCh_.p, alpha_.p, qm_, nQ, K, nPS_, imgObj_.p, imgCh_.p, g_.nCh);
+                               phiCh_.p, alpha_.p, qm_.arow, qPhRow, qObj, nQ, K, nPS_);
         });
         otf_chain(phi_.p, qm_.img, nQ, S_.p);
         const int nChunk = (nOS_ + 255) / 256;

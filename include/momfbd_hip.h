@@ -132,7 +132,8 @@ int mfbd_eval_grad(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alph
 int mfbd_restore(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, float* img);
 /* Diagnostics: copy a device array for patch p ("S" OTF on support, "D", "P", "Q",
  * "ftsum", "phi") or host state ("gamma_host", "noise_host", "otfnorm", "window",
- * "shifts" [nImg][2] integer cut-out shifts (x, y), "tiltoff" [nImg][2]) in double;
+ * "shifts" [nImg][2] integer cut-out shifts (x, y), "tiltoff" [nImg][2], "sizes" = pupil
+ * support, OTF support and stored (Hermitian half) OTF support point counts) in double;
  * returns the element count via *count. */
 int mfbd_debug_get(mfbd_ctx* ctx, const char* what, int p, double* out, int* count);
 /* Per-kernel timing (environment MFBD_KERNEL_TIMING set when the patches were

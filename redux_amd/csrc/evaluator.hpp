@@ -51,19 +51,27 @@ public:
     // raw frames of `n` patches into slots [first, first+n); phiCh [n][nCh][P*P]
     virtual void load(int first, int n, const float* data, const double* phiCh) = 0;
     virtual void set_phi_channel(int slot, const double* phiCh) = 0;
-    // windowing, padding and FFT of the listed images, then ftSum of the touched objects
-    virtual void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window) = 0;
+    // windowing, padding and FFT of the listed images, then ftSum of the touched objects;
+    // lane 2 completes before returning, lane 0/1 only enqueues (see evaluate_begin)
+    virtual void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window, int lane) = 0;
     // windowed-frame standard deviation and noise level (before NOISE_FUDGE) of the
     // listed images at their shifts; patch side <= 128
     virtual void frame_stats(const std::vector<ImgInit>& imgs, const std::vector<double>& window,
                              const std::vector<double>& noiseWindow, double* noise, double* stddev) = 0;
     virtual void set_gamma(int slot, const double* gamma) = 0;
     virtual void evaluate(std::vector<EvalReq>& reqs) = 0;
+    // asynchronous form on lane 0 or 1: begin enqueues and returns, end waits and fills
+    // f / g; the two lanes let the host work on one half of a batch while the device
+    // evaluates the other
+    virtual void evaluate_begin(int lane, std::vector<EvalReq>& reqs) = 0;
+    virtual void evaluate_end(int lane, std::vector<EvalReq>& reqs) = 0;
     // restoration sums at alpha for n patches: P0 = sum conj(D) S, Q0 = sum |S|^2 (no
     // gamma), per object on the OTF support: P0 [n][nObj][nOS] complex, Q0 [n][nObj][nOS];
     // alpha[k] points at [nImg][K] for slots[k]
     virtual void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) = 0;
-    virtual int nOS() const = 0;
+    virtual int nOS() const = 0;            // OTF support points (full)
+    virtual int nOSStored() const = 0;      // points stored on the device (Hermitian half)
+    virtual int nPS() const = 0;            // pupil support points
     virtual const std::vector<int>& otf_support() const = 0;
     virtual void debug_get(const std::string& what, int slot, std::vector<double>& out) = 0;
     virtual std::vector<KernelStat> kernel_stats() = 0;

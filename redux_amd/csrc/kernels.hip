@@ -1111,6 +1111,7 @@ public:
         HIPCHK(hipStreamCreate(&st_));
         HIPCHK(hipEventCreate(&ev0_));
         HIPCHK(hipEventCreate(&ev1_));
+        for (auto& l : lanes_) HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
         std::vector<C> tw(N);
         for (int k = 0; k < N; ++k) {
             long double a = -2.0L * 3.14159265358979323846264338327950288L * k / N;
@@ -1121,27 +1122,21 @@ public:
     }
 
     ~Pipeline() override {
-        for (auto* b : {&Pp_, &Qp_, &part_, &L_, &g_d_, &alpha_, &ftSum_, &gamma_, &win_, &gscale_, &fdOut_, &w_})
-            b->free_();
-        for (auto* b : {&tiles_, &D_, &S_, &Sfd_}) b->free_();
-        for (auto* b : {&phi_, &phiFd_, &B_, &phiCh_, &pupil_, &imgScale_, &v_, &greal_, &fdStep_}) b->free_();
-        for (auto* b : {&hStage_, &hAlpha_, &hL_, &hG_}) b->free_();
-        hInts_.free_();
-        for (auto* b : {&ints_, &fdInts_, &gIdx_, &pyx_, &osyx_, &invPio_, &invPde_, &osNat_, &invOs_, &pde_, &chObj_, &chFirst_, &chCount_,
-                        &imgObj_, &imgCh_})
-            b->free_();
-        rawF_.free_();
-        tw_.free_();
+        (void)hipStreamSynchronize(st_);
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
+        for (auto& l : lanes_) (void)hipEventDestroy(l.done);
         for (auto& e : evPool_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
-        for (auto& e : evPending_) { (void)hipEventDestroy(e.second.first); (void)hipEventDestroy(e.second.second); }
+        for (auto& e : evPending_) { (void)hipEventDestroy(e.ev.first); (void)hipEventDestroy(e.ev.second); }
         (void)hipStreamDestroy(st_);
+        // device and pinned buffers are released by their own destructors
     }
 
     int capacity() const override { return maxSlots_; }
     double* alpha_staging() override { return hStage_.p; }
     int nOS() const override { return nOSfull_; }
+    int nOSStored() const override { return nOS_; }
+    int nPS() const override { return nPS_; }
     const std::vector<int>& otf_support() const override { return os_; }
 
     void set_basis(const double* pupil, const double* modes) override {
@@ -1312,13 +1307,15 @@ public:
 
     void frame_stats(const std::vector<ImgInit>& imgs, const std::vector<double>& window,
                      const std::vector<double>& noiseWindow, double* noise, double* stddev) override {
+        use_lane(2);
         const int nQ = (int)imgs.size();
         if (!nQ) return;
         const int s = g_.s;
         if (s > 128) throw std::runtime_error("frame_stats: patch side above 128");
         std::vector<double> wins(window);
         wins.insert(wins.end(), noiseWindow.begin(), noiseWindow.end());
-        upload(win_, wins, st_);
+        upload(win_, wins, st_);      // the window first: k_window reads win_ as the window
+        winSet_ = true;
         std::vector<int> ints(4 * (size_t)nQ);
         for (int q = 0; q < nQ; ++q) {
             ints[q] = imgs[q].slot;
@@ -1326,18 +1323,18 @@ public:
             ints[2 * nQ + q] = imgs[q].shiftX;
             ints[3 * nQ + q] = imgs[q].shiftY;
         }
-        upload(ints_, ints, st_);
-        statOut_.alloc(2 * (size_t)nQ);
+        upload(sc_->ints, ints, st_);
+        sc_->statOut.alloc(2 * (size_t)nQ);
         const size_t lds = ((size_t)s * s + 6 * (size_t)s) * sizeof(double);
         launch("k_noise", nQ, [&] {
-            hipLaunchKernelGGL(k_noise, dim3(nQ), dim3(256), lds, st_, statOut_.p, statOut_.p + nQ, rawF_.p, win_.p,
-                               win_.p + (size_t)s * s, ints_.p, ints_.p + nQ, ints_.p + 2 * nQ, ints_.p + 3 * nQ,
+            hipLaunchKernelGGL(k_noise, dim3(nQ), dim3(256), lds, st_, sc_->statOut.p, sc_->statOut.p + nQ, rawF_.p, win_.p,
+                               win_.p + (size_t)s * s, sc_->ints.p, sc_->ints.p + nQ, sc_->ints.p + 2 * nQ, sc_->ints.p + 3 * nQ,
                                g_.nImg, g_.dataY, g_.dataX, g_.startY, g_.startX, s);
         });
-        HIPCHK(hipMemcpyAsync(noise, statOut_.p, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
-        HIPCHK(hipMemcpyAsync(stddev, statOut_.p + nQ, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipMemcpyAsync(noise, sc_->statOut.p, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipMemcpyAsync(stddev, sc_->statOut.p + nQ, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipStreamSynchronize(st_));
-        collect();
+        collect(2);
     }
 
     void set_gamma(int slot, const double* gamma) override {
@@ -1346,41 +1343,17 @@ public:
         HIPCHK(hipStreamSynchronize(st_));
     }
 
-    void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window) override {
+    // Lane 2 waits for completion; on lane 0 or 1 the work is only enqueued (after that
+    // lane's finished evaluation, before its next one) from the lane's own pinned staging.
+    void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window, int lane) override {
+        use_lane(lane);
         if (imgs.empty()) return;
-        upload(win_, window, st_);
+        if (!winSet_) {
+            upload(win_, window, st_);
+            winSet_ = true;
+        }
         const int nQ = (int)imgs.size();
         const int chunk = maxTiles();
-        std::vector<int> hs(nQ), hi(nQ), hx(nQ), hy(nQ);
-        for (int q = 0; q < nQ; ++q) {
-            hs[q] = imgs[q].slot;
-            hi[q] = imgs[q].img;
-            hx[q] = imgs[q].shiftX;
-            hy[q] = imgs[q].shiftY;
-        }
-        for (int b = 0; b < nQ; b += chunk) {
-            const int n = std::min(chunk, nQ - b);
-            std::vector<int> ints;
-            ints.insert(ints.end(), hs.begin() + b, hs.begin() + b + n);
-            ints.insert(ints.end(), hi.begin() + b, hi.begin() + b + n);
-            ints.insert(ints.end(), hx.begin() + b, hx.begin() + b + n);
-            ints.insert(ints.end(), hy.begin() + b, hy.begin() + b + n);
-            upload(ints_, ints, st_);
-            const int* ds = ints_.p;
-            const int* di = ds + n;
-            const int* dx = di + n;
-            const int* dy = dx + n;
-            tiles_.alloc((size_t)chunk * N2);
-            launch("k_window", [&] {
-                hipLaunchKernelGGL(k_window<T>, dim3(n), dim3(256), 0, st_, tiles_.p, rawF_.p, win_.p, ds, di, dx, dy,
-                                   g_.nImg, g_.dataY, g_.dataX, g_.startY, g_.startX, g_.s, N);
-            });
-            fft2(tiles_.p, n, -1);
-            launch("k_store_d", [&] {
-                hipLaunchKernelGGL(k_store_d<T>, dim3(nblk((size_t)n * nOS_)), dim3(256), 0, st_, D_.p, tiles_.p,
-                                   osNat_.p, ds, di, n, nOS_, g_.nImg, N2);
-            });
-        }
         // ftSum for every touched (slot, object)
         std::vector<int> pairs;
         std::map<std::pair<int, int>, int> seen;
@@ -1392,29 +1365,82 @@ public:
                 pairs.push_back(key.second);
             }
         }
-        upload(ints_, pairs, st_);
+        std::vector<int> ints(4 * (size_t)nQ);
+        for (int q = 0; q < nQ; ++q) {
+            ints[q] = imgs[q].slot;
+            ints[nQ + q] = imgs[q].img;
+            ints[2 * nQ + q] = imgs[q].shiftX;
+            ints[3 * nQ + q] = imgs[q].shiftY;
+        }
+        ints.insert(ints.end(), pairs.begin(), pairs.end());
+        stage_async(sc_->initInts, sc_->hInit, ints);
+        const int* base = sc_->initInts.p;
+        sc_->tiles.alloc((size_t)std::min(chunk, nQ) * N2);
+        for (int b = 0; b < nQ; b += chunk) {
+            const int n = std::min(chunk, nQ - b);
+            const int* ds = base + b;
+            const int* di = base + nQ + b;
+            const int* dx = base + 2 * nQ + b;
+            const int* dy = base + 3 * nQ + b;
+            launch("k_window", n, [&] {
+                hipLaunchKernelGGL(k_window<T>, dim3(n), dim3(256), 0, st_, sc_->tiles.p, rawF_.p, win_.p, ds, di, dx,
+                                   dy, g_.nImg, g_.dataY, g_.dataX, g_.startY, g_.startX, g_.s, N);
+            });
+            fft2(sc_->tiles.p, n, -1);
+            launch("k_store_d", n, [&] {
+                hipLaunchKernelGGL(k_store_d<T>, dim3(nblk((size_t)n * nOS_)), dim3(256), 0, st_, D_.p, sc_->tiles.p,
+                                   osNat_.p, ds, di, n, nOS_, g_.nImg, N2);
+            });
+        }
         const int nPairs = (int)pairs.size() / 2;
-        launch("k_ftsum", [&] {
+        launch("k_ftsum", nPairs, [&] {
             hipLaunchKernelGGL(k_ftsum<T>, dim3(nblk((size_t)nPairs * nOS_)), dim3(256), 0, st_, ftSum_.p, D_.p,
-                               ints_.p, imgObj_.p, nPairs, g_.nImg, g_.nObj, nOS_);
+                               base + 4 * nQ, imgObj_.p, nPairs, g_.nImg, g_.nObj, nOS_);
         });
-        HIPCHK(hipStreamSynchronize(st_));
-        collect();
-    }
-
-    void evaluate(std::vector<EvalReq>& reqs) override {
-        const int nR = (int)reqs.size();
-        if (!nR) return;
-        const int K = g_.K, nImg = g_.nImg;
-        // requests are processed in chunks that fit the tile workspace
-        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / nImg));
-        for (int b = 0; b < nR; b += perChunk) {
-            const int n = std::min(perChunk, nR - b);
-            eval_chunk(&reqs[b], n, K, nImg);
+        if (lane == 2) {
+            HIPCHK(hipStreamSynchronize(st_));
+            collect(2);
         }
     }
 
+    void evaluate(std::vector<EvalReq>& reqs) override {
+        evaluate_begin(2, reqs);
+        evaluate_end(2, reqs);
+    }
+
+    // Enqueue the evaluation of `reqs` on lane 0 or 1 (lane 2: the synchronous calls) and
+    // return without waiting; evaluate_end waits for that lane and fills f and g.  The
+    // requests' alpha (and the lane's staging) must stay untouched until then.  Requests
+    // beyond one chunk of the tile workspace are evaluated synchronously first.
+    void evaluate_begin(int lane, std::vector<EvalReq>& reqs) override {
+        use_lane(lane);
+        const int nR = (int)reqs.size();
+        sc_->nReq = 0;
+        if (!nR) return;
+        const int K = g_.K, nImg = g_.nImg;
+        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / nImg));
+        int b = 0;
+        for (; nR - b > perChunk; b += perChunk) {
+            eval_chunk_launch(&reqs[b], perChunk, K, nImg);
+            HIPCHK(hipEventSynchronize(sc_->done));
+            eval_chunk_finish(&reqs[b], perChunk, K, nImg);
+        }
+        eval_chunk_launch(&reqs[b], nR - b, K, nImg);
+        sc_->nReq = nR - b;
+        laneFirst_[lane] = b;
+    }
+
+    void evaluate_end(int lane, std::vector<EvalReq>& reqs) override {
+        use_lane(lane);
+        if (!sc_->nReq) return;
+        HIPCHK(hipEventSynchronize(sc_->done));
+        eval_chunk_finish(&reqs[laneFirst_[lane]], sc_->nReq, g_.K, g_.nImg);
+        sc_->nReq = 0;
+        collect(lane);
+    }
+
     void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) override {
+        use_lane(2);
         const size_t per = (size_t)g_.nObj * nOS_;
         const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
         std::vector<EvalReq> rq(n);
@@ -1422,14 +1448,16 @@ public:
             rq[k].slot = slots[k];
             rq[k].alpha = alpha[k];
         }
-        std::vector<double> hP, hQ;
         for (int b = 0; b < n; b += perChunk) {
             const int m = std::min(perChunk, n - b);
             forward(&rq[b], m, /*useGamma=*/0);
-            hP.resize(2 * (size_t)m * per);
-            hQ.resize((size_t)m * per);
-            HIPCHK(hipMemcpyAsync(hP.data(), Pp_.p, hP.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
-            HIPCHK(hipMemcpyAsync(hQ.data(), Qp_.p, hQ.size() * sizeof(double), hipMemcpyDeviceToHost, st_));
+            sc_->hG.alloc(2 * (size_t)m * per);        // pinned: P then Q
+            sc_->hL.alloc((size_t)m * per);
+            const double* hP = sc_->hG.p;
+            const double* hQ = sc_->hL.p;
+            HIPCHK(hipMemcpyAsync(sc_->hG.p, sc_->Pp.p, 2 * (size_t)m * per * sizeof(double), hipMemcpyDeviceToHost,
+                                  st_));
+            HIPCHK(hipMemcpyAsync(sc_->hL.p, sc_->Qp.p, (size_t)m * per * sizeof(double), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipStreamSynchronize(st_));
             // expand the stored half to the full support: P(-u) = P(u)*, Q(-u) = Q(u)
             for (int k = 0; k < m * g_.nObj; ++k)
@@ -1441,10 +1469,11 @@ public:
                     Q0[dst] = hQ[src];
                 }
         }
-        collect();
+        collect(2);
     }
 
     void debug_get(const std::string& what, int slot, std::vector<double>& out) override {
+        use_lane(2);
         auto getc = [&](const C* p, size_t n) {
             std::vector<C> h(n);
             HIPCHK(hipMemcpy(h.data(), p, n * sizeof(C), hipMemcpyDeviceToHost));
@@ -1473,15 +1502,15 @@ public:
         HIPCHK(hipStreamSynchronize(st_));
         const size_t nI = (size_t)g_.nImg * nOS_, nO = (size_t)g_.nObj * nOS_;
         if (what == "D") { getc(D_.p + slot * nI, nI); expand(g_.nImg, true); }
-        else if (what == "S") { getc(S_.p + slot * nI, nI); expand(g_.nImg, true); }   // slot = request index
-        else if (what == "P") { getd(Pp_.p + slot * 2 * nO, 2 * nO); expand(g_.nObj, true); }
-        else if (what == "Q") { getd(Qp_.p + slot * nO, nO); expand(g_.nObj, false); }
+        else if (what == "S") { getc(sc_->S.p + slot * nI, nI); expand(g_.nImg, true); }   // slot = request index
+        else if (what == "P") { getd(sc_->Pp.p + slot * 2 * nO, 2 * nO); expand(g_.nObj, true); }
+        else if (what == "Q") { getd(sc_->Qp.p + slot * nO, nO); expand(g_.nObj, false); }
         else if (what == "ftsum") { getd(ftSum_.p + slot * nO, nO); expand(g_.nObj, false); }
         else if (what == "gamma") getd(gamma_.p + slot * g_.nObj, g_.nObj);
         else if (what == "phi") {
             const size_t n = (size_t)g_.nImg * nPS_;
             std::vector<T> h(n);
-            HIPCHK(hipMemcpy(h.data(), phi_.p + slot * n, n * sizeof(T), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(h.data(), sc_->phi.p + slot * n, n * sizeof(T), hipMemcpyDeviceToHost));
             out.assign(h.begin(), h.end());
         } else out.clear();
     }
@@ -1525,26 +1554,30 @@ private:
         f();
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(pr.second, st_));
-        evPending_.push_back({name, pr});
-        evUnits_.push_back(units);
+        evPending_.push_back({name, pr, units, lane_});
     }
 
-    void collect() {
+    // fold the timings of finished launches into the statistics: every pending launch
+    // (lane < 0) or those of one lane; waits for each launch's end event
+    void collect(int lane = -1) {
         if (evPending_.empty()) return;
-        HIPCHK(hipStreamSynchronize(st_));
-        for (size_t k = 0; k < evPending_.size(); ++k) {
-            auto& e = evPending_[k];
+        std::vector<PendingEv> keep;
+        for (auto& e : evPending_) {
+            if (lane >= 0 && e.lane != lane) {
+                keep.push_back(e);
+                continue;
+            }
+            HIPCHK(hipEventSynchronize(e.ev.second));
             float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
-            auto& s = stats_[e.first];
-            s.name = e.first;
-            s.ms += ms;
-            s.launches++;
-            s.units += evUnits_[k];
-            evPool_.push_back(e.second);
+            HIPCHK(hipEventElapsedTime(&ms, e.ev.first, e.ev.second));
+            auto& st = stats_[e.name];
+            st.name = e.name;
+            st.ms += ms;
+            st.launches++;
+            st.units += e.units;
+            evPool_.push_back(e.ev);
         }
-        evPending_.clear();
-        evUnits_.clear();
+        evPending_.swap(keep);
     }
 
     void fft2(C* tiles, int nTiles, int dir) {
@@ -1568,18 +1601,18 @@ private:
             });
             return;
         }
-        tiles_.alloc((size_t)nQ * N2);
+        sc_->tiles.alloc((size_t)nQ * N2);
         launch("k_field", nQ, [&] {
-            hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, phi, invPio_.p,
+            hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, sc_->tiles.p, phi, invPio_.p,
                                pupil_.p, qImg, imgScale_.p, (const int*)nullptr, nQ, N2, nPS_);
         });
-        fft2(tiles_.p, nQ, -1);
+        fft2(sc_->tiles.p, nQ, -1);
         launch("k_power", nQ, [&] {
-            hipLaunchKernelGGL(k_power<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, tiles_.p, (size_t)nQ * N2);
+            hipLaunchKernelGGL(k_power<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, sc_->tiles.p, (size_t)nQ * N2);
         });
-        fft2(tiles_.p, nQ, +1);
+        fft2(sc_->tiles.p, nQ, +1);
         launch("k_extract", nQ, [&] {
-            hipLaunchKernelGGL(k_extract<T>, dim3(nblk((size_t)nQ * nOS_)), dim3(256), 0, st_, Sout, tiles_.p,
+            hipLaunchKernelGGL(k_extract<T>, dim3(nblk((size_t)nQ * nOS_)), dim3(256), 0, st_, Sout, sc_->tiles.p,
                                osNat_.p, nQ, nOS_, N2, (T)(1.0 / N2));
         });
     }
@@ -1598,8 +1631,8 @@ private:
             hi = std::max(hi, rq[r].slot);
         }
         const size_t nInts = 6 * (size_t)nQ + n;
-        hInts_.alloc(nInts);
-        int* ints = hInts_.p;
+        sc_->hInts.alloc(nInts);
+        int* ints = sc_->hInts.p;
         for (int r = 0; r < n; ++r) {
             for (int j = 0; j < nImg; ++j) {
                 const int q = r * nImg + j;
@@ -1614,68 +1647,86 @@ private:
         }
         if (staged) {
             const size_t cnt = (size_t)(hi - lo + 1) * row;
-            alpha_.alloc(std::max(cnt, alpha_.n));
-            HIPCHK(hipMemcpyAsync(alpha_.p, hStage_.p + (size_t)lo * row, cnt * sizeof(double),
+            sc_->alpha.alloc(std::max(cnt, sc_->alpha.n));
+            HIPCHK(hipMemcpyAsync(sc_->alpha.p, hStage_.p + (size_t)lo * row, cnt * sizeof(double),
                                   hipMemcpyHostToDevice, st_));
         } else {
-            hAlpha_.alloc((size_t)nQ * K);
-            for (int r = 0; r < n; ++r) std::memcpy(hAlpha_.p + (size_t)r * row, rq[r].alpha, sizeof(double) * row);
-            alpha_.alloc((size_t)nQ * K);
-            HIPCHK(hipMemcpyAsync(alpha_.p, hAlpha_.p, (size_t)nQ * K * sizeof(double), hipMemcpyHostToDevice, st_));
+            sc_->hAlpha.alloc((size_t)nQ * K);
+            for (int r = 0; r < n; ++r) std::memcpy(sc_->hAlpha.p + (size_t)r * row, rq[r].alpha, sizeof(double) * row);
+            sc_->alpha.alloc((size_t)nQ * K);
+            HIPCHK(hipMemcpyAsync(sc_->alpha.p, sc_->hAlpha.p, (size_t)nQ * K * sizeof(double), hipMemcpyHostToDevice, st_));
         }
-        ints_.alloc(nInts);
-        HIPCHK(hipMemcpyAsync(ints_.p, ints, nInts * sizeof(int), hipMemcpyHostToDevice, st_));
-        qm_ = {ints_.p, ints_.p + nQ, ints_.p + 2 * nQ};
-        qReq_ = ints_.p + 3 * nQ;
-        reqSlot_ = ints_.p + 4 * nQ;
-        const int* qPhRow = ints_.p + 4 * nQ + n;
-        const int* qObj = ints_.p + 5 * nQ + n;
-        phi_.alloc((size_t)nQ * nPS_);
-        S_.alloc((size_t)nQ * nOS_);
+        sc_->ints.alloc(nInts);
+        HIPCHK(hipMemcpyAsync(sc_->ints.p, ints, nInts * sizeof(int), hipMemcpyHostToDevice, st_));
+        sc_->qm = {sc_->ints.p, sc_->ints.p + nQ, sc_->ints.p + 2 * nQ};
+        sc_->qReq = sc_->ints.p + 3 * nQ;
+        sc_->reqSlot = sc_->ints.p + 4 * nQ;
+        const int* qPhRow = sc_->ints.p + 4 * nQ + n;
+        const int* qObj = sc_->ints.p + 5 * nQ + n;
+        sc_->phi.alloc((size_t)nQ * nPS_);
+        sc_->S.alloc((size_t)nQ * nOS_);
         launch("k_phase_mfma", nQ, [&] {
-            hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_, phi_.p, B_.p,
-                               phiCh_.p, alpha_.p, qm_.arow, qPhRow, qObj, nQ, K, nPS_);
+            hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_, sc_->phi.p, B_.p,
+                               phiCh_.p, sc_->alpha.p, sc_->qm.arow, qPhRow, qObj, nQ, K, nPS_);
         });
-        otf_chain(phi_.p, qm_.img, nQ, S_.p);
+        otf_chain(sc_->phi.p, sc_->qm.img, nQ, sc_->S.p);
         const int nChunk = (nOS_ + 255) / 256;
-        Pp_.alloc((size_t)n * g_.nObj * nOS_ * 2);
-        Qp_.alloc((size_t)n * g_.nObj * nOS_);
-        part_.alloc((size_t)n * g_.nObj * nChunk);
+        sc_->Pp.alloc((size_t)n * g_.nObj * nOS_ * 2);
+        sc_->Qp.alloc((size_t)n * g_.nObj * nOS_);
+        sc_->part.alloc((size_t)n * g_.nObj * nChunk);
         launch("k_pq", nQ, [&] {
-            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, Pp_.p, Qp_.p, part_.p, S_.p, D_.p,
-                               ftSum_.p, gamma_.p, reqSlot_, chObj_.p, chFirst_.p, chCount_.p, wt_.p, g_.nCh, g_.nObj,
+            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, sc_->Pp.p, sc_->Qp.p, sc_->part.p, sc_->S.p, D_.p,
+                               ftSum_.p, gamma_.p, sc_->reqSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, g_.nCh, g_.nObj,
                                nImg, nOS_, useGamma);
         });
     }
 
-    void eval_chunk(EvalReq* rq, int n, int K, int nImg) {
+    // host vector -> pinned staging -> device, without waiting (the lane's staging is not
+    // reused before its evaluation has finished)
+    template <typename U>
+    void stage_async(DBuf<U>& d, HBuf<U>& h, const std::vector<U>& v) {
+        h.alloc(std::max<size_t>(v.size(), 1));
+        std::copy(v.begin(), v.end(), h.p);
+        d.alloc(std::max<size_t>(v.size(), 1));
+        if (!v.empty()) HIPCHK(hipMemcpyAsync(d.p, h.p, v.size() * sizeof(U), hipMemcpyHostToDevice, st_));
+    }
+
+    void use_lane(int lane) {
+        lane_ = lane;
+        sc_ = &lanes_[lane];
+    }
+
+    void eval_chunk_launch(EvalReq* rq, int n, int K, int nImg) {
         forward(rq, n, 1);
         const int nQ = n * nImg;
         const int nChunk = (nOS_ + 255) / 256;
-        L_.alloc(n);
+        sc_->L.alloc(n);
         launch("k_metric", [&] {
-            hipLaunchKernelGGL(k_metric, dim3(nblk(n, 64)), dim3(64), 0, st_, L_.p, part_.p, w_.p, n, g_.nObj, nChunk,
-                               1.0 / ((double)g_.s * g_.s));
+            hipLaunchKernelGGL(k_metric, dim3(nblk(n, 64)), dim3(64), 0, st_, sc_->L.p, sc_->part.p, w_.p, n, g_.nObj,
+                               nChunk, 1.0 / ((double)g_.s * g_.s));
         });
         bool anyGrad = false;
         for (int r = 0; r < n; ++r) anyGrad |= rq[r].wantGrad;
+        sc_->anyGrad = anyGrad;
         if (anyGrad) {
-            g_d_.alloc((size_t)nQ * K);
-            HIPCHK(hipMemsetAsync(g_d_.p, 0, (size_t)nQ * K * sizeof(double), st_));
+            sc_->g_d.alloc((size_t)nQ * K);
+            HIPCHK(hipMemsetAsync(sc_->g_d.p, 0, (size_t)nQ * K * sizeof(double), st_));
             gradients(rq, n, K, nImg);
         }
-        hL_.alloc(n);
-        HIPCHK(hipMemcpyAsync(hL_.p, L_.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
+        sc_->hL.alloc(n);
+        HIPCHK(hipMemcpyAsync(sc_->hL.p, sc_->L.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
         if (anyGrad) {
-            hG_.alloc((size_t)nQ * K);
-            HIPCHK(hipMemcpyAsync(hG_.p, g_d_.p, (size_t)nQ * K * sizeof(double), hipMemcpyDeviceToHost, st_));
+            sc_->hG.alloc((size_t)nQ * K);
+            HIPCHK(hipMemcpyAsync(sc_->hG.p, sc_->g_d.p, (size_t)nQ * K * sizeof(double), hipMemcpyDeviceToHost, st_));
         }
-        HIPCHK(hipStreamSynchronize(st_));
-        collect();
+        HIPCHK(hipEventRecord(sc_->done, st_));
+    }
+
+    void eval_chunk_finish(EvalReq* rq, int n, int K, int nImg) {
         for (int r = 0; r < n; ++r) {
-            rq[r].f = hL_.p[r];
+            rq[r].f = sc_->hL.p[r];
             if (rq[r].wantGrad && rq[r].g)
-                std::memcpy(rq[r].g, hG_.p + (size_t)r * nImg * K, sizeof(double) * nImg * K);
+                std::memcpy(rq[r].g, sc_->hG.p + (size_t)r * nImg * K, sizeof(double) * nImg * K);
         }
     }
 
@@ -1713,54 +1764,54 @@ private:
                 }
             }
             const int nG = (int)gIdx.size();
-            upload(gscale_, sc, st_);
-            upload(gIdx_, gIdx, st_);
-            v_.alloc((size_t)nG * nPS_);
+            stage_async(sc_->gscale, sc_->hGs, sc);
+            stage_async(sc_->gIdx, sc_->hGi, gIdx);
+            sc_->v.alloc((size_t)nG * nPS_);
             if constexpr (FUSED) {
                 launch("k_vogel_fused", nG, [&] {
-                    hipLaunchKernelGGL((k_vogel_fused<N, FNT>), dim3(nG), dim3(FNT), FLDS, st_, v_.p, S_.p, D_.p, Pp_.p,
-                                       Qp_.p, phi_.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, osyx_.p,
-                                       osyx_.p + nOS_, osyx_.p + 2 * nOS_, osyx_.p + 3 * nOS_, qm_, qReq_, imgObj_.p,
-                                       gIdx_.p, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
+                    hipLaunchKernelGGL((k_vogel_fused<N, FNT>), dim3(nG), dim3(FNT), FLDS, st_, sc_->v.p, sc_->S.p, D_.p, sc_->Pp.p,
+                                       sc_->Qp.p, sc_->phi.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, osyx_.p,
+                                       osyx_.p + nOS_, osyx_.p + 2 * nOS_, osyx_.p + 3 * nOS_, sc_->qm, sc_->qReq, imgObj_.p,
+                                       sc_->gIdx.p, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
                 });
             } else {
-                tiles_.alloc((size_t)nG * N2);
-                greal_.alloc((size_t)nG * N2);
+                sc_->tiles.alloc((size_t)nG * N2);
+                sc_->greal.alloc((size_t)nG * N2);
                 launch("k_build_g", nG, [&] {
-                    hipLaunchKernelGGL(k_build_g<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, tiles_.p, S_.p,
-                                       D_.p, Pp_.p, Qp_.p, invOs_.p, qm_, qReq_, imgObj_.p, gIdx_.p, nG, N2, nOS_, nImg,
+                    hipLaunchKernelGGL(k_build_g<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, sc_->tiles.p, sc_->S.p,
+                                       D_.p, sc_->Pp.p, sc_->Qp.p, invOs_.p, sc_->qm, sc_->qReq, imgObj_.p, sc_->gIdx.p, nG, N2, nOS_, nImg,
                                        g_.nObj);
                 });
-                fft2(tiles_.p, nG, +1);
+                fft2(sc_->tiles.p, nG, +1);
                 launch("k_save_re", nG, [&] {
-                    hipLaunchKernelGGL(k_save_re<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, greal_.p,
-                                       tiles_.p, (size_t)nG * N2, (T)(1.0 / N2));
+                    hipLaunchKernelGGL(k_save_re<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, sc_->greal.p,
+                                       sc_->tiles.p, (size_t)nG * N2, (T)(1.0 / N2));
                 });
                 launch("k_field", nG, [&] {
-                    hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, tiles_.p, phi_.p,
-                                       invPde_.p, pupil_.p, qm_.img, (const T*)nullptr, gIdx_.p, nG, N2, nPS_);
+                    hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, sc_->tiles.p, sc_->phi.p,
+                                       invPde_.p, pupil_.p, sc_->qm.img, (const T*)nullptr, sc_->gIdx.p, nG, N2, nPS_);
                 });
-                fft2(tiles_.p, nG, +1);
+                fft2(sc_->tiles.p, nG, +1);
                 launch("k_mul_re", nG, [&] {
-                    hipLaunchKernelGGL(k_mul_re<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, tiles_.p,
-                                       greal_.p, (size_t)nG * N2, (T)(1.0 / N2));
+                    hipLaunchKernelGGL(k_mul_re<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, sc_->tiles.p,
+                                       sc_->greal.p, (size_t)nG * N2, (T)(1.0 / N2));
                 });
-                fft2(tiles_.p, nG, -1);
+                fft2(sc_->tiles.p, nG, -1);
                 launch("k_extract_v", nG, [&] {
-                    hipLaunchKernelGGL(k_extract_v<T>, dim3(nblk((size_t)nG * nPS_)), dim3(256), 0, st_, v_.p,
-                                       tiles_.p, phi_.p, pde_.p, pupil_.p, gIdx_.p, nG, nPS_, N2, extraPupil);
+                    hipLaunchKernelGGL(k_extract_v<T>, dim3(nblk((size_t)nG * nPS_)), dim3(256), 0, st_, sc_->v.p,
+                                       sc_->tiles.p, sc_->phi.p, pde_.p, pupil_.p, sc_->gIdx.p, nG, nPS_, N2, extraPupil);
                 });
             }
             {
                 const int Kt = std::min(K, (nEn + 15) & ~15);      // mode columns actually computed
-                gpart_.alloc((size_t)GR_SLICES * nG * std::max(Kt, 1));
+                sc_->gpart.alloc((size_t)GR_SLICES * nG * std::max(Kt, 1));
                 launch("k_grad_mfma", nG, [&] {
-                    hipLaunchKernelGGL(k_grad_mfma<T>, dim3((nG + 15) / 16, GR_SLICES), dim3(256), 0, st_, gpart_.p,
-                                       v_.p, B_.p, qm_, imgObj_.p, gIdx_.p, nG, K, Kt, nPS_, nEn);
+                    hipLaunchKernelGGL(k_grad_mfma<T>, dim3((nG + 15) / 16, GR_SLICES), dim3(256), 0, st_, sc_->gpart.p,
+                                       sc_->v.p, B_.p, sc_->qm, imgObj_.p, sc_->gIdx.p, nG, K, Kt, nPS_, nEn);
                 });
                 launch("k_grad_sum", [&] {
-                    hipLaunchKernelGGL(k_grad_sum, dim3(nblk((size_t)nG * K)), dim3(256), 0, st_, g_d_.p, gpart_.p,
-                                       gscale_.p, gIdx_.p, nG, K, Kt, nEn);
+                    hipLaunchKernelGGL(k_grad_sum, dim3(nblk((size_t)nG * K)), dim3(256), 0, st_, sc_->g_d.p, sc_->gpart.p,
+                                       sc_->gscale.p, sc_->gIdx.p, nG, K, Kt, nEn);
                 });
             }
         }
@@ -1799,32 +1850,32 @@ private:
             ints.insert(ints.end(), mode.begin() + b, mode.begin() + b + m);
             ints.insert(ints.end(), obj.begin() + b, obj.begin() + b + m);
             ints.insert(ints.end(), pimg.begin() + b, pimg.begin() + b + m);
-            upload(fdInts_, ints, st_);
-            upload(fdStep_, step.data() + b, m, st_);
-            phiFd_.alloc((size_t)m * nPS_);
-            Sfd_.alloc((size_t)m * nOS_);
-            fdOut_.alloc(m);
+            upload(sc_->fdInts, ints, st_);
+            upload(sc_->fdStep, step.data() + b, m, st_);
+            sc_->phiFd.alloc((size_t)m * nPS_);
+            sc_->Sfd.alloc((size_t)m * nOS_);
+            sc_->fdOut.alloc(m);
             launch("k_phase_pert", [&] {
-                hipLaunchKernelGGL(k_phase_pert<T>, dim3(nblk((size_t)m * nPS_)), dim3(256), 0, st_, phiFd_.p, phi_.p,
-                                   B_.p, fdInts_.p, fdInts_.p + m, fdInts_.p + 2 * m, fdStep_.p, m, K, nPS_);
+                hipLaunchKernelGGL(k_phase_pert<T>, dim3(nblk((size_t)m * nPS_)), dim3(256), 0, st_, sc_->phiFd.p, sc_->phi.p,
+                                   B_.p, sc_->fdInts.p, sc_->fdInts.p + m, sc_->fdInts.p + 2 * m, sc_->fdStep.p, m, K, nPS_);
             });
-            otf_chain(phiFd_.p, fdInts_.p + 3 * m, m, Sfd_.p);
+            otf_chain(sc_->phiFd.p, sc_->fdInts.p + 3 * m, m, sc_->Sfd.p);
             launch("k_metric_change", [&] {
-                hipLaunchKernelGGL(k_metric_change<T>, dim3(m), dim3(256), 0, st_, fdOut_.p, Sfd_.p, S_.p, D_.p, Pp_.p,
-                                   Qp_.p, fdInts_.p, qm_, qReq_, imgObj_.p, wt_.p, nOS_, nImg, g_.nObj, 1.0 / N2);
+                hipLaunchKernelGGL(k_metric_change<T>, dim3(m), dim3(256), 0, st_, sc_->fdOut.p, sc_->Sfd.p, sc_->S.p, D_.p, sc_->Pp.p,
+                                   sc_->Qp.p, sc_->fdInts.p, sc_->qm, sc_->qReq, imgObj_.p, wt_.p, nOS_, nImg, g_.nObj, 1.0 / N2);
             });
-            HIPCHK(hipMemcpyAsync(res.data() + b, fdOut_.p, m * sizeof(double), hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipMemcpyAsync(res.data() + b, sc_->fdOut.p, m * sizeof(double), hipMemcpyDeviceToHost, st_));
         }
         HIPCHK(hipStreamSynchronize(st_));
         // g = w * (dl / step * w); merge into the device gradient buffer
         std::vector<double> hg((size_t)n * nImg * K);
-        HIPCHK(hipMemcpy(hg.data(), g_d_.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(hg.data(), sc_->g_d.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost));
         for (int k = 0; k < nP; ++k) {
             const double stp = (double)step[k];
             const double w = wgt[k];
             hg[(size_t)dst[k].first * K + dst[k].second] = (w == 0.0) ? 0.0 : w * (res[k] / stp * w);
         }
-        HIPCHK(hipMemcpy(g_d_.p, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(sc_->g_d.p, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice));
     }
 
     std::vector<int> otf_support_of(const double* pupil) {
@@ -1857,24 +1908,47 @@ private:
     bool timing_ = false;
     std::map<std::string, KernelStat> stats_;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evPool_;
-    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> evPending_;
-    std::vector<long long> evUnits_;
+    struct PendingEv {
+        const char* name;
+        std::pair<hipEvent_t, hipEvent_t> ev;
+        long long units;
+        int lane;
+    };
+    std::vector<PendingEv> evPending_;
     std::vector<int> ps_, os_;
     int nPS_ = 0, nOS_ = 0, nOSfull_ = 0;   // nOS_: stored (half) support, nOSfull_: full support
     std::vector<int> fullMap_;               // full support point -> half index h, or -1-h for a mirror
     double area_ = 0;
     std::vector<double> otfNorm_;
-    QMap qm_{};
-    const int* qReq_ = nullptr;
-    const int* reqSlot_ = nullptr;
-    DBuf<C> tw_, tiles_, D_, S_, Sfd_;
-    DBuf<T> phi_, phiFd_, B_, phiCh_, pupil_, imgScale_, v_, greal_, fdStep_;
-    HBuf<double> hStage_, hAlpha_, hL_, hG_;
-    HBuf<int> hInts_;
-    DBuf<double> Pp_, Qp_, part_, L_, g_d_, alpha_, ftSum_, gamma_, win_, gscale_, fdOut_, w_, statOut_, gpart_, wt_;
-    DBuf<int> ints_, fdInts_, gIdx_, invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_,
-        pyx_, osyx_, hread_, invPs_;
+    // resident: basis, supports, per-patch frames, D, ftSum, gamma, channel phases
+    DBuf<C> tw_, D_;
+    DBuf<T> B_, phiCh_, pupil_, imgScale_;
+    HBuf<double> hStage_;
+    DBuf<double> ftSum_, gamma_, win_, w_, wt_;
+    DBuf<int> invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_, pyx_, osyx_, hread_,
+        invPs_;
     DBuf<float> rawF_;
+    bool winSet_ = false;
+    // per-round work buffers: lanes 0 and 1 take alternate halves of a batch (evaluate_begin
+    // / evaluate_end), lane 2 serves the synchronous calls
+    struct Scratch {
+        DBuf<C> tiles, S, Sfd;
+        DBuf<T> phi, phiFd, v, greal, fdStep;
+        DBuf<double> Pp, Qp, part, L, g_d, alpha, gscale, fdOut, statOut, gpart;
+        DBuf<int> ints, fdInts, gIdx, initInts;
+        HBuf<double> hAlpha, hL, hG, hGs;
+        HBuf<int> hInts, hGi, hInit;
+        QMap qm{};
+        const int* qReq = nullptr;
+        const int* reqSlot = nullptr;
+        hipEvent_t done = nullptr;       // recorded after the lane's result copies
+        int nReq = 0;
+        bool anyGrad = false;
+    };
+    Scratch lanes_[3];
+    Scratch* sc_ = &lanes_[2];
+    int lane_ = 2;
+    int laneFirst_[3] = {0, 0, 0};
 };
 
 std::unique_ptr<DeviceEvaluator> make_device_evaluator(const Geometry& g, int precision, int maxSlots, int maxBatch,

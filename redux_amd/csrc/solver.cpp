@@ -226,20 +226,26 @@ double noise_level(const cd* ft, int ny, int nx) {
     return cnt ? std::sqrt(np / ((double)cnt * ny * nx)) : np;
 }
 
-void smooth2(double* d, int ny, int nx, int n) {             // image::smooth (utils.cpp:304-328)
-    std::vector<double> t((size_t)ny * nx);
-    for (int y = 0; y < ny; ++y) {
-        int yl = std::max(y - n, 0), yh = std::min(y + n, ny);
+// image::smooth (utils.cpp:304-328): mean over the half-open window [y-n, y+n) x
+// [x-n, x+n) clipped to the array; summed separably (rows, then columns)
+void smooth2(double* d, int ny, int nx, int n) {
+    std::vector<double> h((size_t)ny * nx);
+    for (int y = 0; y < ny; ++y)
         for (int x = 0; x < nx; ++x) {
-            int xl = std::max(x - n, 0), xh = std::min(x + n, nx);
+            const int xl = std::max(x - n, 0), xh = std::min(x + n, nx);
             double acc = 0;
-            int c = 0;
-            for (int yy = yl; yy < yh; ++yy)
-                for (int xx = xl; xx < xh; ++xx) { acc += d[yy * nx + xx]; c++; }
-            t[(size_t)y * nx + x] = acc / c;
+            for (int xx = xl; xx < xh; ++xx) acc += d[(size_t)y * nx + xx];
+            h[(size_t)y * nx + x] = acc;
+        }
+    for (int y = 0; y < ny; ++y) {
+        const int yl = std::max(y - n, 0), yh = std::min(y + n, ny);
+        for (int x = 0; x < nx; ++x) {
+            const int xl = std::max(x - n, 0), xh = std::min(x + n, nx);
+            double acc = 0;
+            for (int yy = yl; yy < yh; ++yy) acc += h[(size_t)yy * nx + x];
+            d[(size_t)y * nx + x] = acc / ((yh - yl) * (xh - xl));
         }
     }
-    std::copy(t.begin(), t.end(), d);
 }
 
 // ScharmerFilter (utils.cpp:334-392) with connectedRegion (utils.cpp:49-55, 294-300)
@@ -418,7 +424,7 @@ void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
         c->gamma[p] = gam;
         c->dev->set_gamma(p, gam.data());
     }
-    c->dev->init_images(imgs, c->window);
+    c->dev->init_images(imgs, c->window, 2);
 }
 
 // SubImage::adjustShifts (subimage.cpp:444-480) for one image; returns true if shifted
@@ -485,8 +491,13 @@ void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const 
         if (!c->prm.noFilter) scharmer(ao.data(), dn.data(), N, N, nv, 0.90 * c->st.freqCutoff[o]);
         std::vector<cd> t((size_t)N * N);
         reorder_into(ao.data(), N, N, t.data());
-        fft2d(t.data(), N, N, +1);
+        // inverse 2-D transform; only the columns of the centred s x s crop are needed
         const int off = (N - s) / 2;
+        {
+            HostFFT f(N);
+            for (int y = 0; y < N; ++y) f.run(t.data() + (size_t)y * N, 1, +1);
+            for (int x = off; x < off + s; ++x) f.run(t.data() + x, N, +1);
+        }
         float* out = imgOut + (size_t)o * s * s;
         for (int y = 0; y < s; ++y)
             for (int x = 0; x < s; ++x)
@@ -800,13 +811,13 @@ Task<int> run_patch(PatchRun& r) {
 }
 
 // re-initialisation of the shifted objects' images (Object::reInitialize, doReset = false)
-void shift_inits(mfbd_ctx* c, std::vector<PatchRun*>& rs) {
+void shift_inits(mfbd_ctx* c, std::vector<PatchRun*>& rs, int lane) {
     std::vector<ImgInit> imgs;
     for (auto* r : rs)
         for (int j = 0; j < c->g.nImg; ++j)
             if (r->shiftedObjs[c->st.imgObj[j]])
                 imgs.push_back({r->p, j, c->shifts[r->p][2 * j], c->shifts[r->p][2 * j + 1]});
-    c->dev->init_images(imgs, c->window);
+    c->dev->init_images(imgs, c->window, lane);
 }
 
 int check_ready(mfbd_ctx* c) {
@@ -1111,14 +1122,24 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             if (!c->pool) c->pool = std::make_unique<Pool>(
                 (int)std::min<unsigned>(std::max(2u, std::thread::hardware_concurrency()), 16u) - 1);
             c->pool->run((int)tasks.size(), [&](int k) { tasks[k].h.resume(); });
-            while (true) {
-                // batched re-initialisations after a stage
-                std::vector<PatchRun*> inits;
-                for (auto& r : runs)
-                    if (r->wantShiftInit) inits.push_back(r.get());
-                if (!inits.empty()) {
+            // Two lanes take the two contiguous halves of the batch.  While the device
+            // evaluates one half, the host resumes the other half's patches (their CG /
+            // line-search steps) and enqueues their next requests, so neither side waits
+            // for the other.  Each patch still sees exactly its own sequence of requests.
+            const int nLanes = nb >= 2 ? 2 : 1;
+            const int split = (nb + 1) / 2;
+            auto laneOf = [&](int k) { return nLanes == 1 ? 0 : (k < split ? 0 : 1); };
+            std::vector<EvalReq> rq[2];
+            std::vector<PatchRun*> pend[2];
+            bool inflight[2] = {false, false};
+            auto do_inits = [&](int e) {         // batched re-initialisations after a stage
+                while (true) {
+                    std::vector<PatchRun*> inits;
+                    for (int k = 0; k < nb; ++k)
+                        if (laneOf(k) == e && runs[k]->wantShiftInit) inits.push_back(runs[k].get());
+                    if (inits.empty()) return;
                     auto t0 = clk::now();
-                    shift_inits(c, inits);
+                    shift_inits(c, inits, e);
                     c->hostStats[2].ms += since(t0);
                     c->hostStats[2].launches++;
                     c->hostStats[2].units += (long long)inits.size();
@@ -1127,34 +1148,56 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                         auto h = r->initH;
                         h.resume();
                     }
-                    continue;
                 }
-                std::vector<PatchRun*> pend;
-                for (auto& r : runs)
-                    if (r->ev.pending) pend.push_back(r.get());
-                if (pend.empty()) break;
-                std::vector<EvalReq> rq(pend.size());
-                for (size_t k = 0; k < pend.size(); ++k) {
-                    rq[k].slot = pend[k]->p;
-                    rq[k].alpha = pend[k]->ev.alphaEff;
-                    rq[k].wantGrad = pend[k]->ev.wantGrad;
-                    rq[k].gradMethod = pend[k]->ev.method;
-                    rq[k].nEnabled = pend[k]->ev.nEnabled;
-                    rq[k].g = pend[k]->ev.gAlpha.data();
+            };
+            auto launch_lane = [&](int e) {
+                pend[e].clear();
+                for (int k = 0; k < nb; ++k)
+                    if (laneOf(k) == e && runs[k]->ev.pending) pend[e].push_back(runs[k].get());
+                if (pend[e].empty()) return false;
+                rq[e].assign(pend[e].size(), EvalReq{});
+                for (size_t k = 0; k < pend[e].size(); ++k) {
+                    EvalSlot& ev = pend[e][k]->ev;
+                    rq[e][k].slot = pend[e][k]->p;
+                    rq[e][k].alpha = ev.alphaEff;
+                    rq[e][k].wantGrad = ev.wantGrad;
+                    rq[e][k].gradMethod = ev.method;
+                    rq[e][k].nEnabled = ev.nEnabled;
+                    rq[e][k].g = ev.gAlpha.data();
                 }
                 auto t0 = clk::now();
-                c->dev->evaluate(rq);
+                c->dev->evaluate_begin(e, rq[e]);
                 c->hostStats[0].ms += since(t0);
                 c->hostStats[0].launches++;
-                c->hostStats[0].units += (long long)pend.size();
-                for (size_t k = 0; k < pend.size(); ++k) {
-                    pend[k]->ev.f = rq[k].f;
-                    pend[k]->ev.pending = false;
+                c->hostStats[0].units += (long long)pend[e].size();
+                inflight[e] = true;
+                return true;
+            };
+            auto finish_lane = [&](int e) {
+                auto t0 = clk::now();
+                c->dev->evaluate_end(e, rq[e]);
+                c->hostStats[0].ms += since(t0);
+                for (size_t k = 0; k < pend[e].size(); ++k) {
+                    pend[e][k]->ev.f = rq[e][k].f;
+                    pend[e][k]->ev.pending = false;
                 }
                 t0 = clk::now();
-                c->pool->run((int)pend.size(), [&](int k) { pend[k]->ev.h.resume(); });
+                c->pool->run((int)pend[e].size(), [&](int k) { pend[e][k]->ev.h.resume(); });
                 c->hostStats[1].ms += since(t0);
                 c->hostStats[1].launches++;
+                inflight[e] = false;
+            };
+            while (true) {
+                bool any = false;
+                for (int e = 0; e < nLanes; ++e) {
+                    if (inflight[e]) {
+                        finish_lane(e);
+                        any = true;
+                    }
+                    do_inits(e);
+                    any |= launch_lane(e);
+                }
+                if (!any) break;
             }
             for (size_t k = 0; k < tasks.size(); ++k) {
                 if (!tasks[k].h.done()) throw std::runtime_error("patch state machine stalled");
@@ -1205,6 +1248,7 @@ int mfbd_debug_get(mfbd_ctx* c, const char* what, int p, double* out, int* count
         else if (!strcmp(what, "otfnorm")) v = c->otfNorm;
         else if (!strcmp(what, "window")) v = c->window;
         else if (!strcmp(what, "shifts")) v.assign(c->shifts.at(p).begin(), c->shifts.at(p).end());
+        else if (!strcmp(what, "sizes") && c->dev) v = {(double)c->dev->nPS(), (double)c->dev->nOS(), (double)c->dev->nOSStored()};
         else if (!strcmp(what, "tiltoff")) v = c->tiltOff.at(p);
         else if (c->dev) c->dev->debug_get(what, p, v);
         *count = (int)v.size();

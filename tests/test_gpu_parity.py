@@ -129,3 +129,26 @@ def test_batched_equals_one_at_a_time(golden):
         assert np.array_equal(a["alpha"], b["alpha"])
         assert np.array_equal(a["img"], b["img"])
         assert a["nF"] == b["nF"] and a["iterations"] == b["iterations"]
+
+
+def test_repeated_solve_on_one_context_is_identical(golden):
+    """mfbd_solve restarts from zero alphas and re-initialises the frames, so a second
+    solve on the same context repeats the first exactly (both precisions)."""
+    pb, z = golden
+    from redux_amd.momfbd import Solver
+    rng = np.random.default_rng(9)
+    pbs = []
+    for k in range(3):
+        p = dict(pb)
+        p["data"] = rng.poisson(np.clip(pb["data"], 0, None)).astype(np.float32)
+        pbs.append(p)
+    for prec in ("fp64", "fp32"):
+        sv = Solver(pb, precision=prec, max_batch=3)
+        sv.load(pbs)
+        a = sv.run()
+        b = sv.run()
+        sv.close()
+        for x, y in zip(a, b):
+            assert np.array_equal(x["alpha"], y["alpha"]), prec
+            assert np.array_equal(x["img"], y["img"]), prec
+            assert x["nF"] == y["nF"] and x["finalMetric"] == y["finalMetric"], prec

@@ -946,6 +946,116 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
     }
 }
 
+// FP64 OTF for N = 2P whose tile does not fit in LDS (a 128 x 128 complex double tile
+// is 256 KB): the same algorithm as k_otf_fused in three kernels with one N/2 x N
+// complex intermediate per image in HBM (half a tile), written and read once.
+//   k_otf64_rows: the P = N/2 pupil rows, field a = scale pupil exp(i phi) and the
+//                 forward row FFTs (first pass pruned to the pupil columns)
+//   k_otf64_cols: forward column FFTs of the N/2 non-zero rows (first pass pruned),
+//                 the last pass writing the packed power spectrum |F|^2(y) + i|F|^2(y+N/2)
+//                 in place of its input
+//   k_otf64_inv:  one image per workgroup: inverse FFTs of the packed rows, split into
+//                 half spectra, inverse FFTs of the N/2+1 half-spectrum columns, and the
+//                 stored support points
+template <int N, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_otf64_rows(cx<double>* __restrict__ R, const double* __restrict__ phi,
+                                                            const int* __restrict__ invPs,
+                                                            const double* __restrict__ pupil,
+                                                            const int* __restrict__ qImg,
+                                                            const double* __restrict__ imgScale,
+                                                            const cx<double>* __restrict__ tw, int nPS) {
+    constexpr int P = N / 2, LD = N + 1, NT = LR * N / 16;
+    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
+    __shared__ cx<double> t[LR * LD];
+    const int q = blockIdx.x / (P / LR), r0 = (blockIdx.x % (P / LR)) * LR;
+    const double sc = imgScale[qImg[q]];
+    const double* ph = phi + (size_t)q * nPS;
+    for (int e = threadIdx.x; e < LR * P; e += NT) {
+        const int r = e / P, x = e % P;
+        const int k = invPs[(r0 + r) * P + x];
+        cx<double> val = {0.0, 0.0};
+        if (k >= 0) {
+            double sn, cs;
+            sincos(ph[k], &sn, &cs);
+            const double m = pupil[k] * sc;
+            val = {m * cs, m * sn};
+        }
+        t[r * LD + x + N / 4] = val;
+    }
+    __syncthreads();
+    spass_first_mid<double, N, RF, LR, NT, 1>(t, LD, 1);
+    spass<double, N, RL, RF, -1, LR, NT, 1>(t, LD, 1, tw);
+    cx<double>* out = R + ((size_t)q * P + r0) * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) out[e] = t[(e / N) * LD + e % N];
+}
+
+template <int N, int LC>
+__global__ void __launch_bounds__(LC * N / 16) k_otf64_cols(cx<double>* __restrict__ R, const cx<double>* __restrict__ tw) {
+    constexpr int P = N / 2, NT = LC * N / 16;
+    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
+    __shared__ cx<double> t[N * LC];                       // element (row y, column c) at y * LC + c
+    const int q = blockIdx.x / (N / LC), c0 = (blockIdx.x % (N / LC)) * LC;
+    cx<double>* img = R + (size_t)q * P * N;
+    for (int e = threadIdx.x; e < P * LC; e += NT) {
+        const int y = e / LC, c = e % LC;
+        t[(y + N / 4) * LC + c] = img[(size_t)y * N + c0 + c];   // the pupil rows sit at [N/4, 3N/4)
+    }
+    __syncthreads();
+    spass_first_mid<double, N, RF, LC, NT, 1>(t, 1, LC);
+    spass_last_power<double, N, RL, LC, NT, 1>(t, 1, LC, tw, t, 1, LC);
+    for (int e = threadIdx.x; e < P * LC; e += NT) {
+        const int y = e / LC, c = e % LC;
+        img[(size_t)y * N + c0 + c] = t[y * LC + c];
+    }
+}
+
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_otf64_inv(cx<double>* __restrict__ S, const cx<double>* __restrict__ R,
+                                                  const int* __restrict__ hread, const cx<double>* __restrict__ tw,
+                                                  int nOS) {
+    constexpr int P = N / 2, LD = N + 1, HC = N / 2 + 1;
+    constexpr int PER = (P * HC + NT - 1) / NT;
+    extern __shared__ cx<double> lds64[];
+    cx<double>* t = lds64;                                 // packed rows, stride LD; then N x HC half spectra
+    const int q = blockIdx.x;
+    const cx<double>* img = R + (size_t)q * P * N;
+    for (int e = threadIdx.x; e < P * N; e += NT) t[(e / N) * LD + e % N] = img[e];
+    __syncthreads();
+    fft_lines<double, N, +1, P, NT, 1>(t, LD, 1, tw);
+    // split every packed row into the half spectra of its two real rows; all reads first,
+    // since the half-spectrum layout reuses the same memory
+    cx<double> A[PER], Bv[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (e < P * HC) {
+            const int y = e / HC, k = e % HC;
+            const cx<double> z = t[y * LD + k], zm = t[y * LD + (N - k) % N];
+            A[i] = {0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y)};
+            Bv[i] = {0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x)};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (e < P * HC) {
+            const int y = e / HC, k = e % HC;
+            t[y * HC + k] = A[i];
+            t[(y + P) * HC + k] = Bv[i];
+        }
+    }
+    __syncthreads();
+    fft_lines<double, N, +1, HC, NT, 1>(t, 1, HC, tw);
+    constexpr double inv = 1.0 / ((double)N * N);
+    cx<double>* Sq = S + (size_t)q * nOS;
+    for (int u = threadIdx.x; u < nOS; u += NT) {
+        const int code = hread[u];
+        const cx<double> v = t[code >= 0 ? code : -1 - code];
+        Sq[u] = {v.x * inv, (code >= 0 ? v.y : -v.y) * inv};
+    }
+}
+
 // Vogel / adjoint weights (subimage.cpp:348-390) for one image:
 //   gl = IFFT(G')/N^2 with G' = (PQ D - PS S)/QS de-centred, hj = IFFT(PF')/N^2,
 //   t = FFT(hj * Re gl), v_i = Im(conj(PF_i) t(pde_i)) [* pupil_i]
@@ -1104,6 +1214,11 @@ public:
     static constexpr int FNT = N == 128 ? 1024 : 256;
     static constexpr size_t FLDS = (size_t)Tile<N>::SIZE * sizeof(cx<float>);
     static constexpr size_t OLDS = ((size_t)(N / 2) * Tile<N>::LD + (size_t)N * (N / 2 + 1)) * sizeof(cx<float>);
+    // FP64 three-kernel OTF: one image per workgroup in the last kernel, whose LDS holds
+    // max(N/2 x (N+1), N x (N/2+1)) complex doubles
+    static constexpr bool SPLIT64 = std::is_same_v<T, double> && N == 128;
+    static constexpr int NT64 = 1024;
+    static constexpr size_t ILDS = (size_t)std::max((N / 2) * (N + 1), N * (N / 2 + 1)) * sizeof(cx<double>);
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device) : g_(g), maxSlots_(maxSlots),
                                                                          maxBatch_(maxBatch) {
@@ -1238,6 +1353,10 @@ public:
                                        (int)OLDS));
             HIPCHK(hipFuncSetAttribute((const void*)k_vogel_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)FLDS));
+        }
+        if constexpr (SPLIT64) {
+            HIPCHK(hipFuncSetAttribute((const void*)k_otf64_inv<N, NT64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)ILDS));
         }
         HIPCHK(hipFuncSetAttribute((const void*)k_noise, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)((128 * 128 + 6 * 128) * sizeof(double))));
@@ -1598,6 +1717,23 @@ private:
             launch("k_otf_fused", nQ, [&] {
                 hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(nQ), dim3(FNT), OLDS, st_, Sout, phi, invPs_.p,
                                    pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_);
+            });
+            return;
+        }
+        if constexpr (SPLIT64) {
+            sc_->tiles.alloc((size_t)nQ * N2 / 2);
+            constexpr int LR = 16, LC = 16;
+            launch("k_otf64_rows", nQ, [&] {
+                hipLaunchKernelGGL((k_otf64_rows<N, LR>), dim3(nQ * (N / 2 / LR)), dim3(LR * N / 16), 0, st_,
+                                   sc_->tiles.p, phi, invPs_.p, pupil_.p, qImg, imgScale_.p, tw_.p, nPS_);
+            });
+            launch("k_otf64_cols", nQ, [&] {
+                hipLaunchKernelGGL((k_otf64_cols<N, LC>), dim3(nQ * (N / LC)), dim3(LC * N / 16), 0, st_,
+                                   sc_->tiles.p, tw_.p);
+            });
+            launch("k_otf64_inv", nQ, [&] {
+                hipLaunchKernelGGL((k_otf64_inv<N, NT64>), dim3(nQ), dim3(NT64), ILDS, st_, Sout, sc_->tiles.p,
+                                   hread_.p, tw_.p, nOS_);
             });
             return;
         }

@@ -100,6 +100,20 @@ def run_step(sv, restore=True):
     return res, time.perf_counter() - t0
 
 
+def measured_traffic():
+    """HBM bytes per image of the hot kernels, measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (scaled by a plain copy of known size) and committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", "k_otf_fused_cfg2_traffic.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic-json:"):
+                    return json.loads(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return {}
+
+
 def roofline(stats, N, nPS, nH):
     """Dominant kernel by total time; achieved = algorithmic bytes per launch / mean
     launch time, over every launch of the batched FP32 context (warmup and timed steps).  S, D, P and Q live on the
@@ -122,8 +136,10 @@ def roofline(stats, N, nPS, nH):
         units_per_launch = s["units"] / s["launches"]
         bytes_per_launch = per_unit[name] * units_per_launch
         ach = bytes_per_launch / (mean_ms * 1e-3) / 1e9
+        tr = measured_traffic().get(name)
         out.update(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                   frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, launches=s["launches"],
+                   frac=round(ach / HBM_PEAK_GBS, 4),
+                   traffic=None if tr is None else int(tr * units_per_launch), launches=s["launches"],
                    mean_launch_ms=round(mean_ms, 4), images_per_launch=round(units_per_launch, 1),
                    bytes_per_image=per_unit[name], bytes_per_launch=int(bytes_per_launch))
         if name == "k_otf_fused":
@@ -179,6 +195,40 @@ def cpu_baseline(pbs, evals_f, evals_g, seconds, threads):
                         f"of the FP64 solve)"))
 
 
+COPY_PROBE_BYTES = 256 << 20
+
+
+def pmc_probe():
+    """Workload for the rocprofv3 --pmc runs behind `traffic`: three batched FP32 metric
+    evaluations of 32 config-2 patches (k_otf_fused at 1536 images per launch), then
+    three device-to-device copies of COPY_PROBE_BYTES through the HIP runtime, the
+    plain copy that calibrates FETCH_SIZE / WRITE_SIZE."""
+    import ctypes as C
+    from redux_amd import synth
+    su, pbs = synth.build_field_batch(synth.CFG2_PATCH, 32, seed=7)
+    from redux_amd.momfbd import Solver
+    sv = Solver(pbs[0], precision="fp32", max_batch=len(pbs))
+    sv.load(pbs)
+    sv.init()
+    a = np.stack([p["truth"]["alpha"] * 0.5 for p in pbs])
+    for _ in range(3):
+        sv.metric(a)
+    sv.close()
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(src), C.c_size_t(COPY_PROBE_BYTES)) or hip.hipMalloc(C.byref(dst), C.c_size_t(COPY_PROBE_BYTES)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(src, 1, C.c_size_t(COPY_PROBE_BYTES))
+    for _ in range(3):
+        if hip.hipMemcpy(dst, src, C.c_size_t(COPY_PROBE_BYTES), 3):      # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(json.dumps({"probe": "pmc", "images_per_otf_launch": 32 * len(pbs[0]["imgCh"]),
+                      "copy_bytes": COPY_PROBE_BYTES}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,7 +239,12 @@ def main():
     ap.add_argument("--gradient", choices=["reference", "adjoint"], default="reference")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="run the short workload profiled for HBM traffic (FETCH_SIZE / WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.pmc_probe:
+        pmc_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: patches are independent and shard across GPUs with no "
               "collective (see DESIGN.md); run with --gpus 1", file=sys.stderr)

@@ -184,3 +184,8 @@ CFG1 = dict(s=64, n_frames=8, n_objects=1, modes=list(range(2, 22)), basis="zern
 CFG2_PATCH = dict(s=88, n_frames=24, n_objects=2, modes=list(range(2, 52)), basis="kl", d_r0=20.0,
                   mean_counts=2000.0, diversity=[0.0, math.pi / math.sqrt(3.0)], margin=5,
                   pupil_diameter=64.0, settings=dict(FTOL=1e-3))
+# Config 5 patch: 128 x 128 on a 256 x 256 OTF grid (128-pixel pupil), 150 KL modes,
+# 2 objects x 100 frames (D/r0 and the diversity as in config 2).
+CFG5_PATCH = dict(s=128, n_frames=100, n_objects=2, modes=list(range(2, 152)), basis="kl", d_r0=20.0,
+                  mean_counts=2000.0, diversity=[0.0, math.pi / math.sqrt(3.0)], margin=5,
+                  pupil_diameter=128.0, settings=dict(FTOL=1e-3))

@@ -285,6 +285,14 @@ def main():
     sv.close()
     nRuns = args.warmup + args.steps
     t32 = max(times)
+
+    # mixed precision (BF16 mode GEMM): one solve on its own context
+    sv = make_solver(pbs, "mixed", len(pbs), args.gradient)
+    resmx, tmx = run_step(sv)
+    sv.close()
+    log(f"mixed batched: {tmx:.2f}s")
+    errs_mx = np.array([np.linalg.norm(a["img"].astype(np.float64) - b["img"]) / np.linalg.norm(b["img"])
+                        if a["status"] == 0 else np.inf for a, b in zip(resmx, ref)])
     errs = []
     for a, b in zip(res32, ref):
         e = np.linalg.norm(a["img"].astype(np.float64) - b["img"]) / np.linalg.norm(b["img"])
@@ -316,6 +324,11 @@ def main():
         "fp32_obj_rel_err_median": float(np.median(errs[np.isfinite(errs)])) if np.isfinite(errs).any() else None,
         "fp32_obj_rel_err_max": float(np.max(errs)),
         "fp64_batched_patches_per_s": round(len(pbs) / t64b, 4),
+        "fp64_batched_converged": len(pbs),
+        "mixed_bf16_batched_patches_per_s": round(len(pbs) / tmx, 4),
+        "mixed_obj_rel_err_median": float(np.median(errs_mx[np.isfinite(errs_mx)])) if np.isfinite(errs_mx).any()
+        else None,
+        "mixed_converged": int(np.sum(errs_mx <= 1e-4)),
         "fp64_plain_patches_per_s": round(plain_rate, 4),
         "speedup_fp32_batched_vs_fp64_plain": round((len(pbs) / t32) / plain_rate, 2),
         "evals_per_patch": {"f": ev_f, "grad": ev_g},

@@ -35,6 +35,8 @@ extern "C" {
 /* arithmetic of the hot path */
 #define MFBD_FP64 0          /* all stages in FP64 (the reference's arithmetic)              */
 #define MFBD_FP32 1          /* FP32 phase/FFT/OTF, FP64 P/Q/metric reductions and CG scalars */
+#define MFBD_MIXED 2         /* as MFBD_FP32 with the mode expansion phi = B alpha on BF16 matrix
+                                cores (BF16 basis, alpha as two BF16 parts, FP32 accumulation) */
 
 /* gradient forms for mfbd_eval_grad */
 #define MFBD_GRAD_FD 1       /* SubImage::gradientFiniteDifference (subimage.cpp:281-292)   */
@@ -62,7 +64,7 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
     int minIter, maxIter, nDoneIter, modeStart, modeStep;
     int apodSize;       /* APODIZATION, < 0: patchSize/8 (solver.cpp:117-126)          */
     int noFilter;       /* NO_FILTER: skip the Scharmer filter in restoration           */
-    int precision;      /* MFBD_FP64 or MFBD_FP32                                      */
+    int precision;      /* MFBD_FP64, MFBD_FP32 or MFBD_MIXED                          */
     int maxBatch;       /* patches solved concurrently (1 = the reference's one-at-a-time) */
     int gradient;       /* MFBD_GRAD_SCHEDULE_REFERENCE: finite differences for the first
                            gradient, Vogel after (solver.cpp:385,478); or MFBD_GRAD_ADJOINT

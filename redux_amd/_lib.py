@@ -12,7 +12,7 @@ ip = C.POINTER(C.c_int)
 fp = C.POINTER(C.c_float)
 
 MFBD_OK, MFBD_EINVAL, MFBD_ENOMEM, MFBD_EDEVICE, MFBD_ENONFINITE, MFBD_ESTATE = 0, 1, 2, 3, 4, 5
-FP64, FP32 = 0, 1
+FP64, FP32, MIXED = 0, 1, 2
 GRAD_FD, GRAD_VOGEL, GRAD_ADJOINT = 1, 2, 3
 GRAD_SCHEDULE_REFERENCE = 0
 BASIS_ZERNIKE, BASIS_KL = 0, 1

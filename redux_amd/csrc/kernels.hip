@@ -314,6 +314,80 @@ __global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T
     }
 }
 
+// Mixed-precision mode expansion: phi = phiCh + A B with the basis in BF16 and FP32
+// accumulation on v_mfma_f32_16x16x32_bf16 (gfx950).  Bt holds the basis transposed,
+// [obj][nPS][Kp] BF16 with Kp a multiple of 32, so the 8 consecutive modes a lane
+// feeds per k-step are one 16-byte load.  alpha enters as two BF16 parts, hi =
+// bf16(a) and lo = bf16(a - hi), i.e. about 16 significant bits, at two MFMAs per
+// k-step.  Lane layout (measured on gfx950): A[i][k] from lane i + 16(k/8), element
+// k%8; B[k][j] from lane j + 16(k/8), element k%8; C as the f32 16x16x4 form.  One wave
+// = 16 images x 64 pupil points, independent of the other waves.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__global__ void __launch_bounds__(256) k_phase_bf16(float* __restrict__ phi, const bf16x8* __restrict__ Bt,
+                                                    const float* __restrict__ phiCh, const double* __restrict__ alpha,
+                                                    const int* __restrict__ qArow, const int* __restrict__ qPhRow,
+                                                    const int* __restrict__ qObj, int nQ, int K, int Kp, int nPS) {
+    using V = MfmaVec<float>::v4;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int q0 = blockIdx.y * 64 + 16 * w, c0 = blockIdx.x * 64;
+    if (q0 >= nQ) return;                                    // whole wave exits together
+    const int qa = min(q0 + li, nQ - 1);
+    const int objA = (q0 + li < nQ) ? qObj[qa] : INT_MAX;
+    const double* arow = alpha + (size_t)qArow[qa] * K;
+    V acc[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int q = min(q0 + MfmaVec<float>::row(lk, v), nQ - 1);
+        const float* pc = phiCh + (size_t)qPhRow[q] * nPS;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int col = c0 + 16 * c + li;
+            acc[c][v] = col < nPS ? pc[col] : 0.f;
+        }
+    }
+    int o = INT_MAX;
+    for (int r = 0; r < 16; ++r) o = min(o, __shfl(objA, r));
+    while (o != INT_MAX) {
+        const bf16x8* Bo = Bt + (size_t)o * nPS * (Kp / 8);
+        const bool mine = objA == o;
+        for (int k0 = 0; k0 < Kp; k0 += 32) {
+            bf16x8 ahi, alo;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + 8 * lk + e;
+                const float a = (mine && k < K) ? (float)arow[k] : 0.f;
+                const __bf16 h = (__bf16)a;
+                ahi[e] = h;
+                alo[e] = (__bf16)(a - (float)h);
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int col = c0 + 16 * c + li;
+                bf16x8 b = {};
+                if (col < nPS) b = Bo[(size_t)col * (Kp / 8) + k0 / 8 + lk];
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b, acc[c], 0, 0, 0);
+                acc[c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b, acc[c], 0, 0, 0);
+            }
+        }
+        int next = INT_MAX;
+        for (int r = 0; r < 16; ++r) {
+            const int x = __shfl(objA, r);
+            if (x > o) next = min(next, x);
+        }
+        o = next;
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int q = q0 + MfmaVec<float>::row(lk, v), col = c0 + 16 * c + li;
+            if (q < nQ && col < nPS) phi[(size_t)q * nPS + col] = acc[c][v];
+        }
+    }
+}
+
 // phiOut[q'] = phi[base(q')] + step * B[obj][mode(q')]   (finite-difference probes)
 template <typename T>
 __global__ void k_phase_pert(T* __restrict__ out, const T* __restrict__ phi, const T* __restrict__ B,
@@ -1220,8 +1294,8 @@ public:
     static constexpr int NT64 = 1024;
     static constexpr size_t ILDS = (size_t)std::max((N / 2) * (N + 1), N * (N / 2 + 1)) * sizeof(cx<double>);
 
-    Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device) : g_(g), maxSlots_(maxSlots),
-                                                                         maxBatch_(maxBatch) {
+    Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device, bool bf16Phase = false)
+        : g_(g), maxSlots_(maxSlots), maxBatch_(maxBatch), bf16Phase_(bf16Phase && std::is_same_v<T, float>) {
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreate(&st_));
         HIPCHK(hipEventCreate(&ev0_));
@@ -1369,6 +1443,15 @@ public:
                 for (int k = 0; k < nPS_; ++k)
                     Bc[((size_t)o * K + m) * nPS_ + k] = (T)modes[((size_t)o * K + m) * P2 + ps_[k]];
         upload(B_, Bc, st_);
+        if (bf16Phase_) {       // transposed BF16 copy [obj][nPS][Kp] for k_phase_bf16
+            const int Kp = (K + 31) & ~31;
+            std::vector<__bf16> bt((size_t)g_.nObj * nPS_ * Kp, (__bf16)0.0f);
+            for (int o = 0; o < g_.nObj; ++o)
+                for (int m = 0; m < K; ++m)
+                    for (int k = 0; k < nPS_; ++k)
+                        bt[((size_t)o * nPS_ + k) * Kp + m] = (__bf16)(float)modes[((size_t)o * K + m) * P2 + ps_[k]];
+            upload(Bt_, bt, st_);
+        }
         HIPCHK(hipStreamSynchronize(st_));
     }
 
@@ -1801,10 +1884,20 @@ private:
         const int* qObj = sc_->ints.p + 5 * nQ + n;
         sc_->phi.alloc((size_t)nQ * nPS_);
         sc_->S.alloc((size_t)nQ * nOS_);
-        launch("k_phase_mfma", nQ, [&] {
-            hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_, sc_->phi.p, B_.p,
-                               phiCh_.p, sc_->alpha.p, sc_->qm.arow, qPhRow, qObj, nQ, K, nPS_);
-        });
+        if (bf16Phase_) {
+            if constexpr (std::is_same_v<T, float>) {
+                launch("k_phase_bf16", nQ, [&] {
+                    hipLaunchKernelGGL(k_phase_bf16, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_,
+                                       sc_->phi.p, reinterpret_cast<const bf16x8*>(Bt_.p), phiCh_.p, sc_->alpha.p,
+                                       sc_->qm.arow, qPhRow, qObj, nQ, K, (K + 31) & ~31, nPS_);
+                });
+            }
+        } else {
+            launch("k_phase_mfma", nQ, [&] {
+                hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_,
+                                   sc_->phi.p, B_.p, phiCh_.p, sc_->alpha.p, sc_->qm.arow, qPhRow, qObj, nQ, K, nPS_);
+            });
+        }
         otf_chain(sc_->phi.p, sc_->qm.img, nQ, sc_->S.p);
         const int nChunk = (nOS_ + 255) / 256;
         sc_->Pp.alloc((size_t)n * g_.nObj * nOS_ * 2);
@@ -2059,6 +2152,8 @@ private:
     // resident: basis, supports, per-patch frames, D, ftSum, gamma, channel phases
     DBuf<C> tw_, D_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
+    DBuf<__bf16> Bt_;                 // BF16 basis, transposed, for the mixed-precision form
+    bool bf16Phase_ = false;
     HBuf<double> hStage_;
     DBuf<double> ftSum_, gamma_, win_, w_, wt_;
     DBuf<int> invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_, pyx_, osyx_, hread_,
@@ -2100,9 +2195,10 @@ std::unique_ptr<DeviceEvaluator> make_device_evaluator(const Geometry& g, int pr
             if (g.N == 128) return std::make_unique<Pipeline<double, 128>>(g, maxSlots, maxBatch, device);
             if (g.N == 256) return std::make_unique<Pipeline<double, 256>>(g, maxSlots, maxBatch, device);
         } else {
-            if (g.N == 64) return std::make_unique<Pipeline<float, 64>>(g, maxSlots, maxBatch, device);
-            if (g.N == 128) return std::make_unique<Pipeline<float, 128>>(g, maxSlots, maxBatch, device);
-            if (g.N == 256) return std::make_unique<Pipeline<float, 256>>(g, maxSlots, maxBatch, device);
+            const bool bf = precision == 2;      // MFBD_MIXED: BF16 mode GEMM, FP32 FFTs
+            if (g.N == 64) return std::make_unique<Pipeline<float, 64>>(g, maxSlots, maxBatch, device, bf);
+            if (g.N == 128) return std::make_unique<Pipeline<float, 128>>(g, maxSlots, maxBatch, device, bf);
+            if (g.N == 256) return std::make_unique<Pipeline<float, 256>>(g, maxSlots, maxBatch, device, bf);
         }
         err = "unsupported OTF grid (N must be 64, 128 or 256)";
     } catch (const std::exception& e) {

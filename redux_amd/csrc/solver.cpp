@@ -852,6 +852,7 @@ int mfbd_ctx_create(const mfbd_geometry* geo, const mfbd_params* prm, int device
         geo->nImg < geo->nCh || geo->dataY < geo->s || geo->dataX < geo->s || geo->startY < 0 || geo->startX < 0 ||
         geo->startY + geo->s > geo->dataY || geo->startX + geo->s > geo->dataX || 2 * P < geo->s)
         return MFBD_EINVAL;
+    if (prm->precision < MFBD_FP64 || prm->precision > MFBD_MIXED || prm->maxBatch < 1) return MFBD_EINVAL;
     auto* c = new mfbd_ctx();
     c->g = {geo->s, P, 2 * P, geo->K, geo->nObj, geo->nCh, geo->nImg, geo->dataY, geo->dataX, geo->startY, geo->startX};
     c->prm = *prm;

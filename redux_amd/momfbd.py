@@ -46,7 +46,7 @@ class Solver:
                      pb["start"][0], pb["start"][1])
         prm = Params(st["FTOL"], st["EPS"], st["GRADDIFF_STEP"], st["REG_ALPHA"], st["MIN_ITER"], st["MAX_ITER"],
                      st["N_DONE_ITER"], st["MODE_START"], st["MODE_STEP"], st["APODIZATION"], int(st["NO_FILTER"]),
-                     L.FP64 if precision == "fp64" else L.FP32, max_batch,
+                     {"fp64": L.FP64, "fp32": L.FP32, "mixed": L.MIXED}[precision], max_batch,
                      L.GRAD_ADJOINT if gradient == "adjoint" else L.GRAD_SCHEDULE_REFERENCE)
         h = C.c_void_p()
         self._check(L.lib().mfbd_ctx_create(C.byref(g), C.byref(prm), device, C.byref(h)), "ctx_create", None)

@@ -46,3 +46,20 @@ def test_errors_without_gpu_work():
     h2 = C.c_void_p()
     assert L.mfbd_ctx_create(C.byref(bad), C.byref(prm), 0, C.byref(h2)) == _lib.MFBD_EINVAL
     assert L.mfbd_ctx_destroy(h) == 0
+
+
+def test_header_constants_match_binding():
+    """Every status, precision and gradient constant the header defines has the same
+    value in the ctypes binding."""
+    import re
+    from redux_amd import _lib as L
+    text = open(os.path.join(os.path.dirname(__file__), "..", "include", "momfbd_hip.h")).read()
+    defs = dict(re.findall(r"#define (MFBD_\w+) (\d+)", text))
+    pairs = {"MFBD_OK": L.MFBD_OK, "MFBD_EINVAL": L.MFBD_EINVAL, "MFBD_ENOMEM": L.MFBD_ENOMEM,
+             "MFBD_EDEVICE": L.MFBD_EDEVICE, "MFBD_ENONFINITE": L.MFBD_ENONFINITE, "MFBD_ESTATE": L.MFBD_ESTATE,
+             "MFBD_FP64": L.FP64, "MFBD_FP32": L.FP32, "MFBD_MIXED": L.MIXED, "MFBD_GRAD_FD": L.GRAD_FD,
+             "MFBD_GRAD_VOGEL": L.GRAD_VOGEL, "MFBD_GRAD_ADJOINT": L.GRAD_ADJOINT,
+             "MFBD_GRAD_SCHEDULE_REFERENCE": L.GRAD_SCHEDULE_REFERENCE, "MFBD_BASIS_ZERNIKE": L.BASIS_ZERNIKE,
+             "MFBD_BASIS_KL": L.BASIS_KL}
+    for name, val in pairs.items():
+        assert int(defs[name]) == val, name

@@ -24,16 +24,28 @@ static void fft1d(ocplx* x, int n, int sign, ocplx* tmp)
             j ^= bit;
             if (i < j) { ocplx t = x[i]; x[i] = x[j]; x[j] = t; }
         }
-        for (int k = 0; k < n / 2; ++k) {
-            double a = sign * 2.0 * M_PI * (double)k / (double)n;
-            tmp[k] = cos(a) + I * sin(a);
+        /* twiddles of the last (n, sign) per thread, cached: identical values, computed
+         * once per transform size instead of once per line */
+        static _Thread_local ocplx* tw[2] = {NULL, NULL};
+        static _Thread_local int twn[2] = {0, 0};
+        const int slot = sign < 0 ? 0 : 1;
+        if (twn[slot] != n) {
+            free(tw[slot]);
+            tw[slot] = (ocplx*)malloc(sizeof(ocplx) * (size_t)(n / 2));
+            for (int k = 0; k < n / 2; ++k) {
+                double a = sign * 2.0 * M_PI * (double)k / (double)n;
+                tw[slot][k] = cos(a) + I * sin(a);
+            }
+            twn[slot] = n;
         }
+        const ocplx* twk = tw[slot];
+        (void)tmp;
         for (int len = 2; len <= n; len <<= 1) {
             int half = len >> 1, step = n / len;
             for (int i = 0; i < n; i += len) {
                 for (int j = 0; j < half; ++j) {
                     ocplx u = x[i + j];
-                    ocplx v = x[i + j + half] * tmp[j * step];
+                    ocplx v = x[i + j + half] * twk[j * step];
                     x[i + j] = u + v;
                     x[i + j + half] = u - v;
                 }

@@ -22,7 +22,9 @@ HIP_FLAGS = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=gfx950", "-Wall", "-W
              # packed FP32 instructions are kept out of every kernel
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 CXX_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{ROCM}/include",
-             "-D__HIP_PLATFORM_AMD__"]
+             "-D__HIP_PLATFORM_AMD__",
+             # complex products as (ac - bd, ad + bc) without the NaN/Inf recovery call
+             "-fcx-limited-range"]
 
 
 def _run(cmd):

@@ -66,11 +66,13 @@ public:
     virtual void evaluate_begin(int lane, std::vector<EvalReq>& reqs) = 0;
     virtual void evaluate_end(int lane, std::vector<EvalReq>& reqs) = 0;
     // restoration sums at alpha for n patches: P0 = sum conj(D) S, Q0 = sum |S|^2 (no
-    // gamma), per object on the OTF support: P0 [n][nObj][nOS] complex, Q0 [n][nObj][nOS];
-    // alpha[k] points at [nImg][K] for slots[k]
+    // gamma), per object on the stored half of the OTF support: P0 [n][nObj][nOSStored]
+    // complex, Q0 [n][nObj][nOSStored]; alpha[k] points at [nImg][K] for slots[k]
     virtual void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) = 0;
     virtual int nOS() const = 0;            // OTF support points (full)
     virtual int nOSStored() const = 0;      // points stored on the device (Hermitian half)
+    // full support point u -> stored point h (value as stored), or -1-h (its conjugate)
+    virtual const std::vector<int>& support_map() const = 0;
     virtual int nPS() const = 0;            // pupil support points
     virtual const std::vector<int>& otf_support() const = 0;
     virtual void debug_get(const std::string& what, int slot, std::vector<double>& out) = 0;

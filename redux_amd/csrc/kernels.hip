@@ -1324,6 +1324,7 @@ public:
     int capacity() const override { return maxSlots_; }
     double* alpha_staging() override { return hStage_.p; }
     int nOS() const override { return nOSfull_; }
+    const std::vector<int>& support_map() const override { return fullMap_; }
     int nOSStored() const override { return nOS_; }
     int nPS() const override { return nPS_; }
     const std::vector<int>& otf_support() const override { return os_; }
@@ -1661,15 +1662,8 @@ public:
                                   st_));
             HIPCHK(hipMemcpyAsync(sc_->hL.p, sc_->Qp.p, (size_t)m * per * sizeof(double), hipMemcpyDeviceToHost, st_));
             HIPCHK(hipStreamSynchronize(st_));
-            // expand the stored half to the full support: P(-u) = P(u)*, Q(-u) = Q(u)
-            for (int k = 0; k < m * g_.nObj; ++k)
-                for (int u = 0; u < nOSfull_; ++u) {
-                    const int code = fullMap_[u], h = code >= 0 ? code : -1 - code;
-                    const size_t src = (size_t)k * nOS_ + h, dst = (size_t)(b * g_.nObj + k) * nOSfull_ + u;
-                    P0[2 * dst] = hP[2 * src];
-                    P0[2 * dst + 1] = code >= 0 ? hP[2 * src + 1] : -hP[2 * src + 1];
-                    Q0[dst] = hQ[src];
-                }
+            std::memcpy(P0 + 2 * (size_t)b * per, hP, 2 * (size_t)m * per * sizeof(double));
+            std::memcpy(Q0 + (size_t)b * per, hQ, (size_t)m * per * sizeof(double));
         }
         collect(2);
     }

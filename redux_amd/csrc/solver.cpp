@@ -330,7 +330,7 @@ struct mfbd_ctx {
     bool initDone = false;
     std::unique_ptr<Pool> pool;
     // host-side timing of the batch driver: evaluate / resume / re-init phases
-    std::vector<KernelStat> hostStats = std::vector<KernelStat>(5);
+    std::vector<KernelStat> hostStats = std::vector<KernelStat>(6);
 };
 
 namespace {
@@ -471,15 +471,19 @@ void effective_alpha(const mfbd_ctx* c, int p, const double* alpha, double* out)
 
 // Object::restorePatch after the sums (object.cpp:278-443): conj, safeDivide, Scharmer
 // filter, inverse transform, centred crop
-void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const double* P0, const double* Q0,
-                    float* imgOut) {
+// P0, Q0: the patch's sums on the stored half of the support ([nObj][nH]); `map` takes
+// each full support point to its stored value (code h) or that value's conjugate (-1-h)
+void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const std::vector<int>& map, int nH,
+                    const double* P0, const double* Q0, float* imgOut) {
     const int N = 2 * c->g.P, s = c->g.s, nOS = (int)os.size();
     for (int o = 0; o < c->g.nObj; ++o) {
         std::vector<cd> ao((size_t)N * N, 0.0);
         std::vector<double> dn((size_t)N * N, 0.0);
         for (int u = 0; u < nOS; ++u) {
-            const size_t k = (size_t)o * nOS + u;
-            ao[os[u]] = std::conj(cd(P0[2 * k], P0[2 * k + 1]));
+            const int code = map[u], h = code >= 0 ? code : -1 - code;
+            const size_t k = (size_t)o * nH + h;
+            const cd pv(P0[2 * k], code >= 0 ? P0[2 * k + 1] : -P0[2 * k + 1]);   // P(-u) = P(u)*
+            ao[os[u]] = std::conj(pv);
             dn[os[u]] = Q0[k];
         }
         for (size_t k = 0; k < ao.size(); ++k) ao[k] = (std::fabs(dn[k]) > 0) ? ao[k] / dn[k] : cd(0.0);
@@ -1048,7 +1052,7 @@ int mfbd_restore(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, f
     try {
         ensure_init(c);
         const size_t per = (size_t)c->g.nImg * c->g.K;
-        const size_t po = (size_t)c->g.nObj * c->dev->nOS();
+        const size_t po = (size_t)c->g.nObj * c->dev->nOSStored();
         for (int k = 0; k < n; ++k)
             if (patchIdx[k] < 0 || patchIdx[k] >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
         std::vector<double> ae(per * n), P0(2 * po * n), Q0(po * n);
@@ -1059,8 +1063,8 @@ int mfbd_restore(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, f
         }
         c->dev->restore_sums(n, patchIdx, ap.data(), P0.data(), Q0.data());
         parallel_for(n, [&](int k) {
-            finish_restore(c, patchIdx[k], c->dev->otf_support(), &P0[2 * po * k], &Q0[po * k],
-                           img + (size_t)k * c->g.nObj * c->g.s * c->g.s);
+            finish_restore(c, patchIdx[k], c->dev->otf_support(), c->dev->support_map(), c->dev->nOSStored(),
+                           &P0[2 * po * k], &Q0[po * k], img + (size_t)k * c->g.nObj * c->g.s * c->g.s);
         });
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
@@ -1111,6 +1115,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             c->hostStats[2].name = "host_reinit";
             c->hostStats[3].name = "host_reset_init";
             c->hostStats[4].name = "host_restore";
+            c->hostStats[5].name = "host_restore_sums";
             {
                 auto t0 = clk::now();
                 reset_init(c, ids);
@@ -1205,7 +1210,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 if (tasks[k].h.promise().exc) std::rethrow_exception(tasks[k].h.promise().exc);
             }
             // results + restoration (Solver::run tail, solver.cpp:549-566)
-            const size_t per = (size_t)c->g.nObj * c->dev->nOS();
+            const size_t per = (size_t)c->g.nObj * c->dev->nOSStored();
             for (int k = 0; k < nb; ++k) {
                 auto& r = *runs[k];
                 r.res.nF = r.nF;
@@ -1222,11 +1227,13 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                     ap[k] = &ae[(size_t)k * nPar];
                 }
                 c->dev->restore_sums(nb, ids.data(), ap.data(), P0.data(), Q0.data());
+                c->hostStats[5].ms += since(t0);
+                c->hostStats[5].launches++;
                 const auto& os = c->dev->otf_support();
                 parallel_for(nb, [&](int k) {
                     const int p = runs[k]->p;
-                    finish_restore(c, p, os, &P0[2 * k * per], &Q0[k * per],
-                                   imgOut + (size_t)p * c->g.nObj * c->g.s * c->g.s);
+                    finish_restore(c, p, os, c->dev->support_map(), c->dev->nOSStored(), &P0[2 * k * per],
+                                   &Q0[k * per], imgOut + (size_t)p * c->g.nObj * c->g.s * c->g.s);
                 });
                 c->hostStats[4].ms += since(t0);
                 c->hostStats[4].launches++;

@@ -74,7 +74,7 @@ __device__ __forceinline__ double root_c(int k) {
 }
 __device__ __forceinline__ double root_s(int k) { return -root_c((k + 12) & 15); }   // -sin(2 pi k/16)
 
-// a * exp(DIR 2 pi i k / 16) for a compile-time k in [0, 8): the multiplications by 0
+// a * exp(DIR 2 pi i k / 16) for a compile-time k in [0, 16): the multiplications by 0
 // and 1 of k = 0, 4 and the common factor of k = 2, 6 are written out, since IEEE
 // arithmetic does not let the compiler fold them
 template <typename T, int DIR>
@@ -85,6 +85,7 @@ __device__ __forceinline__ cx<T> rot16(cx<T> a, int k) {
         case 4: return DIR < 0 ? cx<T>{a.y, -a.x} : cx<T>{-a.y, a.x};
         case 2: return DIR < 0 ? cx<T>{(a.x + a.y) * h, (a.y - a.x) * h} : cx<T>{(a.x - a.y) * h, (a.y + a.x) * h};
         case 6: return DIR < 0 ? cx<T>{(a.y - a.x) * h, -(a.x + a.y) * h} : cx<T>{-(a.x + a.y) * h, (a.x - a.y) * h};
+        case 8: return cx<T>{-a.x, -a.y};
         default: {
             const cx<T> w = {(T)root_c(k), (T)(DIR < 0 ? root_s(k) : -root_s(k))};
             return cmul(a, w);
@@ -92,9 +93,49 @@ __device__ __forceinline__ cx<T> rot16(cx<T> a, int k) {
     }
 }
 
+// 4-point DFT in place on v[i0], v[i0+s], v[i0+2s], v[i0+3s]: additions and a
+// multiplication by -i (forward) / +i (backward) only
+template <typename T, int DIR>
+__device__ __forceinline__ void dft4(cx<T>* v, int i0, int s) {
+    const cx<T> a = v[i0], b = v[i0 + s], c = v[i0 + 2 * s], d = v[i0 + 3 * s];
+    const cx<T> t0 = cadd(a, c), t1 = csub(a, c), t2 = cadd(b, d), t3 = csub(b, d);
+    const cx<T> t3r = DIR < 0 ? cx<T>{t3.y, -t3.x} : cx<T>{-t3.y, t3.x};   // t3 * (-+i)
+    v[i0] = cadd(t0, t2);
+    v[i0 + s] = cadd(t1, t3r);
+    v[i0 + 2 * s] = csub(t0, t2);
+    v[i0 + 3 * s] = csub(t1, t3r);
+}
+
+// 16-point DFT as 4 x 4: DFT4 of the stride-4 subsequences, twiddles W16^(m k1),
+// DFT4 across m; output in natural order
+template <typename T, int DIR>
+__device__ __forceinline__ void dft16(cx<T>* v) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) dft4<T, DIR>(v, m, 4);         // Y_m[k1] at v[m + 4 k1]
+#pragma unroll
+    for (int m = 1; m < 4; ++m)
+#pragma unroll
+        for (int k1 = 1; k1 < 4; ++k1) v[m + 4 * k1] = rot16<T, DIR>(v[m + 4 * k1], m * k1);
+    cx<T> w[16];
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) w[4 * k1 + m] = v[m + 4 * k1];
+        dft4<T, DIR>(w, 4 * k1, 1);                             // X[k1 + 4 k2] = w[4 k1 + k2]
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = w[4 * k1 + k2];
+}
+
 // in-register DFT of R points (R power of two <= 16), DIR = -1 forward, +1 backward
 template <typename T, int R, int DIR>
 __device__ __forceinline__ void dft_reg(cx<T>* v) {
+    if constexpr (R == 16) {
+        dft16<T, DIR>(v);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < R; ++i) {
         const int j = bitrev(i, R);

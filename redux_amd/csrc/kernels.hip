@@ -905,11 +905,12 @@ __global__ void __launch_bounds__(256) k_noise(double* __restrict__ noise, doubl
 // One workgroup per image with the whole N x N tile resident in LDS (FP32, N <= 128:
 // 128 x 129 complex floats = 132 KB of gfx950's 160 KB).  Rows are padded by one
 // element so row and column passes both walk consecutive banks (MAP 1).
-// First Stockham pass (NS = 1, forward) of lines whose only non-zero elements are
-// [N/4, 3N/4): element j + r N/R is non-zero only for r in [R/4, 3R/4), so only those
-// are read and the rest of the butterfly input is the constant zero.
-template <typename T, int N, int R, int NL, int NT, int MAP>
-__device__ __forceinline__ void spass_first_mid(cx<T>* buf, int ls, int es) {
+// First Stockham pass (NS = 1) of lines whose non-zero elements are known: the
+// middle [N/4, 3N/4) (ENDS = 0) or the two ends [0, N/4) and [3N/4, N) (ENDS = 1).
+// Element j + r N/R lies in the middle exactly for r in [R/4, 3R/4), so only the
+// non-zero elements are read and the rest of the butterfly input is the constant zero.
+template <typename T, int N, int R, int NL, int NT, int MAP, int DIR, int ENDS>
+__device__ __forceinline__ void spass_first_pruned(cx<T>* buf, int ls, int es) {
     constexpr int NB = NL * (N / R);
     constexpr int BT = (NB + NT - 1) / NT;
     cx<T> v[BT][R];
@@ -920,9 +921,11 @@ __device__ __forceinline__ void spass_first_mid(cx<T>* buf, int ls, int es) {
             const int line = MAP ? b % NL : b / (N / R), j = MAP ? b / NL : b % (N / R);
             const cx<T>* L = buf + line * ls;
 #pragma unroll
-            for (int r = 0; r < R; ++r)
-                v[i][r] = (r >= R / 4 && r < 3 * R / 4) ? L[(j + r * (N / R)) * es] : cx<T>{T(0), T(0)};
-            dft_reg<T, R, -1>(v[i]);
+            for (int r = 0; r < R; ++r) {
+                const bool mid = r >= R / 4 && r < 3 * R / 4;
+                v[i][r] = (mid != (ENDS != 0)) ? L[(j + r * (N / R)) * es] : cx<T>{T(0), T(0)};
+            }
+            dft_reg<T, R, DIR>(v[i]);
         }
     }
     __syncthreads();
@@ -937,6 +940,11 @@ __device__ __forceinline__ void spass_first_mid(cx<T>* buf, int ls, int es) {
         }
     }
     __syncthreads();
+}
+
+template <typename T, int N, int R, int NL, int NT, int MAP>
+__device__ __forceinline__ void spass_first_mid(cx<T>* buf, int ls, int es) {
+    spass_first_pruned<T, N, R, NL, NT, MAP, -1, 0>(buf, ls, es);
 }
 
 // Last forward Stockham pass (NS = N/R) of NL lines that writes, instead of the
@@ -1168,6 +1176,194 @@ __global__ void __launch_bounds__(NT) k_otf64_inv(cx<double>* __restrict__ S, co
         const int code = hread[u];
         const cx<double> v = t[code >= 0 ? code : -1 - code];
         Sq[u] = {v.x * inv, (code >= 0 ? v.y : -v.y) * inv};
+    }
+}
+
+// FP64 Vogel / adjoint weights at N = 2P (N = 128) in five kernels with two
+// intermediates per gradient image in HBM, the same quantities as the unfused chain
+// (k_build_g, IFFT2, k_save_re, k_field, IFFT2, k_mul_re, FFT2, k_extract_v):
+//   k_g64_cols  G' (de-centred, Hermitian) -> inverse column FFTs of the N/2+1 columns
+//               of the half spectrum, C = [N][N/2+1]
+//   k_g64_rows  rows n and n+N/2 of C packed into one complex inverse FFT (both rows are
+//               Hermitian in x, so their transforms are real): greal = Re IFFT2(G') / N^2
+//   k_h64_rows  inverse row FFTs of the N/2 de-centred pupil-field rows (non-zero only at
+//               the two ends of the row), first pass pruned
+//   k_h64_cols  per column: inverse FFT (ends pruned) -> h, times greal / N^2, forward
+//               FFT; only the N/2 pupil rows of the result are kept (in place)
+//   k_h64_final forward row FFTs of those rows, then v_i = Im(conj(PF_i) t(pde_i))
+//               [* pupil_i] at the pupil pixels of the row
+// Pupil-row index i (0..N/2) of the de-centred field <-> grid row g = i for i < N/4,
+// g = i + N/2 otherwise.
+template <int N, int LC>
+__global__ void __launch_bounds__(LC * N / 16) k_g64_cols(cx<double>* __restrict__ Cg, const cx<double>* __restrict__ S,
+                                                          const cx<double>* __restrict__ D,
+                                                          const double* __restrict__ P, const double* __restrict__ Q,
+                                                          const int* __restrict__ invOs, QMap qm,
+                                                          const int* __restrict__ qReq, const int* __restrict__ imgObj,
+                                                          const int* __restrict__ gIdx, const cx<double>* __restrict__ tw,
+                                                          int nOS, int nImg, int nObj) {
+    constexpr int NT = LC * N / 16, HC = N / 2 + 1, NCB = (HC + LC - 1) / LC;
+    __shared__ cx<double> t[N * LC];                           // element (y, c) at y * LC + c
+    const int qt = blockIdx.x / NCB, c0 = (blockIdx.x % NCB) * LC;
+    const int q = gIdx[qt], img = qm.img[q], r = qReq[q], o = imgObj[img];
+    const cx<double>* Dq = D + ((size_t)qm.slot[q] * nImg + img) * nOS;
+    const cx<double>* Sq = S + ((size_t)r * nImg + img) * nOS;
+    const double* Po = P + 2 * ((size_t)r * nObj + o) * nOS;
+    const double* Qo = Q + ((size_t)r * nObj + o) * nOS;
+    // G' on the full support (0 elsewhere), as k_build_g builds it
+    auto gAt = [&](int p) -> cx<double> {
+        const int code = invOs[p];
+        if (code == -1) return {0.0, 0.0};
+        const int u = code >= 0 ? code : -2 - code;
+        const double pr = Po[2 * u], pi = Po[2 * u + 1], qq = Qo[u];
+        const double pqr = pr * qq, pqi = pi * qq, ps = pr * pr + pi * pi, qs = qq * qq;
+        const cx<double> d = Dq[u], sv = Sq[u];
+        const double gr = (pqr * d.x - pqi * d.y) - ps * sv.x;
+        const double gi = (pqr * d.y + pqi * d.x) - ps * sv.y;
+        return {gr / qs, (code >= 0 ? gi : -gi) / qs};
+    };
+    // Re IFFT2(G') = IFFT2 of the Hermitian part (G'(u) + G'(-u)*)/2, which equals G'
+    // wherever the support holds both u and -u
+    for (int e = threadIdx.x; e < N * LC; e += NT) {
+        const int y = e / LC, x = c0 + e % LC;
+        cx<double> val = {0.0, 0.0};
+        if (x < HC) {
+            const cx<double> a = gAt(y * N + x), b = gAt(((N - y) % N) * N + (N - x) % N);
+            val = {0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
+        }
+        t[e] = val;
+    }
+    __syncthreads();
+    fft_lines<double, N, +1, LC, NT, 1>(t, 1, LC, tw);
+    cx<double>* out = Cg + (size_t)qt * N * HC;
+    for (int e = threadIdx.x; e < N * LC; e += NT) {
+        const int n = e / LC, x = c0 + e % LC;
+        if (x < HC) out[(size_t)n * HC + x] = t[e];
+    }
+}
+
+template <int N, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_g64_rows(double* __restrict__ greal, const cx<double>* __restrict__ Cg,
+                                                          const cx<double>* __restrict__ tw) {
+    constexpr int NT = LR * N / 16, HC = N / 2 + 1, LD = N + 1;
+    __shared__ cx<double> t[LR * LD];
+    const int qt = blockIdx.x / (N / 2 / LR), r0 = (blockIdx.x % (N / 2 / LR)) * LR;
+    const cx<double>* C = Cg + (size_t)qt * N * HC;
+    for (int e = threadIdx.x; e < LR * N; e += NT) {
+        const int r = e / N, k = e % N, n = r0 + r;
+        // Z(k) = C(n, k) + i C(n + N/2, k), with C(., k) = C(., N - k)* for k > N/2
+        const int kk = k <= N / 2 ? k : N - k;
+        cx<double> c1 = C[(size_t)n * HC + kk], c2 = C[(size_t)(n + N / 2) * HC + kk];
+        if (k > N / 2) {
+            c1.y = -c1.y;
+            c2.y = -c2.y;
+        }
+        t[r * LD + k] = {c1.x - c2.y, c1.y + c2.x};
+    }
+    __syncthreads();
+    fft_lines<double, N, +1, LR, NT, 1>(t, LD, 1, tw);
+    constexpr double inv = 1.0 / ((double)N * N);
+    double* g = greal + (size_t)qt * N * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) {
+        const int r = e / N, x = e % N;
+        const cx<double> z = t[r * LD + x];
+        g[(size_t)(r0 + r) * N + x] = z.x * inv;
+        g[(size_t)(r0 + r + N / 2) * N + x] = z.y * inv;
+    }
+}
+
+template <int N, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_h64_rows(cx<double>* __restrict__ A, const double* __restrict__ phi,
+                                                          const int* __restrict__ invPde,
+                                                          const double* __restrict__ pupil,
+                                                          const int* __restrict__ gIdx, const cx<double>* __restrict__ tw,
+                                                          int nPS) {
+    constexpr int NT = LR * N / 16, LD = N + 1;
+    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
+    __shared__ cx<double> t[LR * LD];
+    const int qt = blockIdx.x / (N / 2 / LR), i0 = (blockIdx.x % (N / 2 / LR)) * LR;
+    const double* ph = phi + (size_t)gIdx[qt] * nPS;
+    for (int e = threadIdx.x; e < LR * (N / 2); e += NT) {
+        const int r = e / (N / 2), xi = e % (N / 2);
+        const int i = i0 + r, gy = i < N / 4 ? i : i + N / 2;
+        const int x = xi < N / 4 ? xi : xi + N / 2;              // the two ends of the row
+        const int idx = invPde[gy * N + x];
+        cx<double> val = {0.0, 0.0};
+        if (idx >= 0) {
+            double sn, cs;
+            sincos(ph[idx], &sn, &cs);
+            val = {pupil[idx] * cs, pupil[idx] * sn};
+        }
+        t[r * LD + x] = val;
+    }
+    __syncthreads();
+    spass_first_pruned<double, N, RF, LR, NT, 1, +1, 1>(t, LD, 1);
+    spass<double, N, RL, RF, +1, LR, NT, 1>(t, LD, 1, tw);
+    cx<double>* out = A + ((size_t)qt * (N / 2) + i0) * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) out[e] = t[(e / N) * LD + e % N];
+}
+
+template <int N, int LC>
+__global__ void __launch_bounds__(LC * N / 16) k_h64_cols(cx<double>* __restrict__ A, const double* __restrict__ greal,
+                                                          const cx<double>* __restrict__ tw) {
+    constexpr int NT = LC * N / 16;
+    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
+    __shared__ cx<double> t[N * LC];                           // element (row g, column c) at g * LC + c
+    const int qt = blockIdx.x / (N / LC), c0 = (blockIdx.x % (N / LC)) * LC;
+    cx<double>* img = A + (size_t)qt * (N / 2) * N;
+    for (int e = threadIdx.x; e < (N / 2) * LC; e += NT) {
+        const int i = e / LC, c = e % LC, gy = i < N / 4 ? i : i + N / 2;
+        t[gy * LC + c] = img[(size_t)i * N + c0 + c];
+    }
+    __syncthreads();
+    spass_first_pruned<double, N, RF, LC, NT, 1, +1, 1>(t, 1, LC);
+    spass<double, N, RL, RF, +1, LC, NT, 1>(t, 1, LC, tw);
+    // y = h * greal / N^2, then the forward column transform
+    constexpr double inv = 1.0 / ((double)N * N);
+    const double* g = greal + (size_t)qt * N * N;
+    for (int e = threadIdx.x; e < N * LC; e += NT) {
+        const int y = e / LC, c = e % LC;
+        const double f = g[(size_t)y * N + c0 + c] * inv;
+        t[e] = {t[e].x * f, t[e].y * f};
+    }
+    __syncthreads();
+    spass<double, N, RF, 1, -1, LC, NT, 1>(t, 1, LC, tw);
+    spass<double, N, RL, RF, -1, LC, NT, 1>(t, 1, LC, tw);
+    for (int e = threadIdx.x; e < (N / 2) * LC; e += NT) {
+        const int i = e / LC, c = e % LC, gy = i < N / 4 ? i : i + N / 2;
+        img[(size_t)i * N + c0 + c] = t[gy * LC + c];
+    }
+}
+
+template <int N, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_h64_final(double* __restrict__ v, const cx<double>* __restrict__ A,
+                                                           const double* __restrict__ phi,
+                                                           const int* __restrict__ invPs,
+                                                           const double* __restrict__ pupil,
+                                                           const int* __restrict__ gIdx,
+                                                           const cx<double>* __restrict__ tw, int nPS, int extraPupil) {
+    constexpr int NT = LR * N / 16, LD = N + 1, P = N / 2;
+    __shared__ cx<double> t[LR * LD];
+    const int qt = blockIdx.x / (N / 2 / LR), i0 = (blockIdx.x % (N / 2 / LR)) * LR;
+    const cx<double>* src = A + ((size_t)qt * (N / 2) + i0) * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) t[(e / N) * LD + e % N] = src[e];
+    __syncthreads();
+    fft_lines<double, N, -1, LR, NT, 1>(t, LD, 1, tw);
+    const double* ph = phi + (size_t)gIdx[qt] * nPS;
+    double* vq = v + (size_t)qt * nPS;
+    for (int e = threadIdx.x; e < LR * P; e += NT) {
+        const int r = e / P, px = e % P, i = i0 + r;
+        const int gy = i < N / 4 ? i : i + N / 2;
+        const int py = (gy + P / 2) % N;                        // pupil row whose de-centred row is gy
+        const int idx = invPs[py * P + px];
+        if (idx < 0) continue;
+        const int x = ((px - P / 2) % N + N) % N;
+        const cx<double> tv = t[r * LD + x];
+        double sn, cs;
+        sincos(ph[idx], &sn, &cs);
+        const double pu = pupil[idx];
+        const double im = (pu * cs) * tv.y - (pu * sn) * tv.x;   // Im(conj(PF) t)
+        vq[idx] = extraPupil ? im * pu : im;
     }
 }
 
@@ -2037,6 +2233,34 @@ private:
                                        sc_->Qp.p, sc_->phi.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, osyx_.p,
                                        osyx_.p + nOS_, osyx_.p + 2 * nOS_, osyx_.p + 3 * nOS_, sc_->qm, sc_->qReq, imgObj_.p,
                                        sc_->gIdx.p, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
+                });
+            } else if constexpr (SPLIT64) {
+                constexpr int L16 = 16, HC = N / 2 + 1;
+                constexpr size_t per = (size_t)std::max(N * HC, (N / 2) * N);   // C, then the pupil rows
+                sc_->tiles.alloc((size_t)nG * per);
+                sc_->greal.alloc((size_t)nG * N2);
+                C* Cg = sc_->tiles.p;
+                launch("k_g64_cols", nG, [&] {
+                    hipLaunchKernelGGL((k_g64_cols<N, L16>), dim3(nG * ((HC + L16 - 1) / L16)), dim3(L16 * N / 16), 0,
+                                       st_, Cg, sc_->S.p, D_.p, sc_->Pp.p, sc_->Qp.p, invOs_.p, sc_->qm, sc_->qReq,
+                                       imgObj_.p, sc_->gIdx.p, tw_.p, nOS_, nImg, g_.nObj);
+                });
+                launch("k_g64_rows", nG, [&] {
+                    hipLaunchKernelGGL((k_g64_rows<N, L16>), dim3(nG * (N / 2 / L16)), dim3(L16 * N / 16), 0, st_,
+                                       sc_->greal.p, Cg, tw_.p);
+                });
+                launch("k_h64_rows", nG, [&] {
+                    hipLaunchKernelGGL((k_h64_rows<N, L16>), dim3(nG * (N / 2 / L16)), dim3(L16 * N / 16), 0, st_,
+                                       Cg, sc_->phi.p, invPde_.p, pupil_.p, sc_->gIdx.p, tw_.p, nPS_);
+                });
+                launch("k_h64_cols", nG, [&] {
+                    hipLaunchKernelGGL((k_h64_cols<N, L16>), dim3(nG * (N / L16)), dim3(L16 * N / 16), 0, st_, Cg,
+                                       sc_->greal.p, tw_.p);
+                });
+                launch("k_h64_final", nG, [&] {
+                    hipLaunchKernelGGL((k_h64_final<N, L16>), dim3(nG * (N / 2 / L16)), dim3(L16 * N / 16), 0, st_,
+                                       sc_->v.p, Cg, sc_->phi.p, invPs_.p, pupil_.p, sc_->gIdx.p, tw_.p, nPS_,
+                                       extraPupil);
                 });
             } else {
                 sc_->tiles.alloc((size_t)nG * N2);

@@ -707,15 +707,18 @@ __global__ void __launch_bounds__(256) k_grad_mfma(double* __restrict__ part, co
     }
 }
 
+// nEnabled[r]: the enabled-mode count of the request that image r belongs to (each
+// request has its own; part holds the columns of the batch's largest count)
 __global__ void k_grad_sum(double* __restrict__ g, const double* __restrict__ part, const double* __restrict__ scale,
-                           const int* __restrict__ gIdx, int nG, int K, int Kt, int nEnabled) {
+                           const int* __restrict__ gIdx, const int* __restrict__ nEnabled, int nG, int K, int Kt) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nG * K) return;
     const int r = e / K, m = e % K;
+    const bool on = m < nEnabled[r];
     double sum = 0.0;
-    if (m < nEnabled)
+    if (on)
         for (int sl = 0; sl < GR_SLICES; ++sl) sum += part[((size_t)sl * nG + r) * Kt + m];
-    g[(size_t)gIdx[r] * K + m] = (m < nEnabled) ? scale[r] * sum : 0.0;
+    g[(size_t)gIdx[r] * K + m] = on ? scale[r] * sum : 0.0;
 }
 
 // dl = -sum_u (q dn - dq |p|^2) / (q (q + dq)) / N^2 for one perturbed image
@@ -2202,7 +2205,7 @@ private:
         }
         if (anyV) {
             // compact list of the virtual images whose requests want a Vogel/adjoint gradient
-            std::vector<int> gIdx;
+            std::vector<int> gIdx, gEn;
             std::vector<double> sc;
             int nEn = 0, extraPupil = 1;
             for (int r = 0; r < n; ++r) {
@@ -2212,6 +2215,7 @@ private:
                     const int o = st.imgObj[j];
                     const double w = st.weight[o];
                     gIdx.push_back(r * nImg + j);
+                    gEn.push_back(rq[r].nEnabled);
                     if (rq[r].gradMethod == 2) {
                         // gradientVogel: -2 area/N^2 * w, times w again in Solver::gradient
                         sc.push_back((w == 0.0) ? 0.0 : -2.0 * area_ / N2 * w * w);
@@ -2224,6 +2228,7 @@ private:
                 }
             }
             const int nG = (int)gIdx.size();
+            gIdx.insert(gIdx.end(), gEn.begin(), gEn.end());     // [0, nG): images, [nG, 2nG): counts
             stage_async(sc_->gscale, sc_->hGs, sc);
             stage_async(sc_->gIdx, sc_->hGi, gIdx);
             sc_->v.alloc((size_t)nG * nPS_);
@@ -2299,7 +2304,7 @@ private:
                 });
                 launch("k_grad_sum", [&] {
                     hipLaunchKernelGGL(k_grad_sum, dim3(nblk((size_t)nG * K)), dim3(256), 0, st_, sc_->g_d.p, sc_->gpart.p,
-                                       sc_->gscale.p, sc_->gIdx.p, nG, K, Kt, nEn);
+                                       sc_->gscale.p, sc_->gIdx.p, sc_->gIdx.p + nG, nG, K, Kt);
                 });
             }
         }

@@ -93,3 +93,28 @@ def test_restore(patch, ref, prec, tol):
         err = np.linalg.norm(img - want) / np.linalg.norm(want)
         assert err < tol, (prec, err)
     sv.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_batched_solve_equals_one_at_a_time(prec):
+    """Patches of one batch step through the modes at their own pace, so a batched
+    evaluation holds gradient requests with different enabled-mode counts; each must
+    get its own (modes beyond its count have zero gradient, Solver::gradient).  The
+    batched solve of four benchmark-geometry patches repeats the one-at-a-time solves
+    bit for bit."""
+    from redux_amd import synth
+    from redux_amd.momfbd import Solver
+    su, pbs = synth.build_field_batch(synth.CFG2_PATCH, 4, seed=7)
+    sv = Solver(pbs[0], precision=prec, max_batch=len(pbs))
+    sv.load(pbs)
+    batch = sv.run()
+    sv.close()
+    for k, p in enumerate(pbs):
+        sv = Solver(p, precision=prec, max_batch=1)
+        sv.load([p])
+        solo = sv.run()[0]
+        sv.close()
+        b = batch[k]
+        assert solo["nF"] == b["nF"] and solo["nDF"] == b["nDF"], (prec, k, solo["nF"], b["nF"])
+        assert np.array_equal(solo["alpha"], b["alpha"]), (prec, k)
+        assert np.array_equal(solo["img"], b["img"]), (prec, k)

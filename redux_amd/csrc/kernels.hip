@@ -1072,33 +1072,89 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
     }
 }
 
-// FP64 OTF for N = 2P whose tile does not fit in LDS (a 128 x 128 complex double tile
-// is 256 KB): the same algorithm as k_otf_fused in three kernels with one N/2 x N
-// complex intermediate per image in HBM (half a tile), written and read once.
-//   k_otf64_rows: the P = N/2 pupil rows, field a = scale pupil exp(i phi) and the
-//                 forward row FFTs (first pass pruned to the pupil columns)
-//   k_otf64_cols: forward column FFTs of the N/2 non-zero rows (first pass pruned),
-//                 the last pass writing the packed power spectrum |F|^2(y) + i|F|^2(y+N/2)
-//                 in place of its input
-//   k_otf64_inv:  one image per workgroup: inverse FFTs of the packed rows, split into
-//                 half spectra, inverse FFTs of the N/2+1 half-spectrum columns, and the
-//                 stored support points
-template <int N, int LR>
-__global__ void __launch_bounds__(LR * N / 16) k_otf64_rows(cx<double>* __restrict__ R, const double* __restrict__ phi,
-                                                            const int* __restrict__ invPs,
-                                                            const double* __restrict__ pupil,
-                                                            const int* __restrict__ qImg,
-                                                            const double* __restrict__ imgScale,
-                                                            const cx<double>* __restrict__ tw, int nPS) {
-    constexpr int P = N / 2, LD = N + 1, NT = LR * N / 16;
-    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
-    __shared__ cx<double> t[LR * LD];
-    const int q = blockIdx.x / (P / LR), r0 = (blockIdx.x % (P / LR)) * LR;
+// One Stockham pass like spass, over NL lines whose element e of line l sits at
+// buf[addr(l, e)] (any layout: the caller's functor).
+template <typename T, int N, int R, int NS, int DIR, int NL, int NT, typename F>
+__device__ __forceinline__ void spass_at(cx<T>* buf, F addr, const cx<T>* __restrict__ tw) {
+    constexpr int NB = NL * (N / R);
+    constexpr int BT = (NB + NT - 1) / NT;
+    cx<T> v[BT][R];
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = b % NL, j = b / NL, k = j % NS;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                cx<T> x = buf[addr(line, j + r * (N / R))];
+                if (NS > 1 && r > 0) {
+                    cx<T> w = tw[r * k * (N / (NS * R))];
+                    if (DIR > 0) w.y = -w.y;
+                    x = cmul(x, w);
+                }
+                v[i][r] = x;
+            }
+            dft_reg<T, R, DIR>(v[i]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        const int b = threadIdx.x + i * NT;
+        if (NB % NT == 0 || b < NB) {
+            const int line = b % NL, j = b / NL, k = j % NS;
+            const int base = (j / NS) * NS * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[addr(line, base + r * NS)] = v[i][r];
+        }
+    }
+    __syncthreads();
+}
+
+// FP64 OTF with one workgroup per image and every intermediate in one P x (N+1)
+// complex-double LDS buffer (P = N/2; 132 KB at N = 128, where the whole N x N tile
+// would need 256 KB).  The steps of k_otf_fused, arranged to work in place:
+//   1. pupil field in buffer rows 0..P-1 = grid rows [N/4, 3N/4), columns [N/4, 3N/4)
+//   2. forward row FFTs of those rows (first pass pruned to the pupil columns)
+//   3. forward column FFTs (first pass pruned to the pupil rows) and the packed power
+//      spectrum pk[y][x] = |F|^2(y, x) + i |F|^2(y + N/2, x), y < N/2.  A column's
+//      transform has N values between its two passes but its input and output have N/2,
+//      so the columns go in two halves: the first-pass outputs of the left half take
+//      the whole buffer (column c as buffer row c), those of the right half wait in
+//      registers, and the left half's power values wait in registers until both halves
+//      are done
+//   4. inverse row FFTs of the packed rows; each row Z = IFFT(pk[y]) holds the half
+//      spectra of the real rows y and y + N/2, A(k) = (Z(k) + Z*(-k))/2 and
+//      B(k) = (Z(k) - Z*(-k))/2i.  For 0 < k < N/2 A(k) replaces Z(k) and B(k) replaces
+//      Z(N-k); A(0), B(0), A(N/2), B(N/2) are real, and slot 0 takes A(0) + i A(N/2),
+//      slot N/2 takes B(0) + i B(N/2)
+//   5. inverse FFTs over y of the half-spectrum columns: line k (0 < k < N/2) is slot k
+//      of rows y < N/2 and slot N-k of rows y - N/2; line 0 is the complex combination
+//      X = C0 + i C_{N/2} of the two real columns, slot 0 then slot N/2
+//   6. the stored support points: hread codes name (y, k) of the N x (N/2+1) half
+//      array; columns 0 and N/2 separate as C0 = (X(y) + X*(-y))/2,
+//      C_{N/2} = (X(y) - X*(-y))/2i
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, const double* __restrict__ phi,
+                                                    const int* __restrict__ invPs, const double* __restrict__ pupil,
+                                                    const int* __restrict__ qImg, const double* __restrict__ imgScale,
+                                                    const int* __restrict__ hread, const cx<double>* __restrict__ tw,
+                                                    int nPS, int nOS) {
+    constexpr int P = N / 2, LD = N + 1, HC = N / 2 + 1;
+    constexpr int RF = N == 64 ? 8 : 16, RL = 8;             // radices of the two passes (N = RF RL)
+    static_assert(RF * RL == N, "two-pass transforms only (N = 64, 128)");
+    constexpr int NB1 = N * (N / RF);                        // first-pass column butterflies, both halves
+    constexpr int NB2 = P * (N / RL);                        // second-pass butterflies of one half
+    constexpr int IH = NB1 / 2 / NT, I2 = NB2 / NT;          // butterflies per thread: first pass per half, second pass
+    static_assert(NB1 / 2 % NT == 0 && NB2 % NT == 0, "thread count must divide the butterflies of a half");
+    extern __shared__ cx<double> lds64[];
+    cx<double>* t = lds64;
+    const int q = blockIdx.x;
     const double sc = imgScale[qImg[q]];
     const double* ph = phi + (size_t)q * nPS;
-    for (int e = threadIdx.x; e < LR * P; e += NT) {
-        const int r = e / P, x = e % P;
-        const int k = invPs[(r0 + r) * P + x];
+    // 1.
+    for (int e = threadIdx.x; e < P * P; e += NT) {
+        const int k = invPs[e];
         cx<double> val = {0.0, 0.0};
         if (k >= 0) {
             double sn, cs;
@@ -1106,78 +1162,105 @@ __global__ void __launch_bounds__(LR * N / 16) k_otf64_rows(cx<double>* __restri
             const double m = pupil[k] * sc;
             val = {m * cs, m * sn};
         }
-        t[r * LD + x + N / 4] = val;
+        t[(e / P) * LD + e % P + N / 4] = val;
     }
     __syncthreads();
-    spass_first_mid<double, N, RF, LR, NT, 1>(t, LD, 1);
-    spass<double, N, RL, RF, -1, LR, NT, 1>(t, LD, 1, tw);
-    cx<double>* out = R + ((size_t)q * P + r0) * N;
-    for (int e = threadIdx.x; e < LR * N; e += NT) out[e] = t[(e / N) * LD + e % N];
-}
-
-template <int N, int LC>
-__global__ void __launch_bounds__(LC * N / 16) k_otf64_cols(cx<double>* __restrict__ R, const cx<double>* __restrict__ tw) {
-    constexpr int P = N / 2, NT = LC * N / 16;
-    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
-    __shared__ cx<double> t[N * LC];                       // element (row y, column c) at y * LC + c
-    const int q = blockIdx.x / (N / LC), c0 = (blockIdx.x % (N / LC)) * LC;
-    cx<double>* img = R + (size_t)q * P * N;
-    for (int e = threadIdx.x; e < P * LC; e += NT) {
-        const int y = e / LC, c = e % LC;
-        t[(y + N / 4) * LC + c] = img[(size_t)y * N + c0 + c];   // the pupil rows sit at [N/4, 3N/4)
-    }
+    // 2.
+    spass_first_mid<double, N, RF, P, NT, 1>(t, LD, 1);
+    spass<double, N, RL, RF, -1, P, NT, 1>(t, LD, 1, tw);
+    // 3. first pass: butterfly bb of half h (0: left, 1: right) is column bb % P + h P;
+    //    butterfly j of a column reads grid rows j + r N/RF, r in [RF/4, 3RF/4), i.e.
+    //    buffer rows j + (r - RF/4) N/RF.  The inputs of both halves are read before the
+    //    buffer is overwritten; each half's butterflies run just before their outputs are
+    //    stored (the inputs take half the registers of the outputs).
+    cx<double> x1[2][IH][RF / 2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < IH; ++i) {
+            const int bb = threadIdx.x + i * NT, c = bb % P + h * P, j = bb / P;
+#pragma unroll
+            for (int r = 0; r < RF / 2; ++r) x1[h][i][r] = t[(j + r * (N / RF)) * LD + c];
+        }
     __syncthreads();
-    spass_first_mid<double, N, RF, LC, NT, 1>(t, 1, LC);
-    spass_last_power<double, N, RL, LC, NT, 1>(t, 1, LC, tw, t, 1, LC);
-    for (int e = threadIdx.x; e < P * LC; e += NT) {
-        const int y = e / LC, c = e % LC;
-        img[(size_t)y * N + c0 + c] = t[y * LC + c];
+    cx<double> pw[2][I2][RL / 2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        // first-pass outputs of half h: column c as buffer row c % P, element j RF + r
+#pragma unroll
+        for (int i = 0; i < IH; ++i) {
+            const int bb = threadIdx.x + i * NT, c = bb % P, j = bb / P;
+            cx<double> v[RF];
+#pragma unroll
+            for (int r = 0; r < RF; ++r)
+                v[r] = (r >= RF / 4 && r < 3 * RF / 4) ? x1[h][i][r - RF / 4] : cx<double>{0.0, 0.0};
+            dft_reg<double, RF, -1>(v);
+#pragma unroll
+            for (int r = 0; r < RF; ++r) t[c * LD + j * RF + r] = v[r];
+        }
+        __syncthreads();
+        // second pass (NS = RF) and |F|^2 of outputs y = j + r RF and y + N/2
+#pragma unroll
+        for (int i = 0; i < I2; ++i) {
+            const int b = threadIdx.x + i * NT, c = b % P, j = b / P;
+            cx<double> v[RL];
+#pragma unroll
+            for (int r = 0; r < RL; ++r) {
+                cx<double> x = t[c * LD + j + r * RF];
+                if (r > 0) x = cmul(x, tw[r * j * (N / (RF * RL))]);
+                v[r] = x;
+            }
+            dft_reg<double, RL, -1>(v);
+#pragma unroll
+            for (int r = 0; r < RL / 2; ++r) {
+                const cx<double> a = v[r], d = v[r + RL / 2];
+                pw[h][i][r] = {a.x * a.x + a.y * a.y, d.x * d.x + d.y * d.y};
+            }
+        }
+        __syncthreads();
     }
-}
-
-template <int N, int NT>
-__global__ void __launch_bounds__(NT) k_otf64_inv(cx<double>* __restrict__ S, const cx<double>* __restrict__ R,
-                                                  const int* __restrict__ hread, const cx<double>* __restrict__ tw,
-                                                  int nOS) {
-    constexpr int P = N / 2, LD = N + 1, HC = N / 2 + 1;
-    constexpr int PER = (P * HC + NT - 1) / NT;
-    extern __shared__ cx<double> lds64[];
-    cx<double>* t = lds64;                                 // packed rows, stride LD; then N x HC half spectra
-    const int q = blockIdx.x;
-    const cx<double>* img = R + (size_t)q * P * N;
-    for (int e = threadIdx.x; e < P * N; e += NT) t[(e / N) * LD + e % N] = img[e];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < I2; ++i) {
+            const int b = threadIdx.x + i * NT, c = b % P + h * P, j = b / P;
+#pragma unroll
+            for (int r = 0; r < RL / 2; ++r) t[(j + r * RF) * LD + c] = pw[h][i][r];
+        }
     __syncthreads();
+    // 4.
     fft_lines<double, N, +1, P, NT, 1>(t, LD, 1, tw);
-    // split every packed row into the half spectra of its two real rows; all reads first,
-    // since the half-spectrum layout reuses the same memory
-    cx<double> A[PER], Bv[PER];
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * NT;
-        if (e < P * HC) {
-            const int y = e / HC, k = e % HC;
-            const cx<double> z = t[y * LD + k], zm = t[y * LD + (N - k) % N];
-            A[i] = {0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y)};
-            Bv[i] = {0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x)};
+    for (int e = threadIdx.x; e < P * P; e += NT) {
+        const int y = e / P, k = e % P;
+        cx<double>* row = t + y * LD;
+        const cx<double> z = row[k], zm = row[k == 0 ? P : N - k];
+        if (k == 0) {
+            row[0] = {z.x, zm.x};
+            row[P] = {z.y, zm.y};
+        } else {
+            row[k] = {0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y)};
+            row[N - k] = {0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x)};
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int i = 0; i < PER; ++i) {
-        const int e = threadIdx.x + i * NT;
-        if (e < P * HC) {
-            const int y = e / HC, k = e % HC;
-            t[y * HC + k] = A[i];
-            t[(y + P) * HC + k] = Bv[i];
-        }
-    }
-    __syncthreads();
-    fft_lines<double, N, +1, HC, NT, 1>(t, 1, HC, tw);
+    // 5.
+    auto at = [](int k, int y) { return y < P ? y * LD + k : (y - P) * LD + (k == 0 ? P : N - k); };
+    spass_at<double, N, RF, 1, +1, P, NT>(t, at, tw);
+    spass_at<double, N, RL, RF, +1, P, NT>(t, at, tw);
+    // 6.
     constexpr double inv = 1.0 / ((double)N * N);
     cx<double>* Sq = S + (size_t)q * nOS;
     for (int u = threadIdx.x; u < nOS; u += NT) {
         const int code = hread[u];
-        const cx<double> v = t[code >= 0 ? code : -1 - code];
+        const int idx = code >= 0 ? code : -1 - code, y = idx / HC, k = idx % HC;
+        cx<double> v;
+        if (k != 0 && k != P) {
+            v = t[at(k, y)];
+        } else {
+            const cx<double> x = t[at(0, y)], xm = t[at(0, (N - y) % N)];
+            v = k == 0 ? cx<double>{0.5 * (x.x + xm.x), 0.5 * (x.y - xm.y)}
+                       : cx<double>{0.5 * (x.y + xm.y), -0.5 * (x.x - xm.x)};
+        }
         Sq[u] = {v.x * inv, (code >= 0 ? v.y : -v.y) * inv};
     }
 }
@@ -1528,11 +1611,12 @@ public:
     static constexpr int FNT = N == 128 ? 1024 : 256;
     static constexpr size_t FLDS = (size_t)Tile<N>::SIZE * sizeof(cx<float>);
     static constexpr size_t OLDS = ((size_t)(N / 2) * Tile<N>::LD + (size_t)N * (N / 2 + 1)) * sizeof(cx<float>);
-    // FP64 three-kernel OTF: one image per workgroup in the last kernel, whose LDS holds
-    // max(N/2 x (N+1), N x (N/2+1)) complex doubles
+    // FP64 OTF in one kernel with an N/2 x (N+1) complex-double LDS buffer per image
+    static constexpr bool FUSED64 = std::is_same_v<T, double> && N <= 128;
+    static constexpr int F64NT = N == 128 ? 512 : 256;
+    static constexpr size_t F64LDS = (size_t)(N / 2) * (N + 1) * sizeof(cx<double>);
+    // FP64 gradient weights at N = 128 in five kernels
     static constexpr bool SPLIT64 = std::is_same_v<T, double> && N == 128;
-    static constexpr int NT64 = 1024;
-    static constexpr size_t ILDS = (size_t)std::max((N / 2) * (N + 1), N * (N / 2 + 1)) * sizeof(cx<double>);
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device, bool bf16Phase = false)
         : g_(g), maxSlots_(maxSlots), maxBatch_(maxBatch), bf16Phase_(bf16Phase && std::is_same_v<T, float>) {
@@ -1669,9 +1753,9 @@ public:
             HIPCHK(hipFuncSetAttribute((const void*)k_vogel_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)FLDS));
         }
-        if constexpr (SPLIT64) {
-            HIPCHK(hipFuncSetAttribute((const void*)k_otf64_inv<N, NT64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)ILDS));
+        if constexpr (FUSED64) {
+            HIPCHK(hipFuncSetAttribute((const void*)k_otf64_fused<N, F64NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)F64LDS));
         }
         HIPCHK(hipFuncSetAttribute((const void*)k_noise, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)((128 * 128 + 6 * 128) * sizeof(double))));
@@ -2037,20 +2121,10 @@ private:
             });
             return;
         }
-        if constexpr (SPLIT64) {
-            sc_->tiles.alloc((size_t)nQ * N2 / 2);
-            constexpr int LR = 16, LC = 16;
-            launch("k_otf64_rows", nQ, [&] {
-                hipLaunchKernelGGL((k_otf64_rows<N, LR>), dim3(nQ * (N / 2 / LR)), dim3(LR * N / 16), 0, st_,
-                                   sc_->tiles.p, phi, invPs_.p, pupil_.p, qImg, imgScale_.p, tw_.p, nPS_);
-            });
-            launch("k_otf64_cols", nQ, [&] {
-                hipLaunchKernelGGL((k_otf64_cols<N, LC>), dim3(nQ * (N / LC)), dim3(LC * N / 16), 0, st_,
-                                   sc_->tiles.p, tw_.p);
-            });
-            launch("k_otf64_inv", nQ, [&] {
-                hipLaunchKernelGGL((k_otf64_inv<N, NT64>), dim3(nQ), dim3(NT64), ILDS, st_, Sout, sc_->tiles.p,
-                                   hread_.p, tw_.p, nOS_);
+        if constexpr (FUSED64) {
+            launch("k_otf64_fused", nQ, [&] {
+                hipLaunchKernelGGL((k_otf64_fused<N, F64NT>), dim3(nQ), dim3(F64NT), F64LDS, st_, Sout, phi,
+                                   invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_);
             });
             return;
         }

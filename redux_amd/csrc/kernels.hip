@@ -1111,26 +1111,28 @@ __device__ __forceinline__ void spass_at(cx<T>* buf, F addr, const cx<T>* __rest
     __syncthreads();
 }
 
-// FP64 OTF with one workgroup per image and every intermediate in one P x (N+1)
-// complex-double LDS buffer (P = N/2; 132 KB at N = 128, where the whole N x N tile
-// would need 256 KB).  The steps of k_otf_fused, arranged to work in place:
-//   1. pupil field in buffer rows 0..P-1 = grid rows [N/4, 3N/4), columns [N/4, 3N/4)
-//   2. forward row FFTs of those rows (first pass pruned to the pupil columns)
+// FP64 OTF with every intermediate of an image in one P x (N+1) complex-double LDS
+// buffer (P = N/2; 132 KB at N = 128, where the whole N x N tile would need 256 KB),
+// plus the N twiddles.  Persistent: each workgroup loops over images q = blockIdx.x,
+// + gridDim.x, ..., keeping its pupil indices and amplitudes in registers and loading
+// the next image's phases while the current image is transformed.  The steps of
+// k_otf_fused, arranged to work in place:
+//   1-2. forward row FFTs of the P pupil rows (grid rows [N/4, 3N/4)): the first pass
+//      reads only the pupil columns and forms a = scale pupil exp(i phi) on the fly
 //   3. forward column FFTs (first pass pruned to the pupil rows) and the packed power
 //      spectrum pk[y][x] = |F|^2(y, x) + i |F|^2(y + N/2, x), y < N/2.  A column's
 //      transform has N values between its two passes but its input and output have N/2,
-//      so the columns go in two halves: the first-pass outputs of the left half take
-//      the whole buffer (column c as buffer row c), those of the right half wait in
-//      registers, and the left half's power values wait in registers until both halves
-//      are done
-//   4. inverse row FFTs of the packed rows; each row Z = IFFT(pk[y]) holds the half
-//      spectra of the real rows y and y + N/2, A(k) = (Z(k) + Z*(-k))/2 and
-//      B(k) = (Z(k) - Z*(-k))/2i.  For 0 < k < N/2 A(k) replaces Z(k) and B(k) replaces
-//      Z(N-k); A(0), B(0), A(N/2), B(N/2) are real, and slot 0 takes A(0) + i A(N/2),
-//      slot N/2 takes B(0) + i B(N/2)
-//   5. inverse FFTs over y of the half-spectrum columns: line k (0 < k < N/2) is slot k
-//      of rows y < N/2 and slot N-k of rows y - N/2; line 0 is the complex combination
-//      X = C0 + i C_{N/2} of the two real columns, slot 0 then slot N/2
+//      so the columns go in two halves: the inputs of both are read into registers,
+//      the first-pass outputs of one half take the whole buffer (column c as buffer row
+//      c), and the left half's power values wait in registers until both are done
+//   4. inverse row FFTs of the packed rows: Z_y = IFFT(pk[y]) holds the half spectra
+//      of the real rows y and y + N/2, A(k) = (Z(k) + Z*(-k))/2, B(k) = (Z(k) - Z*(-k))/2i
+//   5. inverse FFTs over y of the half-spectrum columns.  Line k (0 < k < N/2) is stored
+//      in slot k of rows y < N/2 and slot N-k of rows y - N/2; line 0 is the complex
+//      combination X = C0 + i C_{N/2} of the two real columns, slots 0 and N/2.  The
+//      first pass forms its inputs A_y(k), B_y(k) (line 0: A_y(0) + i A_y(N/2),
+//      B_y(0) + i B_y(N/2)) from Z_y(k) and Z_y(N-k) as it reads them: the two values
+//      one butterfly needs from row y are exactly that pair
 //   6. the stored support points: hread codes name (y, k) of the N x (N/2+1) half
 //      array; columns 0 and N/2 separate as C0 = (X(y) + X*(-y))/2,
 //      C_{N/2} = (X(y) - X*(-y))/2i
@@ -1138,130 +1140,181 @@ template <int N, int NT>
 __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, const double* __restrict__ phi,
                                                     const int* __restrict__ invPs, const double* __restrict__ pupil,
                                                     const int* __restrict__ qImg, const double* __restrict__ imgScale,
-                                                    const int* __restrict__ hread, const cx<double>* __restrict__ tw,
-                                                    int nPS, int nOS) {
+                                                    const int* __restrict__ hread, const cx<double>* __restrict__ twg,
+                                                    int nPS, int nOS, int nQ) {
     constexpr int P = N / 2, LD = N + 1, HC = N / 2 + 1;
     constexpr int RF = N == 64 ? 8 : 16, RL = 8;             // radices of the two passes (N = RF RL)
     static_assert(RF * RL == N, "two-pass transforms only (N = 64, 128)");
+    constexpr int NBR = P * (N / RF);                        // first-pass row butterflies
     constexpr int NB1 = N * (N / RF);                        // first-pass column butterflies, both halves
     constexpr int NB2 = P * (N / RL);                        // second-pass butterflies of one half
-    constexpr int IH = NB1 / 2 / NT, I2 = NB2 / NT;          // butterflies per thread: first pass per half, second pass
-    static_assert(NB1 / 2 % NT == 0 && NB2 % NT == 0, "thread count must divide the butterflies of a half");
+    constexpr int IR = NBR / NT, IH = NB1 / 2 / NT, I2 = NB2 / NT;
+    static_assert(NBR % NT == 0 && NB1 / 2 % NT == 0 && NB2 % NT == 0, "thread count must divide the butterflies");
     extern __shared__ cx<double> lds64[];
     cx<double>* t = lds64;
-    const int q = blockIdx.x;
-    const double sc = imgScale[qImg[q]];
-    const double* ph = phi + (size_t)q * nPS;
-    // 1.
-    for (int e = threadIdx.x; e < P * P; e += NT) {
-        const int k = invPs[e];
-        cx<double> val = {0.0, 0.0};
-        if (k >= 0) {
-            double sn, cs;
-            sincos(ph[k], &sn, &cs);
-            const double m = pupil[k] * sc;
-            val = {m * cs, m * sn};
-        }
-        t[(e / P) * LD + e % P + N / 4] = val;
-    }
-    __syncthreads();
-    // 2.
-    spass_first_mid<double, N, RF, P, NT, 1>(t, LD, 1);
-    spass<double, N, RL, RF, -1, P, NT, 1>(t, LD, 1, tw);
-    // 3. first pass: butterfly bb of half h (0: left, 1: right) is column bb % P + h P;
-    //    butterfly j of a column reads grid rows j + r N/RF, r in [RF/4, 3RF/4), i.e.
-    //    buffer rows j + (r - RF/4) N/RF.  The inputs of both halves are read before the
-    //    buffer is overwritten; each half's butterflies run just before their outputs are
-    //    stored (the inputs take half the registers of the outputs).
-    cx<double> x1[2][IH][RF / 2];
+    cx<double>* tw = lds64 + P * LD;
+    for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
+    // row butterfly (line, j) reads pupil row `line`, columns j + r N/RF, r < RF/2
+    int kk[IR][RF / 2];
+    double amp[IR][RF / 2], ph[IR][RF / 2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h)
+    for (int i = 0; i < IR; ++i) {
+        const int b = threadIdx.x + i * NT, line = b % P, j = b / P;
 #pragma unroll
-        for (int i = 0; i < IH; ++i) {
-            const int bb = threadIdx.x + i * NT, c = bb % P + h * P, j = bb / P;
-#pragma unroll
-            for (int r = 0; r < RF / 2; ++r) x1[h][i][r] = t[(j + r * (N / RF)) * LD + c];
-        }
-    __syncthreads();
-    cx<double> pw[2][I2][RL / 2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        // first-pass outputs of half h: column c as buffer row c % P, element j RF + r
-#pragma unroll
-        for (int i = 0; i < IH; ++i) {
-            const int bb = threadIdx.x + i * NT, c = bb % P, j = bb / P;
-            cx<double> v[RF];
-#pragma unroll
-            for (int r = 0; r < RF; ++r)
-                v[r] = (r >= RF / 4 && r < 3 * RF / 4) ? x1[h][i][r - RF / 4] : cx<double>{0.0, 0.0};
-            dft_reg<double, RF, -1>(v);
-#pragma unroll
-            for (int r = 0; r < RF; ++r) t[c * LD + j * RF + r] = v[r];
-        }
-        __syncthreads();
-        // second pass (NS = RF) and |F|^2 of outputs y = j + r RF and y + N/2
-#pragma unroll
-        for (int i = 0; i < I2; ++i) {
-            const int b = threadIdx.x + i * NT, c = b % P, j = b / P;
-            cx<double> v[RL];
-#pragma unroll
-            for (int r = 0; r < RL; ++r) {
-                cx<double> x = t[c * LD + j + r * RF];
-                if (r > 0) x = cmul(x, tw[r * j * (N / (RF * RL))]);
-                v[r] = x;
-            }
-            dft_reg<double, RL, -1>(v);
-#pragma unroll
-            for (int r = 0; r < RL / 2; ++r) {
-                const cx<double> a = v[r], d = v[r + RL / 2];
-                pw[h][i][r] = {a.x * a.x + a.y * a.y, d.x * d.x + d.y * d.y};
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int i = 0; i < I2; ++i) {
-            const int b = threadIdx.x + i * NT, c = b % P + h * P, j = b / P;
-#pragma unroll
-            for (int r = 0; r < RL / 2; ++r) t[(j + r * RF) * LD + c] = pw[h][i][r];
-        }
-    __syncthreads();
-    // 4.
-    fft_lines<double, N, +1, P, NT, 1>(t, LD, 1, tw);
-    for (int e = threadIdx.x; e < P * P; e += NT) {
-        const int y = e / P, k = e % P;
-        cx<double>* row = t + y * LD;
-        const cx<double> z = row[k], zm = row[k == 0 ? P : N - k];
-        if (k == 0) {
-            row[0] = {z.x, zm.x};
-            row[P] = {z.y, zm.y};
-        } else {
-            row[k] = {0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y)};
-            row[N - k] = {0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x)};
+        for (int r = 0; r < RF / 2; ++r) {
+            kk[i][r] = invPs[line * P + j + r * (N / RF)];
+            amp[i][r] = kk[i][r] >= 0 ? pupil[kk[i][r]] : 0.0;
         }
     }
-    __syncthreads();
-    // 5.
+    auto fetch = [&](int qq) {
+#pragma unroll
+        for (int i = 0; i < IR; ++i)
+#pragma unroll
+            for (int r = 0; r < RF / 2; ++r)
+                ph[i][r] = (qq < nQ && kk[i][r] >= 0) ? phi[(size_t)qq * nPS + kk[i][r]] : 0.0;
+    };
+    fetch(blockIdx.x);
     auto at = [](int k, int y) { return y < P ? y * LD + k : (y - P) * LD + (k == 0 ? P : N - k); };
-    spass_at<double, N, RF, 1, +1, P, NT>(t, at, tw);
-    spass_at<double, N, RL, RF, +1, P, NT>(t, at, tw);
-    // 6.
-    constexpr double inv = 1.0 / ((double)N * N);
-    cx<double>* Sq = S + (size_t)q * nOS;
-    for (int u = threadIdx.x; u < nOS; u += NT) {
-        const int code = hread[u];
-        const int idx = code >= 0 ? code : -1 - code, y = idx / HC, k = idx % HC;
-        cx<double> v;
-        if (k != 0 && k != P) {
-            v = t[at(k, y)];
-        } else {
-            const cx<double> x = t[at(0, y)], xm = t[at(0, (N - y) % N)];
-            v = k == 0 ? cx<double>{0.5 * (x.x + xm.x), 0.5 * (x.y - xm.y)}
-                       : cx<double>{0.5 * (x.y + xm.y), -0.5 * (x.x - xm.x)};
+    for (int q = blockIdx.x; q < nQ; q += gridDim.x) {
+        // 1-2.
+        const double sc = imgScale[qImg[q]];
+        {
+            cx<double> v[IR][RF];
+#pragma unroll
+            for (int i = 0; i < IR; ++i) {
+#pragma unroll
+                for (int r = 0; r < RF; ++r) v[i][r] = {0.0, 0.0};
+#pragma unroll
+                for (int r = 0; r < RF / 2; ++r) {
+                    double sn, cs;
+                    sincos(ph[i][r], &sn, &cs);
+                    const double m = amp[i][r] * sc;
+                    v[i][r + RF / 4] = {m * cs, m * sn};
+                }
+                dft_reg<double, RF, -1>(v[i]);
+            }
+            fetch(q + gridDim.x);
+#pragma unroll
+            for (int i = 0; i < IR; ++i) {
+                const int b = threadIdx.x + i * NT, line = b % P, j = b / P;
+#pragma unroll
+                for (int r = 0; r < RF; ++r) t[line * LD + j * RF + r] = v[i][r];
+            }
         }
-        Sq[u] = {v.x * inv, (code >= 0 ? v.y : -v.y) * inv};
+        __syncthreads();
+        spass<double, N, RL, RF, -1, P, NT, 1>(t, LD, 1, tw);
+        // 3. butterfly bb of half h (0: left, 1: right) is column bb % P + h P; butterfly j
+        //    reads grid rows j + r N/RF, r in [RF/4, 3RF/4), i.e. buffer rows j + (r - RF/4) N/RF
+        cx<double> x1[2][IH][RF / 2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < IH; ++i) {
+                const int bb = threadIdx.x + i * NT, c = bb % P + h * P, j = bb / P;
+#pragma unroll
+                for (int r = 0; r < RF / 2; ++r) x1[h][i][r] = t[(j + r * (N / RF)) * LD + c];
+            }
+        __syncthreads();
+        cx<double> pw[2][I2][RL / 2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int i = 0; i < IH; ++i) {
+                const int bb = threadIdx.x + i * NT, c = bb % P, j = bb / P;
+                cx<double> v[RF];
+#pragma unroll
+                for (int r = 0; r < RF; ++r)
+                    v[r] = (r >= RF / 4 && r < 3 * RF / 4) ? x1[h][i][r - RF / 4] : cx<double>{0.0, 0.0};
+                dft_reg<double, RF, -1>(v);
+#pragma unroll
+                for (int r = 0; r < RF; ++r) t[c * LD + j * RF + r] = v[r];     // column c as buffer row c
+            }
+            __syncthreads();
+            // second pass (NS = RF); outputs y = j + r RF and y + N/2 give one power pair
+#pragma unroll
+            for (int i = 0; i < I2; ++i) {
+                const int b = threadIdx.x + i * NT, c = b % P, j = b / P;
+                cx<double> v[RL];
+#pragma unroll
+                for (int r = 0; r < RL; ++r) {
+                    cx<double> x = t[c * LD + j + r * RF];
+                    if (r > 0) x = cmul(x, tw[r * j * (N / (RF * RL))]);
+                    v[r] = x;
+                }
+                dft_reg<double, RL, -1>(v);
+#pragma unroll
+                for (int r = 0; r < RL / 2; ++r) {
+                    const cx<double> a = v[r], d = v[r + RL / 2];
+                    pw[h][i][r] = {a.x * a.x + a.y * a.y, d.x * d.x + d.y * d.y};
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < I2; ++i) {
+                const int b = threadIdx.x + i * NT, c = b % P + h * P, j = b / P;
+#pragma unroll
+                for (int r = 0; r < RL / 2; ++r) t[(j + r * RF) * LD + c] = pw[h][i][r];
+            }
+        __syncthreads();
+        // 4.
+        fft_lines<double, N, +1, P, NT, 1>(t, LD, 1, tw);
+        // 5. first pass with the half-spectrum split on its inputs, then the second pass
+        {
+            constexpr int NB = P * (N / RF), BT = (NB + NT - 1) / NT;
+            cx<double> v[BT][RF];
+#pragma unroll
+            for (int i = 0; i < BT; ++i) {
+                const int b = threadIdx.x + i * NT;
+                if (NB % NT == 0 || b < NB) {
+                    const int k = b % P, j = b / P;
+#pragma unroll
+                    for (int r = 0; r < RF / 2; ++r) {
+                        const cx<double>* row = t + (j + r * (N / RF)) * LD;
+                        const cx<double> z = row[k], zm = row[k == 0 ? P : N - k];
+                        if (k == 0) {
+                            v[i][r] = {z.x, zm.x};
+                            v[i][r + RF / 2] = {z.y, zm.y};
+                        } else {
+                            v[i][r] = {0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y)};
+                            v[i][r + RF / 2] = {0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x)};
+                        }
+                    }
+                    dft_reg<double, RF, +1>(v[i]);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < BT; ++i) {
+                const int b = threadIdx.x + i * NT;
+                if (NB % NT == 0 || b < NB) {
+                    const int k = b % P, j = b / P;
+#pragma unroll
+                    for (int r = 0; r < RF; ++r) t[at(k, j * RF + r)] = v[i][r];
+                }
+            }
+            __syncthreads();
+        }
+        spass_at<double, N, RL, RF, +1, P, NT>(t, at, tw);
+        // 6.
+        constexpr double inv = 1.0 / ((double)N * N);
+        cx<double>* Sq = S + (size_t)q * nOS;
+        for (int u = threadIdx.x; u < nOS; u += NT) {
+            const int code = hread[u];
+            const int idx = code >= 0 ? code : -1 - code, y = idx / HC, k = idx % HC;
+            cx<double> v;
+            if (k != 0 && k != P) {
+                v = t[at(k, y)];
+            } else {
+                const cx<double> x = t[at(0, y)], xm = t[at(0, (N - y) % N)];
+                v = k == 0 ? cx<double>{0.5 * (x.x + xm.x), 0.5 * (x.y - xm.y)}
+                           : cx<double>{0.5 * (x.y + xm.y), -0.5 * (x.x - xm.x)};
+            }
+            Sq[u] = {v.x * inv, (code >= 0 ? v.y : -v.y) * inv};
+        }
+        __syncthreads();
     }
 }
 
@@ -1614,7 +1667,7 @@ public:
     // FP64 OTF in one kernel with an N/2 x (N+1) complex-double LDS buffer per image
     static constexpr bool FUSED64 = std::is_same_v<T, double> && N <= 128;
     static constexpr int F64NT = N == 128 ? 512 : 256;
-    static constexpr size_t F64LDS = (size_t)(N / 2) * (N + 1) * sizeof(cx<double>);
+    static constexpr size_t F64LDS = ((size_t)(N / 2) * (N + 1) + N) * sizeof(cx<double>);
     // FP64 gradient weights at N = 128 in five kernels
     static constexpr bool SPLIT64 = std::is_same_v<T, double> && N == 128;
 
@@ -1756,6 +1809,10 @@ public:
         if constexpr (FUSED64) {
             HIPCHK(hipFuncSetAttribute((const void*)k_otf64_fused<N, F64NT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)F64LDS));
+            int nCU = 0, dev = 0;
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
+            otfGrid_ = std::max(nCU, 1);         // persistent workgroups: one per CU (the LDS fits one)
         }
         HIPCHK(hipFuncSetAttribute((const void*)k_noise, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)((128 * 128 + 6 * 128) * sizeof(double))));
@@ -2123,8 +2180,8 @@ private:
         }
         if constexpr (FUSED64) {
             launch("k_otf64_fused", nQ, [&] {
-                hipLaunchKernelGGL((k_otf64_fused<N, F64NT>), dim3(nQ), dim3(F64NT), F64LDS, st_, Sout, phi,
-                                   invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_);
+                hipLaunchKernelGGL((k_otf64_fused<N, F64NT>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT), F64LDS, st_,
+                                   Sout, phi, invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_, nQ);
             });
             return;
         }
@@ -2268,7 +2325,6 @@ private:
     }
 
     void gradients(EvalReq* rq, int n, int K, int nImg) {
-        const int nQ = n * nImg;
         // Vogel / adjoint for requests that ask for them (all virtual images are
         // processed; results of other requests are discarded by the zeroed scale)
         bool anyV = false, anyFd = false;
@@ -2470,6 +2526,7 @@ public:
 private:
     Geometry g_;
     int maxSlots_, maxBatch_;
+    int otfGrid_ = 1;                        // workgroups of the persistent FP64 OTF kernel
     hipStream_t st_;
     hipEvent_t ev0_, ev1_;
     bool timing_ = false;

@@ -134,11 +134,9 @@ def roofline(stats, N, nPS, nH, form):
       k_pq: read S and D                                             2 c nH
       k_phase_mfma: write phi                                        r nPS
       k_vogel_fused: read S, D, P, Q (FP64), phi; write v            2 c nH + 24 nH + 2 r nPS
-      k_g64_cols: read S, D; write the N x (N/2+1) column transforms 2 c nH + c N (N/2+1)
-      k_g64_rows: read those, write the real N x N G                c N (N/2+1) + r N^2
-      k_h64_rows: read phi, write the N/2 pupil-row transforms       r nPS + c N^2/2
-      k_h64_cols: read them and G, write the pupil rows back         2 c N^2/2 + r N^2
-      k_h64_final: read the pupil rows and phi, write v              c N^2/2 + 2 r nPS
+      k_grad64_fused: read S, D, phi, P and Q (FP64, shared by the
+        object's images); write and read back gl (N x N real); write v
+                                                                     2 c nH + 24 nH + r nPS + 2 r N^2 + r nPS
       unfused FFT pass: read and write the N x N tile                2 c N^2
     For the fused OTF kernels the FFT arithmetic is reported beside it: two N x N complex
     transforms, 5 N^2 log2(N^2) flops each, against the vector peak (78.6 TFLOP/s for
@@ -150,9 +148,7 @@ def roofline(stats, N, nPS, nH, form):
                 "k_power": 2 * c * N * N, "k_extract": 2 * c * nH, "k_phase_mfma": r * nPS, "k_pq": 2 * c * nH,
                 "k_otf_fused": r * nPS + c * nH, "k_otf64_fused": r * nPS + c * nH,
                 "k_vogel_fused": 2 * c * nH + 24 * nH + 2 * r * nPS,
-                "k_g64_cols": 2 * c * nH + c * N * (N // 2 + 1), "k_g64_rows": c * N * (N // 2 + 1) + r * N * N,
-                "k_h64_rows": r * nPS + c * half, "k_h64_cols": 2 * c * half + r * N * N,
-                "k_h64_final": c * half + 2 * r * nPS}
+                "k_grad64_fused": 2 * c * nH + 24 * nH + 2 * r * nPS + 2 * r * N * N}
     kern = {k: v for k, v in stats.items() if not k.startswith("host") and not k.startswith("_")}
     name, s = max(kern.items(), key=lambda kv: kv[1]["ms"])
     out = dict(kernel=name)

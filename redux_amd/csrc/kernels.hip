@@ -1318,191 +1318,257 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
     }
 }
 
-// FP64 Vogel / adjoint weights at N = 2P (N = 128) in five kernels with two
-// intermediates per gradient image in HBM, the same quantities as the unfused chain
-// (k_build_g, IFFT2, k_save_re, k_field, IFFT2, k_mul_re, FFT2, k_extract_v):
-//   k_g64_cols  G' (de-centred, Hermitian) -> inverse column FFTs of the N/2+1 columns
-//               of the half spectrum, C = [N][N/2+1]
-//   k_g64_rows  rows n and n+N/2 of C packed into one complex inverse FFT (both rows are
-//               Hermitian in x, so their transforms are real): greal = Re IFFT2(G') / N^2
-//   k_h64_rows  inverse row FFTs of the N/2 de-centred pupil-field rows (non-zero only at
-//               the two ends of the row), first pass pruned
-//   k_h64_cols  per column: inverse FFT (ends pruned) -> h, times greal / N^2, forward
-//               FFT; only the N/2 pupil rows of the result are kept (in place)
-//   k_h64_final forward row FFTs of those rows, then v_i = Im(conj(PF_i) t(pde_i))
-//               [* pupil_i] at the pupil pixels of the row
-// Pupil-row index i (0..N/2) of the de-centred field <-> grid row g = i for i < N/4,
-// g = i + N/2 otherwise.
-template <int N, int LC>
-__global__ void __launch_bounds__(LC * N / 16) k_g64_cols(cx<double>* __restrict__ Cg, const cx<double>* __restrict__ S,
-                                                          const cx<double>* __restrict__ D,
-                                                          const double* __restrict__ P, const double* __restrict__ Q,
-                                                          const int* __restrict__ invOs, QMap qm,
-                                                          const int* __restrict__ qReq, const int* __restrict__ imgObj,
-                                                          const int* __restrict__ gIdx, const cx<double>* __restrict__ tw,
-                                                          int nOS, int nImg, int nObj) {
-    constexpr int NT = LC * N / 16, HC = N / 2 + 1, NCB = (HC + LC - 1) / LC;
-    __shared__ cx<double> t[N * LC];                           // element (y, c) at y * LC + c
-    const int qt = blockIdx.x / NCB, c0 = (blockIdx.x % NCB) * LC;
-    const int q = gIdx[qt], img = qm.img[q], r = qReq[q], o = imgObj[img];
-    const cx<double>* Dq = D + ((size_t)qm.slot[q] * nImg + img) * nOS;
-    const cx<double>* Sq = S + ((size_t)r * nImg + img) * nOS;
-    const double* Po = P + 2 * ((size_t)r * nObj + o) * nOS;
-    const double* Qo = Q + ((size_t)r * nObj + o) * nOS;
-    // G' on the full support (0 elsewhere), as k_build_g builds it
-    auto gAt = [&](int p) -> cx<double> {
-        const int code = invOs[p];
-        if (code == -1) return {0.0, 0.0};
-        const int u = code >= 0 ? code : -2 - code;
-        const double pr = Po[2 * u], pi = Po[2 * u + 1], qq = Qo[u];
-        const double pqr = pr * qq, pqi = pi * qq, ps = pr * pr + pi * pi, qs = qq * qq;
-        const cx<double> d = Dq[u], sv = Sq[u];
-        const double gr = (pqr * d.x - pqi * d.y) - ps * sv.x;
-        const double gi = (pqr * d.y + pqi * d.x) - ps * sv.y;
-        return {gr / qs, (code >= 0 ? gi : -gi) / qs};
-    };
-    // Re IFFT2(G') = IFFT2 of the Hermitian part (G'(u) + G'(-u)*)/2, which equals G'
-    // wherever the support holds both u and -u
-    for (int e = threadIdx.x; e < N * LC; e += NT) {
-        const int y = e / LC, x = c0 + e % LC;
-        cx<double> val = {0.0, 0.0};
-        if (x < HC) {
-            const cx<double> a = gAt(y * N + x), b = gAt(((N - y) % N) * N + (N - x) % N);
-            val = {0.5 * (a.x + b.x), 0.5 * (a.y - b.y)};
-        }
-        t[e] = val;
-    }
-    __syncthreads();
-    fft_lines<double, N, +1, LC, NT, 1>(t, 1, LC, tw);
-    cx<double>* out = Cg + (size_t)qt * N * HC;
-    for (int e = threadIdx.x; e < N * LC; e += NT) {
-        const int n = e / LC, x = c0 + e % LC;
-        if (x < HC) out[(size_t)n * HC + x] = t[e];
-    }
-}
-
-template <int N, int LR>
-__global__ void __launch_bounds__(LR * N / 16) k_g64_rows(double* __restrict__ greal, const cx<double>* __restrict__ Cg,
-                                                          const cx<double>* __restrict__ tw) {
-    constexpr int NT = LR * N / 16, HC = N / 2 + 1, LD = N + 1;
-    __shared__ cx<double> t[LR * LD];
-    const int qt = blockIdx.x / (N / 2 / LR), r0 = (blockIdx.x % (N / 2 / LR)) * LR;
-    const cx<double>* C = Cg + (size_t)qt * N * HC;
-    for (int e = threadIdx.x; e < LR * N; e += NT) {
-        const int r = e / N, k = e % N, n = r0 + r;
-        // Z(k) = C(n, k) + i C(n + N/2, k), with C(., k) = C(., N - k)* for k > N/2
-        const int kk = k <= N / 2 ? k : N - k;
-        cx<double> c1 = C[(size_t)n * HC + kk], c2 = C[(size_t)(n + N / 2) * HC + kk];
-        if (k > N / 2) {
-            c1.y = -c1.y;
-            c2.y = -c2.y;
-        }
-        t[r * LD + k] = {c1.x - c2.y, c1.y + c2.x};
-    }
-    __syncthreads();
-    fft_lines<double, N, +1, LR, NT, 1>(t, LD, 1, tw);
+// FP64 Vogel / adjoint weights in one persistent kernel per gradient image (N = 64,
+// 128), the quantities of the unfused chain (k_build_g, IFFT2, k_save_re, k_field,
+// IFFT2, k_mul_re, FFT2, k_extract_v) with every intermediate but one in
+// the P x (N+1) complex-double LDS buffer of k_otf64_fused (line/slot layout `at` as
+// there).  gl = Re IFFT2(G'_H)/N^2 goes through HBM (`greal`, one N x N slice per
+// gradient image, written by this workgroup and read back by it once L2 has acknowledged
+// the stores), since it is needed while the buffer holds the pupil field.
+//   A1. G'_H = (G'(u) + G'(-u)*)/2, the Hermitian part of G' = (PQ D - PS S)/QS on the
+//       support (de-centred natural layout), as lines over y: line k (0 < k < N/2) is
+//       column k, line 0 is G'_H(., 0) + i G'_H(., N/2).  G' is evaluated once per
+//       stored point and written where the host table gw puts it: G' at u and G'* at -u
+//       for a pair, G'/2 and G'*/2 when -u lies outside the support, Re G' when u = -u
+//   A2. inverse FFTs of the lines: C(n, k), line 0 holding C(n, 0) + i C(n, N/2) (real
+//       columns, since G'_H is Hermitian)
+//   A3. rows n and n + N/2 packed into one complex row, Z_n(k) = C(n, k) + i C(n + N/2, k),
+//       extended to k > N/2 by C(n, N-k) = C(n, k)*
+//   A4. inverse row FFTs: Re Z_n = N^2 gl(n, .), Im Z_n = N^2 gl(n + N/2, .)
+//   B1. inverse row FFTs of the P rows of the de-centred pupil field PF' (non-zero in
+//       rows and columns [0, N/4) and [3N/4, N); buffer row i is grid row i for i < N/4,
+//       i + N/2 otherwise), first pass pruned to the two ends and formed from phi
+//   B2. per column: inverse FFT (first pass pruned to the pupil rows) -> hj, times
+//       gl / N^2, forward FFT, keeping the output rows of the pupil; the columns go in
+//       two halves as in k_otf64_fused
+//   B3. forward row FFTs; v_i = Im(conj(PF_i) t(pde_i)) [* pupil_i] at the pupil pixels
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_grad64_fused(double* __restrict__ v, double* __restrict__ greal,
+                                                     const cx<double>* __restrict__ S, const cx<double>* __restrict__ D,
+                                                     const double* __restrict__ P, const double* __restrict__ Q,
+                                                     const int* __restrict__ gw, QMap qm,
+                                                     const int* __restrict__ qReq, const int* __restrict__ imgObj,
+                                                     const int* __restrict__ gIdx, const double* __restrict__ phi,
+                                                     const int* __restrict__ invPs, const double* __restrict__ pupil,
+                                                     const cx<double>* __restrict__ twg, int nPS, int nOS, int nImg,
+                                                     int nObj, int nG, int extraPupil) {
+    constexpr int H = N / 2, LD = N + 1, Q4 = N / 4;
+    constexpr int RF = N == 64 ? 8 : 16, RL = 8;
+    static_assert(RF * RL == N, "two-pass transforms only (N = 64, 128)");
+    constexpr int NB1 = N * (N / RF), NB2 = H * (N / RL);
+    constexpr int IH = NB1 / 2 / NT, I2 = NB2 / NT;
+    static_assert(NB1 / 2 % NT == 0 && NB2 % NT == 0, "thread count must divide the butterflies");
     constexpr double inv = 1.0 / ((double)N * N);
-    double* g = greal + (size_t)qt * N * N;
-    for (int e = threadIdx.x; e < LR * N; e += NT) {
-        const int r = e / N, x = e % N;
-        const cx<double> z = t[r * LD + x];
-        g[(size_t)(r0 + r) * N + x] = z.x * inv;
-        g[(size_t)(r0 + r + N / 2) * N + x] = z.y * inv;
-    }
-}
-
-template <int N, int LR>
-__global__ void __launch_bounds__(LR * N / 16) k_h64_rows(cx<double>* __restrict__ A, const double* __restrict__ phi,
-                                                          const int* __restrict__ invPde,
-                                                          const double* __restrict__ pupil,
-                                                          const int* __restrict__ gIdx, const cx<double>* __restrict__ tw,
-                                                          int nPS) {
-    constexpr int NT = LR * N / 16, LD = N + 1;
-    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
-    __shared__ cx<double> t[LR * LD];
-    const int qt = blockIdx.x / (N / 2 / LR), i0 = (blockIdx.x % (N / 2 / LR)) * LR;
-    const double* ph = phi + (size_t)gIdx[qt] * nPS;
-    for (int e = threadIdx.x; e < LR * (N / 2); e += NT) {
-        const int r = e / (N / 2), xi = e % (N / 2);
-        const int i = i0 + r, gy = i < N / 4 ? i : i + N / 2;
-        const int x = xi < N / 4 ? xi : xi + N / 2;              // the two ends of the row
-        const int idx = invPde[gy * N + x];
-        cx<double> val = {0.0, 0.0};
-        if (idx >= 0) {
+    extern __shared__ cx<double> lds64[];
+    cx<double>* t = lds64;
+    cx<double>* tw = lds64 + H * LD;
+    cx<double>* side = tw + N;                                   // G'_H columns 0 and N/2
+    for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
+    auto at = [](int k, int y) { return y < H ? y * LD + k : (y - H) * LD + (k == 0 ? H : N - k); };
+    for (int qt = blockIdx.x; qt < nG; qt += gridDim.x) {
+        const int q = gIdx[qt], img = qm.img[q], r = qReq[q], o = imgObj[img];
+        const cx<double>* Dq = D + ((size_t)qm.slot[q] * nImg + img) * nOS;
+        const cx<double>* Sq = S + ((size_t)r * nImg + img) * nOS;
+        const double* Po = P + 2 * ((size_t)r * nObj + o) * nOS;
+        const double* Qo = Q + ((size_t)r * nObj + o) * nOS;
+        // A1. zero the lines and the side area, write G'_H of every stored point to its
+        //     targets, then line 0 = G'_H(., 0) + i G'_H(., N/2)
+        for (int e = threadIdx.x; e < H * LD; e += NT) t[e] = {0.0, 0.0};
+        for (int e = threadIdx.x; e < 2 * N; e += NT) side[e] = {0.0, 0.0};
+        __syncthreads();
+        for (int u = threadIdx.x; u < nOS; u += NT) {
+            const double pr = Po[2 * u], pi = Po[2 * u + 1], qq = Qo[u];
+            const cx<double> d = Dq[u], sv = Sq[u];
+            const double pqr = pr * qq, pqi = pi * qq, ps = pr * pr + pi * pi, qs = qq * qq;
+            const double gr = ((pqr * d.x - pqi * d.y) - ps * sv.x) / qs;
+            const double gi = ((pqr * d.y + pqi * d.x) - ps * sv.y) / qs;
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int c = gw[w * nOS + u];
+                if (c < 0) continue;
+                const int f = c & 7;
+                const double sc = (f & 2) ? 0.5 : 1.0;
+                t[c >> 3] = {sc * gr, (f & 4) ? 0.0 : ((f & 1) ? -sc * gi : sc * gi)};
+            }
+        }
+        __syncthreads();
+        for (int y = threadIdx.x; y < N; y += NT) {
+            const cx<double> a = side[y], b = side[N + y];
+            t[at(0, y)] = {a.x - b.y, a.y + b.x};
+        }
+        __syncthreads();
+        // A2.
+        spass_at<double, N, RF, 1, +1, H, NT>(t, at, tw);
+        spass_at<double, N, RL, RF, +1, H, NT>(t, at, tw);
+        // A3. row n: slot k holds C(n, k), slot N-k holds C(n + N/2, k); slot 0 holds
+        //     C(n, 0) + i C(n, N/2), slot N/2 holds C(n + N/2, 0) + i C(n + N/2, N/2)
+        for (int e = threadIdx.x; e < H * H; e += NT) {
+            const int n = e / H, k = e % H;
+            cx<double>* row = t + n * LD;
+            const cx<double> a = row[k], b = row[k == 0 ? H : N - k];
+            if (k == 0) {
+                row[0] = {a.x, b.x};
+                row[H] = {a.y, b.y};
+            } else {
+                row[k] = {a.x - b.y, a.y + b.x};
+                row[N - k] = {a.x + b.y, b.x - a.y};
+            }
+        }
+        __syncthreads();
+        // A4.
+        fft_lines<double, N, +1, H, NT, 1>(t, LD, 1, tw);
+        double* gq = greal + (size_t)qt * N * N;
+        for (int e = threadIdx.x; e < H * N; e += NT) {
+            const int n = e / N, x = e % N;
+            const cx<double> z = t[n * LD + x];
+            gq[(size_t)n * N + x] = z.x * inv;
+            gq[(size_t)(n + H) * N + x] = z.y * inv;
+        }
+        // gl is read back only by this workgroup, from a slice that nothing else touches
+        // during the launch (so no L1 line of it can be stale): its stores must have been
+        // acknowledged by L2 before any wave passes the barrier.  A device-scope fence
+        // would also write back the whole L2, which this does not need.
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        // B1. butterfly (row i, j) of the first pass reads columns j + m N/RF, m in the
+        //     two ends [0, RF/4) and [3RF/4, RF)
+        const double* ph = phi + (size_t)q * nPS;
+        {
+            constexpr int NB = H * (N / RF), BT = NB / NT;
+            static_assert(NB % NT == 0, "");
+            cx<double> w[BT][RF];
+#pragma unroll
+            for (int i = 0; i < BT; ++i) {
+                const int b = threadIdx.x + i * NT, row = b % H, j = b / H;
+                const int py = (row + Q4) % H;                      // pupil row of buffer row `row`
+#pragma unroll
+                for (int m = 0; m < RF; ++m) {
+                    w[i][m] = {0.0, 0.0};
+                    if (m < RF / 4 || m >= 3 * RF / 4) {
+                        const int x = j + m * (N / RF), px = x < Q4 ? x + Q4 : x - 3 * Q4;
+                        const int k = invPs[py * H + px];
+                        if (k >= 0) {
+                            double sn, cs;
+                            sincos(ph[k], &sn, &cs);
+                            w[i][m] = {pupil[k] * cs, pupil[k] * sn};
+                        }
+                    }
+                }
+                dft_reg<double, RF, +1>(w[i]);
+            }
+#pragma unroll
+            for (int i = 0; i < BT; ++i) {
+                const int b = threadIdx.x + i * NT, row = b % H, j = b / H;
+#pragma unroll
+                for (int m = 0; m < RF; ++m) t[row * LD + j * RF + m] = w[i][m];
+            }
+        }
+        __syncthreads();
+        spass<double, N, RL, RF, +1, H, NT, 1>(t, LD, 1, tw);
+        // B2. column butterfly (c, j) of the first pass reads grid rows j + m N/RF for m in
+        //     the two ends, i.e. buffer rows j + m' N/RF, m' = m for m < RF/4, m - RF/2 after
+        cx<double> x1[2][IH][RF / 2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < IH; ++i) {
+                const int bb = threadIdx.x + i * NT, c = bb % H + h * H, j = bb / H;
+#pragma unroll
+                for (int m = 0; m < RF / 2; ++m) x1[h][i][m] = t[(j + m * (N / RF)) * LD + c];
+            }
+        __syncthreads();
+        cx<double> keep[2][I2][RL / 2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            // inverse first pass: column c as buffer row c % H, element j RF + m
+#pragma unroll
+            for (int i = 0; i < IH; ++i) {
+                const int bb = threadIdx.x + i * NT, c = bb % H, j = bb / H;
+                cx<double> w[RF];
+#pragma unroll
+                for (int m = 0; m < RF; ++m)
+                    w[m] = m < RF / 4 ? x1[h][i][m] : m >= 3 * RF / 4 ? x1[h][i][m - RF / 2] : cx<double>{0.0, 0.0};
+                dft_reg<double, RF, +1>(w);
+#pragma unroll
+                for (int m = 0; m < RF; ++m) t[c * LD + j * RF + m] = w[m];
+            }
+            __syncthreads();
+            // inverse second pass; output y = j + m RF times gl(y, column) / N^2.  A last
+            // pass writes exactly the elements it read (j + m RF), so each butterfly
+            // finishes in place without waiting for the others.
+#pragma unroll
+            for (int i = 0; i < I2; ++i) {
+                const int b = threadIdx.x + i * NT, c = b % H, j = b / H;
+                cx<double> w[RL];
+                double f[RL];
+#pragma unroll
+                for (int m = 0; m < RL; ++m) f[m] = gq[(size_t)(j + m * RF) * N + c + h * H] * inv;
+#pragma unroll
+                for (int m = 0; m < RL; ++m) {
+                    cx<double> x = t[c * LD + j + m * RF];
+                    if (m > 0) {
+                        cx<double> tv = tw[m * j * (N / (RF * RL))];
+                        tv.y = -tv.y;
+                        x = cmul(x, tv);
+                    }
+                    w[m] = x;
+                }
+                dft_reg<double, RL, +1>(w);
+#pragma unroll
+                for (int m = 0; m < RL; ++m) t[c * LD + j + m * RF] = {w[m].x * f[m], w[m].y * f[m]};
+            }
+            __syncthreads();
+            // forward transform of the natural-order column
+            spass<double, N, RF, 1, -1, H, NT, 1>(t, LD, 1, tw);
+            // forward second pass, keeping outputs y = j + m RF in the two ends
+#pragma unroll
+            for (int i = 0; i < I2; ++i) {
+                const int b = threadIdx.x + i * NT, c = b % H, j = b / H;
+                cx<double> w[RL];
+#pragma unroll
+                for (int m = 0; m < RL; ++m) {
+                    cx<double> x = t[c * LD + j + m * RF];
+                    if (m > 0) x = cmul(x, tw[m * j * (N / (RF * RL))]);
+                    w[m] = x;
+                }
+                dft_reg<double, RL, -1>(w);
+#pragma unroll
+                for (int m = 0; m < RL / 4; ++m) {
+                    keep[h][i][m] = w[m];
+                    keep[h][i][m + RL / 4] = w[m + 3 * RL / 4];
+                }
+            }
+            __syncthreads();
+        }
+        // kept row y = j + m RF (ends) -> buffer row y (y < N/4) or y - N/2
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int i = 0; i < I2; ++i) {
+                const int b = threadIdx.x + i * NT, c = b % H + h * H, j = b / H;
+#pragma unroll
+                for (int m = 0; m < RL / 4; ++m) {
+                    t[(j + m * RF) * LD + c] = keep[h][i][m];
+                    t[(j + (m + 3 * RL / 4) * RF - H) * LD + c] = keep[h][i][m + RL / 4];
+                }
+            }
+        __syncthreads();
+        // B3.
+        fft_lines<double, N, -1, H, NT, 1>(t, LD, 1, tw);
+        double* vq = v + (size_t)qt * nPS;
+        for (int e = threadIdx.x; e < H * H; e += NT) {
+            const int py = e / H, px = e % H;
+            const int k = invPs[e];
+            if (k < 0) continue;
+            const int row = (py + Q4) % H, x = ((px - Q4) % N + N) % N;
+            const cx<double> tv = t[row * LD + x];
             double sn, cs;
-            sincos(ph[idx], &sn, &cs);
-            val = {pupil[idx] * cs, pupil[idx] * sn};
+            sincos(ph[k], &sn, &cs);
+            const double pu = pupil[k];
+            const double im = (pu * cs) * tv.y - (pu * sn) * tv.x;   // Im(conj(PF) t)
+            vq[k] = extraPupil ? im * pu : im;
         }
-        t[r * LD + x] = val;
-    }
-    __syncthreads();
-    spass_first_pruned<double, N, RF, LR, NT, 1, +1, 1>(t, LD, 1);
-    spass<double, N, RL, RF, +1, LR, NT, 1>(t, LD, 1, tw);
-    cx<double>* out = A + ((size_t)qt * (N / 2) + i0) * N;
-    for (int e = threadIdx.x; e < LR * N; e += NT) out[e] = t[(e / N) * LD + e % N];
-}
-
-template <int N, int LC>
-__global__ void __launch_bounds__(LC * N / 16) k_h64_cols(cx<double>* __restrict__ A, const double* __restrict__ greal,
-                                                          const cx<double>* __restrict__ tw) {
-    constexpr int NT = LC * N / 16;
-    constexpr int RF = N == 64 ? 8 : 16, RL = N == 256 ? 16 : 8;
-    __shared__ cx<double> t[N * LC];                           // element (row g, column c) at g * LC + c
-    const int qt = blockIdx.x / (N / LC), c0 = (blockIdx.x % (N / LC)) * LC;
-    cx<double>* img = A + (size_t)qt * (N / 2) * N;
-    for (int e = threadIdx.x; e < (N / 2) * LC; e += NT) {
-        const int i = e / LC, c = e % LC, gy = i < N / 4 ? i : i + N / 2;
-        t[gy * LC + c] = img[(size_t)i * N + c0 + c];
-    }
-    __syncthreads();
-    spass_first_pruned<double, N, RF, LC, NT, 1, +1, 1>(t, 1, LC);
-    spass<double, N, RL, RF, +1, LC, NT, 1>(t, 1, LC, tw);
-    // y = h * greal / N^2, then the forward column transform
-    constexpr double inv = 1.0 / ((double)N * N);
-    const double* g = greal + (size_t)qt * N * N;
-    for (int e = threadIdx.x; e < N * LC; e += NT) {
-        const int y = e / LC, c = e % LC;
-        const double f = g[(size_t)y * N + c0 + c] * inv;
-        t[e] = {t[e].x * f, t[e].y * f};
-    }
-    __syncthreads();
-    spass<double, N, RF, 1, -1, LC, NT, 1>(t, 1, LC, tw);
-    spass<double, N, RL, RF, -1, LC, NT, 1>(t, 1, LC, tw);
-    for (int e = threadIdx.x; e < (N / 2) * LC; e += NT) {
-        const int i = e / LC, c = e % LC, gy = i < N / 4 ? i : i + N / 2;
-        img[(size_t)i * N + c0 + c] = t[gy * LC + c];
-    }
-}
-
-template <int N, int LR>
-__global__ void __launch_bounds__(LR * N / 16) k_h64_final(double* __restrict__ v, const cx<double>* __restrict__ A,
-                                                           const double* __restrict__ phi,
-                                                           const int* __restrict__ invPs,
-                                                           const double* __restrict__ pupil,
-                                                           const int* __restrict__ gIdx,
-                                                           const cx<double>* __restrict__ tw, int nPS, int extraPupil) {
-    constexpr int NT = LR * N / 16, LD = N + 1, P = N / 2;
-    __shared__ cx<double> t[LR * LD];
-    const int qt = blockIdx.x / (N / 2 / LR), i0 = (blockIdx.x % (N / 2 / LR)) * LR;
-    const cx<double>* src = A + ((size_t)qt * (N / 2) + i0) * N;
-    for (int e = threadIdx.x; e < LR * N; e += NT) t[(e / N) * LD + e % N] = src[e];
-    __syncthreads();
-    fft_lines<double, N, -1, LR, NT, 1>(t, LD, 1, tw);
-    const double* ph = phi + (size_t)gIdx[qt] * nPS;
-    double* vq = v + (size_t)qt * nPS;
-    for (int e = threadIdx.x; e < LR * P; e += NT) {
-        const int r = e / P, px = e % P, i = i0 + r;
-        const int gy = i < N / 4 ? i : i + N / 2;
-        const int py = (gy + P / 2) % N;                        // pupil row whose de-centred row is gy
-        const int idx = invPs[py * P + px];
-        if (idx < 0) continue;
-        const int x = ((px - P / 2) % N + N) % N;
-        const cx<double> tv = t[r * LD + x];
-        double sn, cs;
-        sincos(ph[idx], &sn, &cs);
-        const double pu = pupil[idx];
-        const double im = (pu * cs) * tv.y - (pu * sn) * tv.x;   // Im(conj(PF) t)
-        vq[idx] = extraPupil ? im * pu : im;
+        __syncthreads();
     }
 }
 
@@ -1668,8 +1734,7 @@ public:
     static constexpr bool FUSED64 = std::is_same_v<T, double> && N <= 128;
     static constexpr int F64NT = N == 128 ? 512 : 256;
     static constexpr size_t F64LDS = ((size_t)(N / 2) * (N + 1) + N) * sizeof(cx<double>);
-    // FP64 gradient weights at N = 128 in five kernels
-    static constexpr bool SPLIT64 = std::is_same_v<T, double> && N == 128;
+    static constexpr size_t G64LDS = F64LDS + 2 * N * sizeof(cx<double>);     // plus two G'_H columns
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device, bool bf16Phase = false)
         : g_(g), maxSlots_(maxSlots), maxBatch_(maxBatch), bf16Phase_(bf16Phase && std::is_same_v<T, float>) {
@@ -1763,6 +1828,31 @@ public:
                 hread[h] = nx <= N / 2 ? ny * (N / 2 + 1) + nx : -1 - (((N - ny) % N) * (N / 2 + 1) + (N - nx));
             }
             upload(hread_, hread, st_);
+            // where k_grad64_fused writes G'_H of stored point h (two targets, -1 for none):
+            // target * 8 + flags (1: conjugate, 2: half, 4: real part), in its LDS line
+            // layout, columns 0 and N/2 in a side area after the buffer and twiddles
+            std::vector<int> gw(2 * (size_t)nOS_, -1);
+            auto target = [&](int nat) {
+                const int y = nat / N, x = nat % N, base = (N / 2) * (N + 1) + N;
+                if (x > N / 2) return -1;
+                if (x == 0) return base + y;
+                if (x == N / 2) return base + N + y;
+                return y < N / 2 ? y * (N + 1) + x : (y - N / 2) * (N + 1) + (N - x);
+            };
+            auto enc = [](int t, int f) { return t < 0 ? -1 : t * 8 + f; };
+            for (int h = 0; h < nOS_; ++h) {
+                const int pn = natOf(rep[h]), mn = ((N - pn / N) % N) * N + (N - pn % N) % N;
+                if (mirNat[h] >= 0) {                       // paired: G'_H = G' at p, G'* at -p
+                    gw[h] = enc(target(pn), 0);
+                    gw[nOS_ + h] = enc(target(mn), 1);
+                } else if (mn == pn) {                      // its own mirror: Re G'
+                    gw[h] = enc(target(pn), 4);
+                } else {                                    // -p outside the support
+                    gw[h] = enc(target(pn), 2);
+                    gw[nOS_ + h] = enc(target(mn), 3);
+                }
+            }
+            upload(gw_, gw, st_);
             std::vector<int> invPs(P2, -1);
             for (int k = 0; k < nPS_; ++k) invPs[ps_[k]] = k;
             upload(invPs_, invPs, st_);
@@ -1809,6 +1899,8 @@ public:
         if constexpr (FUSED64) {
             HIPCHK(hipFuncSetAttribute((const void*)k_otf64_fused<N, F64NT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)F64LDS));
+            HIPCHK(hipFuncSetAttribute((const void*)k_grad64_fused<N, F64NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)G64LDS));
             int nCU = 0, dev = 0;
             HIPCHK(hipGetDevice(&dev));
             HIPCHK(hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2369,33 +2461,13 @@ private:
                                        osyx_.p + nOS_, osyx_.p + 2 * nOS_, osyx_.p + 3 * nOS_, sc_->qm, sc_->qReq, imgObj_.p,
                                        sc_->gIdx.p, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
                 });
-            } else if constexpr (SPLIT64) {
-                constexpr int L16 = 16, HC = N / 2 + 1;
-                constexpr size_t per = (size_t)std::max(N * HC, (N / 2) * N);   // C, then the pupil rows
-                sc_->tiles.alloc((size_t)nG * per);
+            } else if constexpr (FUSED64) {
                 sc_->greal.alloc((size_t)nG * N2);
-                C* Cg = sc_->tiles.p;
-                launch("k_g64_cols", nG, [&] {
-                    hipLaunchKernelGGL((k_g64_cols<N, L16>), dim3(nG * ((HC + L16 - 1) / L16)), dim3(L16 * N / 16), 0,
-                                       st_, Cg, sc_->S.p, D_.p, sc_->Pp.p, sc_->Qp.p, invOs_.p, sc_->qm, sc_->qReq,
-                                       imgObj_.p, sc_->gIdx.p, tw_.p, nOS_, nImg, g_.nObj);
-                });
-                launch("k_g64_rows", nG, [&] {
-                    hipLaunchKernelGGL((k_g64_rows<N, L16>), dim3(nG * (N / 2 / L16)), dim3(L16 * N / 16), 0, st_,
-                                       sc_->greal.p, Cg, tw_.p);
-                });
-                launch("k_h64_rows", nG, [&] {
-                    hipLaunchKernelGGL((k_h64_rows<N, L16>), dim3(nG * (N / 2 / L16)), dim3(L16 * N / 16), 0, st_,
-                                       Cg, sc_->phi.p, invPde_.p, pupil_.p, sc_->gIdx.p, tw_.p, nPS_);
-                });
-                launch("k_h64_cols", nG, [&] {
-                    hipLaunchKernelGGL((k_h64_cols<N, L16>), dim3(nG * (N / L16)), dim3(L16 * N / 16), 0, st_, Cg,
-                                       sc_->greal.p, tw_.p);
-                });
-                launch("k_h64_final", nG, [&] {
-                    hipLaunchKernelGGL((k_h64_final<N, L16>), dim3(nG * (N / 2 / L16)), dim3(L16 * N / 16), 0, st_,
-                                       sc_->v.p, Cg, sc_->phi.p, invPs_.p, pupil_.p, sc_->gIdx.p, tw_.p, nPS_,
-                                       extraPupil);
+                launch("k_grad64_fused", nG, [&] {
+                    hipLaunchKernelGGL((k_grad64_fused<N, F64NT>), dim3(std::min(nG, otfGrid_)), dim3(F64NT), G64LDS,
+                                       st_, sc_->v.p, sc_->greal.p, sc_->S.p, D_.p, sc_->Pp.p, sc_->Qp.p, gw_.p,
+                                       sc_->qm, sc_->qReq, imgObj_.p, sc_->gIdx.p, sc_->phi.p, invPs_.p, pupil_.p,
+                                       tw_.p, nPS_, nOS_, nImg, g_.nObj, nG, extraPupil);
                 });
             } else {
                 sc_->tiles.alloc((size_t)nG * N2);
@@ -2551,6 +2623,7 @@ private:
     bool bf16Phase_ = false;
     HBuf<double> hStage_;
     DBuf<double> ftSum_, gamma_, win_, w_, wt_;
+    DBuf<int> gw_;
     DBuf<int> invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_, pyx_, osyx_, hread_,
         invPs_;
     DBuf<float> rawF_;

@@ -58,7 +58,8 @@ public:
     // listed images at their shifts; patch side <= 128
     virtual void frame_stats(const std::vector<ImgInit>& imgs, const std::vector<double>& window,
                              const std::vector<double>& noiseWindow, double* noise, double* stddev) = 0;
-    virtual void set_gamma(int slot, const double* gamma) = 0;
+    // regularisation gamma of n slots, gamma[k * nObj + o] for slot slots[k]
+    virtual void set_gamma(int n, const int* slots, const double* gamma) = 0;
     virtual void evaluate(std::vector<EvalReq>& reqs) = 0;
     // asynchronous form on lane 0 or 1: begin enqueues and returns, end waits and fills
     // f / g; the two lanes let the host work on one half of a batch while the device

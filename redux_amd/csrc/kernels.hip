@@ -707,10 +707,11 @@ __global__ void __launch_bounds__(256) k_grad_mfma(double* __restrict__ part, co
     }
 }
 
-// nEnabled[r]: the enabled-mode count of the request that image r belongs to (each
-// request has its own; part holds the columns of the batch's largest count)
+// Row r of g is gradient image r (the order of gIdx); nEnabled[r] is the enabled-mode
+// count of its request (each request has its own; part holds the columns of the
+// batch's largest count)
 __global__ void k_grad_sum(double* __restrict__ g, const double* __restrict__ part, const double* __restrict__ scale,
-                           const int* __restrict__ gIdx, const int* __restrict__ nEnabled, int nG, int K, int Kt) {
+                           const int* __restrict__ nEnabled, int nG, int K, int Kt) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nG * K) return;
     const int r = e / K, m = e % K;
@@ -718,7 +719,7 @@ __global__ void k_grad_sum(double* __restrict__ g, const double* __restrict__ pa
     double sum = 0.0;
     if (on)
         for (int sl = 0; sl < GR_SLICES; ++sl) sum += part[((size_t)sl * nG + r) * Kt + m];
-    g[(size_t)gIdx[r] * K + m] = on ? scale[r] * sum : 0.0;
+    g[(size_t)r * K + m] = on ? scale[r] * sum : 0.0;
 }
 
 // dl = -sum_u (q dn - dq |p|^2) / (q (q + dq)) / N^2 for one perturbed image
@@ -2013,9 +2014,14 @@ public:
         collect(2);
     }
 
-    void set_gamma(int slot, const double* gamma) override {
-        HIPCHK(hipMemcpyAsync(gamma_.p + (size_t)slot * g_.nObj, gamma, g_.nObj * sizeof(double),
-                              hipMemcpyHostToDevice, st_));
+    // gamma[k * nObj + o] of slot slots[k], all in one copy
+    void set_gamma(int n, const int* slots, const double* gamma) override {
+        if (n <= 0) return;
+        hGamma_.resize((size_t)maxSlots_ * g_.nObj, 0.0);
+        for (int k = 0; k < n; ++k)
+            std::copy(gamma + (size_t)k * g_.nObj, gamma + (size_t)(k + 1) * g_.nObj,
+                      hGamma_.begin() + (size_t)slots[k] * g_.nObj);
+        HIPCHK(hipMemcpyAsync(gamma_.p, hGamma_.data(), hGamma_.size() * sizeof(double), hipMemcpyHostToDevice, st_));
         HIPCHK(hipStreamSynchronize(st_));
     }
 
@@ -2296,7 +2302,8 @@ private:
     // phases, OTFs, P/Q and the metric for n requests (virtual images q = r*nImg + j).
     // Alpha rows come from the slot-indexed pinned staging when every request is
     // staged there (the solver's case), else they are packed per request.
-    void forward(EvalReq* rq, int n, int useGamma) {
+    void forward(EvalReq* rq, int n, int useGamma, const std::vector<int>* xi = nullptr,
+                 const std::vector<double>* xd = nullptr) {
         const int K = g_.K, nImg = g_.nImg, nQ = n * nImg;
         const size_t row = (size_t)nImg * K;
         bool staged = true;
@@ -2306,9 +2313,13 @@ private:
             lo = std::min(lo, rq[r].slot);
             hi = std::max(hi, rq[r].slot);
         }
-        const size_t nInts = 6 * (size_t)nQ + n;
-        sc_->hInts.alloc(nInts);
-        int* ints = sc_->hInts.p;
+        // one pinned block and one copy per evaluation: the caller's doubles xd, then the
+        // request tables (6 nQ + n ints), then the caller's ints xi
+        const size_t nd = xd ? xd->size() : 0, nx = xi ? xi->size() : 0;
+        const size_t nInts = 6 * (size_t)nQ + n + nx, nBlk = nd + (nInts + 1) / 2;
+        sc_->hTab.alloc(nBlk);
+        if (nd) std::memcpy(sc_->hTab.p, xd->data(), nd * sizeof(double));
+        int* ints = reinterpret_cast<int*>(sc_->hTab.p + nd);
         for (int r = 0; r < n; ++r) {
             for (int j = 0; j < nImg; ++j) {
                 const int q = r * nImg + j;
@@ -2321,6 +2332,7 @@ private:
             }
             ints[4 * nQ + r] = rq[r].slot;
         }
+        if (nx) std::memcpy(ints + 6 * (size_t)nQ + n, xi->data(), nx * sizeof(int));
         if (staged) {
             const size_t cnt = (size_t)(hi - lo + 1) * row;
             sc_->alpha.alloc(std::max(cnt, sc_->alpha.n));
@@ -2332,13 +2344,16 @@ private:
             sc_->alpha.alloc((size_t)nQ * K);
             HIPCHK(hipMemcpyAsync(sc_->alpha.p, sc_->hAlpha.p, (size_t)nQ * K * sizeof(double), hipMemcpyHostToDevice, st_));
         }
-        sc_->ints.alloc(nInts);
-        HIPCHK(hipMemcpyAsync(sc_->ints.p, ints, nInts * sizeof(int), hipMemcpyHostToDevice, st_));
-        sc_->qm = {sc_->ints.p, sc_->ints.p + nQ, sc_->ints.p + 2 * nQ};
-        sc_->qReq = sc_->ints.p + 3 * nQ;
-        sc_->reqSlot = sc_->ints.p + 4 * nQ;
-        const int* qPhRow = sc_->ints.p + 4 * nQ + n;
-        const int* qObj = sc_->ints.p + 5 * nQ + n;
+        sc_->tab.alloc(nBlk);
+        HIPCHK(hipMemcpyAsync(sc_->tab.p, sc_->hTab.p, nBlk * sizeof(double), hipMemcpyHostToDevice, st_));
+        const int* ti = reinterpret_cast<const int*>(sc_->tab.p + nd);
+        sc_->qm = {ti, ti + nQ, ti + 2 * nQ};
+        sc_->qReq = ti + 3 * nQ;
+        sc_->reqSlot = ti + 4 * nQ;
+        const int* qPhRow = ti + 4 * nQ + n;
+        const int* qObj = ti + 5 * nQ + n;
+        sc_->tx = ti + 6 * nQ + n;
+        sc_->td = sc_->tab.p;
         sc_->phi.alloc((size_t)nQ * nPS_);
         sc_->S.alloc((size_t)nQ * nOS_);
         if (bf16Phase_) {
@@ -2382,91 +2397,88 @@ private:
         sc_ = &lanes_[lane];
     }
 
+    // Gradient tables of the requests that want a Vogel / adjoint gradient: their virtual
+    // images (gIdx, request order), enabled-mode counts (gEn) and output scales (gs)
+    void grad_tables(const EvalReq* rq, int n, int nImg, std::vector<int>& gIdx, std::vector<int>& gEn,
+                     std::vector<double>& gs, int& nEn, int& extraPupil) {
+        nEn = 0;
+        extraPupil = 1;
+        for (int r = 0; r < n; ++r) {
+            if (!rq[r].wantGrad || rq[r].gradMethod == 1) continue;
+            nEn = std::max(nEn, rq[r].nEnabled);
+            for (int j = 0; j < nImg; ++j) {
+                const int o = st.imgObj[j];
+                const double w = st.weight[o];
+                gIdx.push_back(r * nImg + j);
+                gEn.push_back(rq[r].nEnabled);
+                if (rq[r].gradMethod == 2) {
+                    // gradientVogel: -2 area/N^2 * w, times w again in Solver::gradient
+                    gs.push_back((w == 0.0) ? 0.0 : -2.0 * area_ / N2 * w * w);
+                } else {
+                    // exact dL/dalpha = -(4 c^2 / s^2) w sum Im(conj(a) (G * a)), t = (G * PF)/N^2
+                    const double c = otfNorm_[st.imgCh[j]];
+                    gs.push_back(-4.0 * c * c * N2 / ((double)g_.s * g_.s) * w);
+                    extraPupil = 0;
+                }
+            }
+        }
+    }
+
+    // One chunk of requests: one table upload, the forward chain, the metric, the
+    // gradients, and one download of [L (n) | Vogel / adjoint gradient rows (nG x K)]
     void eval_chunk_launch(EvalReq* rq, int n, int K, int nImg) {
-        forward(rq, n, 1);
-        const int nQ = n * nImg;
+        std::vector<int> gIdx, gEn;
+        std::vector<double> gs;
+        int nEn = 0, extraPupil = 1;
+        grad_tables(rq, n, nImg, gIdx, gEn, gs, nEn, extraPupil);
+        const int nG = (int)gIdx.size();
+        gIdx.insert(gIdx.end(), gEn.begin(), gEn.end());       // [0, nG): images, [nG, 2nG): counts
+        forward(rq, n, 1, &gIdx, &gs);
         const int nChunk = (nOS_ + 255) / 256;
-        sc_->L.alloc(n);
+        const size_t nOut = (size_t)n + (size_t)nG * K;
+        sc_->out.alloc(nOut);
         launch("k_metric", [&] {
-            hipLaunchKernelGGL(k_metric, dim3(nblk(n, 64)), dim3(64), 0, st_, sc_->L.p, sc_->part.p, w_.p, n, g_.nObj,
+            hipLaunchKernelGGL(k_metric, dim3(nblk(n, 64)), dim3(64), 0, st_, sc_->out.p, sc_->part.p, w_.p, n, g_.nObj,
                                nChunk, 1.0 / ((double)g_.s * g_.s));
         });
-        bool anyGrad = false;
-        for (int r = 0; r < n; ++r) anyGrad |= rq[r].wantGrad;
-        sc_->anyGrad = anyGrad;
-        if (anyGrad) {
-            sc_->g_d.alloc((size_t)nQ * K);
-            HIPCHK(hipMemsetAsync(sc_->g_d.p, 0, (size_t)nQ * K * sizeof(double), st_));
-            gradients(rq, n, K, nImg);
-        }
-        sc_->hL.alloc(n);
-        HIPCHK(hipMemcpyAsync(sc_->hL.p, sc_->L.p, n * sizeof(double), hipMemcpyDeviceToHost, st_));
-        if (anyGrad) {
-            sc_->hG.alloc((size_t)nQ * K);
-            HIPCHK(hipMemcpyAsync(sc_->hG.p, sc_->g_d.p, (size_t)nQ * K * sizeof(double), hipMemcpyDeviceToHost, st_));
-        }
+        if (nG) vogel_gradients(n, K, nImg, nG, nEn, extraPupil);
+        bool anyFd = false;
+        for (int r = 0; r < n; ++r) anyFd |= rq[r].wantGrad && rq[r].gradMethod == 1;
+        if (anyFd) fd_gradient(rq, n, K, nImg);
+        sc_->hOut.alloc(nOut);
+        HIPCHK(hipMemcpyAsync(sc_->hOut.p, sc_->out.p, nOut * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipEventRecord(sc_->done, st_));
     }
 
     void eval_chunk_finish(EvalReq* rq, int n, int K, int nImg) {
+        size_t row = 0;                                          // Vogel / adjoint gradient rows, request order
         for (int r = 0; r < n; ++r) {
-            rq[r].f = sc_->hL.p[r];
-            if (rq[r].wantGrad && rq[r].g)
-                std::memcpy(rq[r].g, sc_->hG.p + (size_t)r * nImg * K, sizeof(double) * nImg * K);
+            rq[r].f = sc_->hOut.p[r];
+            if (!rq[r].wantGrad || rq[r].gradMethod == 1) continue;   // finite differences: filled already
+            if (rq[r].g) std::memcpy(rq[r].g, sc_->hOut.p + n + row * K, sizeof(double) * nImg * K);
+            row += nImg;
         }
     }
 
-    void gradients(EvalReq* rq, int n, int K, int nImg) {
-        // Vogel / adjoint for requests that ask for them (all virtual images are
-        // processed; results of other requests are discarded by the zeroed scale)
-        bool anyV = false, anyFd = false;
-        for (int r = 0; r < n; ++r) {
-            if (!rq[r].wantGrad) continue;
-            if (rq[r].gradMethod == 1) anyFd = true;
-            else anyV = true;
-        }
-        if (anyV) {
-            // compact list of the virtual images whose requests want a Vogel/adjoint gradient
-            std::vector<int> gIdx, gEn;
-            std::vector<double> sc;
-            int nEn = 0, extraPupil = 1;
-            for (int r = 0; r < n; ++r) {
-                if (!rq[r].wantGrad || rq[r].gradMethod == 1) continue;
-                nEn = std::max(nEn, rq[r].nEnabled);
-                for (int j = 0; j < nImg; ++j) {
-                    const int o = st.imgObj[j];
-                    const double w = st.weight[o];
-                    gIdx.push_back(r * nImg + j);
-                    gEn.push_back(rq[r].nEnabled);
-                    if (rq[r].gradMethod == 2) {
-                        // gradientVogel: -2 area/N^2 * w, times w again in Solver::gradient
-                        sc.push_back((w == 0.0) ? 0.0 : -2.0 * area_ / N2 * w * w);
-                    } else {
-                        // exact dL/dalpha = -(4 c^2 / s^2) w sum Im(conj(a) (G * a)), t = (G * PF)/N^2
-                        const double c = otfNorm_[st.imgCh[j]];
-                        sc.push_back(-4.0 * c * c * N2 / ((double)g_.s * g_.s) * w);
-                        extraPupil = 0;
-                    }
-                }
-            }
-            const int nG = (int)gIdx.size();
-            gIdx.insert(gIdx.end(), gEn.begin(), gEn.end());     // [0, nG): images, [nG, 2nG): counts
-            stage_async(sc_->gscale, sc_->hGs, sc);
-            stage_async(sc_->gIdx, sc_->hGi, gIdx);
+    // Vogel / adjoint weights and their mode projection for the nG gradient images whose
+    // tables forward() uploaded (sc_->tx: gIdx then counts, sc_->td: scales)
+    void vogel_gradients(int n, int K, int nImg, int nG, int nEn, int extraPupil) {
+        const int* gIdxD = sc_->tx;
+        {
             sc_->v.alloc((size_t)nG * nPS_);
             if constexpr (FUSED) {
                 launch("k_vogel_fused", nG, [&] {
                     hipLaunchKernelGGL((k_vogel_fused<N, FNT>), dim3(nG), dim3(FNT), FLDS, st_, sc_->v.p, sc_->S.p, D_.p, sc_->Pp.p,
                                        sc_->Qp.p, sc_->phi.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, osyx_.p,
                                        osyx_.p + nOS_, osyx_.p + 2 * nOS_, osyx_.p + 3 * nOS_, sc_->qm, sc_->qReq, imgObj_.p,
-                                       sc_->gIdx.p, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
+                                       gIdxD, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
                 });
             } else if constexpr (FUSED64) {
                 sc_->greal.alloc((size_t)nG * N2);
                 launch("k_grad64_fused", nG, [&] {
                     hipLaunchKernelGGL((k_grad64_fused<N, F64NT>), dim3(std::min(nG, otfGrid_)), dim3(F64NT), G64LDS,
                                        st_, sc_->v.p, sc_->greal.p, sc_->S.p, D_.p, sc_->Pp.p, sc_->Qp.p, gw_.p,
-                                       sc_->qm, sc_->qReq, imgObj_.p, sc_->gIdx.p, sc_->phi.p, invPs_.p, pupil_.p,
+                                       sc_->qm, sc_->qReq, imgObj_.p, gIdxD, sc_->phi.p, invPs_.p, pupil_.p,
                                        tw_.p, nPS_, nOS_, nImg, g_.nObj, nG, extraPupil);
                 });
             } else {
@@ -2474,7 +2486,7 @@ private:
                 sc_->greal.alloc((size_t)nG * N2);
                 launch("k_build_g", nG, [&] {
                     hipLaunchKernelGGL(k_build_g<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, sc_->tiles.p, sc_->S.p,
-                                       D_.p, sc_->Pp.p, sc_->Qp.p, invOs_.p, sc_->qm, sc_->qReq, imgObj_.p, sc_->gIdx.p, nG, N2, nOS_, nImg,
+                                       D_.p, sc_->Pp.p, sc_->Qp.p, invOs_.p, sc_->qm, sc_->qReq, imgObj_.p, gIdxD, nG, N2, nOS_, nImg,
                                        g_.nObj);
                 });
                 fft2(sc_->tiles.p, nG, +1);
@@ -2484,7 +2496,7 @@ private:
                 });
                 launch("k_field", nG, [&] {
                     hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nG * N2)), dim3(256), 0, st_, sc_->tiles.p, sc_->phi.p,
-                                       invPde_.p, pupil_.p, sc_->qm.img, (const T*)nullptr, sc_->gIdx.p, nG, N2, nPS_);
+                                       invPde_.p, pupil_.p, sc_->qm.img, (const T*)nullptr, gIdxD, nG, N2, nPS_);
                 });
                 fft2(sc_->tiles.p, nG, +1);
                 launch("k_mul_re", nG, [&] {
@@ -2494,7 +2506,7 @@ private:
                 fft2(sc_->tiles.p, nG, -1);
                 launch("k_extract_v", nG, [&] {
                     hipLaunchKernelGGL(k_extract_v<T>, dim3(nblk((size_t)nG * nPS_)), dim3(256), 0, st_, sc_->v.p,
-                                       sc_->tiles.p, sc_->phi.p, pde_.p, pupil_.p, sc_->gIdx.p, nG, nPS_, N2, extraPupil);
+                                       sc_->tiles.p, sc_->phi.p, pde_.p, pupil_.p, gIdxD, nG, nPS_, N2, extraPupil);
                 });
             }
             {
@@ -2502,15 +2514,14 @@ private:
                 sc_->gpart.alloc((size_t)GR_SLICES * nG * std::max(Kt, 1));
                 launch("k_grad_mfma", nG, [&] {
                     hipLaunchKernelGGL(k_grad_mfma<T>, dim3((nG + 15) / 16, GR_SLICES), dim3(256), 0, st_, sc_->gpart.p,
-                                       sc_->v.p, B_.p, sc_->qm, imgObj_.p, sc_->gIdx.p, nG, K, Kt, nPS_, nEn);
+                                       sc_->v.p, B_.p, sc_->qm, imgObj_.p, gIdxD, nG, K, Kt, nPS_, nEn);
                 });
                 launch("k_grad_sum", [&] {
-                    hipLaunchKernelGGL(k_grad_sum, dim3(nblk((size_t)nG * K)), dim3(256), 0, st_, sc_->g_d.p, sc_->gpart.p,
-                                       sc_->gscale.p, sc_->gIdx.p, sc_->gIdx.p + nG, nG, K, Kt);
+                    hipLaunchKernelGGL(k_grad_sum, dim3(nblk((size_t)nG * K)), dim3(256), 0, st_, sc_->out.p + n,
+                                       sc_->gpart.p, sc_->td, gIdxD + nG, nG, K, Kt);
                 });
             }
         }
-        if (anyFd) fd_gradient(rq, n, K, nImg);
     }
 
     void fd_gradient(EvalReq* rq, int n, int K, int nImg) {
@@ -2562,15 +2573,16 @@ private:
             HIPCHK(hipMemcpyAsync(res.data() + b, sc_->fdOut.p, m * sizeof(double), hipMemcpyDeviceToHost, st_));
         }
         HIPCHK(hipStreamSynchronize(st_));
-        // g = w * (dl / step * w); merge into the device gradient buffer
-        std::vector<double> hg((size_t)n * nImg * K);
-        HIPCHK(hipMemcpy(hg.data(), sc_->g_d.p, hg.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // g = w * (dl / step * w) for the probed modes, 0 for the others, straight into the
+        // requests' gradient arrays (their patches are waiting for this evaluation)
+        for (int r = 0; r < n; ++r)
+            if (rq[r].wantGrad && rq[r].gradMethod == 1 && rq[r].g) std::fill(rq[r].g, rq[r].g + (size_t)nImg * K, 0.0);
         for (int k = 0; k < nP; ++k) {
             const double stp = (double)step[k];
             const double w = wgt[k];
-            hg[(size_t)dst[k].first * K + dst[k].second] = (w == 0.0) ? 0.0 : w * (res[k] / stp * w);
+            const int r = dst[k].first / nImg, j = dst[k].first % nImg;
+            if (rq[r].g) rq[r].g[(size_t)j * K + dst[k].second] = (w == 0.0) ? 0.0 : w * (res[k] / stp * w);
         }
-        HIPCHK(hipMemcpy(sc_->g_d.p, hg.data(), hg.size() * sizeof(double), hipMemcpyHostToDevice));
     }
 
     std::vector<int> otf_support_of(const double* pupil) {
@@ -2623,6 +2635,7 @@ private:
     bool bf16Phase_ = false;
     HBuf<double> hStage_;
     DBuf<double> ftSum_, gamma_, win_, w_, wt_;
+    std::vector<double> hGamma_;             // host copy of gamma_ (slot, object)
     DBuf<int> gw_;
     DBuf<int> invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_, pyx_, osyx_, hread_,
         invPs_;
@@ -2633,16 +2646,17 @@ private:
     struct Scratch {
         DBuf<C> tiles, S, Sfd;
         DBuf<T> phi, phiFd, v, greal, fdStep;
-        DBuf<double> Pp, Qp, part, L, g_d, alpha, gscale, fdOut, statOut, gpart;
-        DBuf<int> ints, fdInts, gIdx, initInts;
-        HBuf<double> hAlpha, hL, hG, hGs;
-        HBuf<int> hInts, hGi, hInit;
-        QMap qm{};
+        DBuf<double> Pp, Qp, part, alpha, fdOut, statOut, gpart, tab, out;
+        DBuf<int> ints, fdInts, initInts;
+        HBuf<double> hAlpha, hL, hG, hTab, hOut;   // hTab: the evaluation's tables, hOut: its results
+        HBuf<int> hInit;
+        QMap qm{};                       // device tables of the current evaluation (in tab)
         const int* qReq = nullptr;
         const int* reqSlot = nullptr;
+        const int* tx = nullptr;         // the caller's ints after the request tables
+        const double* td = nullptr;      // the caller's doubles before them
         hipEvent_t done = nullptr;       // recorded after the lane's result copies
         int nReq = 0;
-        bool anyGrad = false;
     };
     Scratch lanes_[3];
     Scratch* sc_ = &lanes_[2];

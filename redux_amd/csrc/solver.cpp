@@ -413,17 +413,18 @@ void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
             frame_noise(c, p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1], noiseAll[k], sdAll[k]);
         });
     }
+    std::vector<double> gamAll(patches.size() * nObj, 0.0);
     for (size_t a = 0; a < patches.size(); ++a) {
         const int p = patches[a];
         c->noise[p].assign(noiseAll.begin() + a * nImg, noiseAll.begin() + (a + 1) * nImg);
-        std::vector<double> gam(nObj, 0.0);
+        double* gam = &gamAll[a * nObj];
         for (int j = 0; j < nImg; ++j) {                      // Object::addRegGamma
             const int o = c->st.imgObj[j];
             gam[o] += 0.10 * (noiseAll[a * nImg + j] / sdAll[a * nImg + j]) / c->st.nImgObj[o];
         }
-        c->gamma[p] = gam;
-        c->dev->set_gamma(p, gam.data());
+        c->gamma[p].assign(gam, gam + nObj);
     }
+    c->dev->set_gamma((int)patches.size(), patches.data(), gamAll.data());
     c->dev->init_images(imgs, c->window, 2);
 }
 

@@ -70,6 +70,12 @@ public:
     // gamma), per object on the stored half of the OTF support: P0 [n][nObj][nOSStored]
     // complex, Q0 [n][nObj][nOSStored]; alpha[k] points at [nImg][K] for slots[k]
     virtual void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) = 0;
+    // restore_sums in two halves on the synchronous lane, for use while lanes 0/1 evaluate:
+    // begin enqueues the sums and their download and returns; ready says whether they are
+    // done; end waits and copies them out.  One restoration at a time.
+    virtual void restore_begin(int n, const int* slots, const double* const* alpha) = 0;
+    virtual bool restore_ready() = 0;
+    virtual void restore_end(double* P0, double* Q0) = 0;
     virtual int nOS() const = 0;            // OTF support points (full)
     virtual int nOSStored() const = 0;      // points stored on the device (Hermitian half)
     // full support point u -> stored point h (value as stored), or -1-h (its conjugate)

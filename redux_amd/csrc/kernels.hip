@@ -2122,6 +2122,14 @@ public:
     }
 
     void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) override {
+        restore_begin(n, slots, alpha);
+        restore_end(P0, Q0);
+    }
+
+    // P0, Q0 (no gamma) of n slots at the given alphas, on lane 2: the sums and their
+    // download are enqueued behind whatever the stream holds; lane 2's pinned buffers
+    // receive [n][nObj][nH] P (complex) and Q
+    void restore_begin(int n, const int* slots, const double* const* alpha) override {
         use_lane(2);
         const size_t per = (size_t)g_.nObj * nOS_;
         const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
@@ -2130,20 +2138,34 @@ public:
             rq[k].slot = slots[k];
             rq[k].alpha = alpha[k];
         }
+        sc_->hG.alloc(2 * (size_t)n * per);        // pinned: P then Q
+        sc_->hL.alloc((size_t)n * per);
         for (int b = 0; b < n; b += perChunk) {
             const int m = std::min(perChunk, n - b);
             forward(&rq[b], m, /*useGamma=*/0);
-            sc_->hG.alloc(2 * (size_t)m * per);        // pinned: P then Q
-            sc_->hL.alloc((size_t)m * per);
-            const double* hP = sc_->hG.p;
-            const double* hQ = sc_->hL.p;
-            HIPCHK(hipMemcpyAsync(sc_->hG.p, sc_->Pp.p, 2 * (size_t)m * per * sizeof(double), hipMemcpyDeviceToHost,
-                                  st_));
-            HIPCHK(hipMemcpyAsync(sc_->hL.p, sc_->Qp.p, (size_t)m * per * sizeof(double), hipMemcpyDeviceToHost, st_));
-            HIPCHK(hipStreamSynchronize(st_));
-            std::memcpy(P0 + 2 * (size_t)b * per, hP, 2 * (size_t)m * per * sizeof(double));
-            std::memcpy(Q0 + (size_t)b * per, hQ, (size_t)m * per * sizeof(double));
+            HIPCHK(hipMemcpyAsync(sc_->hG.p + 2 * (size_t)b * per, sc_->Pp.p, 2 * (size_t)m * per * sizeof(double),
+                                  hipMemcpyDeviceToHost, st_));
+            HIPCHK(hipMemcpyAsync(sc_->hL.p + (size_t)b * per, sc_->Qp.p, (size_t)m * per * sizeof(double),
+                                  hipMemcpyDeviceToHost, st_));
         }
+        HIPCHK(hipEventRecord(sc_->done, st_));
+        restoreN_ = n;
+    }
+
+    bool restore_ready() override {
+        const hipError_t e = hipEventQuery(lanes_[2].done);
+        if (e == hipErrorNotReady) return false;
+        HIPCHK(e);
+        return true;
+    }
+
+    void restore_end(double* P0, double* Q0) override {
+        use_lane(2);
+        const size_t per = (size_t)g_.nObj * nOS_;
+        HIPCHK(hipEventSynchronize(sc_->done));
+        std::memcpy(P0, sc_->hG.p, 2 * (size_t)restoreN_ * per * sizeof(double));
+        std::memcpy(Q0, sc_->hL.p, (size_t)restoreN_ * per * sizeof(double));
+        restoreN_ = 0;
         collect(2);
     }
 
@@ -2611,6 +2633,7 @@ private:
     Geometry g_;
     int maxSlots_, maxBatch_;
     int otfGrid_ = 1;                        // workgroups of the persistent FP64 OTF kernel
+    int restoreN_ = 0;                       // slots of the restoration in flight on lane 2
     hipStream_t st_;
     hipEvent_t ev0_, ev1_;
     bool timing_ = false;

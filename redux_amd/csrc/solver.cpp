@@ -74,8 +74,8 @@ struct Task {
 };
 
 // ------------------------------------------------------------ parallel for ----
-void parallel_for(int n, const std::function<void(int)>& f) {
-    int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+void parallel_for(int n, const std::function<void(int)>& f, int maxThreads = 16) {
+    int nt = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), (unsigned)std::max(1, maxThreads));
     nt = std::min(nt, n);
     if (nt <= 1) {
         for (int i = 0; i < n; ++i) f(i);
@@ -1194,6 +1194,79 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 c->hostStats[1].launches++;
                 inflight[e] = false;
             };
+            // Restoration (Object::restorePatch) of the patches that have finished, while the
+            // others still iterate: their P0/Q0 sums go to the device on lane 2 behind the
+            // lanes' evaluations, in groups of RESTORE_MIN..RESTORE_MAX patches, and the
+            // filter and inverse transform run on a host thread.  A patch's restoration
+            // depends on its own final alpha only, so grouping leaves it unchanged.
+            constexpr int RESTORE_MIN = 16, RESTORE_MAX = 32;
+            const size_t per = (size_t)c->g.nObj * c->dev->nOSStored();
+            struct Restore {
+                std::vector<int> ks;
+                std::vector<double> ae, P0, Q0;
+                std::thread th;
+            };
+            std::vector<std::unique_ptr<Restore>> restores;
+            struct JoinAll {                     // no host restoration outlives the solve, errors included
+                std::vector<std::unique_ptr<Restore>>& v;
+                ~JoinAll() {
+                    for (auto& r : v)
+                        if (r->th.joinable()) r->th.join();
+                }
+            } joinAll{restores};
+            Restore* onDevice = nullptr;
+            std::vector<char> queued(nb, 0);
+            std::exception_ptr restoreExc;
+            std::mutex restoreMu;
+            auto device_done = [&](bool wait) {
+                if (!onDevice || (!wait && !c->dev->restore_ready())) return;
+                auto t0 = clk::now();
+                Restore* rs = onDevice;
+                rs->P0.resize(2 * rs->ks.size() * per);
+                rs->Q0.resize(rs->ks.size() * per);
+                c->dev->restore_end(rs->P0.data(), rs->Q0.data());
+                onDevice = nullptr;
+                c->hostStats[5].ms += since(t0);
+                rs->th = std::thread([&, rs] {
+                    try {
+                        const auto& os = c->dev->otf_support();
+                        parallel_for((int)rs->ks.size(), [&](int i) {
+                            const int p = runs[rs->ks[i]]->p;
+                            finish_restore(c, p, os, c->dev->support_map(), c->dev->nOSStored(), &rs->P0[2 * i * per],
+                                           &rs->Q0[i * per], imgOut + (size_t)p * c->g.nObj * c->g.s * c->g.s);
+                        }, 4);
+                    } catch (...) {
+                        std::lock_guard<std::mutex> l(restoreMu);
+                        if (!restoreExc) restoreExc = std::current_exception();
+                    }
+                });
+            };
+            auto poll_restores = [&](bool flush) {
+                if (!imgOut) return;
+                device_done(flush);
+                if (onDevice) return;
+                std::vector<int> ks;
+                for (int k = 0; k < nb && (int)ks.size() < RESTORE_MAX; ++k)
+                    if (!queued[k] && tasks[k].h.done()) ks.push_back(k);
+                if (ks.empty() || (!flush && (int)ks.size() < RESTORE_MIN)) return;
+                auto t0 = clk::now();
+                auto rs = std::make_unique<Restore>();
+                rs->ks = ks;
+                rs->ae.resize(ks.size() * (size_t)nPar);
+                std::vector<const double*> ap(ks.size());
+                std::vector<int> slots(ks.size());
+                for (size_t i = 0; i < ks.size(); ++i) {
+                    queued[ks[i]] = 1;
+                    slots[i] = runs[ks[i]]->p;
+                    effective_alpha(c, slots[i], runs[ks[i]]->alpha.data(), &rs->ae[i * nPar]);
+                    ap[i] = &rs->ae[i * nPar];
+                }
+                c->dev->restore_begin((int)ks.size(), slots.data(), ap.data());
+                onDevice = rs.get();
+                restores.push_back(std::move(rs));
+                c->hostStats[5].ms += since(t0);
+                c->hostStats[5].launches++;
+            };
             while (true) {
                 bool any = false;
                 for (int e = 0; e < nLanes; ++e) {
@@ -1203,6 +1276,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                     }
                     do_inits(e);
                     any |= launch_lane(e);
+                    poll_restores(false);
                 }
                 if (!any) break;
             }
@@ -1210,35 +1284,25 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 if (!tasks[k].h.done()) throw std::runtime_error("patch state machine stalled");
                 if (tasks[k].h.promise().exc) std::rethrow_exception(tasks[k].h.promise().exc);
             }
-            // results + restoration (Solver::run tail, solver.cpp:549-566)
-            const size_t per = (size_t)c->g.nObj * c->dev->nOSStored();
+            if (imgOut) {
+                auto t0 = clk::now();
+                while (true) {
+                    poll_restores(true);
+                    if (!onDevice) break;
+                }
+                for (auto& rs : restores) rs->th.join();
+                c->hostStats[4].ms += since(t0);
+                c->hostStats[4].launches++;
+                c->hostStats[4].units += nb;
+                if (restoreExc) std::rethrow_exception(restoreExc);
+            }
+            // results (Solver::run tail, solver.cpp:549-566); the objects were restored above
             for (int k = 0; k < nb; ++k) {
                 auto& r = *runs[k];
                 r.res.nF = r.nF;
                 r.res.nDF = r.nDF;
                 if (results) results[r.p] = r.res;
                 if (alphaOut) std::copy(r.alpha.begin(), r.alpha.end(), alphaOut + (size_t)r.p * nPar);
-            }
-            if (imgOut) {
-                auto t0 = clk::now();
-                std::vector<double> ae((size_t)nb * nPar), P0(2 * nb * per), Q0(nb * per);
-                std::vector<const double*> ap(nb);
-                for (int k = 0; k < nb; ++k) {
-                    effective_alpha(c, runs[k]->p, runs[k]->alpha.data(), &ae[(size_t)k * nPar]);
-                    ap[k] = &ae[(size_t)k * nPar];
-                }
-                c->dev->restore_sums(nb, ids.data(), ap.data(), P0.data(), Q0.data());
-                c->hostStats[5].ms += since(t0);
-                c->hostStats[5].launches++;
-                const auto& os = c->dev->otf_support();
-                parallel_for(nb, [&](int k) {
-                    const int p = runs[k]->p;
-                    finish_restore(c, p, os, c->dev->support_map(), c->dev->nOSStored(), &P0[2 * k * per],
-                                   &Q0[k * per], imgOut + (size_t)p * c->g.nObj * c->g.s * c->g.s);
-                });
-                c->hostStats[4].ms += since(t0);
-                c->hostStats[4].launches++;
-                c->hostStats[4].units += nb;
             }
         }
         c->initDone = false;    // shifts moved during the solve

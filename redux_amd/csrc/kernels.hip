@@ -1319,6 +1319,115 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
     }
 }
 
+// FP64 frame transform D = FFT2(padded windowed frame) on the stored half of the OTF
+// support, one image per workgroup, in k_otf64_fused's P x (N+1) LDS buffer (N <= 128):
+// the steps of k_window (cut s x s at the shifted start, (x - avg) window + avg, pad to
+// N x N with the windowed mean), then the forward transform of a real image:
+//   rows y and y + N/2 packed as one complex row, forward row FFTs, the split into the
+//   half spectra of the two real rows folded into the first pass of the column FFTs
+//   (line layout `at` of k_otf64_fused: column 0 combined with column N/2), second pass,
+//   and the stored points through hread (unnormalised, as FourierTransform's forward FFT)
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_init64_fused(cx<double>* __restrict__ D, const float* __restrict__ raw,
+                                                     const double* __restrict__ window, const int* __restrict__ slot,
+                                                     const int* __restrict__ img, const int* __restrict__ sx,
+                                                     const int* __restrict__ sy, const int* __restrict__ hread,
+                                                     const cx<double>* __restrict__ twg, int nImg, int dataY, int dataX,
+                                                     int startY, int startX, int s, int nOS) {
+    constexpr int P = N / 2, LD = N + 1, HC = N / 2 + 1;
+    constexpr int RF = N == 64 ? 8 : 16, RL = 8;
+    extern __shared__ cx<double> lds64[];
+    cx<double>* t = lds64;
+    cx<double>* tw = lds64 + P * LD;
+    double* red = reinterpret_cast<double*>(tw + N);          // NT doubles for the block sums
+    for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
+    auto bsum = [&](double v) {                                 // fixed-order tree over NT threads
+        red[threadIdx.x] = v;
+        __syncthreads();
+        for (int w = NT / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        const double r = red[0];
+        __syncthreads();
+        return r;
+    };
+    const int q = blockIdx.x;
+    const float* blk = raw + ((size_t)slot[q] * nImg + img[q]) * dataY * dataX;
+    const int y0 = startY + sy[q], x0 = startX + sx[q];
+    double acc = 0;
+    for (int e = threadIdx.x; e < s * s; e += NT) acc += (double)blk[(size_t)(y0 + e / s) * dataX + x0 + e % s];
+    const double avg = bsum(acc) / (double)(s * s);
+    acc = 0;
+    for (int e = threadIdx.x; e < s * s; e += NT)
+        acc += ((double)blk[(size_t)(y0 + e / s) * dataX + x0 + e % s] - avg) * window[e] + avg;
+    const double mean = bsum(acc) / (double)(s * s);
+    const int off = (N - s) / 2;
+    auto val = [&](int y, int x) {
+        const int yy = y - off, xx = x - off;
+        return (yy >= 0 && yy < s && xx >= 0 && xx < s)
+                   ? ((double)blk[(size_t)(y0 + yy) * dataX + x0 + xx] - avg) * window[yy * s + xx] + avg
+                   : mean;
+    };
+    for (int e = threadIdx.x; e < P * N; e += NT) {
+        const int y = e / N, x = e % N;
+        t[y * LD + x] = {val(y, x), val(y + P, x)};
+    }
+    __syncthreads();
+    fft_lines<double, N, -1, P, NT, 1>(t, LD, 1, tw);
+    auto at = [](int k, int y) { return y < P ? y * LD + k : (y - P) * LD + (k == 0 ? P : N - k); };
+    {
+        constexpr int NB = P * (N / RF), BT = (NB + NT - 1) / NT;
+        cx<double> v[BT][RF];
+#pragma unroll
+        for (int i = 0; i < BT; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (NB % NT == 0 || b < NB) {
+                const int k = b % P, j = b / P;
+#pragma unroll
+                for (int r = 0; r < RF / 2; ++r) {
+                    const cx<double>* row = t + (j + r * (N / RF)) * LD;
+                    const cx<double> z = row[k], zm = row[k == 0 ? P : N - k];
+                    if (k == 0) {
+                        v[i][r] = {z.x, zm.x};
+                        v[i][r + RF / 2] = {z.y, zm.y};
+                    } else {
+                        v[i][r] = {0.5 * (z.x + zm.x), 0.5 * (z.y - zm.y)};
+                        v[i][r + RF / 2] = {0.5 * (z.y + zm.y), -0.5 * (z.x - zm.x)};
+                    }
+                }
+                dft_reg<double, RF, -1>(v[i]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < BT; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (NB % NT == 0 || b < NB) {
+                const int k = b % P, j = b / P;
+#pragma unroll
+                for (int r = 0; r < RF; ++r) t[at(k, j * RF + r)] = v[i][r];
+            }
+        }
+        __syncthreads();
+    }
+    spass_at<double, N, RL, RF, -1, P, NT>(t, at, tw);
+    cx<double>* Dq = D + ((size_t)slot[q] * nImg + img[q]) * nOS;
+    for (int u = threadIdx.x; u < nOS; u += NT) {
+        const int code = hread[u];
+        const int idx = code >= 0 ? code : -1 - code, y = idx / HC, k = idx % HC;
+        cx<double> v;
+        if (k != 0 && k != P) {
+            v = t[at(k, y)];
+        } else {
+            const cx<double> x = t[at(0, y)], xm = t[at(0, (N - y) % N)];
+            v = k == 0 ? cx<double>{0.5 * (x.x + xm.x), 0.5 * (x.y - xm.y)}
+                       : cx<double>{0.5 * (x.y + xm.y), -0.5 * (x.x - xm.x)};
+        }
+        Dq[u] = {v.x, code >= 0 ? v.y : -v.y};
+    }
+}
+
 // FP64 Vogel / adjoint weights in one persistent kernel per gradient image (N = 64,
 // 128), the quantities of the unfused chain (k_build_g, IFFT2, k_save_re, k_field,
 // IFFT2, k_mul_re, FFT2, k_extract_v) with every intermediate but one in
@@ -1736,6 +1845,7 @@ public:
     static constexpr int F64NT = N == 128 ? 512 : 256;
     static constexpr size_t F64LDS = ((size_t)(N / 2) * (N + 1) + N) * sizeof(cx<double>);
     static constexpr size_t G64LDS = F64LDS + 2 * N * sizeof(cx<double>);     // plus two G'_H columns
+    static constexpr size_t I64LDS = F64LDS + F64NT * sizeof(double);           // plus the block-sum scratch
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device, bool bf16Phase = false)
         : g_(g), maxSlots_(maxSlots), maxBatch_(maxBatch), bf16Phase_(bf16Phase && std::is_same_v<T, float>) {
@@ -1902,6 +2012,8 @@ public:
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)F64LDS));
             HIPCHK(hipFuncSetAttribute((const void*)k_grad64_fused<N, F64NT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)G64LDS));
+            HIPCHK(hipFuncSetAttribute((const void*)k_init64_fused<N, F64NT>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)I64LDS));
             int nCU = 0, dev = 0;
             HIPCHK(hipGetDevice(&dev));
             HIPCHK(hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2057,22 +2169,30 @@ public:
         ints.insert(ints.end(), pairs.begin(), pairs.end());
         stage_async(sc_->initInts, sc_->hInit, ints);
         const int* base = sc_->initInts.p;
-        sc_->tiles.alloc((size_t)std::min(chunk, nQ) * N2);
-        for (int b = 0; b < nQ; b += chunk) {
-            const int n = std::min(chunk, nQ - b);
-            const int* ds = base + b;
-            const int* di = base + nQ + b;
-            const int* dx = base + 2 * nQ + b;
-            const int* dy = base + 3 * nQ + b;
-            launch("k_window", n, [&] {
-                hipLaunchKernelGGL(k_window<T>, dim3(n), dim3(256), 0, st_, sc_->tiles.p, rawF_.p, win_.p, ds, di, dx,
-                                   dy, g_.nImg, g_.dataY, g_.dataX, g_.startY, g_.startX, g_.s, N);
+        if constexpr (FUSED64) {
+            launch("k_init64_fused", nQ, [&] {
+                hipLaunchKernelGGL((k_init64_fused<N, F64NT>), dim3(nQ), dim3(F64NT), I64LDS, st_, D_.p, rawF_.p, win_.p,
+                                   base, base + nQ, base + 2 * nQ, base + 3 * nQ, hread_.p, tw_.p, g_.nImg, g_.dataY,
+                                   g_.dataX, g_.startY, g_.startX, g_.s, nOS_);
             });
-            fft2(sc_->tiles.p, n, -1);
-            launch("k_store_d", n, [&] {
-                hipLaunchKernelGGL(k_store_d<T>, dim3(nblk((size_t)n * nOS_)), dim3(256), 0, st_, D_.p, sc_->tiles.p,
-                                   osNat_.p, ds, di, n, nOS_, g_.nImg, N2);
-            });
+        } else {
+            sc_->tiles.alloc((size_t)std::min(chunk, nQ) * N2);
+            for (int b = 0; b < nQ; b += chunk) {
+                const int n = std::min(chunk, nQ - b);
+                const int* ds = base + b;
+                const int* di = base + nQ + b;
+                const int* dx = base + 2 * nQ + b;
+                const int* dy = base + 3 * nQ + b;
+                launch("k_window", n, [&] {
+                    hipLaunchKernelGGL(k_window<T>, dim3(n), dim3(256), 0, st_, sc_->tiles.p, rawF_.p, win_.p, ds, di, dx,
+                                       dy, g_.nImg, g_.dataY, g_.dataX, g_.startY, g_.startX, g_.s, N);
+                });
+                fft2(sc_->tiles.p, n, -1);
+                launch("k_store_d", n, [&] {
+                    hipLaunchKernelGGL(k_store_d<T>, dim3(nblk((size_t)n * nOS_)), dim3(256), 0, st_, D_.p, sc_->tiles.p,
+                                       osNat_.p, ds, di, n, nOS_, g_.nImg, N2);
+                });
+            }
         }
         const int nPairs = (int)pairs.size() / 2;
         launch("k_ftsum", nPairs, [&] {

@@ -69,9 +69,14 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
     int gradient;       /* MFBD_GRAD_SCHEDULE_REFERENCE: finite differences for the first
                            gradient, Vogel after (solver.cpp:385,478); or MFBD_GRAD_ADJOINT
                            for the exact dL/dalpha throughout                              */
+    int getstep;        /* GETSTEP: MFBD_GETSTEP_CONJUGATE or MFBD_GETSTEP_BFGS_INV
+                           (the stepper switch of Solver::run, solver.cpp:352-382)         */
 } mfbd_params;
 
 #define MFBD_GRAD_SCHEDULE_REFERENCE 0
+/* steppers, numbered as the reference's getstepMethod (config.cpp:37, 102-114) */
+#define MFBD_GETSTEP_CONJUGATE 2   /* GSM_CNJG: conjugate_rdx (conjugate_gradient.cpp:128-256)   */
+#define MFBD_GETSTEP_BFGS_INV 4    /* GSM_BFGS_inv, the reference default: GSL vector_bfgs2      */
 
 typedef struct {
     int status;             /* MFBD_PATCH_*                                  */

@@ -61,6 +61,7 @@ typedef struct {
     int noFilter;
     const double* alphaInit;      /* [nImg][K] or NULL */
     int maxMetrics;
+    int getstep;                  /* 0: conjugate_rdx (GETSTEP=conjugate), 4: GSL vector_bfgs2 (BFGS_inv) */
 } OProblem;
 
 typedef struct {
@@ -93,5 +94,20 @@ void o_state_restore(OState* st, const double* alpha, double* img);
 /* export per-image/per-object arrays after the last evaluation */
 void o_state_get(OState* st, const char* what, double* out);
 double o_state_scalar(OState* st, const char* what, int idx);
+
+
+
+/* ---- GSL 1.16 vector_bfgs2 restatement (solver.c), generic over f / df / fdf ---- */
+typedef struct {
+    void* ctx;
+    double (*f)(void* ctx, const double* x);
+    void (*df)(void* ctx, const double* x, double* g);
+    void (*fdf)(void* ctx, const double* x, double* f, double* g);
+} OFdf;
+typedef struct OBfgs2 OBfgs2;
+OBfgs2* o_bfgs2_new(int n);
+void o_bfgs2_free(OBfgs2* w);
+void o_bfgs2_set(OBfgs2* w, const OFdf* fn, const double* x, double* f, double* g, double stepSize, double tol);
+int o_bfgs2_iterate(OBfgs2* w, double* x, double* f, double* g, double* dx);
 
 #endif

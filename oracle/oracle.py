@@ -35,8 +35,20 @@ class OProblem(C.Structure):
         ("FTOL", C.c_double), ("EPS", C.c_double), ("graddiffStep", C.c_double), ("regAlpha", C.c_double),
         ("minIter", C.c_int), ("maxIter", C.c_int), ("nDoneIter", C.c_int), ("modeStart", C.c_int),
         ("modeStep", C.c_int), ("apodSize", C.c_int), ("noFilter", C.c_int),
-        ("alphaInit", dp), ("maxMetrics", C.c_int),
+        ("alphaInit", dp), ("maxMetrics", C.c_int), ("getstep", C.c_int),
     ]
+
+
+GETSTEP = {"conjugate": 0, "conjugate_gradient": 0, "bfgs_inv": 4}    # getstepMap (config.cpp:102-114)
+
+
+class OFdf(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("f", C.c_void_p), ("df", C.c_void_p), ("fdf", C.c_void_p)]
+
+
+F_T = C.CFUNCTYPE(C.c_double, C.c_void_p, dp)
+DF_T = C.CFUNCTYPE(None, C.c_void_p, dp, dp)
+FDF_T = C.CFUNCTYPE(None, C.c_void_p, dp, dp, dp)
 
 
 class OResult(C.Structure):
@@ -61,6 +73,12 @@ def lib():
         L.o_solve.argtypes = [C.POINTER(OProblem), C.POINTER(OResult)]
         L.o_solve_many.argtypes = [C.POINTER(OProblem), C.POINTER(OResult), C.c_int, C.c_int]
         L.o_state_new.argtypes = [C.POINTER(OProblem)]
+        L.o_bfgs2_new.argtypes = [C.c_int]
+        L.o_bfgs2_new.restype = C.c_void_p
+        L.o_bfgs2_free.argtypes = [C.c_void_p]
+        L.o_bfgs2_set.argtypes = [C.c_void_p, C.POINTER(OFdf), dp, dp, dp, C.c_double, C.c_double]
+        L.o_bfgs2_iterate.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.o_bfgs2_iterate.restype = C.c_int
         L.o_state_new.restype = C.c_void_p
         L.o_state_free.argtypes = [C.c_void_p]
         L.o_state_init.argtypes = [C.c_void_p]
@@ -164,6 +182,7 @@ class Problem:
         else:
             o.alphaInit = dp()
         o.maxMetrics = 100000
+        o.getstep = GETSTEP[str(st.get("GETSTEP", "conjugate")).lower()]
         self.o = o
         self.nObj, self.nImg, self.K, self.s = o.nObj, o.nImg, K, o.s
         self.nPar = o.nPar
@@ -255,3 +274,46 @@ class State:
 
     def scalar(self, what, idx=0):
         return lib().o_state_scalar(self.h, what.encode(), idx)
+
+
+def bfgs2_minimize(f, grad, x0, step=1.0, tol=1e-6, iters=100):
+    """Drive the vector_bfgs2 restatement on a Python function (tests): f(x) -> float,
+    grad(x) -> array.  Returns the iterates [(x, f, g, status, evals)] after set and
+    after each iteration; evals counts f and gradient calls so far."""
+    L = lib()
+    n = len(x0)
+    calls = {"f": 0, "df": 0}
+
+    def _f(ctx, x):
+        calls["f"] += 1
+        return float(f(np.ctypeslib.as_array(x, (n,)).copy()))
+
+    def _df(ctx, x, g):
+        calls["df"] += 1
+        np.ctypeslib.as_array(g, (n,))[:] = grad(np.ctypeslib.as_array(x, (n,)).copy())
+
+    def _fdf(ctx, x, fo, g):
+        calls["f"] += 1
+        calls["df"] += 1
+        xv = np.ctypeslib.as_array(x, (n,)).copy()
+        fo[0] = float(f(xv))
+        np.ctypeslib.as_array(g, (n,))[:] = grad(xv)
+
+    cf, cdf, cfdf = F_T(_f), DF_T(_df), FDF_T(_fdf)
+    fn = OFdf(None, C.cast(cf, C.c_void_p), C.cast(cdf, C.c_void_p), C.cast(cfdf, C.c_void_p))
+    w = L.o_bfgs2_new(n)
+    x = np.array(x0, dtype=np.float64)
+    g = np.zeros(n)
+    dx = np.zeros(n)
+    fv = np.zeros(1)
+    try:
+        L.o_bfgs2_set(w, C.byref(fn), _p(x), _p(fv), _p(g), step, tol)
+        out = [(x.copy(), float(fv[0]), g.copy(), 0, dict(calls))]
+        for _ in range(iters):
+            st = L.o_bfgs2_iterate(w, _p(x), _p(fv), _p(g), _p(dx))
+            out.append((x.copy(), float(fv[0]), g.copy(), st, dict(calls)))
+            if st != 0:
+                break
+    finally:
+        L.o_bfgs2_free(w)
+    return out

@@ -609,6 +609,21 @@ static void my_df(OState* st, const double* beta, double* gb)
     ns_apply(st->pr, st->gradAlpha, gb);
 }
 
+/* Solver::my_fdf (solver.cpp:237-243): metric, then the gradient from the same P, Q */
+static void my_fdf(OState* st, const double* beta, double* f, double* gb)
+{
+    ns_reverse_add(st->pr, beta, st->alphaOffset, st->alpha);
+    apply_alpha(st, st->alpha);
+    *f = metric_now(st);
+    calc_helpers(st);
+    gradient_now(st);
+    ns_apply(st->pr, st->gradAlpha, gb);
+}
+
+static double cb_f(void* ctx, const double* x) { return my_f((OState*)ctx, x); }
+static void cb_df(void* ctx, const double* x, double* g) { my_df((OState*)ctx, x, g); }
+static void cb_fdf(void* ctx, const double* x, double* f, double* g) { my_fdf((OState*)ctx, x, f, g); }
+
 /* ---------- conjugate_rdx + bracket + brent ---------- */
 
 typedef struct {
@@ -771,6 +786,296 @@ static int cg_iterate(OState* st, CGState* cs, double* x, double* f, double* g, 
     return GSL_SUCCESS;
 }
 
+/* ---------- GETSTEP = BFGS_inv: GSL 1.16 vector_bfgs2 ----------
+   The reference's default stepper (config.cpp:632; solver.cpp:362-368 selects
+   gsl_multimin_fdfminimizer_vector_bfgs2 with init_step = 1e-4 lambda_max, tol = 1e-6).
+   GSL is not in this tree; its published algorithm is restated here:
+   - direction: BFGS update of the inverse Hessian applied implicitly,
+     p' = g1 - A dx - B dg, B = dx.g1/dx.dg, A = -(1 + dg.dg/dx.dg) B + dg.g1/dx.dg,
+     then p <- -sign(p.g1) p/|p| (multimin/vector_bfgs2.c);
+   - line search: Fletcher's bracketing and sectioning (Practical Methods of
+     Optimization, algorithms 2.6.2 and 2.6.4) with rho = 0.01, sigma = tol,
+     tau1 = 9, tau2 = 0.05, tau3 = 0.5 and cubic interpolation where both ends have a
+     slope (quadratic otherwise), at most 100 trial points (multimin/linear_minimize.c);
+   - a line wrapper that caches f, the gradient and the slope at the last trial point
+     and at alpha = 0, so no point is evaluated twice (multimin/linear_wrapper.c).
+   Parity with GSL's exact arithmetic is unpinned (no GSL build here); the tests pin the
+   mathematics: finite termination on quadratics and the Wolfe conditions at every
+   accepted step. */
+
+#define O_DBL_EPSILON 2.2204460492503131e-16
+
+struct OBfgs2 {
+    /* caching line wrapper over x + alpha p */
+    const OFdf* fn;
+    double *x, *g, *p, *xa, *ga;
+    double fa, dfa, xKey, fKey, gKey, dfKey;
+    /* stepper state */
+    double step, g0norm, pnorm, deltaF, fp0;
+    double *dx0, *dg0;
+    double rho, sigma, tau1, tau2, tau3;
+    int n;
+};
+
+static double o_dot(const double* a, const double* b, int n) { double s = 0; for (int k = 0; k < n; ++k) s += a[k] * b[k]; return s; }
+
+static void lw_moveto(OBfgs2* w, double alpha)
+{
+    if (alpha == w->xKey) return;
+    for (int k = 0; k < w->n; ++k) w->xa[k] = w->x[k] + alpha * w->p[k];
+    w->xKey = alpha;
+}
+
+static double lw_f(OBfgs2* w, double alpha)
+{
+    if (alpha == w->fKey) return w->fa;
+    lw_moveto(w, alpha);
+    w->fa = w->fn->f(w->fn->ctx, w->xa);
+    w->fKey = alpha;
+    return w->fa;
+}
+
+static double lw_df(OBfgs2* w, double alpha)
+{
+    if (alpha == w->dfKey) return w->dfa;
+    lw_moveto(w, alpha);
+    if (alpha != w->gKey) { w->fn->df(w->fn->ctx, w->xa, w->ga); w->gKey = alpha; }
+    w->dfa = o_dot(w->ga, w->p, w->n);
+    w->dfKey = alpha;
+    return w->dfa;
+}
+
+static void lw_fdf(OBfgs2* w, double alpha, double* f, double* df)
+{
+    if (alpha == w->fKey && alpha == w->dfKey) { *f = w->fa; *df = w->dfa; return; }
+    if (alpha == w->fKey || alpha == w->dfKey) { *f = lw_f(w, alpha); *df = lw_df(w, alpha); return; }
+    lw_moveto(w, alpha);
+    w->fn->fdf(w->fn->ctx, w->xa, &w->fa, w->ga);
+    w->fKey = alpha; w->gKey = alpha;
+    w->dfa = o_dot(w->ga, w->p, w->n);
+    w->dfKey = alpha;
+    *f = w->fa; *df = w->dfa;
+}
+
+/* new origin alpha = 0 at the end point of the last line search, along the new p */
+static void lw_change_direction(OBfgs2* w)
+{
+    memcpy(w->xa, w->x, sizeof(double) * w->n);
+    w->xKey = 0.0;
+    w->fKey = 0.0;
+    memcpy(w->ga, w->g, sizeof(double) * w->n);
+    w->gKey = 0.0;
+    w->dfa = o_dot(w->ga, w->p, w->n);
+    w->dfKey = 0.0;
+}
+
+/* real roots of a z^2 + b z + c, ascending (gsl_poly_solve_quadratic) */
+static int o_solve_quadratic(double a, double b, double c, double* z0, double* z1)
+{
+    if (a == 0) {
+        if (b == 0) return 0;
+        *z0 = -c / b;
+        return 1;
+    }
+    double disc = b * b - 4 * a * c;
+    if (disc > 0) {
+        if (b == 0) { double r = sqrt(-c / a); *z0 = -r; *z1 = r; }
+        else {
+            double sgnb = (b > 0 ? 1 : -1), temp = -0.5 * (b + sgnb * sqrt(disc));
+            double r1 = temp / a, r2 = c / temp;
+            if (r1 < r2) { *z0 = r1; *z1 = r2; } else { *z0 = r2; *z1 = r1; }
+        }
+        return 2;
+    }
+    if (disc == 0) { *z0 = -0.5 * b / a; *z1 = -0.5 * b / a; return 2; }
+    return 0;
+}
+
+/* minimum on [zl, zh] of the quadratic through (0, f0) with slope fp0 and (1, f1) */
+static double o_interp_quad(double f0, double fp0, double f1, double zl, double zh)
+{
+    double fl = f0 + zl * (fp0 + zl * (f1 - f0 - fp0));
+    double fh = f0 + zh * (fp0 + zh * (f1 - f0 - fp0));
+    double c = 2 * (f1 - f0 - fp0);
+    double zmin = zl, fmin = fl;
+    if (fh < fmin) { zmin = zh; fmin = fh; }
+    if (c > 0) {
+        double z = -fp0 / c;
+        if (z > zl && z < zh) {
+            double f = f0 + z * (fp0 + z * (f1 - f0 - fp0));
+            if (f < fmin) { zmin = z; fmin = f; }
+        }
+    }
+    return zmin;
+}
+
+static double o_cubic(double c0, double c1, double c2, double c3, double z) { return c0 + z * (c1 + z * (c2 + z * c3)); }
+
+/* minimum on [zl, zh] of the cubic through (0, f0), (1, f1) with slopes fp0, fp1 */
+static double o_interp_cubic(double f0, double fp0, double f1, double fp1, double zl, double zh)
+{
+    double eta = 3 * (f1 - f0) - 2 * fp0 - fp1, xi = fp0 + fp1 - 2 * (f1 - f0);
+    double c0 = f0, c1 = fp0, c2 = eta, c3 = xi, z0, z1;
+    double zmin = zl, fmin = o_cubic(c0, c1, c2, c3, zl), y;
+    y = o_cubic(c0, c1, c2, c3, zh);
+    if (y < fmin) { zmin = zh; fmin = y; }
+    int nr = o_solve_quadratic(3 * c3, 2 * c2, c1, &z0, &z1);
+    if (nr >= 1 && z0 > zl && z0 < zh) { y = o_cubic(c0, c1, c2, c3, z0); if (y < fmin) { zmin = z0; fmin = y; } }
+    if (nr == 2 && z1 > zl && z1 < zh) { y = o_cubic(c0, c1, c2, c3, z1); if (y < fmin) { zmin = z1; fmin = y; } }
+    return zmin;
+}
+
+/* trial point in [xmin, xmax] from the end points a, b (slope fpb may be NaN: quadratic) */
+static double o_interpolate(double a, double fa, double fpa, double b, double fb, double fpb, double xmin,
+                            double xmax)
+{
+    double zmin = (xmin - a) / (b - a), zmax = (xmax - a) / (b - a), z;
+    if (zmin > zmax) { double t = zmin; zmin = zmax; zmax = t; }
+    if (isfinite(fpb)) z = o_interp_cubic(fa, fpa * (b - a), fb, fpb * (b - a), zmin, zmax);
+    else z = o_interp_quad(fa, fpa * (b - a), fb, zmin, zmax);
+    return a + z * (b - a);
+}
+
+/* Fletcher's line search along the wrapper's p from alpha = 0 (cached); *alphaNew stays
+   unchanged when the trial budget runs out */
+static int o_line_minimize(OBfgs2* w, double alpha1, double* alphaNew)
+{
+    const double rho = w->rho, sigma = w->sigma, tau1 = w->tau1, tau2 = w->tau2, tau3 = w->tau3;
+    double f0, fp0, falpha, falphaPrev, fpalpha, fpalphaPrev, delta, alphaNext;
+    double alpha = alpha1, alphaPrev = 0.0, a, b, fa, fb, fpa, fpb;
+    int i = 0;
+    lw_fdf(w, 0.0, &f0, &fp0);
+    falphaPrev = f0;
+    fpalphaPrev = fp0;
+    a = 0.0; b = alpha; fa = f0; fb = 0.0; fpa = fp0; fpb = 0.0;
+    while (i++ < 100) {                                     /* bracketing */
+        falpha = lw_f(w, alpha);
+        if (falpha > f0 + alpha * rho * fp0 || falpha >= falphaPrev) {
+            a = alphaPrev; fa = falphaPrev; fpa = fpalphaPrev;
+            b = alpha; fb = falpha; fpb = NAN;
+            break;
+        }
+        fpalpha = lw_df(w, alpha);
+        if (fabs(fpalpha) <= -sigma * fp0) { *alphaNew = alpha; return GSL_SUCCESS; }
+        if (fpalpha >= 0) {
+            a = alpha; fa = falpha; fpa = fpalpha;
+            b = alphaPrev; fb = falphaPrev; fpb = fpalphaPrev;
+            break;
+        }
+        delta = alpha - alphaPrev;
+        alphaNext = o_interpolate(alphaPrev, falphaPrev, fpalphaPrev, alpha, falpha, fpalpha, alpha + delta,
+                                  alpha + tau1 * delta);
+        alphaPrev = alpha; falphaPrev = falpha; fpalphaPrev = fpalpha;
+        alpha = alphaNext;
+    }
+    while (i++ < 100) {                                     /* sectioning of [a, b] */
+        delta = b - a;
+        alpha = o_interpolate(a, fa, fpa, b, fb, fpb, a + tau2 * delta, b - tau3 * delta);
+        falpha = lw_f(w, alpha);
+        if ((a - alpha) * fpa <= O_DBL_EPSILON) return GSL_ENOPROG;     /* roundoff prevents progress */
+        if (falpha > f0 + rho * alpha * fp0 || falpha >= fa) {
+            b = alpha; fb = falpha; fpb = NAN;
+        } else {
+            fpalpha = lw_df(w, alpha);
+            if (fabs(fpalpha) <= -sigma * fp0) { *alphaNew = alpha; return GSL_SUCCESS; }
+            if (((b - a) >= 0 && fpalpha >= 0) || ((b - a) <= 0 && fpalpha <= 0)) {
+                b = a; fb = fa; fpb = fpa;
+            }
+            a = alpha; fa = falpha; fpa = fpalpha;
+        }
+    }
+    return GSL_SUCCESS;
+}
+
+OBfgs2* o_bfgs2_new(int n)
+{
+    OBfgs2* w = calloc(1, sizeof(OBfgs2));
+    w->n = n;
+    w->x = calloc(n, sizeof(double)); w->g = calloc(n, sizeof(double)); w->p = calloc(n, sizeof(double));
+    w->xa = calloc(n, sizeof(double)); w->ga = calloc(n, sizeof(double));
+    w->dx0 = calloc(n, sizeof(double)); w->dg0 = calloc(n, sizeof(double));
+    return w;
+}
+
+void o_bfgs2_free(OBfgs2* w)
+{
+    if (!w) return;
+    free(w->x); free(w->g); free(w->p); free(w->xa); free(w->ga); free(w->dx0); free(w->dg0); free(w);
+}
+
+/* vector_bfgs2_set: f and g at x, first direction -g/|g| */
+void o_bfgs2_set(OBfgs2* w, const OFdf* fn, const double* x, double* f, double* g, double stepSize, double tol)
+{
+    const int n = w->n;
+    w->fn = fn;
+    w->step = stepSize;
+    w->deltaF = 0;
+    fn->fdf(fn->ctx, x, f, g);
+    memcpy(w->x, x, sizeof(double) * n);
+    memcpy(w->g, g, sizeof(double) * n);
+    w->g0norm = dnrm2(w->g, n);
+    for (int k = 0; k < n; ++k) w->p[k] = g[k] * (-1 / w->g0norm);
+    w->pnorm = dnrm2(w->p, n);
+    w->fp0 = -w->g0norm;
+    memcpy(w->xa, w->x, sizeof(double) * n);              /* the wrapper at alpha = 0 */
+    w->xKey = 0.0;
+    w->fa = *f;
+    w->fKey = 0.0;
+    memcpy(w->ga, w->g, sizeof(double) * n);
+    w->gKey = 0.0;
+    w->dfa = o_dot(w->ga, w->p, n);
+    w->dfKey = 0.0;
+    w->rho = 0.01; w->sigma = tol; w->tau1 = 9; w->tau2 = 0.05; w->tau3 = 0.5;
+}
+
+/* vector_bfgs2_iterate: x, f, g move to the line minimum; dx = the step taken */
+int o_bfgs2_iterate(OBfgs2* w, double* x, double* f, double* g, double* dx)
+{
+    const int n = w->n;
+    double alpha = 0.0, alpha1;
+    const double f0 = *f;
+    if (w->pnorm == 0.0 || w->g0norm == 0.0 || w->fp0 == 0) {
+        memset(dx, 0, sizeof(double) * n);
+        return GSL_ENOPROG;
+    }
+    if (w->deltaF < 0) {
+        double del = fmax(-w->deltaF, 10 * O_DBL_EPSILON * fabs(f0));
+        alpha1 = fmin(1.0, 2.0 * del / (-w->fp0));
+    } else {
+        alpha1 = fabs(w->step);
+    }
+    int status = o_line_minimize(w, alpha1, &alpha);
+    if (status != GSL_SUCCESS) return status;
+    {   /* update_position: f and g at alpha, cached or evaluated together */
+        double fa, dfa;
+        lw_fdf(w, alpha, &fa, &dfa);
+        *f = w->fa;
+        memcpy(x, w->xa, sizeof(double) * n);
+        memcpy(g, w->ga, sizeof(double) * n);
+    }
+    w->deltaF = *f - f0;
+    for (int k = 0; k < n; ++k) { w->dx0[k] = x[k] - w->x[k]; dx[k] = w->dx0[k]; w->dg0[k] = g[k] - w->g[k]; }
+    double dxg = o_dot(w->dx0, g, n), dgg = o_dot(w->dg0, g, n), dxdg = o_dot(w->dx0, w->dg0, n);
+    double dgnorm = dnrm2(w->dg0, n), A, B;
+    if (dxdg != 0) {
+        B = dxg / dxdg;
+        A = -(1.0 + dgnorm * dgnorm / dxdg) * B + dgg / dxdg;
+    } else {
+        B = 0; A = 0;
+    }
+    for (int k = 0; k < n; ++k) w->p[k] = g[k] + (-A) * w->dx0[k] + (-B) * w->dg0[k];
+    memcpy(w->g, g, sizeof(double) * n);
+    memcpy(w->x, x, sizeof(double) * n);
+    w->g0norm = dnrm2(w->g, n);
+    w->pnorm = dnrm2(w->p, n);
+    double pg = o_dot(w->p, g, n), dir = (pg >= 0.0) ? -1.0 : +1.0;
+    for (int k = 0; k < n; ++k) w->p[k] *= dir / w->pnorm;
+    w->pnorm = dnrm2(w->p, n);
+    w->fp0 = o_dot(w->p, w->g, n);
+    lw_change_direction(w);
+    return GSL_SUCCESS;
+}
+
 /* ---------- restoration (object.cpp:278-443, utils.cpp:304-392) ---------- */
 
 static void smooth2(double* d, int ny, int nx, int n, double* tmp)
@@ -899,6 +1204,8 @@ int o_solve(const OProblem* pr, OResult* res)
     double f = 0;
 
     double init_step = 1E-4 * maxWl, init_tol = 1E-6;
+    OBfgs2* bfgs = pr->getstep == 4 ? o_bfgs2_new(nFree) : NULL;
+    OFdf fn = { st, cb_f, cb_df, cb_fdf };
     st->gradMethod = 1;
     double previousMetric = 0, thisMetric = 0, gradNorm = 0, tol = 1.0E-1;
     int failCount = 0, totalIterations = 0, maxIterations = 10, maxFails = 3, status = 0;
@@ -917,15 +1224,20 @@ int o_solve(const OProblem* pr, OResult* res)
         if (modeCount == K || pr->maxIter == 0) {
             exitLoop = 1; tol = pr->FTOL; maxIterations = pr->maxIter;
         }
-        /* gsl_multimin_fdfminimizer_set + conjugate_rdx_set (conjugate_gradient.cpp:72-86) */
+        /* gsl_multimin_fdfminimizer_set + the stepper's set: conjugate_rdx_set
+           (conjugate_gradient.cpp:72-86) or vector_bfgs2_set */
         memcpy(x, beta0, sizeof(double) * nFree);
         memset(dx, 0, sizeof(double) * nFree);
-        cs.iter = 0; cs.step = init_step; cs.tol = init_tol;
-        f = my_f(st, x);
+        if (bfgs) {
+            o_bfgs2_set(bfgs, &fn, x, &f, g, init_step, init_tol);
+        } else {
+            cs.iter = 0; cs.step = init_step; cs.tol = init_tol;
+            f = my_f(st, x);
+        }
         int iter = 0, successCount = 0, done = 0;
         while ((!done || iter < pr->minIter) && iter < maxIterations) {
             iter++;
-            status = cg_iterate(st, &cs, x, &f, g, dx, nFree);
+            status = bfgs ? o_bfgs2_iterate(bfgs, x, &f, g, dx) : cg_iterate(st, &cs, x, &f, g, dx, nFree);
             if (failCount > maxFails) { status = GSL_FAILURE; exitLoop = 1; done = 1; break; }
             st->gradMethod = 2;
             if (status == GSL_ENOPROG) { failCount++; break; }
@@ -960,6 +1272,7 @@ int o_solve(const OProblem* pr, OResult* res)
     if (res->shifts) memcpy(res->shifts, st->curShift, sizeof(int) * 2 * pr->nImg);
     if (res->img) restore_objects(st, res->img);
     free(x); free(g); free(dx); free(beta0); free(cs.x1); free(cs.p); free(cs.prevG);
+    o_bfgs2_free(bfgs);
     o_state_free(st);
     return 0;
 }

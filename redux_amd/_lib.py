@@ -15,6 +15,7 @@ MFBD_OK, MFBD_EINVAL, MFBD_ENOMEM, MFBD_EDEVICE, MFBD_ENONFINITE, MFBD_ESTATE = 
 FP64, FP32, MIXED = 0, 1, 2
 GRAD_FD, GRAD_VOGEL, GRAD_ADJOINT = 1, 2, 3
 GRAD_SCHEDULE_REFERENCE = 0
+GETSTEP_CONJUGATE, GETSTEP_BFGS_INV = 2, 4
 BASIS_ZERNIKE, BASIS_KL = 0, 1
 
 
@@ -28,7 +29,8 @@ class Params(C.Structure):
     _fields_ = [("FTOL", C.c_double), ("EPS", C.c_double), ("graddiffStep", C.c_double),
                 ("regAlpha", C.c_double), ("minIter", C.c_int), ("maxIter", C.c_int), ("nDoneIter", C.c_int),
                 ("modeStart", C.c_int), ("modeStep", C.c_int), ("apodSize", C.c_int), ("noFilter", C.c_int),
-                ("precision", C.c_int), ("maxBatch", C.c_int), ("gradient", C.c_int)]
+                ("precision", C.c_int), ("maxBatch", C.c_int), ("gradient", C.c_int),
+                ("getstep", C.c_int)]
 
 
 class Result(C.Structure):

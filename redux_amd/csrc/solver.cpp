@@ -21,6 +21,7 @@
 #include <cstring>
 #include <exception>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -536,6 +537,13 @@ struct PatchRun {
     int nF = 0, nDF = 0;
     mfbd_result res{};
     std::vector<double> metrics;
+    // vector_bfgs2 (GETSTEP = BFGS_inv): the line wrapper over x0 + alpha p caches f, the
+    // gradient and the slope at its last trial point (keys: the alpha they belong to)
+    struct Bfgs2 {
+        std::vector<double> x0, g0, p, xa, ga, dx0, dg0;
+        double fa = 0, dfa = 0, xKey = 0, fKey = 0, gKey = 0, dfKey = 0;
+        double step = 0, g0norm = 0, pnorm = 0, deltaF = 0, fp0 = 0, sigma = 0;
+    } bf;
 };
 
 struct EvalAwait {
@@ -738,12 +746,294 @@ Task<int> cg_iterate(PatchRun& r, double& f) {
     co_return ST_SUCCESS;
 }
 
+// Solver::my_fdf (solver.cpp:237-243): the metric and the gradient of one point, one request
+Task<double> my_fdf(PatchRun& r, const double* beta, double* gb) {
+    ns_reverse_add(r.c, beta, r.alphaOffset.data(), r.alpha.data());
+    r.ev.wantGrad = true;
+    r.ev.method = r.gradMethod;
+    r.ev.nEnabled = r.nEnabled;
+    effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
+    co_await EvalAwait{&r};
+    r.nF++;
+    r.nDF++;
+    ns_apply(r.c, r.ev.gAlpha.data(), gb);
+    co_return r.ev.f;
+}
+
+// ---- GETSTEP = BFGS_inv: GSL 1.16 vector_bfgs2, the reference's default stepper
+// (config.cpp:632, solver.cpp:362-368).  GSL is not part of the reference tree; its
+// published algorithm is restated (multimin/vector_bfgs2.c, linear_minimize.c,
+// linear_wrapper.c): BFGS direction p' = g1 - A dx - B dg (B = dx.g1/dx.dg,
+// A = -(1 + |dg|^2/dx.dg) B + dg.g1/dx.dg), normalised downhill; Fletcher's line search
+// (bracketing, then sectioning; rho 0.01, sigma = tol, tau 9 / 0.05 / 0.5, cubic
+// interpolation when both ends carry a slope, 100 trial points); a line wrapper caching
+// f, g and the slope at the last trial point.
+constexpr double DBL_EPS = 2.2204460492503131e-16;
+
+double dotv(const std::vector<double>& a, const std::vector<double>& b) {
+    double s = 0;
+    for (size_t k = 0; k < a.size(); ++k) s += a[k] * b[k];
+    return s;
+}
+
+void lw_moveto(PatchRun& r, double alpha) {
+    auto& w = r.bf;
+    if (alpha == w.xKey) return;
+    for (size_t k = 0; k < w.xa.size(); ++k) w.xa[k] = w.x0[k] + alpha * w.p[k];
+    w.xKey = alpha;
+}
+
+Task<double> lw_f(PatchRun& r, double alpha) {
+    auto& w = r.bf;
+    if (alpha == w.fKey) co_return w.fa;
+    lw_moveto(r, alpha);
+    w.fa = co_await my_f(r, w.xa.data());
+    w.fKey = alpha;
+    co_return w.fa;
+}
+
+Task<double> lw_df(PatchRun& r, double alpha) {
+    auto& w = r.bf;
+    if (alpha == w.dfKey) co_return w.dfa;
+    lw_moveto(r, alpha);
+    if (alpha != w.gKey) {
+        co_await my_df(r, w.xa.data(), w.ga.data());
+        w.gKey = alpha;
+    }
+    w.dfa = dotv(w.ga, w.p);
+    w.dfKey = alpha;
+    co_return w.dfa;
+}
+
+struct FDf { double f, df; };
+
+Task<FDf> lw_fdf(PatchRun& r, double alpha) {
+    auto& w = r.bf;
+    if (alpha == w.fKey && alpha == w.dfKey) co_return FDf{w.fa, w.dfa};
+    if (alpha == w.fKey || alpha == w.dfKey) {
+        const double f = co_await lw_f(r, alpha);
+        const double df = co_await lw_df(r, alpha);
+        co_return FDf{f, df};
+    }
+    lw_moveto(r, alpha);
+    w.fa = co_await my_fdf(r, w.xa.data(), w.ga.data());
+    w.fKey = alpha;
+    w.gKey = alpha;
+    w.dfa = dotv(w.ga, w.p);
+    w.dfKey = alpha;
+    co_return FDf{w.fa, w.dfa};
+}
+
+// real roots of a z^2 + b z + c, ascending (gsl_poly_solve_quadratic)
+int solve_quadratic(double a, double b, double c, double& z0, double& z1) {
+    if (a == 0) {
+        if (b == 0) return 0;
+        z0 = -c / b;
+        return 1;
+    }
+    const double disc = b * b - 4 * a * c;
+    if (disc > 0) {
+        if (b == 0) {
+            const double rr = std::sqrt(-c / a);
+            z0 = -rr;
+            z1 = rr;
+        } else {
+            const double sgnb = b > 0 ? 1 : -1, temp = -0.5 * (b + sgnb * std::sqrt(disc));
+            const double r1 = temp / a, r2 = c / temp;
+            z0 = std::min(r1, r2);
+            z1 = std::max(r1, r2);
+        }
+        return 2;
+    }
+    if (disc == 0) {
+        z0 = z1 = -0.5 * b / a;
+        return 2;
+    }
+    return 0;
+}
+
+double interp_quad(double f0, double fp0, double f1, double zl, double zh) {
+    const double fl = f0 + zl * (fp0 + zl * (f1 - f0 - fp0)), fh = f0 + zh * (fp0 + zh * (f1 - f0 - fp0));
+    const double c = 2 * (f1 - f0 - fp0);
+    double zmin = zl, fmin = fl;
+    if (fh < fmin) { zmin = zh; fmin = fh; }
+    if (c > 0) {
+        const double z = -fp0 / c;
+        if (z > zl && z < zh) {
+            const double f = f0 + z * (fp0 + z * (f1 - f0 - fp0));
+            if (f < fmin) { zmin = z; fmin = f; }
+        }
+    }
+    return zmin;
+}
+
+double interp_cubic(double f0, double fp0, double f1, double fp1, double zl, double zh) {
+    const double c0 = f0, c1 = fp0, c2 = 3 * (f1 - f0) - 2 * fp0 - fp1, c3 = fp0 + fp1 - 2 * (f1 - f0);
+    auto cub = [&](double z) { return c0 + z * (c1 + z * (c2 + z * c3)); };
+    double zmin = zl, fmin = cub(zl), z0 = 0, z1 = 0;
+    auto check = [&](double z) {
+        const double y = cub(z);
+        if (y < fmin) { zmin = z; fmin = y; }
+    };
+    check(zh);
+    const int nr = solve_quadratic(3 * c3, 2 * c2, c1, z0, z1);
+    if (nr >= 1 && z0 > zl && z0 < zh) check(z0);
+    if (nr == 2 && z1 > zl && z1 < zh) check(z1);
+    return zmin;
+}
+
+double interpolate(double a, double fa, double fpa, double b, double fb, double fpb, double xmin, double xmax) {
+    double zmin = (xmin - a) / (b - a), zmax = (xmax - a) / (b - a);
+    if (zmin > zmax) std::swap(zmin, zmax);
+    const double z = std::isfinite(fpb) ? interp_cubic(fa, fpa * (b - a), fb, fpb * (b - a), zmin, zmax)
+                                        : interp_quad(fa, fpa * (b - a), fb, zmin, zmax);
+    return a + z * (b - a);
+}
+
+// linear_minimize.c: Fletcher's line search from alpha = 0 (cached); alphaNew stays as it
+// was when the trial budget runs out
+Task<int> line_minimize(PatchRun& r, double alpha1, double& alphaNew) {
+    const double rho = 0.01, sigma = r.bf.sigma, tau1 = 9, tau2 = 0.05, tau3 = 0.5;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    const FDf z = co_await lw_fdf(r, 0.0);
+    const double f0 = z.f, fp0 = z.df;
+    double falphaPrev = f0, fpalphaPrev = fp0, alpha = alpha1, alphaPrev = 0.0;
+    double a = 0.0, b = alpha, fa = f0, fb = 0.0, fpa = fp0, fpb = 0.0, falpha, fpalpha;
+    int i = 0;
+    while (i++ < 100) {                                     // bracketing
+        falpha = co_await lw_f(r, alpha);
+        if (falpha > f0 + alpha * rho * fp0 || falpha >= falphaPrev) {
+            a = alphaPrev; fa = falphaPrev; fpa = fpalphaPrev;
+            b = alpha; fb = falpha; fpb = nan;
+            break;
+        }
+        fpalpha = co_await lw_df(r, alpha);
+        if (std::fabs(fpalpha) <= -sigma * fp0) {
+            alphaNew = alpha;
+            co_return ST_SUCCESS;
+        }
+        if (fpalpha >= 0) {
+            a = alpha; fa = falpha; fpa = fpalpha;
+            b = alphaPrev; fb = falphaPrev; fpb = fpalphaPrev;
+            break;
+        }
+        const double delta = alpha - alphaPrev;
+        const double next = interpolate(alphaPrev, falphaPrev, fpalphaPrev, alpha, falpha, fpalpha, alpha + delta,
+                                        alpha + tau1 * delta);
+        alphaPrev = alpha; falphaPrev = falpha; fpalphaPrev = fpalpha;
+        alpha = next;
+    }
+    while (i++ < 100) {                                     // sectioning of [a, b]
+        const double delta = b - a;
+        alpha = interpolate(a, fa, fpa, b, fb, fpb, a + tau2 * delta, b - tau3 * delta);
+        falpha = co_await lw_f(r, alpha);
+        if ((a - alpha) * fpa <= DBL_EPS) co_return ST_ENOPROG;    // roundoff prevents progress
+        if (falpha > f0 + rho * alpha * fp0 || falpha >= fa) {
+            b = alpha; fb = falpha; fpb = nan;
+        } else {
+            fpalpha = co_await lw_df(r, alpha);
+            if (std::fabs(fpalpha) <= -sigma * fp0) {
+                alphaNew = alpha;
+                co_return ST_SUCCESS;
+            }
+            if ((delta >= 0 && fpalpha >= 0) || (delta <= 0 && fpalpha <= 0)) {
+                b = a; fb = fa; fpb = fpa;
+            }
+            a = alpha; fa = falpha; fpa = fpalpha;
+        }
+    }
+    co_return ST_SUCCESS;
+}
+
+// vector_bfgs2_set at x = r.x: f and g (both used), first direction -g/|g|
+Task<int> bfgs2_set(PatchRun& r, double& f, double stepSize, double tol) {
+    auto& w = r.bf;
+    const size_t n = r.x.size();
+    for (auto* v : {&w.x0, &w.g0, &w.p, &w.xa, &w.ga, &w.dx0, &w.dg0}) v->assign(n, 0.0);
+    w.step = stepSize;
+    w.deltaF = 0;
+    w.sigma = tol;
+    f = co_await my_fdf(r, r.x.data(), r.g.data());
+    w.x0 = r.x;
+    w.g0 = r.g;
+    w.g0norm = dnrm2(w.g0);
+    for (size_t k = 0; k < n; ++k) w.p[k] = r.g[k] * (-1 / w.g0norm);
+    w.pnorm = dnrm2(w.p);
+    w.fp0 = -w.g0norm;
+    w.xa = w.x0;                                         // the wrapper at alpha = 0
+    w.xKey = 0.0;
+    w.fa = f;
+    w.fKey = 0.0;
+    w.ga = w.g0;
+    w.gKey = 0.0;
+    w.dfa = dotv(w.ga, w.p);
+    w.dfKey = 0.0;
+    co_return ST_SUCCESS;
+}
+
+// vector_bfgs2_iterate: r.x, f, r.g move to the line minimum, r.dx is the step
+Task<int> bfgs2_iterate(PatchRun& r, double& f) {
+    auto& w = r.bf;
+    const size_t n = r.x.size();
+    double alpha = 0.0, alpha1;
+    const double f0 = f;
+    if (w.pnorm == 0.0 || w.g0norm == 0.0 || w.fp0 == 0) {
+        std::fill(r.dx.begin(), r.dx.end(), 0.0);
+        co_return ST_ENOPROG;
+    }
+    if (w.deltaF < 0) {
+        const double del = std::max(-w.deltaF, 10 * DBL_EPS * std::fabs(f0));
+        alpha1 = std::min(1.0, 2.0 * del / (-w.fp0));
+    } else {
+        alpha1 = std::fabs(w.step);
+    }
+    const int status = co_await line_minimize(r, alpha1, alpha);
+    if (status != ST_SUCCESS) co_return status;
+    co_await lw_fdf(r, alpha);                           // update_position: f and g at alpha
+    f = w.fa;
+    r.x = w.xa;
+    r.g = w.ga;
+    w.deltaF = f - f0;
+    for (size_t k = 0; k < n; ++k) {
+        w.dx0[k] = r.x[k] - w.x0[k];
+        r.dx[k] = w.dx0[k];
+        w.dg0[k] = r.g[k] - w.g0[k];
+    }
+    const double dxg = dotv(w.dx0, r.g), dgg = dotv(w.dg0, r.g), dxdg = dotv(w.dx0, w.dg0), dgnorm = dnrm2(w.dg0);
+    double A = 0, B = 0;
+    if (dxdg != 0) {
+        B = dxg / dxdg;
+        A = -(1.0 + dgnorm * dgnorm / dxdg) * B + dgg / dxdg;
+    }
+    for (size_t k = 0; k < n; ++k) w.p[k] = r.g[k] + (-A) * w.dx0[k] + (-B) * w.dg0[k];
+    w.g0 = r.g;
+    w.x0 = r.x;
+    w.g0norm = dnrm2(w.g0);
+    w.pnorm = dnrm2(w.p);
+    const double pg = dotv(w.p, r.g), dir = (pg >= 0.0) ? -1.0 : +1.0;
+    for (size_t k = 0; k < n; ++k) w.p[k] *= dir / w.pnorm;
+    w.pnorm = dnrm2(w.p);
+    w.fp0 = dotv(w.p, w.g0);
+    w.xa = w.x0;                                         // change_direction: new origin, same f
+    w.xKey = 0.0;
+    w.fKey = 0.0;
+    w.ga = w.g0;
+    w.gKey = 0.0;
+    w.dfa = dotv(w.ga, w.p);
+    w.dfKey = 0.0;
+    co_return ST_SUCCESS;
+}
+
 // Solver::run (solver.cpp:330-580), starting after the reset initialisation
 Task<int> run_patch(PatchRun& r) {
     mfbd_ctx* c = r.c;
     const mfbd_params& pr = c->prm;
     const int K = c->g.K, nFree = c->nFree, nPar = c->nPar;
     const bool adjoint = pr.gradient == MFBD_GRAD_ADJOINT;
+    const bool bfgs = pr.getstep == MFBD_GETSTEP_BFGS_INV;
+    double maxWl = 0;                                   // init_step = 1e-4 lambda_max (solver.cpp:351)
+    for (int o = 0; o < c->g.nObj; ++o) maxWl = std::max(maxWl, c->st.wavelength[o]);
+    const double initStep = 1E-4 * maxWl;
     r.gradMethod = adjoint ? MFBD_GRAD_ADJOINT : MFBD_GRAD_FD;   // GM_DIFF for the first gradient
     double prev = 0, thisMetric = 0, gradNorm = 0, tol = 1.0E-1;
     int failCount = 0, totalIter = 0, maxIterations = 10, status = 0;
@@ -762,17 +1052,20 @@ Task<int> run_patch(PatchRun& r) {
             tol = pr.FTOL;
             maxIterations = pr.maxIter;
         }
-        // gsl_multimin_fdfminimizer_set + conjugate_rdx_set: x = 0, f(x); the gradient
-        // the reference also computes here is overwritten before use.
+        // gsl_multimin_fdfminimizer_set at x = 0 with the stepper's set: conjugate_rdx_set
+        // (f(x); the gradient the reference also computes here is overwritten before use)
+        // or vector_bfgs2_set (f and g, both used)
         std::fill(r.x.begin(), r.x.end(), 0.0);
         std::fill(r.dx.begin(), r.dx.end(), 0.0);
         r.iterCG = 0;
-        double f = co_await my_f(r, r.x.data());
+        double f = 0;
+        if (bfgs) co_await bfgs2_set(r, f, initStep, 1E-6);
+        else f = co_await my_f(r, r.x.data());
         int iter = 0, successCount = 0;
         bool done = false;
         while ((!done || iter < pr.minIter) && iter < maxIterations) {
             iter++;
-            status = co_await cg_iterate(r, f);
+            status = bfgs ? co_await bfgs2_iterate(r, f) : co_await cg_iterate(r, f);
             if (failCount > maxFails) { status = ST_FAILURE; exitLoop = true; done = true; break; }
             if (!adjoint) r.gradMethod = MFBD_GRAD_VOGEL;   // Vogel from here on (solver.cpp:478)
             if (status == ST_ENOPROG) { failCount++; break; }
@@ -863,6 +1156,12 @@ int mfbd_ctx_create(const mfbd_geometry* geo, const mfbd_params* prm, int device
     c->prm = *prm;
     if (c->prm.maxBatch < 1) c->prm.maxBatch = 1;
     if (c->prm.gradient != MFBD_GRAD_SCHEDULE_REFERENCE && c->prm.gradient != MFBD_GRAD_ADJOINT) {
+        delete c;
+        return MFBD_EINVAL;
+    }
+    // 0 (unset) falls to the conjugate gradient, as the reference's switch default does
+    if (c->prm.getstep == 0) c->prm.getstep = MFBD_GETSTEP_CONJUGATE;
+    if (c->prm.getstep != MFBD_GETSTEP_CONJUGATE && c->prm.getstep != MFBD_GETSTEP_BFGS_INV) {
         delete c;
         return MFBD_EINVAL;
     }

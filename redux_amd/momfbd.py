@@ -21,11 +21,18 @@ class SolverError(RuntimeError):
 
 
 def default_params(**kw):
-    """GlobalCfg defaults (config.cpp:629-639) with GETSTEP=conjugate."""
+    """GlobalCfg defaults (config.cpp:629-639), except GETSTEP: conjugate here (the
+    reference's default is BFGS_inv, GSL vector_bfgs2; "bfgs_inv" selects it)."""
     p = dict(FTOL=1e-3, EPS=1e-10, GRADDIFF_STEP=1e-2, REG_ALPHA=0.0, MIN_ITER=5, MAX_ITER=500, N_DONE_ITER=3,
-             MODE_START=5, MODE_STEP=5, APODIZATION=-1, NO_FILTER=0)
+             MODE_START=5, MODE_STEP=5, APODIZATION=-1, NO_FILTER=0, GETSTEP="conjugate")
     p.update(kw)
     return p
+
+
+# getstepMap (config.cpp:102-114), the steppers built here
+GETSTEP = {"conjugate": L.GETSTEP_CONJUGATE, "conjugate_gradient": L.GETSTEP_CONJUGATE,
+           "getstep_conjugate_gradient": L.GETSTEP_CONJUGATE, "bfgs_inv": L.GETSTEP_BFGS_INV,
+           "bfgsinv": L.GETSTEP_BFGS_INV, "getstep_bfgs_inv": L.GETSTEP_BFGS_INV}
 
 
 class Solver:
@@ -47,7 +54,8 @@ class Solver:
         prm = Params(st["FTOL"], st["EPS"], st["GRADDIFF_STEP"], st["REG_ALPHA"], st["MIN_ITER"], st["MAX_ITER"],
                      st["N_DONE_ITER"], st["MODE_START"], st["MODE_STEP"], st["APODIZATION"], int(st["NO_FILTER"]),
                      {"fp64": L.FP64, "fp32": L.FP32, "mixed": L.MIXED}[precision], max_batch,
-                     L.GRAD_ADJOINT if gradient == "adjoint" else L.GRAD_SCHEDULE_REFERENCE)
+                     L.GRAD_ADJOINT if gradient == "adjoint" else L.GRAD_SCHEDULE_REFERENCE,
+                     GETSTEP[str(st["GETSTEP"]).lower()])
         h = C.c_void_p()
         self._check(L.lib().mfbd_ctx_create(C.byref(g), C.byref(prm), device, C.byref(h)), "ctx_create", None)
         self.h = h

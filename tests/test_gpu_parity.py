@@ -152,3 +152,53 @@ def test_repeated_solve_on_one_context_is_identical(golden):
             assert np.array_equal(x["alpha"], y["alpha"]), prec
             assert np.array_equal(x["img"], y["img"]), prec
             assert x["nF"] == y["nF"] and x["finalMetric"] == y["finalMetric"], prec
+
+
+def _bfgs(pb):
+    p = dict(pb)
+    p["settings"] = dict(pb["settings"], GETSTEP="bfgs_inv")
+    return p
+
+
+def test_bfgs_inv_reproduces_the_oracle_outcome(golden):
+    """GETSTEP = bfgs_inv (GSL vector_bfgs2, restated in both the driver and the oracle)
+    with the reference's gradient schedule: both stall at alpha = 0, because the Vogel
+    slope cannot meet the line search's strong Wolfe test (tests/test_oracle_bfgs2.py)."""
+    pb, z = golden
+    from oracle import oracle as O
+    from redux_amd.momfbd import Solver
+    pbb = _bfgs(pb)
+    ro = O.solve(pbb, restore=False)
+    sv = Solver(pbb, precision="fp64", max_batch=1)
+    sv.load([pbb])
+    r = sv.run()[0]
+    sv.close()
+    assert ro["status"] == 27 and ro["finalMetric"] == ro["initialMetric"]
+    assert r["failCount"] == ro["failCount"], (r["failCount"], ro["failCount"])
+    assert r["finalMetric"] == r["initialMetric"]
+    assert abs(r["initialMetric"] - ro["initialMetric"]) <= 1e-12 * ro["initialMetric"]
+
+
+def test_bfgs_inv_with_the_adjoint_gradient_converges(golden):
+    """The same stepper with the exact dL/dalpha (MFBD_GRAD_ADJOINT) meets its line-search
+    tests and converges: the final metric is within 2 % of the conjugate-gradient solve's
+    and the patch and batch results agree bit for bit."""
+    pb, z = golden
+    from redux_amd.momfbd import Solver
+    pbb = _bfgs(pb)
+    out = {}
+    for batch in (1, 3):
+        sv = Solver(pbb, precision="fp64", max_batch=batch, gradient="adjoint")
+        sv.load([pbb] * batch)
+        out[batch] = sv.run()
+        sv.close()
+    sv = Solver(pb, precision="fp64", max_batch=1)
+    sv.load([pb])
+    rc = sv.run()[0]
+    sv.close()
+    r = out[1][0]
+    assert r["status"] != 2, r
+    assert r["finalMetric"] < 0.9 * r["initialMetric"]
+    assert abs(r["finalMetric"] - rc["finalMetric"]) <= 2e-2 * rc["finalMetric"], (r["finalMetric"], rc["finalMetric"])
+    for b in out[3]:
+        assert np.array_equal(b["alpha"], r["alpha"]) and np.array_equal(b["img"], r["img"])

@@ -148,6 +148,20 @@ int mfbd_debug_get(mfbd_ctx* ctx, const char* what, int p, double* out, int* cou
  * over all calls; returns the number of entries (<= cap). */
 int mfbd_kernel_stats(mfbd_ctx* ctx, int cap, char (*names)[48], double* ms, int* launches, long long* units);
 
+/* ---- after the solve: mosaic of restored patches (Object::doMozaic, object.cpp:1160-1207,
+ *      which the ANA/FITS writers call, object.cpp:1213-1310) ---- */
+/* Untrimmed extent of the mosaic of patches at (posY, posX): maxPosY + patchSize + 1 rows,
+ * maxPosX + patchSize + 1 columns (object.cpp:1187-1188). */
+int mfbd_mosaic_size(int nPatches, int patchSize, const int* posY, const int* posX, int* imgRows, int* imgCols);
+/* redux::image::mozaic (utils.hpp:440-502) on the device: patches [n][patchSize][patchSize]
+ * weighted by apodizeInPlace(ones, blend, blend, margin, margin) with margin = patchSize/8,
+ * blend = (patchSize - 2 margin)/3, FP64 sums divided where the weight sum exceeds 1e-6;
+ * with clip (RF_NO_CLIP unset) img_trim(1e-15) drops empty border rows and columns
+ * (utils.hpp:521-553).  img must hold the untrimmed extent; on return it holds
+ * imgRows x imgCols, the window starting at (firstRow, firstCol) of the untrimmed mosaic. */
+int mfbd_mosaic(int nPatches, int patchSize, const float* patches, const int* posY, const int* posX, int clip,
+                int device, float* img, int* imgRows, int* imgCols, int* firstRow, int* firstCol);
+
 /* ---- per-job setup (host; GlobalData/Object::initObject equivalents) ---- */
 const char* mfbd_setup_last_error(void);
 /* Pupil::calculatePupilSize (pupil.cpp:78-100) */

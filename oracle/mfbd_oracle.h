@@ -110,4 +110,9 @@ void o_bfgs2_free(OBfgs2* w);
 void o_bfgs2_set(OBfgs2* w, const OFdf* fn, const double* x, double* f, double* g, double stepSize, double tol);
 int o_bfgs2_iterate(OBfgs2* w, double* x, double* f, double* g, double* dx);
 
+/* ---- mosaic of restored patches (mosaic.c; object.cpp:1160-1207) ---- */
+void o_blend_window(double* w, int nRows, int nCols, int blend, int margin);
+int o_mosaic(int n, int s, const float* patches, const int* posY, const int* posX, int clip,
+             float* img, int* rows, int* cols, int* r0, int* c0);
+
 #endif

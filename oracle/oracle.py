@@ -96,6 +96,10 @@ def lib():
         L.o_autocorrelate_c.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.o_noise.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double]
         L.o_noise.restype = C.c_double
+        ip = C.POINTER(C.c_int)
+        L.o_blend_window.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.o_mosaic.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_float), ip, ip, C.c_int, C.POINTER(C.c_float),
+                               ip, ip, ip, ip]
         _lib = L
     return _lib
 
@@ -126,6 +130,29 @@ def autocorrelate(a, center=False):
     a = np.ascontiguousarray(a, dtype=np.float64).copy()
     lib().o_autocorrelate_d(_p(a), a.shape[0], a.shape[1], int(center))
     return a
+
+
+def blend_window(n_rows, n_cols, blend, margin):
+    """apodizeInPlace of an all-ones array (src/lib/image/utils.cpp:584-613)."""
+    w = np.zeros((n_rows, n_cols))
+    lib().o_blend_window(_p(w), n_rows, n_cols, blend, margin)
+    return w
+
+
+def mosaic(patches, pos_y, pos_x, clip=True):
+    """Object::doMozaic (src/lib/momfbd/object.cpp:1160-1207): blended mosaic of patches
+    [n][s][s] at (pos_y, pos_x), trimmed unless clip is False, and its first (row, column)."""
+    pt = np.ascontiguousarray(patches, dtype=np.float32)
+    n, s = pt.shape[0], pt.shape[1]
+    py = np.ascontiguousarray(pos_y, dtype=np.int32)
+    px = np.ascontiguousarray(pos_x, dtype=np.int32)
+    img = np.zeros((int(py.max()) + s + 1) * (int(px.max()) + s + 1), dtype=np.float32)
+    o = [np.zeros(1, np.int32) for _ in range(4)]
+    ip, fp = C.POINTER(C.c_int), C.POINTER(C.c_float)
+    rc = lib().o_mosaic(n, s, _p(pt, fp), _p(py, ip), _p(px, ip), int(clip), _p(img, fp), *[_p(a, ip) for a in o])
+    assert rc == 0
+    rows, cols = int(o[0][0]), int(o[1][0])
+    return img[: rows * cols].reshape(rows, cols), (int(o[2][0]), int(o[3][0]))
 
 
 def noise(ft, mask=-1, limit=-1.0):

@@ -56,6 +56,8 @@ SIGNATURES = {
     "mfbd_restore": (C.c_int, [C.c_void_p, C.c_int, ip, dp, fp]),
     "mfbd_debug_get": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, dp, ip]),
     "mfbd_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, dp, ip, C.POINTER(C.c_longlong)]),
+    "mfbd_mosaic_size": (C.c_int, [C.c_int, C.c_int, ip, ip, ip, ip]),
+    "mfbd_mosaic": (C.c_int, [C.c_int, C.c_int, fp, ip, ip, C.c_int, C.c_int, fp, ip, ip, ip, ip]),
     "mfbd_setup_last_error": (C.c_char_p, []),
     "mfbd_pupil_size": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double, dp, dp, ip]),
     "mfbd_make_pupil": (C.c_int, [C.c_int, C.c_double, C.c_double, dp, dp]),

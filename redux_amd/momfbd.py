@@ -169,3 +169,24 @@ class Solver:
                                       units.ctypes.data_as(C.POINTER(C.c_longlong)))
         return {names[i].value.decode(): dict(ms=float(ms[i]), launches=int(launches[i]), units=int(units[i]))
                 for i in range(n)}
+
+
+def mosaic(patches, pos_y, pos_x, clip=True, device=0):
+    """Object::doMozaic of restored patches [n][s][s] placed at (pos_y, pos_x): the blended
+    mosaic (trimmed of empty borders unless clip is False) and the (row, column) of its
+    first pixel in the untrimmed frame."""
+    pt = np.ascontiguousarray(patches, dtype=np.float32)
+    n, s = pt.shape[0], pt.shape[1]
+    py = np.ascontiguousarray(pos_y, dtype=np.int32)
+    px = np.ascontiguousarray(pos_x, dtype=np.int32)
+    rows, cols = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    rc = L.lib().mfbd_mosaic_size(n, s, ptr(py, ip), ptr(px, ip), ptr(rows, ip), ptr(cols, ip))
+    if rc:
+        raise SolverError(f"mosaic_size failed ({rc})")
+    img = np.zeros(int(rows[0]) * int(cols[0]), dtype=np.float32)
+    r0, c0 = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    rc = L.lib().mfbd_mosaic(n, s, ptr(pt, fp), ptr(py, ip), ptr(px, ip), int(clip), device, ptr(img, fp),
+                             ptr(rows, ip), ptr(cols, ip), ptr(r0, ip), ptr(c0, ip))
+    if rc:
+        raise SolverError(f"mosaic failed ({rc})")
+    return img[: int(rows[0]) * int(cols[0])].reshape(int(rows[0]), int(cols[0])), (int(r0[0]), int(c0[0]))

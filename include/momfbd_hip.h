@@ -118,6 +118,17 @@ int mfbd_set_constraints(mfbd_ctx* ctx, int nPar, int nFree, int nnz, const int*
 int mfbd_load_patches(mfbd_ctx* ctx, int n, const float* data, const double* phiFixed, const double* resOffset,
                       const double* noiseFudge);
 
+/* Ragged batches (uneven mode and frame counts in one context): patch `patch` of the last
+ * load solves for its first nModes modes only, has only the images with imgOn[j] != 0
+ * ([nImg]; at least one per object; the frames of the others are ignored) and its own null
+ * space N ((row, col, value) triplets as in mfbd_set_constraints, rows only on its images
+ * and modes).  The reference runs such patches as separate jobs of their own geometry
+ * (object.cpp:789-790); within a context of the largest K and frame counts this gives the
+ * same results, since a missing image has a zero OTF and a missing mode a zero alpha.
+ * Call after mfbd_load_patches and before the solve; a new load resets every patch. */
+int mfbd_set_patch_shape(mfbd_ctx* ctx, int patch, int nModes, const unsigned char* imgOn, int nFree, int nnz,
+                         const int* rows, const int* cols, const double* vals);
+
 /* Solver::run for every loaded patch (batched maxBatch at a time).  alpha0
  * [n][nImg][K] warm start (Solver::loadInit) or NULL; results[n]; alphaOut
  * [n][nImg][K]; imgOut [n][nObj][s][s] restored objects (Object::restorePatch,

@@ -60,6 +60,9 @@ public:
                              const std::vector<double>& noiseWindow, double* noise, double* stddev) = 0;
     // regularisation gamma of n slots, gamma[k * nObj + o] for slot slots[k]
     virtual void set_gamma(int n, const int* slots, const double* gamma) = 0;
+    // which of the nImg images the patch in `slot` has (ragged frame counts): a missing
+    // image gets a zero pupil amplitude, so it adds nothing to P, Q, the metric or gradient
+    virtual void set_images_enabled(int slot, const unsigned char* on) = 0;
     virtual void evaluate(std::vector<EvalReq>& reqs) = 0;
     // asynchronous form on lane 0 or 1: begin enqueues and returns, end waits and fills
     // f / g; the two lanes let the host work on one half of a batch while the device

@@ -2061,11 +2061,14 @@ public:
         upload(imgObj_, s.imgObj, st_);
         upload(imgCh_, s.imgCh, st_);
         upload(w_, s.weight, st_);
-        {
-            std::vector<T> isc(g_.nImg);
-            for (int j = 0; j < g_.nImg; ++j) isc[j] = (T)otfNorm[s.imgCh[j]];
-            upload(imgScale_, isc, st_);
-        }
+        // pupil amplitude scale per (slot, image): the channel's otfNormalization, 0 for an
+        // image a patch does not have (its OTF, and with it its share of P, Q and the
+        // gradient, is then exactly zero)
+        hScale_.assign((size_t)maxSlots_ * g_.nImg, T(0));
+        imgOn_.assign((size_t)maxSlots_ * g_.nImg, 1);
+        for (int sl = 0; sl < maxSlots_; ++sl)
+            for (int j = 0; j < g_.nImg; ++j) hScale_[(size_t)sl * g_.nImg + j] = (T)otfNorm[s.imgCh[j]];
+        upload(imgScale_, hScale_, st_);
         hStage_.alloc((size_t)maxSlots_ * g_.nImg * g_.K);
         std::fill(hStage_.p, hStage_.p + hStage_.n, 0.0);
         const size_t nS = maxSlots_;
@@ -2127,6 +2130,16 @@ public:
     }
 
     // gamma[k * nObj + o] of slot slots[k], all in one copy
+    void set_images_enabled(int slot, const unsigned char* on) override {
+        const size_t b = (size_t)slot * g_.nImg;
+        for (int j = 0; j < g_.nImg; ++j) {
+            imgOn_[b + j] = on[j] ? 1 : 0;
+            hScale_[b + j] = on[j] ? (T)otfNorm_[st.imgCh[j]] : T(0);
+        }
+        HIPCHK(hipMemcpyAsync(imgScale_.p + b, hScale_.data() + b, g_.nImg * sizeof(T), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipStreamSynchronize(st_));
+    }
+
     void set_gamma(int n, const int* slots, const double* gamma) override {
         if (n <= 0) return;
         hGamma_.resize((size_t)maxSlots_ * g_.nObj, 0.0);
@@ -2458,7 +2471,7 @@ private:
         // one pinned block and one copy per evaluation: the caller's doubles xd, then the
         // request tables (6 nQ + n ints), then the caller's ints xi
         const size_t nd = xd ? xd->size() : 0, nx = xi ? xi->size() : 0;
-        const size_t nInts = 6 * (size_t)nQ + n + nx, nBlk = nd + (nInts + 1) / 2;
+        const size_t nInts = 7 * (size_t)nQ + n + nx, nBlk = nd + (nInts + 1) / 2;
         sc_->hTab.alloc(nBlk);
         if (nd) std::memcpy(sc_->hTab.p, xd->data(), nd * sizeof(double));
         int* ints = reinterpret_cast<int*>(sc_->hTab.p + nd);
@@ -2471,10 +2484,11 @@ private:
                 ints[3 * nQ + q] = r;                                          // request
                 ints[4 * nQ + n + q] = rq[r].slot * g_.nCh + st.imgCh[j];     // phiCh row
                 ints[5 * nQ + n + q] = st.imgObj[j];                           // object
+                ints[6 * nQ + n + q] = rq[r].slot * nImg + j;                  // pupil scale row
             }
             ints[4 * nQ + r] = rq[r].slot;
         }
-        if (nx) std::memcpy(ints + 6 * (size_t)nQ + n, xi->data(), nx * sizeof(int));
+        if (nx) std::memcpy(ints + 7 * (size_t)nQ + n, xi->data(), nx * sizeof(int));
         if (staged) {
             const size_t cnt = (size_t)(hi - lo + 1) * row;
             sc_->alpha.alloc(std::max(cnt, sc_->alpha.n));
@@ -2494,7 +2508,8 @@ private:
         sc_->reqSlot = ti + 4 * nQ;
         const int* qPhRow = ti + 4 * nQ + n;
         const int* qObj = ti + 5 * nQ + n;
-        sc_->tx = ti + 6 * nQ + n;
+        const int* qScale = ti + 6 * nQ + n;
+        sc_->tx = ti + 7 * nQ + n;
         sc_->td = sc_->tab.p;
         sc_->phi.alloc((size_t)nQ * nPS_);
         sc_->S.alloc((size_t)nQ * nOS_);
@@ -2512,7 +2527,7 @@ private:
                                    sc_->phi.p, B_.p, phiCh_.p, sc_->alpha.p, sc_->qm.arow, qPhRow, qObj, nQ, K, nPS_);
             });
         }
-        otf_chain(sc_->phi.p, sc_->qm.img, nQ, sc_->S.p);
+        otf_chain(sc_->phi.p, qScale, nQ, sc_->S.p);
         const int nChunk = (nOS_ + 255) / 256;
         sc_->Pp.alloc((size_t)n * g_.nObj * nOS_ * 2);
         sc_->Qp.alloc((size_t)n * g_.nObj * nOS_);
@@ -2676,12 +2691,13 @@ private:
             if (!rq[r].wantGrad || rq[r].gradMethod != 1) continue;
             for (int j = 0; j < nImg; ++j) {
                 const int o = st.imgObj[j];
+                if (!imgOn_[(size_t)rq[r].slot * nImg + j]) continue;   // an image the patch lacks
                 for (int m = 0; m < rq[r].nEnabled; ++m) {
                     base.push_back(r * nImg + j);
                     mode.push_back(m);
                     obj.push_back(o);
                     step.push_back((T)(st.graddiffStep * st.wavelength[o]));
-                    pimg.push_back(j);
+                    pimg.push_back(rq[r].slot * nImg + j);
                     wgt.push_back(st.weight[o]);
                     dst.push_back({r * nImg + j, m});
                 }
@@ -2774,6 +2790,8 @@ private:
     // resident: basis, supports, per-patch frames, D, ftSum, gamma, channel phases
     DBuf<C> tw_, D_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
+    std::vector<T> hScale_;                 // host copy of imgScale_ [maxSlots][nImg]
+    std::vector<unsigned char> imgOn_;      // images each slot's patch has [maxSlots][nImg]
     DBuf<__bf16> Bt_;                 // BF16 basis, transposed, for the mixed-precision form
     bool bf16Phase_ = false;
     HBuf<double> hStage_;

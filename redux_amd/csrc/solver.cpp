@@ -303,6 +303,14 @@ void scharmer(cd* data, const double* q2, int ny, int nx, double noise_power, do
 // ------------------------------------------------------------------ context ----
 
 
+// alpha = alpha_offset + N beta of one patch (Constraints::reverseAndAdd / apply,
+// constraints.hpp:128-159): N as (row, col, value) triplets
+struct NullSpace {
+    int nFree = 0;
+    std::vector<int> row, col;
+    std::vector<double> val;
+};
+
 struct mfbd_ctx {
     Geometry g{};
     mfbd_params prm{};
@@ -316,8 +324,12 @@ struct mfbd_ctx {
     std::vector<int> tilt;
     // constraints
     int nPar = 0, nFree = 0;
-    std::vector<int> nsRow, nsCol;
-    std::vector<double> nsVal;
+    NullSpace ns;                                // the job's
+    // ragged batches: per patch, the modes it solves for, the images it has and its own
+    // null space (nFree 0: the job's), all within the context's K and nImg
+    std::vector<int> nModesP;
+    std::vector<std::vector<unsigned char>> imgOn;
+    std::vector<NullSpace> nsP;
     // patches
     int nPatch = 0;
     std::vector<float> data;          // host copy of raw frames [n][nImg][dataY][dataX]
@@ -340,6 +352,12 @@ int set_err(mfbd_ctx* c, int code, const std::string& m) {
     if (c) c->err = m;
     return code;
 }
+
+const NullSpace& ns_of(const mfbd_ctx* c, int p) {
+    return c->nsP.empty() || c->nsP[p].nFree == 0 ? c->ns : c->nsP[p];
+}
+
+bool img_on(const mfbd_ctx* c, int p, int j) { return c->imgOn.empty() || c->imgOn[p][j]; }
 
 // phi_channel per patch: phi_fixed - residual-tilt (Channel::initPatch, channel.cpp:908-939)
 void phi_channels(const mfbd_ctx* c, int p, std::vector<double>& out) {
@@ -415,13 +433,18 @@ void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
         });
     }
     std::vector<double> gamAll(patches.size() * nObj, 0.0);
+    std::vector<int> nIO(nObj);
     for (size_t a = 0; a < patches.size(); ++a) {
         const int p = patches[a];
         c->noise[p].assign(noiseAll.begin() + a * nImg, noiseAll.begin() + (a + 1) * nImg);
         double* gam = &gamAll[a * nObj];
+        std::fill(nIO.begin(), nIO.end(), 0);
+        for (int j = 0; j < nImg; ++j)
+            if (img_on(c, p, j)) nIO[c->st.imgObj[j]]++;
         for (int j = 0; j < nImg; ++j) {                      // Object::addRegGamma
+            if (!img_on(c, p, j)) continue;
             const int o = c->st.imgObj[j];
-            gam[o] += 0.10 * (noiseAll[a * nImg + j] / sdAll[a * nImg + j]) / c->st.nImgObj[o];
+            gam[o] += 0.10 * (noiseAll[a * nImg + j] / sdAll[a * nImg + j]) / nIO[o];
         }
         c->gamma[p].assign(gam, gam + nObj);
     }
@@ -431,6 +454,7 @@ void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
 
 // SubImage::adjustShifts (subimage.cpp:444-480) for one image; returns true if shifted
 bool adjust_shift(mfbd_ctx* c, int p, int j, const double* a) {
+    if (!img_on(c, p, j)) return false;
     const int o = c->st.imgObj[j];
     const double* J = &c->jac[8 * o];
     const int tx = c->tilt[2 * o], ty = c->tilt[2 * o + 1];
@@ -492,7 +516,7 @@ void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const 
         double nv = 0;
         int cnt = 0;
         for (int j = 0; j < c->g.nImg; ++j)
-            if (c->st.imgObj[j] == o) { nv += c->noise[p][j] * c->noise[p][j]; cnt++; }
+            if (c->st.imgObj[j] == o && img_on(c, p, j)) { nv += c->noise[p][j] * c->noise[p][j]; cnt++; }
         nv /= cnt;
         if (!c->prm.noFilter) scharmer(ao.data(), dn.data(), N, N, nv, 0.90 * c->st.freqCutoff[o]);
         std::vector<cd> t((size_t)N * N);
@@ -525,6 +549,8 @@ struct EvalSlot {
 struct PatchRun {
     mfbd_ctx* c = nullptr;
     int p = 0;
+    const NullSpace* ns = nullptr;  // this patch's null space
+    int K = 0;                      // modes this patch solves for
     EvalSlot ev;
     bool wantShiftInit = false;     // pending re-initialisation after a stage
     std::vector<int> shiftedObjs;
@@ -560,18 +586,18 @@ struct InitAwait {
     void await_resume() {}
 };
 
-void ns_reverse_add(const mfbd_ctx* c, const double* in, const double* offset, double* out) {
+void ns_reverse_add(const mfbd_ctx* c, const NullSpace& ns, const double* in, const double* offset, double* out) {
     std::memcpy(out, offset, sizeof(double) * c->nPar);
-    for (size_t e = 0; e < c->nsRow.size(); ++e) out[c->nsRow[e]] += c->nsVal[e] * in[c->nsCol[e]];
+    for (size_t e = 0; e < ns.row.size(); ++e) out[ns.row[e]] += ns.val[e] * in[ns.col[e]];
 }
 
-void ns_apply(const mfbd_ctx* c, const double* in, double* out) {
-    std::fill(out, out + c->nFree, 0.0);
-    for (size_t e = 0; e < c->nsRow.size(); ++e) out[c->nsCol[e]] += c->nsVal[e] * in[c->nsRow[e]];
+void ns_apply(const NullSpace& ns, const double* in, double* out) {
+    std::fill(out, out + ns.nFree, 0.0);
+    for (size_t e = 0; e < ns.row.size(); ++e) out[ns.col[e]] += ns.val[e] * in[ns.row[e]];
 }
 
 Task<double> my_f(PatchRun& r, const double* beta) {   // Solver::my_f (solver.cpp:219-225)
-    ns_reverse_add(r.c, beta, r.alphaOffset.data(), r.alpha.data());
+    ns_reverse_add(r.c, *r.ns, beta, r.alphaOffset.data(), r.alpha.data());
     r.ev.wantGrad = false;
     effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
     co_await EvalAwait{&r};
@@ -580,20 +606,20 @@ Task<double> my_f(PatchRun& r, const double* beta) {   // Solver::my_f (solver.c
 }
 
 Task<int> my_df(PatchRun& r, const double* beta, double* gb) {   // Solver::my_df (solver.cpp:228-234)
-    ns_reverse_add(r.c, beta, r.alphaOffset.data(), r.alpha.data());
+    ns_reverse_add(r.c, *r.ns, beta, r.alphaOffset.data(), r.alpha.data());
     r.ev.wantGrad = true;
     r.ev.method = r.gradMethod;
     r.ev.nEnabled = r.nEnabled;
     effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
     co_await EvalAwait{&r};
     r.nDF++;
-    ns_apply(r.c, r.ev.gAlpha.data(), gb);
+    ns_apply(*r.ns, r.ev.gAlpha.data(), gb);
     co_return 0;
 }
 
 // value_at / take_step (conjugate_gradient.cpp:110-125)
 Task<double> value_at(PatchRun& r, double stp, double lambda) {
-    const int n = r.c->nFree;
+    const int n = r.ns->nFree;
     for (int k = 0; k < n; ++k) r.dx[k] = 0.0 + (-stp * lambda) * r.p_[k];
     for (int k = 0; k < n; ++k) r.x1[k] = r.x[k] + 1.0 * r.dx[k];
     double f = co_await my_f(r, r.x1.data());
@@ -705,7 +731,7 @@ double dnrm2(const std::vector<double>& v) {
 
 // conjugate_rdx_iterate (conjugate_gradient.cpp:128-256)
 Task<int> cg_iterate(PatchRun& r, double& f) {
-    const int n = r.c->nFree;
+    const int n = r.ns->nFree;
     co_await my_df(r, r.x.data(), r.g.data());
     double pnorm = r.pnorm, g0norm = r.g0norm;
     const double g1norm = dnrm2(r.g);
@@ -748,7 +774,7 @@ Task<int> cg_iterate(PatchRun& r, double& f) {
 
 // Solver::my_fdf (solver.cpp:237-243): the metric and the gradient of one point, one request
 Task<double> my_fdf(PatchRun& r, const double* beta, double* gb) {
-    ns_reverse_add(r.c, beta, r.alphaOffset.data(), r.alpha.data());
+    ns_reverse_add(r.c, *r.ns, beta, r.alphaOffset.data(), r.alpha.data());
     r.ev.wantGrad = true;
     r.ev.method = r.gradMethod;
     r.ev.nEnabled = r.nEnabled;
@@ -756,7 +782,7 @@ Task<double> my_fdf(PatchRun& r, const double* beta, double* gb) {
     co_await EvalAwait{&r};
     r.nF++;
     r.nDF++;
-    ns_apply(r.c, r.ev.gAlpha.data(), gb);
+    ns_apply(*r.ns, r.ev.gAlpha.data(), gb);
     co_return r.ev.f;
 }
 
@@ -1028,7 +1054,7 @@ Task<int> bfgs2_iterate(PatchRun& r, double& f) {
 Task<int> run_patch(PatchRun& r) {
     mfbd_ctx* c = r.c;
     const mfbd_params& pr = c->prm;
-    const int K = c->g.K, nFree = c->nFree, nPar = c->nPar;
+    const int K = c->g.K, nModes = r.K;
     const bool adjoint = pr.gradient == MFBD_GRAD_ADJOINT;
     const bool bfgs = pr.getstep == MFBD_GETSTEP_BFGS_INV;
     double maxWl = 0;                                   // init_step = 1e-4 lambda_max (solver.cpp:351)
@@ -1041,13 +1067,15 @@ Task<int> run_patch(PatchRun& r) {
     bool lastDone = false;
     double initial = co_await my_f(r, r.beta0.data());
     r.metrics.push_back(initial);
-    const double nTotalPixels = (double)c->g.nImg * c->g.s * c->g.s;
+    int nImgP = 0;
+    for (int j = 0; j < c->g.nImg; ++j) nImgP += img_on(c, r.p, j);
+    const double nTotalPixels = (double)nImgP * c->g.s * c->g.s;
     const double gradScale = 1.0 / (nTotalPixels * nTotalPixels);
     bool exitLoop = false;
     for (int modeCount = pr.modeStart; !exitLoop; modeCount += pr.modeStep) {
-        modeCount = std::min(modeCount, K);
+        modeCount = std::min(modeCount, nModes);
         r.nEnabled = std::max(r.nEnabled, modeCount);
-        if (modeCount == K || pr.maxIter == 0) {
+        if (modeCount == nModes || pr.maxIter == 0) {
             exitLoop = true;
             tol = pr.FTOL;
             maxIterations = pr.maxIter;
@@ -1097,8 +1125,6 @@ Task<int> run_patch(PatchRun& r) {
             r.alphaOffset = r.alpha;
         }
     }
-    (void)nFree;
-    (void)nPar;
     r.res.initialMetric = initial;
     r.res.finalMetric = thisMetric;
     r.res.iterations = totalIter;
@@ -1113,7 +1139,7 @@ void shift_inits(mfbd_ctx* c, std::vector<PatchRun*>& rs, int lane) {
     std::vector<ImgInit> imgs;
     for (auto* r : rs)
         for (int j = 0; j < c->g.nImg; ++j)
-            if (r->shiftedObjs[c->st.imgObj[j]])
+            if (r->shiftedObjs[c->st.imgObj[j]] && img_on(c, r->p, j))
                 imgs.push_back({r->p, j, c->shifts[r->p][2 * j], c->shifts[r->p][2 * j + 1]});
     c->dev->init_images(imgs, c->window, lane);
 }
@@ -1237,9 +1263,10 @@ int mfbd_set_constraints(mfbd_ctx* c, int nPar, int nFree, int nnz, const int* r
             return set_err(c, MFBD_EINVAL, "nullspace entry out of range");
     c->nPar = nPar;
     c->nFree = nFree;
-    c->nsRow.assign(rows, rows + nnz);
-    c->nsCol.assign(cols, cols + nnz);
-    c->nsVal.assign(vals, vals + nnz);
+    c->ns.nFree = nFree;
+    c->ns.row.assign(rows, rows + nnz);
+    c->ns.col.assign(cols, cols + nnz);
+    c->ns.val.assign(vals, vals + nnz);
     c->haveCons = true;
     return MFBD_OK;
 }
@@ -1259,6 +1286,9 @@ int mfbd_load_patches(mfbd_ctx* c, int n, const float* data, const double* phiFi
     c->tiltOff.assign(n, {});
     c->noise.assign(n, {});
     c->gamma.assign(n, {});
+    c->nModesP.clear();
+    c->imgOn.clear();
+    c->nsP.clear();
     c->initDone = false;
     try {
         // device slots: the evaluator keeps every loaded patch resident
@@ -1276,6 +1306,53 @@ int mfbd_load_patches(mfbd_ctx* c, int n, const float* data, const double* phiFi
             std::copy(phi.begin(), phi.end(), all.begin() + perPhi * p);
         }
         c->dev->load(0, n, data, all.data());
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_set_patch_shape(mfbd_ctx* c, int patch, int nModes, const unsigned char* imgOn, int nFree, int nnz,
+                         const int* rows, const int* cols, const double* vals) {
+    if (!c || !imgOn || !rows || !cols || !vals) return set_err(c, MFBD_EINVAL, "null argument");
+    if (!c->dev || !c->haveCons) return set_err(c, MFBD_ESTATE, "constraints and patches must be set first");
+    if (patch < 0 || patch >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
+    const int K = c->g.K, nImg = c->g.nImg, nPar = c->nPar;
+    if (nModes < 1 || nModes > K) return set_err(c, MFBD_EINVAL, "nModes out of range");
+    std::vector<int> perObj(c->g.nObj, 0);
+    for (int j = 0; j < nImg; ++j) perObj[c->st.imgObj[j]] += imgOn[j] ? 1 : 0;
+    for (int v : perObj)
+        if (v == 0) return set_err(c, MFBD_EINVAL, "every object needs at least one image");
+    if (nFree < 1 || nFree > nPar || nnz < 1) return set_err(c, MFBD_EINVAL, "nullspace has the wrong number of entries");
+    for (int e = 0; e < nnz; ++e) {
+        if (rows[e] < 0 || rows[e] >= nPar || cols[e] < 0 || cols[e] >= nFree)
+            return set_err(c, MFBD_EINVAL, "nullspace entry out of range");
+        if (!imgOn[rows[e] / K] || rows[e] % K >= nModes)
+            return set_err(c, MFBD_EINVAL, "nullspace entry outside the patch's images and modes");
+    }
+    try {
+        if (c->nModesP.empty()) {
+            c->nModesP.assign(c->nPatch, K);
+            c->imgOn.assign(c->nPatch, std::vector<unsigned char>(nImg, 1));
+            c->nsP.assign(c->nPatch, NullSpace{});
+        }
+        c->nModesP[patch] = nModes;
+        for (int j = 0; j < nImg; ++j) c->imgOn[patch][j] = imgOn[j] ? 1 : 0;
+        NullSpace& ns = c->nsP[patch];
+        ns.nFree = nFree;
+        ns.row.assign(rows, rows + nnz);
+        ns.col.assign(cols, cols + nnz);
+        ns.val.assign(vals, vals + nnz);
+        // the frames of a missing image are zero, so its transform D is zero too
+        const size_t fr = (size_t)c->g.dataY * c->g.dataX;
+        float* blk = &c->data[(size_t)patch * nImg * fr];
+        for (int j = 0; j < nImg; ++j)
+            if (!imgOn[j]) std::fill(blk + j * fr, blk + (j + 1) * fr, 0.0f);
+        std::vector<double> phi;
+        phi_channels(c, patch, phi);
+        c->dev->load(patch, 1, blk, phi.data());
+        c->dev->set_images_enabled(patch, c->imgOn[patch].data());
+        c->initDone = false;
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
     }
@@ -1376,7 +1453,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
     int rc = check_ready(c);
     if (rc) return rc;
     try {
-        const int K = c->g.K, nImg = c->g.nImg, nPar = c->nPar, nFree = c->nFree;
+        const int K = c->g.K, nImg = c->g.nImg, nPar = c->nPar;
         const int batch = std::max(1, std::min(c->prm.maxBatch, c->nPatch));
         for (int b0 = 0; b0 < c->nPatch; b0 += batch) {
             const int nb = std::min(batch, c->nPatch - b0);
@@ -1386,6 +1463,9 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 auto r = std::make_unique<PatchRun>();
                 r->c = c;
                 r->p = b0 + k;
+                r->ns = &ns_of(c, r->p);
+                r->K = c->nModesP.empty() ? K : c->nModesP[r->p];
+                const int nFree = r->ns->nFree;
                 r->alpha.assign(nPar, 0.0);
                 r->alphaOffset.assign(nPar, 0.0);
                 if (alpha0) std::copy(alpha0 + (size_t)r->p * nPar, alpha0 + (size_t)(r->p + 1) * nPar, r->alphaOffset.begin());

@@ -109,6 +109,20 @@ class Solver:
         self.nPatch = len(patches)
         self._data = data
 
+    def set_shapes(self, shapes):
+        """Ragged batch: per loaded patch, a dict(nModes, imgOn, nFree, rows, cols, vals) (or
+        None to keep the context's) -- the modes and images it has and its null space."""
+        for p, sh in enumerate(shapes):
+            if sh is None:
+                continue
+            on = np.ascontiguousarray(sh["imgOn"], dtype=np.uint8)
+            rows = np.ascontiguousarray(sh["rows"], dtype=np.int32)
+            cols = np.ascontiguousarray(sh["cols"], dtype=np.int32)
+            vals = np.ascontiguousarray(sh["vals"], dtype=np.float64)
+            self._check(L.lib().mfbd_set_patch_shape(self.h, p, int(sh["nModes"]), on.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                                     int(sh["nFree"]), len(rows), ptr(rows, ip), ptr(cols, ip),
+                                                     ptr(vals)), "set_patch_shape")
+
     def run(self, alpha0=None, restore=True):
         n = self.nPatch
         res = (Result * n)()

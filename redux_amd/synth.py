@@ -176,6 +176,38 @@ def build_field_batch(cfg: dict, n_patches: int, seed: int = 2024, field_size: i
     return su, out
 
 
+def ragged_patch(su: dict, seed: int, n_modes: int, frames, field=None, field_offset=(0, 0)):
+    """A patch of the class `su` (whose K and n_frames are the class maxima) that solves for
+    its first `n_modes` modes with frames[o] frames of object o.  Returns the patch as it is
+    loaded into a context of the class geometry, the `shape` mfbd_set_patch_shape takes for
+    it, and the same patch as a problem of its own geometry (the reference runs a patch of
+    another mode or frame count as a job of its own)."""
+    cfg = su["cfg"]
+    nO, F, K = cfg.get("n_objects", 1), cfg["n_frames"], su["K"]
+    if not 1 <= n_modes <= K or len(frames) != nO or not all(1 <= f <= F for f in frames):
+        raise ValueError("ragged patch outside the class geometry")
+    big = build_patch(su, seed, field, field_offset)
+    img_on = np.zeros(nO * F, dtype=np.uint8)
+    sel = []
+    for o in range(nO):                       # object o's images are o*F .. o*F+F-1 (one channel each)
+        img_on[o * F:o * F + frames[o]] = 1
+        sel += list(range(o * F, o * F + frames[o]))
+    sel = np.asarray(sel)
+    S = _setup_default()
+    wfs = [[list(range(frames[o]))] for o in range(nO)]
+    nPar, nFree, rows, cols, vals = S.constraints([[frames[o]] for o in range(nO)], wfs, n_modes)
+    shape = dict(nModes=n_modes, imgOn=img_on, nFree=nFree,
+                 rows=(sel[rows // n_modes] * K + rows % n_modes).astype(np.int32), cols=cols, vals=vals)
+    small = dict(big)
+    small["K"] = n_modes
+    small["objects"] = [dict(o, modes=o["modes"][:n_modes]) for o in big["objects"]]
+    small["imgCh"] = big["imgCh"][sel]
+    small["data"] = big["data"][sel]
+    small["nullspace"] = dict(nPar=nPar, nFree=nFree, rows=rows, cols=cols, vals=vals)
+    small["truth"] = dict(alpha=big["truth"]["alpha"][sel][:, :n_modes])
+    return big, shape, small
+
+
 # Configurations (sizes follow the benchmark definitions; cfg2 keeps N = 128 with a
 # 64-pixel pupil because the reference ties N = 2P and a 128 grid cannot hold a 100-px
 # pupil without aliasing the OTF).

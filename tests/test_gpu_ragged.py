@@ -1,0 +1,101 @@
+"""Ragged batches: patches of different mode and frame counts solved in one context of the
+largest counts (mfbd_set_patch_shape) against each patch solved alone in a context of its
+own geometry, which is how the reference runs them (one job per geometry,
+object.cpp:789-790).  A missing image has a zero OTF and a missing mode a zero alpha, every
+reduction sums a patch's images in the same order, so the results are identical bit for
+bit.  One ragged patch is also checked against the FP64 oracle's solve of its own geometry
+(converged quantities, as for every full solve)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+CFG = dict(s=64, n_frames=6, n_objects=2, modes=list(range(2, 26)), basis="kl", d_r0=10.0, mean_counts=1000.0,
+           diversity=[0.0, 1.0], margin=5, settings=dict(FTOL=1e-3))
+# (modes, frames per object): the full shape, mode counts off a multiple of 4, one object short
+SHAPES = [(24, [6, 6]), (13, [4, 6]), (17, [6, 2]), (20, [3, 5]), (5, [2, 2])]
+
+
+@pytest.fixture(scope="module")
+def ragged():
+    from redux_amd import synth
+    su = synth.build_setup(CFG)
+    return [synth.ragged_patch(su, 40 + k, m, f) for k, (m, f) in enumerate(SHAPES)]
+
+
+def _alone(small, prec):
+    from redux_amd.momfbd import Solver
+    sv = Solver(small, precision=prec, max_batch=1)
+    sv.load([small])
+    out = sv.run()[0]
+    sv.close()
+    return out
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_ragged_batch_equals_each_patch_solved_alone(ragged, prec):
+    from redux_amd.momfbd import Solver
+    print("ragged batch", prec, flush=True)
+    bigs = [b for b, _, _ in ragged]
+    sv = Solver(bigs[0], precision=prec, max_batch=len(bigs))
+    sv.load(bigs)
+    sv.set_shapes([sh for _, sh, _ in ragged])
+    got = sv.run()
+    sv.close()
+    K = bigs[0]["K"]
+    for k, (big, sh, small) in enumerate(ragged):
+        want = _alone(small, prec)
+        g = got[k]
+        sel = np.flatnonzero(sh["imgOn"])
+        m = sh["nModes"]
+        assert (g["status"], g["iterations"], g["nF"], g["nDF"]) == \
+            (want["status"], want["iterations"], want["nF"], want["nDF"]), (k, prec)
+        assert g["finalMetric"] == want["finalMetric"], (k, prec)
+        np.testing.assert_array_equal(g["alpha"][sel][:, :m], want["alpha"], err_msg=f"patch {k}")
+        rest = np.ones_like(g["alpha"], dtype=bool)
+        rest[np.ix_(sel, np.arange(m))] = False
+        assert np.all(g["alpha"][rest] == 0), k
+        np.testing.assert_array_equal(g["img"], want["img"], err_msg=f"patch {k}")
+        assert g["alpha"].shape == (len(big["imgCh"]), K)
+
+
+def test_ragged_patch_against_the_oracle(ragged, oracle_lib):
+    """Patch 1 (13 modes, 4 + 6 frames) of the ragged batch against the oracle's solve of
+    that geometry: the same initial metric (1e-12), converged, a final metric no worse than
+    the oracle's (1e-5) and within 1e-2, restored objects within 1e-2 (the CG trajectories
+    of the two are not pinned)."""
+    from redux_amd.momfbd import Solver
+    print("ragged vs oracle", flush=True)
+    bigs = [b for b, _, _ in ragged]
+    sv = Solver(bigs[0], precision="fp64", max_batch=len(bigs))
+    sv.load(bigs)
+    sv.set_shapes([sh for _, sh, _ in ragged])
+    got = sv.run()[1]
+    sv.close()
+    small = ragged[1][2]
+    want = oracle_lib.solve(small)
+    assert got["status"] == 0
+    assert abs(got["initialMetric"] - want["initialMetric"]) <= 1e-12 * abs(want["initialMetric"])
+    rel = (got["finalMetric"] - want["finalMetric"]) / abs(want["finalMetric"])
+    assert -1e-2 < rel < 1e-5, rel
+    err = np.linalg.norm(got["img"] - want["img"]) / np.linalg.norm(want["img"])
+    assert err <= 1e-2, err
+
+
+def test_patch_shape_validation(ragged):
+    """Shapes outside the context are refused with MFBD_EINVAL before any device work."""
+    from redux_amd.momfbd import Solver, SolverError
+    bigs = [b for b, _, _ in ragged]
+    sv = Solver(bigs[0], precision="fp64", max_batch=2)
+    sv.load(bigs[:2])
+    _, sh, _ = ragged[1]
+    bad = [dict(sh, nModes=0), dict(sh, nModes=bigs[0]["K"] + 1),
+           dict(sh, imgOn=np.r_[np.zeros(6, np.uint8), np.ones(6, np.uint8)]),   # object 0 without images
+           dict(sh, rows=np.r_[sh["rows"][:-1], 13]),                            # a mode the patch lacks
+           dict(sh, cols=np.r_[sh["cols"][:-1], sh["nFree"]])]
+    for b in bad:
+        with pytest.raises(SolverError, match=r"\(1\)"):
+            sv.set_shapes([None, b])
+    with pytest.raises(SolverError):
+        sv.set_shapes([None, None, sh])
+    sv.close()

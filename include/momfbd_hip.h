@@ -135,6 +135,14 @@ int mfbd_set_patch_shape(mfbd_ctx* ctx, int patch, int nModes, const unsigned ch
  * object.cpp:278-443), either output may be NULL. */
 int mfbd_solve(mfbd_ctx* ctx, const double* alpha0, mfbd_result* results, double* alphaOut, float* imgOut);
 
+/* Solve the loaded patches of n contexts at once, one host thread per context, so their
+ * launches overlap on the device: the grid classes of an uneven batch (patches of another
+ * size need a context of their own geometry).  Arguments per context as for mfbd_solve
+ * (alpha0 or imgOut may be NULL as a whole or per context); each context's results are
+ * those mfbd_solve gives it alone.  Returns the first failing context's code. */
+int mfbd_solve_group(int n, mfbd_ctx* const* ctxs, const double* const* alpha0, mfbd_result* const* results,
+                     double* const* alphaOut, float* const* imgOut);
+
 /* ---- stage-wise entry points (parity checks, autograd harness) ---- */
 /* First SubImage::initialize (doReset) of every loaded patch at zero alpha
  * (Solver::shiftAndInit(alpha, true), solver.cpp:583-605). */

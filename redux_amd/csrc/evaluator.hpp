@@ -89,6 +89,7 @@ public:
     virtual std::vector<KernelStat> kernel_stats() = 0;
     virtual void reset_stats() = 0;
     virtual int capacity() const = 0;
+    virtual void bind_thread() = 0;          // make the context's device current on this thread
     // pinned [slots][nImg][K] buffer: a request whose alpha points at its slot's row is
     // uploaded from here without a host-side copy
     virtual double* alpha_staging() = 0;

@@ -1849,6 +1849,7 @@ public:
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device, bool bf16Phase = false)
         : g_(g), maxSlots_(maxSlots), maxBatch_(maxBatch), bf16Phase_(bf16Phase && std::is_same_v<T, float>) {
+        device_ = device;
         HIPCHK(hipSetDevice(device));
         HIPCHK(hipStreamCreate(&st_));
         HIPCHK(hipEventCreate(&ev0_));
@@ -1875,6 +1876,7 @@ public:
     }
 
     int capacity() const override { return maxSlots_; }
+    void bind_thread() override { HIPCHK(hipSetDevice(device_)); }
     double* alpha_staging() override { return hStage_.p; }
     int nOS() const override { return nOSfull_; }
     const std::vector<int>& support_map() const override { return fullMap_; }
@@ -2790,6 +2792,7 @@ private:
     // resident: basis, supports, per-patch frames, D, ftSum, gamma, channel phases
     DBuf<C> tw_, D_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
+    int device_ = 0;
     std::vector<T> hScale_;                 // host copy of imgScale_ [maxSlots][nImg]
     std::vector<unsigned char> imgOn_;      // images each slot's patch has [maxSlots][nImg]
     DBuf<__bf16> Bt_;                 // BF16 basis, transposed, for the mixed-precision form

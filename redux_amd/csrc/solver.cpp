@@ -1691,6 +1691,37 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
     return MFBD_OK;
 }
 
+int mfbd_solve_group(int n, mfbd_ctx* const* ctxs, const double* const* alpha0, mfbd_result* const* results,
+                     double* const* alphaOut, float* const* imgOut) {
+    if (n < 1 || !ctxs || !results || !alphaOut) return MFBD_EINVAL;
+    for (int k = 0; k < n; ++k) {
+        if (!ctxs[k] || !results[k] || !alphaOut[k]) return MFBD_EINVAL;
+        for (int k2 = 0; k2 < k; ++k2)
+            if (ctxs[k2] == ctxs[k]) return set_err(ctxs[k], MFBD_EINVAL, "a context appears twice in the group");
+        const int rc = check_ready(ctxs[k]);
+        if (rc) return rc;
+    }
+    // one host thread per context: each drives its own stream(s), batch and restoration
+    // threads, so the contexts' launches overlap on the device
+    std::vector<int> rcs(n, MFBD_OK);
+    std::vector<std::thread> th;
+    for (int k = 1; k < n; ++k)
+        th.emplace_back([&, k] {
+            try {
+                ctxs[k]->dev->bind_thread();
+            } catch (const std::exception& e) {
+                rcs[k] = set_err(ctxs[k], MFBD_EDEVICE, e.what());
+                return;
+            }
+            rcs[k] = mfbd_solve(ctxs[k], alpha0 ? alpha0[k] : nullptr, results[k], alphaOut[k], imgOut ? imgOut[k] : nullptr);
+        });
+    rcs[0] = mfbd_solve(ctxs[0], alpha0 ? alpha0[0] : nullptr, results[0], alphaOut[0], imgOut ? imgOut[0] : nullptr);
+    for (auto& t : th) t.join();
+    for (int rc : rcs)
+        if (rc) return rc;
+    return MFBD_OK;
+}
+
 int mfbd_debug_get(mfbd_ctx* c, const char* what, int p, double* out, int* count) {
     if (!c || !what || !count) return MFBD_EINVAL;
     try {

@@ -185,6 +185,43 @@ class Solver:
                 for i in range(n)}
 
 
+def solve_group(solvers, restore=True):
+    """mfbd_solve_group: the loaded patches of several contexts (the grid classes of an uneven
+    batch) solved at once, one host thread per context.  Returns each context's results as
+    Solver.run does."""
+    n = len(solvers)
+    res = [(Result * sv.nPatch)() for sv in solvers]
+    alphas = [np.zeros((sv.nPatch, sv.nImg, sv.K)) for sv in solvers]
+    imgs = [np.zeros((sv.nPatch, sv.nObj, sv.s, sv.s), dtype=np.float32) for sv in solvers] if restore else None
+    H = (C.c_void_p * n)(*[sv.h.value for sv in solvers])
+    R = (C.c_void_p * n)(*[C.cast(r, C.c_void_p).value for r in res])
+    A = (dp * n)(*[ptr(a) for a in alphas])
+    I = (fp * n)(*[ptr(i, fp) for i in imgs]) if restore else C.POINTER(fp)()
+    rc = L.lib().mfbd_solve_group(n, H, C.POINTER(dp)(), R, A, I)
+    if rc:
+        msgs = "; ".join(L.lib().mfbd_last_error(sv.h).decode() for sv in solvers)
+        raise SolverError(f"solve_group failed ({rc}): {msgs}")
+    out = []
+    for k, sv in enumerate(solvers):
+        out.append([dict(alpha=alphas[k][i], img=None if imgs is None else imgs[k][i], status=r.status,
+                         iterations=r.iterations, nF=r.nF, nDF=r.nDF, failCount=r.failCount,
+                         initialMetric=r.initialMetric, finalMetric=r.finalMetric) for i, r in enumerate(res[k])])
+    return out
+
+
+def uneven_solvers(groups, precision="fp64", max_batch=None, gradient="reference"):
+    """One loaded context per grid class of an uneven batch: `groups` is a list of
+    (patches, shapes), the patches of one class (loaded with the class's largest mode and
+    frame counts) and their mfbd_set_patch_shape shapes."""
+    out = []
+    for pbs, shapes in groups:
+        sv = Solver(pbs[0], precision=precision, max_batch=max_batch or len(pbs), gradient=gradient)
+        sv.load(pbs)
+        sv.set_shapes(shapes)
+        out.append(sv)
+    return out
+
+
 def mosaic(patches, pos_y, pos_x, clip=True, device=0):
     """Object::doMozaic of restored patches [n][s][s] placed at (pos_y, pos_x): the blended
     mosaic (trimmed of empty borders unless clip is False) and the (row, column) of its

@@ -208,6 +208,31 @@ def ragged_patch(su: dict, seed: int, n_modes: int, frames, field=None, field_of
     return big, shape, small
 
 
+def uneven_batch(classes, n_patches, seed=2024, n_modes=(36, 100), n_frames=(12, 48), workers=1):
+    """An uneven batch: n_patches drawn from `seed`, each of a grid class picked at random
+    from `classes` (configurations whose modes and n_frames are the class maxima), with a
+    mode count in n_modes and a frame count per object in n_frames (inclusive ranges,
+    clipped to the class).  Returns per class (setup, patches, shapes, own-geometry
+    problems, indices into the batch)."""
+    rng = np.random.default_rng(seed)
+    sus = [build_setup(c) for c in classes]
+    draws = []
+    for k in range(n_patches):
+        c = int(rng.integers(len(classes)))
+        cfg, K = classes[c], sus[c]["K"]
+        m = int(rng.integers(min(n_modes[0], K), min(n_modes[1], K) + 1))
+        F = cfg["n_frames"]
+        fr = [int(rng.integers(min(n_frames[0], F), min(n_frames[1], F) + 1)) for _ in range(cfg.get("n_objects", 1))]
+        draws.append((c, m, fr, seed * 7 + k))
+    out = []
+    for c in range(len(classes)):
+        idx = [k for k, d in enumerate(draws) if d[0] == c]
+        items = [ragged_patch(sus[c], draws[k][3], draws[k][1], draws[k][2]) for k in idx]
+        out.append(dict(setup=sus[c], patches=[b for b, _, _ in items], shapes=[sh for _, sh, _ in items],
+                        own=[sm for _, _, sm in items], index=idx))
+    return out
+
+
 # Configurations (sizes follow the benchmark definitions; cfg2 keeps N = 128 with a
 # 64-pixel pupil because the reference ties N = 2P and a 128 grid cannot hold a 100-px
 # pupil without aliasing the OTF).
@@ -221,3 +246,12 @@ CFG2_PATCH = dict(s=88, n_frames=24, n_objects=2, modes=list(range(2, 52)), basi
 CFG5_PATCH = dict(s=128, n_frames=100, n_objects=2, modes=list(range(2, 152)), basis="kl", d_r0=20.0,
                   mean_counts=2000.0, diversity=[0.0, math.pi / math.sqrt(3.0)], margin=5,
                   pupil_diameter=128.0, settings=dict(FTOL=1e-3))
+# Config 3 (uneven batch): grid classes 64 / 128 / 256 (patches 64, 88, 128 with 32-, 64-
+# and 128-pixel pupils), each with the largest counts a patch may draw: 100 KL modes and
+# 48 frames per object; patches draw 36-100 modes and 12-48 frames per object.
+UNEVEN_CLASSES = [
+    dict(s=64, n_frames=48, n_objects=2, modes=list(range(2, 102)), basis="kl", d_r0=20.0, mean_counts=2000.0,
+         diversity=[0.0, math.pi / math.sqrt(3.0)], margin=5, pupil_diameter=32.0, settings=dict(FTOL=1e-3)),
+    dict(CFG2_PATCH, n_frames=48, modes=list(range(2, 102))),
+    dict(CFG5_PATCH, n_frames=48, modes=list(range(2, 102))),
+]

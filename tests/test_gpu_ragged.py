@@ -99,3 +99,32 @@ def test_patch_shape_validation(ragged):
     with pytest.raises(SolverError):
         sv.set_shapes([None, None, sh])
     sv.close()
+
+
+def test_uneven_batch_over_three_grids_equals_each_patch_solved_alone():
+    """An uneven batch over the grids 64, 128 and 256 (the last on the N = 256 transform
+    path), ragged mode and frame counts within each, solved by mfbd_solve_group with one
+    context per grid running at once: every patch as if solved alone in its own geometry."""
+    from redux_amd import synth
+    from redux_amd.momfbd import solve_group, uneven_solvers
+    print("uneven batch, three grids", flush=True)
+    small = dict(n_objects=2, basis="kl", d_r0=10.0, mean_counts=1000.0, diversity=[0.0, 1.0], margin=5,
+                 settings=dict(FTOL=1e-3))
+    classes = [dict(small, s=64, pupil_diameter=32.0, n_frames=4, modes=list(range(2, 14))),
+               dict(small, s=88, pupil_diameter=64.0, n_frames=4, modes=list(range(2, 14))),
+               dict(small, s=128, pupil_diameter=128.0, n_frames=3, modes=list(range(2, 10)))]
+    groups = synth.uneven_batch(classes, 9, seed=5, n_modes=(5, 12), n_frames=(2, 4))
+    assert sorted(k for g in groups for k in g["index"]) == list(range(9))
+    assert [g["setup"]["N"] for g in groups] == [64, 128, 256]
+    solvers = uneven_solvers([(g["patches"], g["shapes"]) for g in groups], precision="fp64")
+    got = solve_group(solvers)
+    for sv in solvers:
+        sv.close()
+    for g, res in zip(groups, got):
+        for k, (sh, own) in enumerate(zip(g["shapes"], g["own"])):
+            want = _alone(own, "fp64")
+            sel = np.flatnonzero(sh["imgOn"])
+            assert res[k]["finalMetric"] == want["finalMetric"], (g["setup"]["N"], k)
+            assert res[k]["iterations"] == want["iterations"]
+            np.testing.assert_array_equal(res[k]["alpha"][sel][:, :sh["nModes"]], want["alpha"])
+            np.testing.assert_array_equal(res[k]["img"], want["img"])

@@ -23,6 +23,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -1862,6 +1863,7 @@ public:
         }
         upload(tw_, tw, st_);
         timing_ = getenv("MFBD_KERNEL_TIMING") != nullptr;
+        autotune_fft();
     }
 
     ~Pipeline() override {
@@ -2305,6 +2307,10 @@ public:
     }
 
     void debug_get(const std::string& what, int slot, std::vector<double>& out) override {
+        if (what == "fft_lines") {      // the transform passes' lines per workgroup (rows, columns)
+            out = {(double)lr_, (double)lc_};
+            return;
+        }
         use_lane(2);
         auto getc = [&](const C* p, size_t n) {
             std::vector<C> h(n);
@@ -2412,15 +2418,90 @@ private:
         evPending_.swap(keep);
     }
 
+    // One row or column pass of L lines per workgroup over nTiles tiles.  Every line's
+    // butterflies are the same whatever L, so the choice changes only the speed.
+    template <int DIR, int L>
+    void fft_pass(C* tiles, int nTiles, bool rows) {
+        const unsigned gr = nTiles * (N / L);
+        if (rows)
+            hipLaunchKernelGGL((k_fft_rows<T, N, DIR, L>), dim3(gr), dim3(L * N / 16), 0, st_, tiles, tw_.p);
+        else
+            hipLaunchKernelGGL((k_fft_cols<T, N, DIR, L>), dim3(gr), dim3(L * N / 16), 0, st_, tiles, tw_.p);
+    }
+
+    template <int DIR>
+    void fft_pass_any(C* tiles, int nTiles, bool rows, int l) {
+        if constexpr (N == 256) {
+            if (l == LINES / 4) return fft_pass<DIR, LINES / 4>(tiles, nTiles, rows);
+            if (l == LINES / 2) return fft_pass<DIR, LINES / 2>(tiles, nTiles, rows);
+        }
+        fft_pass<DIR, LINES>(tiles, nTiles, rows);
+    }
+
     void fft2(C* tiles, int nTiles, int dir) {
-        const unsigned gr = nTiles * (N / LINES);
-        constexpr int NT = LINES * N / 16;
         if (dir < 0) {
-            launch("fft_rows", nTiles, [&] { hipLaunchKernelGGL((k_fft_rows<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
-            launch("fft_cols", nTiles, [&] { hipLaunchKernelGGL((k_fft_cols<T, N, -1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_rows", nTiles, [&] { fft_pass_any<-1>(tiles, nTiles, true, lr_); });
+            launch("fft_cols", nTiles, [&] { fft_pass_any<-1>(tiles, nTiles, false, lc_); });
         } else {
-            launch("fft_rows", nTiles, [&] { hipLaunchKernelGGL((k_fft_rows<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
-            launch("fft_cols", nTiles, [&] { hipLaunchKernelGGL((k_fft_cols<T, N, +1, LINES>), dim3(gr), dim3(NT), 0, st_, tiles, tw_.p); });
+            launch("fft_rows", nTiles, [&] { fft_pass_any<+1>(tiles, nTiles, true, lr_); });
+            launch("fft_cols", nTiles, [&] { fft_pass_any<+1>(tiles, nTiles, false, lc_); });
+        }
+    }
+
+    // Tile shape of the N = 256 transform passes (lines per workgroup: LINES/4, LINES/2 or
+    // LINES, separately for rows and columns), picked once per precision and device by
+    // timing each candidate on 32 scratch tiles; MFBD_FFT_LINES="rows,cols" fixes it.
+    void autotune_fft() {
+        lr_ = lc_ = LINES;
+        if constexpr (N == 256) {
+            const int cand[3] = {LINES / 4, LINES / 2, LINES};
+            auto valid = [&](int l) { return l == cand[0] || l == cand[1] || l == cand[2]; };
+            if (const char* e = getenv("MFBD_FFT_LINES")) {
+                int a = 0, b = 0;
+                if (sscanf(e, "%d,%d", &a, &b) != 2 || !valid(a) || !valid(b))
+                    throw std::runtime_error("MFBD_FFT_LINES must be two of " + std::to_string(cand[0]) + ", " +
+                                             std::to_string(cand[1]) + ", " + std::to_string(cand[2]));
+                lr_ = a;
+                lc_ = b;
+                return;
+            }
+            static std::mutex mu;
+            static std::map<std::pair<int, int>, std::pair<int, int>> chosen;   // (bytes of T, device) -> (rows, cols)
+            std::lock_guard<std::mutex> lock(mu);
+            const auto key = std::make_pair((int)sizeof(T), device_);
+            auto it = chosen.find(key);
+            if (it == chosen.end()) {
+                const int nT = 32;
+                DBuf<C> scratch;
+                scratch.alloc((size_t)nT * N2);
+                HIPCHK(hipMemsetAsync(scratch.p, 0, (size_t)nT * N2 * sizeof(C), st_));
+                std::pair<int, int> best{LINES, LINES};
+                for (int pass = 0; pass < 2; ++pass) {
+                    float bestMs = 1e30f;
+                    for (int l : cand) {
+                        fft_pass_any<-1>(scratch.p, nT, pass == 0, l);      // warm-up
+                        float ms = 1e30f;
+                        for (int rep = 0; rep < 3; ++rep) {
+                            HIPCHK(hipEventRecord(ev0_, st_));
+                            fft_pass_any<-1>(scratch.p, nT, pass == 0, l);
+                            HIPCHK(hipEventRecord(ev1_, st_));
+                            HIPCHK(hipEventSynchronize(ev1_));
+                            float t = 0;
+                            HIPCHK(hipEventElapsedTime(&t, ev0_, ev1_));
+                            ms = std::min(ms, t);
+                        }
+                        HIPCHK(hipGetLastError());
+                        if (ms < bestMs) {
+                            bestMs = ms;
+                            (pass == 0 ? best.first : best.second) = l;
+                        }
+                    }
+                }
+                HIPCHK(hipStreamSynchronize(st_));
+                it = chosen.emplace(key, best).first;
+            }
+            lr_ = it->second.first;
+            lc_ = it->second.second;
         }
     }
 
@@ -2793,6 +2874,7 @@ private:
     DBuf<C> tw_, D_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
     int device_ = 0;
+    int lr_ = LINES, lc_ = LINES;           // lines per workgroup of the transform passes
     std::vector<T> hScale_;                 // host copy of imgScale_ [maxSlots][nImg]
     std::vector<unsigned char> imgOn_;      // images each slot's patch has [maxSlots][nImg]
     DBuf<__bf16> Bt_;                 // BF16 basis, transposed, for the mixed-precision form

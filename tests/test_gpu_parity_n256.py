@@ -83,3 +83,49 @@ def test_restore_fp64(patch, ref):
     want = st.restore(a)
     assert np.linalg.norm(img - want) / np.linalg.norm(want) < 1e-6
     sv.close()
+
+
+def _with_lines(patch, prec, lines):
+    import os
+    from redux_amd.momfbd import Solver
+    if lines:
+        os.environ["MFBD_FFT_LINES"] = lines
+    try:
+        sv = Solver(patch, precision=prec, max_batch=1)
+        sv.load([patch])
+    finally:
+        os.environ.pop("MFBD_FFT_LINES", None)
+    sv.init()
+    return sv
+
+
+@pytest.mark.parametrize("prec,shapes", [("fp64", ["4,4", "4,16", "8,8", "16,4", "16,16"]),
+                                         ("fp32", ["8,8", "8,32", "16,16", "32,8", "32,32"])])
+def test_transform_tile_shapes_give_identical_results(patch, ref, prec, shapes):
+    """The autotuner's candidates (lines per workgroup of the row and column passes) only
+    redistribute the same butterflies: metric, Vogel and adjoint gradients and the
+    restored objects are bit-identical for every shape, and the autotuned choice is one
+    of them."""
+    a = ref["alphas"][2]
+    sv = _with_lines(patch, prec, None)
+    cand = {"fp64": (4, 8, 16), "fp32": (8, 16, 32)}[prec]
+    lr, lc = (int(v) for v in sv.debug("fft_lines"))
+    assert lr in cand and lc in cand, (lr, lc)
+    base = (sv.metric(a[None]), sv.grad(a[None], method="vogel"), sv.grad(a[None], method="adjoint"),
+            sv.restore(a[None]))
+    sv.close()
+    for sh in shapes:
+        print("lines", prec, sh, flush=True)
+        sv = _with_lines(patch, prec, sh)
+        assert [int(v) for v in sv.debug("fft_lines")] == [int(v) for v in sh.split(",")]
+        got = (sv.metric(a[None]), sv.grad(a[None], method="vogel"), sv.grad(a[None], method="adjoint"),
+               sv.restore(a[None]))
+        sv.close()
+        for x, y in zip(got, base):
+            np.testing.assert_array_equal(x, y, err_msg=sh)
+
+
+def test_transform_tile_shape_outside_the_candidates_is_refused(patch):
+    from redux_amd.momfbd import SolverError
+    with pytest.raises(SolverError, match="MFBD_FFT_LINES"):
+        _with_lines(patch, "fp64", "12,16")

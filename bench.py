@@ -255,6 +255,79 @@ def pmc_probe(form):
                       "copy_bytes": COPY_PROBE_BYTES}), flush=True)
 
 
+def uneven(args):
+    """The uneven batch: patches of the grids 64 / 128 / 256 with 36-100 KL modes and 12-48
+    frames per object drawn per patch from the seed, one context per grid solved at once
+    (mfbd_solve_group), FP64; beside it the same patches solved the way the reference runs
+    them, one at a time in a context of each patch's own geometry (a bounded sample)."""
+    from redux_amd import synth
+    from redux_amd.momfbd import Solver, solve_group, uneven_solvers
+    workers = min(16, os.cpu_count() or 1)
+    t0 = time.perf_counter()
+    groups = synth.uneven_batch(synth.UNEVEN_CLASSES, args.patches, seed=args.seed, workers=workers)
+    log(f"generated {args.patches} uneven patches in {time.perf_counter() - t0:.1f}s: "
+        + ", ".join(f"N={g['setup']['N']}: {len(g['patches'])}" for g in groups))
+    groups = [g for g in groups if g["patches"]]
+    solvers = uneven_solvers([(g["patches"], g["shapes"]) for g in groups], precision="fp64")
+    times, res = [], None
+    for k in range(args.warmup + args.steps):
+        t1 = time.perf_counter()
+        out = solve_group(solvers)
+        dt = time.perf_counter() - t1
+        log(f"uneven fp64 {'warmup' if k < args.warmup else 'step'}: {dt:.2f}s")
+        if k >= args.warmup:
+            times.append(dt)
+            if res is not None:
+                assert all(np.array_equal(a["img"], b["img"]) for ra, rb in zip(res, out) for a, b in zip(ra, rb))
+            res = out
+    lines = [(int(x) for x in sv.debug("fft_lines")) for sv in solvers]
+    fft_lines = {f"N={g['setup']['N']}": list(v) for g, v in zip(groups, lines)}
+    for sv in solvers:
+        sv.close()
+    dt = max(times)
+    n = sum(len(g["patches"]) for g in groups)
+    conv = sum(1 for ra in res for r in ra if r["status"] == 0)
+    # one patch at a time in its own geometry: the first few patches of every grid, their
+    # mean time per patch scaled by that grid's patch count
+    plain_est, same, per_grid = 0.0, True, {}
+    for g, ra in zip(groups, res):
+        ts = []
+        for k in range(min(args.plain_patches, len(g["own"]))):
+            own = g["own"][k]
+            sv = Solver(own, precision="fp64", max_batch=1)
+            sv.load([own])
+            t1 = time.perf_counter()
+            r = sv.run()[0]
+            ts.append(time.perf_counter() - t1)
+            sv.close()
+            sel = np.flatnonzero(g["shapes"][k]["imgOn"])
+            same &= bool(np.array_equal(r["img"], ra[k]["img"])
+                         and np.array_equal(r["alpha"], ra[k]["alpha"][sel][:, :g["shapes"][k]["nModes"]]))
+        plain_est += float(np.mean(ts)) * len(g["patches"])
+        per_grid[f"N={g['setup']['N']}"] = round(1.0 / float(np.mean(ts)), 4)
+        log(f"plain N={g['setup']['N']}: {np.mean(ts):.3f} s per patch")
+    line = {
+        "metric": "converged patches/s, uneven batch (FP64)",
+        "value": round(conv / dt, 4),
+        "unit": "patches/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{n} patches, grids 64/128/256 (patches 64/88/128, pupils 32/64/128 px), "
+                               "36-100 KL modes and 12-48 frames per object drawn per patch, 2 objects",
+                   "patches_per_grid": {f"N={g['setup']['N']}": len(g["patches"]) for g in groups},
+                   "form": "one context per grid, ragged shapes inside, contexts solved at once"},
+        "converged": conv,
+        "plain_one_at_a_time_patches_per_s": round(n / plain_est, 4),
+        "plain_per_grid_patches_per_s": per_grid,
+        "plain_sample": f"the first {args.plain_patches} patches of every grid, each solved alone in a context of "
+                        "its own geometry; the batch's plain time = per grid, mean time per patch x patches",
+        "plain_identical_to_batched": same,
+        "speedup_vs_plain": round(plain_est / dt, 2),
+        "fft_lines_autotuned": fft_lines,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -265,6 +338,9 @@ def main():
     ap.add_argument("--gradient", choices=["reference", "adjoint"], default="reference")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--workload", choices=["cfg2", "uneven"], default="cfg2",
+                    help="cfg2: the headline (256 patches of the benchmark geometry); uneven: the uneven batch "
+                         "over three grids (its own JSON line)")
     ap.add_argument("--pmc-probe", choices=["fp32", "fp64"], default=None,
                     help="run the short workload profiled for HBM traffic (FETCH_SIZE / WRITE_SIZE) in one "
                          "precision form and exit")
@@ -276,6 +352,9 @@ def main():
         print("bench.py measures one GPU: patches are independent and shard across GPUs with no "
               "collective (see DESIGN.md); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "uneven":
+        uneven(args)
+        return
     workers = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     su, pbs = make_batch(args.patches, args.seed, workers)

@@ -252,6 +252,88 @@ __global__ void __launch_bounds__(LC * N / 16) k_fft_cols(cx<T>* __restrict__ da
     }
 }
 
+// The multi-pass OTF (grids too large for one tile in LDS) in four passes over the tile
+// workspace, the elementwise stages folded into the transform passes:
+//   k_field_rows       pupil field a = mag exp(i phi) built in LDS, forward row FFTs
+//   k_cols_power       forward column FFTs, then |.|^2
+//   k_fft_rows (+1)    inverse row FFTs
+//   k_cols_extract     inverse column FFTs, the stored support points scaled by 1/N^2
+//                      written straight to S (invOs: stored index h >= 0, else not stored)
+template <typename T, int N, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_field_rows(cx<T>* __restrict__ data, const T* __restrict__ phi,
+                                                            const int* __restrict__ inv, const T* __restrict__ pupil,
+                                                            const int* __restrict__ qImg, const T* __restrict__ imgScale,
+                                                            int nPS, const cx<T>* __restrict__ tw) {
+    constexpr int NT = LR * N / 16;
+    __shared__ cx<T> lds[LR * N];
+    const int chunks = N / LR;
+    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    const T sc = imgScale[qImg[tile]];
+    const T* ph = phi + tile * nPS;
+    const int p0 = (int)chunk * LR * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) {
+        const int idx = inv[p0 + e];
+        cx<T> v = {T(0), T(0)};
+        if (idx >= 0) {
+            T sn, cs;
+            sincos_t(ph[idx], &sn, &cs);
+            const T mag = pupil[idx] * sc;
+            v = {mag * cs, mag * sn};
+        }
+        lds[e] = v;
+    }
+    __syncthreads();
+    fft_lines<T, N, -1, LR, NT>(lds, N, 1, tw);
+    cx<T>* g = data + tile * N * N + p0;
+    for (int e = threadIdx.x; e < LR * N; e += NT) g[e] = lds[e];
+}
+
+template <typename T, int N, int LC>
+__global__ void __launch_bounds__(LC * N / 16) k_cols_power(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw) {
+    constexpr int NT = LC * N / 16;
+    __shared__ cx<T> lds[LC * N];
+    const int chunks = N / LC;
+    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    cx<T>* g = data + tile * N * N + chunk * LC;
+    for (int e = threadIdx.x; e < LC * N; e += NT) {
+        const int r = e / LC, c = e % LC;
+        lds[e] = g[(size_t)r * N + c];
+    }
+    __syncthreads();
+    fft_lines<T, N, -1, LC, NT, 1>(lds, 1, LC, tw);
+    for (int e = threadIdx.x; e < LC * N; e += NT) {
+        const int r = e / LC, c = e % LC;
+        const cx<T> v = lds[e];
+        g[(size_t)r * N + c] = {v.x * v.x + v.y * v.y, T(0)};
+    }
+}
+
+template <typename T, int N, int LC>
+__global__ void __launch_bounds__(LC * N / 16) k_cols_extract(cx<T>* __restrict__ S, const cx<T>* __restrict__ data,
+                                                              const int* __restrict__ invOs, int nOS, T scale,
+                                                              const cx<T>* __restrict__ tw) {
+    constexpr int NT = LC * N / 16;
+    __shared__ cx<T> lds[LC * N];
+    const int chunks = N / LC;
+    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    const cx<T>* g = data + tile * N * N + chunk * LC;
+    for (int e = threadIdx.x; e < LC * N; e += NT) {
+        const int r = e / LC, c = e % LC;
+        lds[e] = g[(size_t)r * N + c];
+    }
+    __syncthreads();
+    fft_lines<T, N, +1, LC, NT, 1>(lds, 1, LC, tw);
+    cx<T>* So = S + tile * nOS;
+    for (int e = threadIdx.x; e < LC * N; e += NT) {
+        const int r = e / LC, c = e % LC;
+        const int h = invOs[r * N + (int)chunk * LC + c];
+        if (h >= 0) {
+            const cx<T> v = lds[e];
+            So[h] = {v.x * scale, v.y * scale};
+        }
+    }
+}
+
 // ------------------------------------------------------------ elementwise ----
 
 struct QMap {             // virtual image q -> source
@@ -2438,6 +2520,32 @@ private:
         fft_pass<DIR, LINES>(tiles, nTiles, rows);
     }
 
+    // The four passes of the multi-pass OTF (see k_field_rows) at the tuned tile shapes
+    template <int L>
+    void otf_stage(int stage, C* tiles, int nQ, const T* phi, const int* qImg, C* Sout) {
+        const unsigned gr = nQ * (N / L);
+        const dim3 bs(L * N / 16);
+        if (stage == 0)
+            hipLaunchKernelGGL((k_field_rows<T, N, L>), dim3(gr), bs, 0, st_, tiles, phi, invPio_.p, pupil_.p, qImg,
+                               imgScale_.p, nPS_, tw_.p);
+        else if (stage == 1)
+            hipLaunchKernelGGL((k_cols_power<T, N, L>), dim3(gr), bs, 0, st_, tiles, tw_.p);
+        else if (stage == 2)
+            hipLaunchKernelGGL((k_fft_rows<T, N, +1, L>), dim3(gr), bs, 0, st_, tiles, tw_.p);
+        else
+            hipLaunchKernelGGL((k_cols_extract<T, N, L>), dim3(gr), bs, 0, st_, Sout, tiles, invOs_.p, nOS_,
+                               (T)(1.0 / N2), tw_.p);
+    }
+
+    void otf_stage_any(int stage, C* tiles, int nQ, const T* phi, const int* qImg, C* Sout) {
+        const int l = (stage == 0 || stage == 2) ? lr_ : lc_;
+        if constexpr (N == 256) {
+            if (l == LINES / 4) return otf_stage<LINES / 4>(stage, tiles, nQ, phi, qImg, Sout);
+            if (l == LINES / 2) return otf_stage<LINES / 2>(stage, tiles, nQ, phi, qImg, Sout);
+        }
+        otf_stage<LINES>(stage, tiles, nQ, phi, qImg, Sout);
+    }
+
     void fft2(C* tiles, int nTiles, int dir) {
         if (dir < 0) {
             launch("fft_rows", nTiles, [&] { fft_pass_any<-1>(tiles, nTiles, true, lr_); });
@@ -2522,19 +2630,9 @@ private:
             return;
         }
         sc_->tiles.alloc((size_t)nQ * N2);
-        launch("k_field", nQ, [&] {
-            hipLaunchKernelGGL(k_field<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, sc_->tiles.p, phi, invPio_.p,
-                               pupil_.p, qImg, imgScale_.p, (const int*)nullptr, nQ, N2, nPS_);
-        });
-        fft2(sc_->tiles.p, nQ, -1);
-        launch("k_power", nQ, [&] {
-            hipLaunchKernelGGL(k_power<T>, dim3(nblk((size_t)nQ * N2)), dim3(256), 0, st_, sc_->tiles.p, (size_t)nQ * N2);
-        });
-        fft2(sc_->tiles.p, nQ, +1);
-        launch("k_extract", nQ, [&] {
-            hipLaunchKernelGGL(k_extract<T>, dim3(nblk((size_t)nQ * nOS_)), dim3(256), 0, st_, Sout, sc_->tiles.p,
-                               osNat_.p, nQ, nOS_, N2, (T)(1.0 / N2));
-        });
+        static const char* names[4] = {"k_field_rows", "k_cols_power", "fft_rows_inv", "k_cols_extract"};
+        for (int stage = 0; stage < 4; ++stage)
+            launch(names[stage], nQ, [&] { otf_stage_any(stage, sc_->tiles.p, nQ, phi, qImg, Sout); });
     }
 
     // phases, OTFs, P/Q and the metric for n requests (virtual images q = r*nImg + j).

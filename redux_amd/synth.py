@@ -208,6 +208,14 @@ def ragged_patch(su: dict, seed: int, n_modes: int, frames, field=None, field_of
     return big, shape, small
 
 
+_UNEVEN = {}
+
+
+def _uneven_item(d):
+    c, m, fr, sd = d
+    return ragged_patch(_UNEVEN["setups"][c], sd, m, fr)
+
+
 def uneven_batch(classes, n_patches, seed=2024, n_modes=(36, 100), n_frames=(12, 48), workers=1):
     """An uneven batch: n_patches drawn from `seed`, each of a grid class picked at random
     from `classes` (configurations whose modes and n_frames are the class maxima), with a
@@ -224,10 +232,19 @@ def uneven_batch(classes, n_patches, seed=2024, n_modes=(36, 100), n_frames=(12,
         F = cfg["n_frames"]
         fr = [int(rng.integers(min(n_frames[0], F), min(n_frames[1], F) + 1)) for _ in range(cfg.get("n_objects", 1))]
         draws.append((c, m, fr, seed * 7 + k))
+    if workers > 1:
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+        _UNEVEN["setups"] = sus
+        with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+            made = list(ex.map(_uneven_item, draws, chunksize=4))
+        _UNEVEN.clear()
+    else:
+        made = [ragged_patch(sus[c], sd, m, fr) for c, m, fr, sd in draws]
     out = []
     for c in range(len(classes)):
         idx = [k for k, d in enumerate(draws) if d[0] == c]
-        items = [ragged_patch(sus[c], draws[k][3], draws[k][1], draws[k][2]) for k in idx]
+        items = [made[k] for k in idx]
         out.append(dict(setup=sus[c], patches=[b for b, _, _ in items], shapes=[sh for _, sh, _ in items],
                         own=[sm for _, _, sm in items], index=idx))
     return out

@@ -232,6 +232,22 @@ __global__ void __launch_bounds__(LR * N / 16) k_fft_rows(cx<T>* __restrict__ da
     for (int e = threadIdx.x; e < LR * N; e += NT) g[e] = lds[e];
 }
 
+// Inverse row pass of the multi-pass OTF: as k_fft_rows, skipping images a patch lacks
+template <typename T, int N, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_rows_inv(cx<T>* __restrict__ data, const int* __restrict__ qImg,
+                                                          const T* __restrict__ imgScale, const cx<T>* __restrict__ tw) {
+    constexpr int NT = LR * N / 16;
+    __shared__ cx<T> lds[LR * N];
+    const int chunks = N / LR;
+    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    if (imgScale[qImg[tile]] == T(0)) return;
+    cx<T>* g = data + tile * N * N + chunk * LR * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) lds[e] = g[e];
+    __syncthreads();
+    fft_lines<T, N, +1, LR, NT>(lds, N, 1, tw);
+    for (int e = threadIdx.x; e < LR * N; e += NT) g[e] = lds[e];
+}
+
 // Column pass: each block transforms LC consecutive columns of one tile in place.
 template <typename T, int N, int DIR, int LC>
 __global__ void __launch_bounds__(LC * N / 16) k_fft_cols(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw) {
@@ -269,6 +285,7 @@ __global__ void __launch_bounds__(LR * N / 16) k_field_rows(cx<T>* __restrict__ 
     const int chunks = N / LR;
     const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
     const T sc = imgScale[qImg[tile]];
+    if (sc == T(0)) return;     // an image the patch lacks: k_cols_extract writes its zero OTF
     const T* ph = phi + tile * nPS;
     const int p0 = (int)chunk * LR * N;
     for (int e = threadIdx.x; e < LR * N; e += NT) {
@@ -289,11 +306,13 @@ __global__ void __launch_bounds__(LR * N / 16) k_field_rows(cx<T>* __restrict__ 
 }
 
 template <typename T, int N, int LC>
-__global__ void __launch_bounds__(LC * N / 16) k_cols_power(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(LC * N / 16) k_cols_power(cx<T>* __restrict__ data, const int* __restrict__ qImg,
+                                                            const T* __restrict__ imgScale, const cx<T>* __restrict__ tw) {
     constexpr int NT = LC * N / 16;
     __shared__ cx<T> lds[LC * N];
     const int chunks = N / LC;
     const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    if (imgScale[qImg[tile]] == T(0)) return;
     cx<T>* g = data + tile * N * N + chunk * LC;
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
@@ -311,11 +330,20 @@ __global__ void __launch_bounds__(LC * N / 16) k_cols_power(cx<T>* __restrict__ 
 template <typename T, int N, int LC>
 __global__ void __launch_bounds__(LC * N / 16) k_cols_extract(cx<T>* __restrict__ S, const cx<T>* __restrict__ data,
                                                               const int* __restrict__ invOs, int nOS, T scale,
+                                                              const int* __restrict__ qImg, const T* __restrict__ imgScale,
                                                               const cx<T>* __restrict__ tw) {
     constexpr int NT = LC * N / 16;
     __shared__ cx<T> lds[LC * N];
     const int chunks = N / LC;
     const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    cx<T>* So = S + tile * nOS;
+    if (imgScale[qImg[tile]] == T(0)) {       // an image the patch lacks: zero OTF
+        for (int e = threadIdx.x; e < LC * N; e += NT) {
+            const int h = invOs[(e / LC) * N + (int)chunk * LC + e % LC];
+            if (h >= 0) So[h] = {T(0), T(0)};
+        }
+        return;
+    }
     const cx<T>* g = data + tile * N * N + chunk * LC;
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
@@ -323,7 +351,6 @@ __global__ void __launch_bounds__(LC * N / 16) k_cols_extract(cx<T>* __restrict_
     }
     __syncthreads();
     fft_lines<T, N, +1, LC, NT, 1>(lds, 1, LC, tw);
-    cx<T>* So = S + tile * nOS;
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
         const int h = invOs[r * N + (int)chunk * LC + c];
@@ -1117,6 +1144,10 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
     cx<float>* hs = lds_dyn + (N / 2) * LD;                  // N x HC half spectra, element (y, k) at y*HC + k
     const int q = blockIdx.x;
     const float sc = imgScale[qImg[q]];
+    if (sc == 0.f) {            // an image the patch lacks: its OTF is zero
+        for (int u = threadIdx.x; u < nOS; u += NT) S[(size_t)q * nOS + u] = {0.f, 0.f};
+        return;
+    }
     const float* ph = phi + (size_t)q * nPS;
     for (int e = threadIdx.x; e < P * P; e += NT) {
         const int k = invPs[e];
@@ -1260,8 +1291,13 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
     fetch(blockIdx.x);
     auto at = [](int k, int y) { return y < P ? y * LD + k : (y - P) * LD + (k == 0 ? P : N - k); };
     for (int q = blockIdx.x; q < nQ; q += gridDim.x) {
-        // 1-2.
         const double sc = imgScale[qImg[q]];
+        if (sc == 0.0) {        // an image the patch lacks: its OTF is zero (t is not touched)
+            for (int u = threadIdx.x; u < nOS; u += NT) S[(size_t)q * nOS + u] = {0.0, 0.0};
+            fetch(q + gridDim.x);
+            continue;
+        }
+        // 1-2.
         {
             cx<double> v[IR][RF];
 #pragma unroll
@@ -2529,12 +2565,12 @@ private:
             hipLaunchKernelGGL((k_field_rows<T, N, L>), dim3(gr), bs, 0, st_, tiles, phi, invPio_.p, pupil_.p, qImg,
                                imgScale_.p, nPS_, tw_.p);
         else if (stage == 1)
-            hipLaunchKernelGGL((k_cols_power<T, N, L>), dim3(gr), bs, 0, st_, tiles, tw_.p);
+            hipLaunchKernelGGL((k_cols_power<T, N, L>), dim3(gr), bs, 0, st_, tiles, qImg, imgScale_.p, tw_.p);
         else if (stage == 2)
-            hipLaunchKernelGGL((k_fft_rows<T, N, +1, L>), dim3(gr), bs, 0, st_, tiles, tw_.p);
+            hipLaunchKernelGGL((k_rows_inv<T, N, L>), dim3(gr), bs, 0, st_, tiles, qImg, imgScale_.p, tw_.p);
         else
             hipLaunchKernelGGL((k_cols_extract<T, N, L>), dim3(gr), bs, 0, st_, Sout, tiles, invOs_.p, nOS_,
-                               (T)(1.0 / N2), tw_.p);
+                               (T)(1.0 / N2), qImg, imgScale_.p, tw_.p);
     }
 
     void otf_stage_any(int stage, C* tiles, int nQ, const T* phi, const int* qImg, C* Sout) {
@@ -2745,6 +2781,7 @@ private:
             if (!rq[r].wantGrad || rq[r].gradMethod == 1) continue;
             nEn = std::max(nEn, rq[r].nEnabled);
             for (int j = 0; j < nImg; ++j) {
+                if (!imgOn_[(size_t)rq[r].slot * nImg + j]) continue;   // its gradient row is zero
                 const int o = st.imgObj[j];
                 const double w = st.weight[o];
                 gIdx.push_back(r * nImg + j);
@@ -2793,8 +2830,15 @@ private:
         for (int r = 0; r < n; ++r) {
             rq[r].f = sc_->hOut.p[r];
             if (!rq[r].wantGrad || rq[r].gradMethod == 1) continue;   // finite differences: filled already
-            if (rq[r].g) std::memcpy(rq[r].g, sc_->hOut.p + n + row * K, sizeof(double) * nImg * K);
-            row += nImg;
+            const unsigned char* on = &imgOn_[(size_t)rq[r].slot * nImg];
+            for (int j = 0; j < nImg; ++j) {
+                if (!on[j]) {               // an image the patch lacks has no gradient row
+                    if (rq[r].g) std::fill(rq[r].g + (size_t)j * K, rq[r].g + (size_t)(j + 1) * K, 0.0);
+                    continue;
+                }
+                if (rq[r].g) std::memcpy(rq[r].g + (size_t)j * K, sc_->hOut.p + n + row * K, sizeof(double) * K);
+                row++;
+            }
         }
     }
 

@@ -287,12 +287,13 @@ def uneven(args):
     dt = max(times)
     n = sum(len(g["patches"]) for g in groups)
     conv = sum(1 for ra in res for r in ra if r["status"] == 0)
-    # one patch at a time in its own geometry: the first few patches of every grid, their
-    # mean time per patch scaled by that grid's patch count
+    # one patch at a time in its own geometry: a few patches spread evenly over every
+    # grid's list, their mean time per patch scaled by that grid's patch count
     plain_est, same, per_grid = 0.0, True, {}
     for g, ra in zip(groups, res):
         ts = []
-        for k in range(min(args.plain_patches, len(g["own"]))):
+        picks = sorted(set(np.linspace(0, len(g["own"]) - 1, min(args.plain_patches, len(g["own"]))).round().astype(int)))
+        for k in picks:
             own = g["own"][k]
             sv = Solver(own, precision="fp64", max_batch=1)
             sv.load([own])
@@ -319,7 +320,7 @@ def uneven(args):
         "converged": conv,
         "plain_one_at_a_time_patches_per_s": round(n / plain_est, 4),
         "plain_per_grid_patches_per_s": per_grid,
-        "plain_sample": f"the first {args.plain_patches} patches of every grid, each solved alone in a context of "
+        "plain_sample": f"{args.plain_patches} patches spread over every grid, each solved alone in a context of "
                         "its own geometry; the batch's plain time = per grid, mean time per patch x patches",
         "plain_identical_to_batched": same,
         "speedup_vs_plain": round(plain_est / dt, 2),

@@ -232,22 +232,6 @@ __global__ void __launch_bounds__(LR * N / 16) k_fft_rows(cx<T>* __restrict__ da
     for (int e = threadIdx.x; e < LR * N; e += NT) g[e] = lds[e];
 }
 
-// Inverse row pass of the multi-pass OTF: as k_fft_rows, skipping images a patch lacks
-template <typename T, int N, int LR>
-__global__ void __launch_bounds__(LR * N / 16) k_rows_inv(cx<T>* __restrict__ data, const int* __restrict__ qImg,
-                                                          const T* __restrict__ imgScale, const cx<T>* __restrict__ tw) {
-    constexpr int NT = LR * N / 16;
-    __shared__ cx<T> lds[LR * N];
-    const int chunks = N / LR;
-    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
-    if (imgScale[qImg[tile]] == T(0)) return;
-    cx<T>* g = data + tile * N * N + chunk * LR * N;
-    for (int e = threadIdx.x; e < LR * N; e += NT) lds[e] = g[e];
-    __syncthreads();
-    fft_lines<T, N, +1, LR, NT>(lds, N, 1, tw);
-    for (int e = threadIdx.x; e < LR * N; e += NT) g[e] = lds[e];
-}
-
 // Column pass: each block transforms LC consecutive columns of one tile in place.
 template <typename T, int N, int DIR, int LC>
 __global__ void __launch_bounds__(LC * N / 16) k_fft_cols(cx<T>* __restrict__ data, const cx<T>* __restrict__ tw) {
@@ -269,23 +253,32 @@ __global__ void __launch_bounds__(LC * N / 16) k_fft_cols(cx<T>* __restrict__ da
 }
 
 // The multi-pass OTF (grids too large for one tile in LDS) in four passes over the tile
-// workspace, the elementwise stages folded into the transform passes:
-//   k_field_rows       pupil field a = mag exp(i phi) built in LDS, forward row FFTs
-//   k_cols_power       forward column FFTs, then |.|^2
-//   k_fft_rows (+1)    inverse row FFTs
-//   k_cols_extract     inverse column FFTs, the stored support points scaled by 1/N^2
-//                      written straight to S (invOs: stored index h >= 0, else not stored)
+// workspace, the elementwise stages folded into the transform passes and the real and
+// Hermitian structure used:
+//   k_field_rows     pupil field a = mag exp(i phi) built in LDS, forward row FFTs; only
+//                    the rows the pupil touches (rowNZ) exist, the others are zero
+//   k_cols_power     forward column FFTs (zero rows not read), |.|^2 to a real array
+//   k_rows_inv_real  inverse row FFTs of the real rows, two per complex transform; the
+//                    Hermitian rows are written in columns [0, N/2] only
+//   k_cols_extract   inverse column FFTs of the columns [0, N/2] (where the stored half
+//                    of the support lies), its points scaled by 1/N^2 written to S
+//                    (invOs: stored index h >= 0, else not stored)
+// Images a patch lacks (imgScale 0) skip every pass; k_cols_extract writes their zero OTF.
 template <typename T, int N, int LR>
 __global__ void __launch_bounds__(LR * N / 16) k_field_rows(cx<T>* __restrict__ data, const T* __restrict__ phi,
                                                             const int* __restrict__ inv, const T* __restrict__ pupil,
                                                             const int* __restrict__ qImg, const T* __restrict__ imgScale,
-                                                            int nPS, const cx<T>* __restrict__ tw) {
+                                                            const unsigned char* __restrict__ rowNZ, int nPS,
+                                                            const cx<T>* __restrict__ tw) {
     constexpr int NT = LR * N / 16;
     __shared__ cx<T> lds[LR * N];
     const int chunks = N / LR;
     const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
     const T sc = imgScale[qImg[tile]];
-    if (sc == T(0)) return;     // an image the patch lacks: k_cols_extract writes its zero OTF
+    if (sc == T(0)) return;
+    bool any = false;
+    for (int l = 0; l < LR; ++l) any |= rowNZ[chunk * LR + l] != 0;
+    if (!any) return;
     const T* ph = phi + tile * nPS;
     const int p0 = (int)chunk * LR * N;
     for (int e = threadIdx.x; e < LR * N; e += NT) {
@@ -306,54 +299,88 @@ __global__ void __launch_bounds__(LR * N / 16) k_field_rows(cx<T>* __restrict__ 
 }
 
 template <typename T, int N, int LC>
-__global__ void __launch_bounds__(LC * N / 16) k_cols_power(cx<T>* __restrict__ data, const int* __restrict__ qImg,
-                                                            const T* __restrict__ imgScale, const cx<T>* __restrict__ tw) {
+__global__ void __launch_bounds__(LC * N / 16) k_cols_power(T* __restrict__ power, const cx<T>* __restrict__ data,
+                                                            const unsigned char* __restrict__ rowNZ,
+                                                            const int* __restrict__ qImg, const T* __restrict__ imgScale,
+                                                            const cx<T>* __restrict__ tw) {
     constexpr int NT = LC * N / 16;
     __shared__ cx<T> lds[LC * N];
     const int chunks = N / LC;
     const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
     if (imgScale[qImg[tile]] == T(0)) return;
-    cx<T>* g = data + tile * N * N + chunk * LC;
+    const cx<T>* g = data + tile * N * N + chunk * LC;
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
-        lds[e] = g[(size_t)r * N + c];
+        lds[e] = rowNZ[r] ? g[(size_t)r * N + c] : cx<T>{T(0), T(0)};
     }
     __syncthreads();
     fft_lines<T, N, -1, LC, NT, 1>(lds, 1, LC, tw);
+    T* pw = power + tile * N * N + chunk * LC;
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
         const cx<T> v = lds[e];
-        g[(size_t)r * N + c] = {v.x * v.x + v.y * v.y, T(0)};
+        pw[(size_t)r * N + c] = v.x * v.x + v.y * v.y;
     }
 }
 
+// LR complex lines per block, each the pair of real rows (2l, 2l+1) of the block's 2 LR rows:
+// z = x1 + i x2, Z = IFFT(z), X1(k) = (Z(k) + Z(N-k)*)/2, X2(k) = (Z(k) - Z(N-k)*)/(2i)
+template <typename T, int N, int LR>
+__global__ void __launch_bounds__(LR * N / 16) k_rows_inv_real(cx<T>* __restrict__ data, const T* __restrict__ power,
+                                                               const int* __restrict__ qImg,
+                                                               const T* __restrict__ imgScale,
+                                                               const cx<T>* __restrict__ tw) {
+    constexpr int NT = LR * N / 16, H = N / 2 + 1;
+    __shared__ cx<T> lds[LR * N];
+    const int chunks = N / (2 * LR);
+    const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    if (imgScale[qImg[tile]] == T(0)) return;
+    const size_t r0 = chunk * 2 * LR;
+    const T* pw = power + tile * N * N + r0 * N;
+    for (int e = threadIdx.x; e < LR * N; e += NT) {
+        const int l = e / N, n = e % N;
+        lds[e] = {pw[(size_t)(2 * l) * N + n], pw[(size_t)(2 * l + 1) * N + n]};
+    }
+    __syncthreads();
+    fft_lines<T, N, +1, LR, NT>(lds, N, 1, tw);
+    cx<T>* g = data + tile * N * N + r0 * N;
+    for (int e = threadIdx.x; e < LR * H; e += NT) {
+        const int l = e / H, k = e % H;
+        const cx<T> zk = lds[l * N + k], zm = lds[l * N + (N - k) % N];
+        g[(size_t)(2 * l) * N + k] = {T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y)};
+        g[(size_t)(2 * l + 1) * N + k] = {T(0.5) * (zk.y + zm.y), T(0.5) * (zm.x - zk.x)};
+    }
+}
+
+// columns [0, (N/2)/LC * LC + LC) in `chunks` blocks of LC per tile; those beyond N/2 read as
+// zero (no stored point lies there)
 template <typename T, int N, int LC>
 __global__ void __launch_bounds__(LC * N / 16) k_cols_extract(cx<T>* __restrict__ S, const cx<T>* __restrict__ data,
                                                               const int* __restrict__ invOs, int nOS, T scale,
                                                               const int* __restrict__ qImg, const T* __restrict__ imgScale,
-                                                              const cx<T>* __restrict__ tw) {
+                                                              int chunks, const cx<T>* __restrict__ tw) {
     constexpr int NT = LC * N / 16;
     __shared__ cx<T> lds[LC * N];
-    const int chunks = N / LC;
     const size_t tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    const int c0 = (int)chunk * LC;
     cx<T>* So = S + tile * nOS;
     if (imgScale[qImg[tile]] == T(0)) {       // an image the patch lacks: zero OTF
         for (int e = threadIdx.x; e < LC * N; e += NT) {
-            const int h = invOs[(e / LC) * N + (int)chunk * LC + e % LC];
+            const int h = invOs[(e / LC) * N + c0 + e % LC];
             if (h >= 0) So[h] = {T(0), T(0)};
         }
         return;
     }
-    const cx<T>* g = data + tile * N * N + chunk * LC;
+    const cx<T>* g = data + tile * N * N + c0;
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
-        lds[e] = g[(size_t)r * N + c];
+        lds[e] = c0 + c <= N / 2 ? g[(size_t)r * N + c] : cx<T>{T(0), T(0)};
     }
     __syncthreads();
     fft_lines<T, N, +1, LC, NT, 1>(lds, 1, LC, tw);
     for (int e = threadIdx.x; e < LC * N; e += NT) {
         const int r = e / LC, c = e % LC;
-        const int h = invOs[r * N + (int)chunk * LC + c];
+        const int h = invOs[r * N + c0 + c];
         if (h >= 0) {
             const cx<T> v = lds[e];
             So[h] = {v.x * scale, v.y * scale};
@@ -2054,6 +2081,8 @@ public:
             mirNat.push_back(paired ? natOf(m) : -1);
         }
         nOS_ = (int)rep.size();
+        for (int c : rep)       // k_cols_extract transforms only the natural columns [0, N/2]
+            if (natOf(c) % N > N / 2) throw std::runtime_error("OTF support is not centro-symmetric");
         {   // where each stored point sits in the fused kernel's (N) x (N/2+1) half array
             std::vector<int> hread(nOS_);
             for (int h = 0; h < nOS_; ++h) {
@@ -2098,6 +2127,11 @@ public:
         }
         upload(wt_, wt, st_);
         upload(invPio_, invPio, st_);
+        {
+            std::vector<unsigned char> rowNZ(N, 0);
+            for (int k = 0; k < nPS_; ++k) rowNZ[pio[k] / N] = 1;
+            upload(rowNZ_, rowNZ, st_);
+        }
         upload(invPde_, invPde, st_);
         {
             std::vector<int> yx(4 * (size_t)nPS_);
@@ -2480,8 +2514,8 @@ public:
 
 private:
     int maxTiles() const {
-        // tile workspace bounded to ~2 GiB
-        const size_t cap = (size_t)2 << 30;
+        // tile workspace (per lane, plus a real array of half its size) bounded to 4 GiB
+        const size_t cap = (size_t)4 << 30;
         const size_t per = (size_t)N2 * sizeof(C);
         return (int)std::max<size_t>(16, std::min<size_t>(cap / per, (size_t)maxBatch_ * g_.nImg));
     }
@@ -2559,18 +2593,22 @@ private:
     // The four passes of the multi-pass OTF (see k_field_rows) at the tuned tile shapes
     template <int L>
     void otf_stage(int stage, C* tiles, int nQ, const T* phi, const int* qImg, C* Sout) {
-        const unsigned gr = nQ * (N / L);
         const dim3 bs(L * N / 16);
+        T* power = sc_->power.p;
         if (stage == 0)
-            hipLaunchKernelGGL((k_field_rows<T, N, L>), dim3(gr), bs, 0, st_, tiles, phi, invPio_.p, pupil_.p, qImg,
-                               imgScale_.p, nPS_, tw_.p);
+            hipLaunchKernelGGL((k_field_rows<T, N, L>), dim3(nQ * (N / L)), bs, 0, st_, tiles, phi, invPio_.p,
+                               pupil_.p, qImg, imgScale_.p, rowNZ_.p, nPS_, tw_.p);
         else if (stage == 1)
-            hipLaunchKernelGGL((k_cols_power<T, N, L>), dim3(gr), bs, 0, st_, tiles, qImg, imgScale_.p, tw_.p);
+            hipLaunchKernelGGL((k_cols_power<T, N, L>), dim3(nQ * (N / L)), bs, 0, st_, power, tiles, rowNZ_.p, qImg,
+                               imgScale_.p, tw_.p);
         else if (stage == 2)
-            hipLaunchKernelGGL((k_rows_inv<T, N, L>), dim3(gr), bs, 0, st_, tiles, qImg, imgScale_.p, tw_.p);
-        else
-            hipLaunchKernelGGL((k_cols_extract<T, N, L>), dim3(gr), bs, 0, st_, Sout, tiles, invOs_.p, nOS_,
-                               (T)(1.0 / N2), qImg, imgScale_.p, tw_.p);
+            hipLaunchKernelGGL((k_rows_inv_real<T, N, L>), dim3(nQ * (N / (2 * L))), bs, 0, st_, tiles, power, qImg,
+                               imgScale_.p, tw_.p);
+        else {
+            const int chunks = (N / 2) / L + 1;
+            hipLaunchKernelGGL((k_cols_extract<T, N, L>), dim3(nQ * chunks), bs, 0, st_, Sout, tiles, invOs_.p, nOS_,
+                               (T)(1.0 / N2), qImg, imgScale_.p, chunks, tw_.p);
+        }
     }
 
     void otf_stage_any(int stage, C* tiles, int nQ, const T* phi, const int* qImg, C* Sout) {
@@ -2666,7 +2704,8 @@ private:
             return;
         }
         sc_->tiles.alloc((size_t)nQ * N2);
-        static const char* names[4] = {"k_field_rows", "k_cols_power", "fft_rows_inv", "k_cols_extract"};
+        sc_->power.alloc((size_t)nQ * N2);
+        static const char* names[4] = {"k_field_rows", "k_cols_power", "k_rows_inv_real", "k_cols_extract"};
         for (int stage = 0; stage < 4; ++stage)
             launch(names[stage], nQ, [&] { otf_stage_any(stage, sc_->tiles.p, nQ, phi, qImg, Sout); });
     }
@@ -3025,6 +3064,7 @@ private:
     DBuf<double> ftSum_, gamma_, win_, w_, wt_;
     std::vector<double> hGamma_;             // host copy of gamma_ (slot, object)
     DBuf<int> gw_;
+    DBuf<unsigned char> rowNZ_;             // natural rows of the OTF grid the pupil field touches
     DBuf<int> invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_, pyx_, osyx_, hread_,
         invPs_;
     DBuf<float> rawF_;
@@ -3033,7 +3073,7 @@ private:
     // / evaluate_end), lane 2 serves the synchronous calls
     struct Scratch {
         DBuf<C> tiles, S, Sfd;
-        DBuf<T> phi, phiFd, v, greal, fdStep;
+        DBuf<T> phi, phiFd, v, greal, fdStep, power;
         DBuf<double> Pp, Qp, part, alpha, fdOut, statOut, gpart, tab, out;
         DBuf<int> ints, fdInts, initInts;
         HBuf<double> hAlpha, hL, hG, hTab, hOut;   // hTab: the evaluation's tables, hOut: its results

@@ -639,8 +639,9 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
                                             const cx<T>* __restrict__ D, const double* __restrict__ ftSum,
                                             const double* __restrict__ gamma, const int* __restrict__ reqSlot,
                                             const int* __restrict__ chObj, const int* __restrict__ chFirst,
-                                            const int* __restrict__ chCount, const double* __restrict__ wt, int nCh,
-                                            int nObj, int nImg, int nOS, int useGamma) {
+                                            const int* __restrict__ chCount, const double* __restrict__ wt,
+                                            const T* __restrict__ imgScale, int nCh, int nObj, int nImg, int nOS,
+                                            int useGamma) {
     __shared__ double sh[256];
     const int r = blockIdx.z, o = blockIdx.y;
     const int u = blockIdx.x * 256 + threadIdx.x;
@@ -656,6 +657,7 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
                 double tr = 0, ti = 0, tq = 0;
 #pragma unroll 8
                 for (int j = f0 + b; j < f0 + e; ++j) {
+                    if (imgScale[(size_t)slot * nImg + j] == T(0)) continue;   // an image the patch lacks
                     const cx<T> s = S[((size_t)r * nImg + j) * nOS + u];
                     const cx<T> d = D[((size_t)slot * nImg + j) * nOS + u];
                     const double sx = s.x, sy = s.y, dx = d.x, dy = d.y;
@@ -2790,8 +2792,8 @@ private:
         sc_->part.alloc((size_t)n * g_.nObj * nChunk);
         launch("k_pq", nQ, [&] {
             hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, sc_->Pp.p, sc_->Qp.p, sc_->part.p, sc_->S.p, D_.p,
-                               ftSum_.p, gamma_.p, sc_->reqSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, g_.nCh, g_.nObj,
-                               nImg, nOS_, useGamma);
+                               ftSum_.p, gamma_.p, sc_->reqSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, imgScale_.p,
+                               g_.nCh, g_.nObj, nImg, nOS_, useGamma);
         });
     }
 

@@ -195,6 +195,14 @@ int mfbd_noll_covariance(int i, int j, double* cov);
 /* ModeSet::generate (modes.cpp:348-439): unit-RMS Zernike or KL modes [K][P*P] */
 int mfbd_make_modes(int P, double radius, double angle, int nModes, const int* modeIds, int basis, int klMin,
                     int klMax, double klCutoff, double* modes, double* atmRms, int* tiltXY);
+/* mfbd_make_modes with the Zernike polynomials, their normalisation and the KL sums on the
+ * device (FP64; the host version evaluates the radial polynomials in long double): the
+ * same plan of modes (ModeSet / PupilMode, modes.cpp:158-205; karhunenLoeveExpansion,
+ * zernike.cpp:367-479), for jobs whose KL_MAX_MODE (2000 by default) makes the host
+ * evaluation of every Zernike the startup cost.  Errors: mfbd_basis_last_error(). */
+int mfbd_make_modes_device(int P, double radius, double angle, int nModes, const int* modeIds, int basis, int klMin,
+                           int klMax, double klCutoff, int device, double* modes, double* atmRms, int* tiltXY);
+const char* mfbd_basis_last_error(void);
 /* ModeSet::getNorms + normalize(scale) (modes.cpp:443-535) */
 int mfbd_normalize_modes(int P, int K, double* modes, const double* pupil, double area, double scale);
 /* ModeSet::measureJacobian (modes.cpp:464-512) */

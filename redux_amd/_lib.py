@@ -68,6 +68,9 @@ SIGNATURES = {
     "mfbd_noll_covariance": (C.c_int, [C.c_int, C.c_int, dp]),
     "mfbd_make_modes": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, ip, C.c_int, C.c_int, C.c_int,
                                   C.c_double, dp, dp, ip]),
+    "mfbd_make_modes_device": (C.c_int, [C.c_int, C.c_double, C.c_double, C.c_int, ip, C.c_int, C.c_int, C.c_int,
+                                         C.c_double, C.c_int, dp, dp, ip]),
+    "mfbd_basis_last_error": (C.c_char_p, []),
     "mfbd_normalize_modes": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, C.c_double]),
     "mfbd_measure_jacobian": (C.c_int, [C.c_int, C.c_int, dp, dp, C.c_double, ip, dp, dp]),
     "mfbd_make_constraints": (C.c_int, [C.c_int, ip, ip, ip, C.c_int, ip, ip, ip, ip, ip, dp, C.c_int]),

@@ -1,6 +1,6 @@
 """Builds redux_amd/_lib/libmomfbd_hip.so in-tree for gfx950.
 
-Device code (csrc/kernels.hip, csrc/mosaic.hip) is compiled with hipcc; the host solver and setup
+Device code (csrc/kernels.hip, csrc/mosaic.hip, csrc/basis.hip) is compiled with hipcc; the host solver and setup
 (C++20) with g++; hipcc links the shared library.  Object files go to
 redux_amd/_build/ (git-ignored).
 """
@@ -40,7 +40,7 @@ def build(verbose: bool = False) -> str:
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
     jobs = []
-    for name in ("kernels.hip", "mosaic.hip"):
+    for name in ("kernels.hip", "mosaic.hip", "basis.hip"):
         obj = os.path.join(BUILD, name + ".o")
         jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, name), "-o", obj])
         objs.append(obj)

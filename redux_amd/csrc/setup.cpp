@@ -13,6 +13,7 @@
 //   Constraints::init + nullspace src/lib/momfbd/constraints.cpp:85-198,302-356,429-469,536-622
 #include "../../include/momfbd_hip.h"
 #include "hostfft.hpp"
+#include "modeplan.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -324,6 +325,49 @@ std::map<int, KL> kl_expansion(int first, int last) {
 
 }  // namespace
 
+namespace mfbd {
+
+void noll_nm(int j, int& n, int& m) { noll_to_nm(j, n, m); }
+
+std::vector<double> mode_pupil(int P, double radius, double& area) {
+    PupilData pd = make_pupil(P, radius, 0.0);
+    area = pupil_support(P, pd.pupil.data()).area;
+    return pd.pupil;
+}
+
+// PupilMode selection (modes.cpp:158-205): tilts (Noll 2, 3) and the Zernike basis are
+// Zernike polynomials, everything else in the KL basis a KL mode of [klMin, klMax]
+int mode_plan(int nModes, const int* modeIds, int basis, int klMin, int klMax, double klCutoff, ModePlan& plan,
+              std::string& err) {
+    std::map<int, KL> kl;
+    if (basis == MFBD_BASIS_KL) kl = kl_expansion(klMin, klMax);
+    plan = ModePlan{};
+    for (int k = 0; k < nModes; ++k) {
+        const int j = modeIds[k];
+        plan.noll.push_back(j);
+        plan.terms.emplace_back();
+        if (j == 2 || j == 3 || basis == MFBD_BASIS_ZERNIKE) {
+            if (j == 2) plan.tilt[0] = k;
+            if (j == 3) plan.tilt[1] = k;
+            plan.kind.push_back(j < 2 ? ModePlan::CONSTANT : ModePlan::ZERNIKE);
+            plan.atmRms.push_back(std::sqrt(noll_cov(j, j)));
+        } else {
+            auto it = kl.find(j);
+            if (it == kl.end()) {
+                err = "mfbd_make_modes: KL index outside [klMin, klMax]";
+                return MFBD_EINVAL;
+            }
+            plan.kind.push_back(ModePlan::KL);
+            for (auto& w : it->second.weights)
+                if (std::fabs(w.second) >= klCutoff) plan.terms.back().push_back(w);
+            plan.atmRms.push_back(std::sqrt(it->second.covariance));
+        }
+    }
+    return MFBD_OK;
+}
+
+}  // namespace mfbd
+
 extern "C" {
 
 const char* mfbd_setup_last_error(void) { return g_setup_err.c_str(); }
@@ -379,8 +423,9 @@ int mfbd_make_modes(int P, double radius, double angle, int nModes, const int* m
                     int klMin, int klMax, double klCutoff, double* modes, double* atmRms, int* tiltXY) {
     if (P < 2 || nModes < 1 || !modeIds || !modes) return fail(MFBD_EINVAL, "mfbd_make_modes: bad arguments");
     size_t P2 = (size_t)P * P;
-    std::map<int, KL> kl;
-    if (basis == MFBD_BASIS_KL) kl = kl_expansion(klMin, klMax);
+    ModePlan plan;
+    std::string err;
+    if (int rc = mode_plan(nModes, modeIds, basis, klMin, klMax, klCutoff, plan, err)) return fail(rc, err);
     std::map<int, std::vector<double>> zcache;
     auto Z = [&](int j) -> const std::vector<double>& {
         auto it = zcache.find(j);
@@ -389,33 +434,28 @@ int mfbd_make_modes(int P, double radius, double angle, int nModes, const int* m
     };
     PupilData pd = make_pupil(P, radius, 0.0);
     Support sp = pupil_support(P, pd.pupil.data());
-    tiltXY[0] = tiltXY[1] = -1;
+    tiltXY[0] = plan.tilt[0];
+    tiltXY[1] = plan.tilt[1];
     for (int k = 0; k < nModes; ++k) {
-        int j = modeIds[k];
+        int j = plan.noll[k];
         double* out = modes + (size_t)k * P2;
-        if (j == 2 || j == 3 || basis == MFBD_BASIS_ZERNIKE) {
-            if (j == 2) tiltXY[0] = k;
-            if (j == 3) tiltXY[1] = k;
-            if (j < 2) for (size_t q = 0; q < P2; ++q) out[q] = j;
-            else std::memcpy(out, Z(j).data(), sizeof(double) * P2);
-            if (atmRms) atmRms[k] = std::sqrt(noll_cov(j, j));
+        if (plan.kind[k] == ModePlan::CONSTANT) {
+            for (size_t q = 0; q < P2; ++q) out[q] = j;
+        } else if (plan.kind[k] == ModePlan::ZERNIKE) {
+            std::memcpy(out, Z(j).data(), sizeof(double) * P2);
         } else {
-            auto it = kl.find(j);
-            if (it == kl.end()) return fail(MFBD_EINVAL, "mfbd_make_modes: KL index outside [klMin, klMax]");
-            std::vector<long double> acc(P2, 0.0L);
             std::vector<double> m(P2, 0.0);
-            for (auto& w : it->second.weights)
-                if (std::fabs(w.second) >= klCutoff) {
-                    const auto& z = Z(w.first);
-                    for (size_t q = 0; q < P2; ++q) m[q] = m[q] + z[q] * w.second;
-                }
+            for (auto& w : plan.terms[k]) {
+                const auto& z = Z(w.first);
+                for (size_t q = 0; q < P2; ++q) m[q] = m[q] + z[q] * w.second;
+            }
             long double nrm = 0;
             for (size_t q = 0; q < P2; ++q)
                 if (pd.pupil[q] > 0) nrm += (long double)m[q] * m[q] * pd.pupil[q];
             nrm = sqrtl((long double)sp.area / nrm);
             for (size_t q = 0; q < P2; ++q) out[q] = (double)(m[q] * nrm);
-            if (atmRms) atmRms[k] = std::sqrt(it->second.covariance);
         }
+        if (atmRms) atmRms[k] = plan.atmRms[k];
     }
     return MFBD_OK;
 }

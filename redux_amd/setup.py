@@ -51,6 +51,20 @@ def mode_set(P, radius, modes, basis="zernike", kl_min=2, kl_max=2000, kl_cutoff
     return B, rms, (int(tilt[0]), int(tilt[1]))
 
 
+def mode_set_device(P, radius, modes, basis="zernike", kl_min=2, kl_max=2000, kl_cutoff=1e-3, angle=0.0, device=0):
+    """mode_set with the Zernike evaluations, normalisation and KL sums on the GPU (FP64)."""
+    ids = np.ascontiguousarray(modes, dtype=np.int32)
+    B = np.zeros((len(ids), P, P))
+    rms = np.zeros(len(ids))
+    tilt = np.zeros(2, dtype=np.int32)
+    b = BASIS_KL if basis == "kl" else BASIS_ZERNIKE
+    rc = lib().mfbd_make_modes_device(P, radius, angle, len(ids), ptr(ids, ip), b, kl_min, kl_max, kl_cutoff, device,
+                                      ptr(B), ptr(rms), ptr(tilt, ip))
+    if rc != 0:
+        raise RuntimeError(f"mfbd_make_modes_device failed ({rc}): {lib().mfbd_basis_last_error().decode()}")
+    return B, rms, (int(tilt[0]), int(tilt[1]))
+
+
 def normalize_modes(B, pupil, area, scale=1.0):
     Bc = np.ascontiguousarray(B, dtype=np.float64).copy()
     K, P = Bc.shape[0], Bc.shape[1]

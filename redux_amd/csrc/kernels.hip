@@ -1999,7 +1999,17 @@ public:
         : g_(g), maxSlots_(maxSlots), maxBatch_(maxBatch), bf16Phase_(bf16Phase && std::is_same_v<T, float>) {
         device_ = device;
         HIPCHK(hipSetDevice(device));
-        HIPCHK(hipStreamCreate(&st_));
+        // one stream per lane, so the two evaluation lanes' kernels overlap on the device
+        // (a latency-bound transform beside a bandwidth-bound reduction); lane 2 (set-up,
+        // synchronous calls, restoration sums) is the pipeline's main stream.  With kernel
+        // timing on, every lane uses the main stream, so each launch's duration is its own.
+        timing_ = getenv("MFBD_KERNEL_TIMING") != nullptr;
+        HIPCHK(hipStreamCreate(&lanes_[2].stream));
+        for (int l = 0; l < 2; ++l) {
+            if (timing_) lanes_[l].stream = lanes_[2].stream;
+            else HIPCHK(hipStreamCreate(&lanes_[l].stream));
+        }
+        st_ = lanes_[2].stream;
         HIPCHK(hipEventCreate(&ev0_));
         HIPCHK(hipEventCreate(&ev1_));
         for (auto& l : lanes_) HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
@@ -2009,18 +2019,18 @@ public:
             tw[k] = {(T)cosl(a), (T)sinl(a)};
         }
         upload(tw_, tw, st_);
-        timing_ = getenv("MFBD_KERNEL_TIMING") != nullptr;
         autotune_fft();
     }
 
     ~Pipeline() override {
-        (void)hipStreamSynchronize(st_);
+        for (auto& l : lanes_) (void)hipStreamSynchronize(l.stream);
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
         for (auto& l : lanes_) (void)hipEventDestroy(l.done);
         for (auto& e : evPool_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         for (auto& e : evPending_) { (void)hipEventDestroy(e.ev.first); (void)hipEventDestroy(e.ev.second); }
-        (void)hipStreamDestroy(st_);
+        for (int l = 0; l < 3; ++l)
+            if (l == 2 || lanes_[l].stream != lanes_[2].stream) (void)hipStreamDestroy(lanes_[l].stream);
         // device and pinned buffers are released by their own destructors
     }
 
@@ -2034,6 +2044,7 @@ public:
     const std::vector<int>& otf_support() const override { return os_; }
 
     void set_basis(const double* pupil, const double* modes) override {
+        use_lane(2);
         const int P = g_.P, P2 = P * P;
         ps_.clear();
         std::vector<int> pio;
@@ -2201,6 +2212,7 @@ public:
     }
 
     void set_structure(const Structure& s, const std::vector<double>& otfNorm) override {
+        use_lane(2);
         st = s;
         otfNorm_ = otfNorm;
         std::vector<int> chFirst(g_.nCh, 0), chCount(g_.nCh, 0);
@@ -2240,6 +2252,7 @@ public:
     }
 
     void load(int first, int n, const float* data, const double* phiCh) override {
+        use_lane(2);
         const size_t per = (size_t)g_.nImg * g_.dataY * g_.dataX;
         HIPCHK(hipMemcpyAsync(rawF_.p + (size_t)first * per, data, n * per * sizeof(float), hipMemcpyHostToDevice, st_));
         for (int k = 0; k < n; ++k) set_phi_channel(first + k, phiCh + (size_t)k * g_.nCh * g_.P * g_.P);
@@ -2247,6 +2260,7 @@ public:
     }
 
     void set_phi_channel(int slot, const double* phiCh) override {
+        use_lane(2);
         std::vector<T> h((size_t)g_.nCh * nPS_);
         for (int c = 0; c < g_.nCh; ++c)
             for (int k = 0; k < nPS_; ++k) h[(size_t)c * nPS_ + k] = (T)phiCh[(size_t)c * g_.P * g_.P + ps_[k]];
@@ -2289,6 +2303,7 @@ public:
 
     // gamma[k * nObj + o] of slot slots[k], all in one copy
     void set_images_enabled(int slot, const unsigned char* on) override {
+        use_lane(2);
         const size_t b = (size_t)slot * g_.nImg;
         for (int j = 0; j < g_.nImg; ++j) {
             imgOn_[b + j] = on[j] ? 1 : 0;
@@ -2299,6 +2314,7 @@ public:
     }
 
     void set_gamma(int n, const int* slots, const double* gamma) override {
+        use_lane(2);
         if (n <= 0) return;
         hGamma_.resize((size_t)maxSlots_ * g_.nObj, 0.0);
         for (int k = 0; k < n; ++k)
@@ -2810,6 +2826,7 @@ private:
     void use_lane(int lane) {
         lane_ = lane;
         sc_ = &lanes_[lane];
+        st_ = sc_->stream;
     }
 
     // Gradient tables of the requests that want a Vogel / adjoint gradient: their virtual
@@ -3086,6 +3103,7 @@ private:
         const int* tx = nullptr;         // the caller's ints after the request tables
         const double* td = nullptr;      // the caller's doubles before them
         hipEvent_t done = nullptr;       // recorded after the lane's result copies
+        hipStream_t stream = nullptr;
         int nReq = 0;
     };
     Scratch lanes_[3];

@@ -9,6 +9,12 @@
 
 namespace mfbd {
 
+// Lanes of the device evaluator: up to kMaxEvalLanes asynchronous evaluation lanes (each a
+// contiguous share of the batch, with its own scratch buffers and stream) and the
+// synchronous lane kSyncLane (set-up, stage-wise calls, restoration sums).
+constexpr int kMaxEvalLanes = 4;
+constexpr int kSyncLane = kMaxEvalLanes;
+
 struct Geometry {
     int s, P, N, K, nObj, nCh, nImg, dataY, dataX, startY, startX;
 };
@@ -52,7 +58,7 @@ public:
     virtual void load(int first, int n, const float* data, const double* phiCh) = 0;
     virtual void set_phi_channel(int slot, const double* phiCh) = 0;
     // windowing, padding and FFT of the listed images, then ftSum of the touched objects;
-    // lane 2 completes before returning, lane 0/1 only enqueues (see evaluate_begin)
+    // the synchronous lane completes before returning, an evaluation lane only enqueues
     virtual void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window, int lane) = 0;
     // windowed-frame standard deviation and noise level (before NOISE_FUDGE) of the
     // listed images at their shifts; patch side <= 128
@@ -64,7 +70,7 @@ public:
     // image gets a zero pupil amplitude, so it adds nothing to P, Q, the metric or gradient
     virtual void set_images_enabled(int slot, const unsigned char* on) = 0;
     virtual void evaluate(std::vector<EvalReq>& reqs) = 0;
-    // asynchronous form on lane 0 or 1: begin enqueues and returns, end waits and fills
+    // asynchronous form on an evaluation lane: begin enqueues and returns, end waits and fills
     // f / g; the two lanes let the host work on one half of a batch while the device
     // evaluates the other
     virtual void evaluate_begin(int lane, std::vector<EvalReq>& reqs) = 0;
@@ -73,7 +79,7 @@ public:
     // gamma), per object on the stored half of the OTF support: P0 [n][nObj][nOSStored]
     // complex, Q0 [n][nObj][nOSStored]; alpha[k] points at [nImg][K] for slots[k]
     virtual void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) = 0;
-    // restore_sums in two halves on the synchronous lane, for use while lanes 0/1 evaluate:
+    // restore_sums in two halves on the synchronous lane, for use while the evaluation lanes run:
     // begin enqueues the sums and their download and returns; ready says whether they are
     // done; end waits and copies them out.  One restoration at a time.
     virtual void restore_begin(int n, const int* slots, const double* const* alpha) = 0;

@@ -2000,16 +2000,16 @@ public:
         device_ = device;
         HIPCHK(hipSetDevice(device));
         // one stream per lane, so the two evaluation lanes' kernels overlap on the device
-        // (a latency-bound transform beside a bandwidth-bound reduction); lane 2 (set-up,
+        // (a latency-bound transform beside a bandwidth-bound reduction); the synchronous lane (set-up,
         // synchronous calls, restoration sums) is the pipeline's main stream.  With kernel
         // timing on, every lane uses the main stream, so each launch's duration is its own.
         timing_ = getenv("MFBD_KERNEL_TIMING") != nullptr;
-        HIPCHK(hipStreamCreate(&lanes_[2].stream));
-        for (int l = 0; l < 2; ++l) {
-            if (timing_) lanes_[l].stream = lanes_[2].stream;
+        HIPCHK(hipStreamCreate(&lanes_[kSyncLane].stream));
+        for (int l = 0; l < kMaxEvalLanes; ++l) {
+            if (timing_) lanes_[l].stream = lanes_[kSyncLane].stream;
             else HIPCHK(hipStreamCreate(&lanes_[l].stream));
         }
-        st_ = lanes_[2].stream;
+        st_ = lanes_[kSyncLane].stream;
         HIPCHK(hipEventCreate(&ev0_));
         HIPCHK(hipEventCreate(&ev1_));
         for (auto& l : lanes_) HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
@@ -2029,8 +2029,8 @@ public:
         for (auto& l : lanes_) (void)hipEventDestroy(l.done);
         for (auto& e : evPool_) { (void)hipEventDestroy(e.first); (void)hipEventDestroy(e.second); }
         for (auto& e : evPending_) { (void)hipEventDestroy(e.ev.first); (void)hipEventDestroy(e.ev.second); }
-        for (int l = 0; l < 3; ++l)
-            if (l == 2 || lanes_[l].stream != lanes_[2].stream) (void)hipStreamDestroy(lanes_[l].stream);
+        for (int l = 0; l <= kSyncLane; ++l)
+            if (l == kSyncLane || lanes_[l].stream != lanes_[kSyncLane].stream) (void)hipStreamDestroy(lanes_[l].stream);
         // device and pinned buffers are released by their own destructors
     }
 
@@ -2044,7 +2044,7 @@ public:
     const std::vector<int>& otf_support() const override { return os_; }
 
     void set_basis(const double* pupil, const double* modes) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         const int P = g_.P, P2 = P * P;
         ps_.clear();
         std::vector<int> pio;
@@ -2212,7 +2212,7 @@ public:
     }
 
     void set_structure(const Structure& s, const std::vector<double>& otfNorm) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         st = s;
         otfNorm_ = otfNorm;
         std::vector<int> chFirst(g_.nCh, 0), chCount(g_.nCh, 0);
@@ -2252,7 +2252,7 @@ public:
     }
 
     void load(int first, int n, const float* data, const double* phiCh) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         const size_t per = (size_t)g_.nImg * g_.dataY * g_.dataX;
         HIPCHK(hipMemcpyAsync(rawF_.p + (size_t)first * per, data, n * per * sizeof(float), hipMemcpyHostToDevice, st_));
         for (int k = 0; k < n; ++k) set_phi_channel(first + k, phiCh + (size_t)k * g_.nCh * g_.P * g_.P);
@@ -2260,7 +2260,7 @@ public:
     }
 
     void set_phi_channel(int slot, const double* phiCh) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         std::vector<T> h((size_t)g_.nCh * nPS_);
         for (int c = 0; c < g_.nCh; ++c)
             for (int k = 0; k < nPS_; ++k) h[(size_t)c * nPS_ + k] = (T)phiCh[(size_t)c * g_.P * g_.P + ps_[k]];
@@ -2271,7 +2271,7 @@ public:
 
     void frame_stats(const std::vector<ImgInit>& imgs, const std::vector<double>& window,
                      const std::vector<double>& noiseWindow, double* noise, double* stddev) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         const int nQ = (int)imgs.size();
         if (!nQ) return;
         const int s = g_.s;
@@ -2298,12 +2298,12 @@ public:
         HIPCHK(hipMemcpyAsync(noise, sc_->statOut.p, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipMemcpyAsync(stddev, sc_->statOut.p + nQ, nQ * sizeof(double), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipStreamSynchronize(st_));
-        collect(2);
+        collect(kSyncLane);
     }
 
     // gamma[k * nObj + o] of slot slots[k], all in one copy
     void set_images_enabled(int slot, const unsigned char* on) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         const size_t b = (size_t)slot * g_.nImg;
         for (int j = 0; j < g_.nImg; ++j) {
             imgOn_[b + j] = on[j] ? 1 : 0;
@@ -2314,7 +2314,7 @@ public:
     }
 
     void set_gamma(int n, const int* slots, const double* gamma) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         if (n <= 0) return;
         hGamma_.resize((size_t)maxSlots_ * g_.nObj, 0.0);
         for (int k = 0; k < n; ++k)
@@ -2324,7 +2324,7 @@ public:
         HIPCHK(hipStreamSynchronize(st_));
     }
 
-    // Lane 2 waits for completion; on lane 0 or 1 the work is only enqueued (after that
+    // The synchronous lane waits for completion; on an evaluation lane the work is only enqueued (after that
     // lane's finished evaluation, before its next one) from the lane's own pinned staging.
     void init_images(const std::vector<ImgInit>& imgs, const std::vector<double>& window, int lane) override {
         use_lane(lane);
@@ -2386,18 +2386,18 @@ public:
             hipLaunchKernelGGL(k_ftsum<T>, dim3(nblk((size_t)nPairs * nOS_)), dim3(256), 0, st_, ftSum_.p, D_.p,
                                base + 4 * nQ, imgObj_.p, nPairs, g_.nImg, g_.nObj, nOS_);
         });
-        if (lane == 2) {
+        if (lane == kSyncLane) {
             HIPCHK(hipStreamSynchronize(st_));
-            collect(2);
+            collect(kSyncLane);
         }
     }
 
     void evaluate(std::vector<EvalReq>& reqs) override {
-        evaluate_begin(2, reqs);
-        evaluate_end(2, reqs);
+        evaluate_begin(kSyncLane, reqs);
+        evaluate_end(kSyncLane, reqs);
     }
 
-    // Enqueue the evaluation of `reqs` on lane 0 or 1 (lane 2: the synchronous calls) and
+    // Enqueue the evaluation of `reqs` on an evaluation lane (kSyncLane: the synchronous calls) and
     // return without waiting; evaluate_end waits for that lane and fills f and g.  The
     // requests' alpha (and the lane's staging) must stay untouched until then.  Requests
     // beyond one chunk of the tile workspace are evaluated synchronously first.
@@ -2433,11 +2433,11 @@ public:
         restore_end(P0, Q0);
     }
 
-    // P0, Q0 (no gamma) of n slots at the given alphas, on lane 2: the sums and their
-    // download are enqueued behind whatever the stream holds; lane 2's pinned buffers
+    // P0, Q0 (no gamma) of n slots at the given alphas, on the synchronous lane: the sums and their
+    // download are enqueued behind whatever the stream holds; that lane's pinned buffers
     // receive [n][nObj][nH] P (complex) and Q
     void restore_begin(int n, const int* slots, const double* const* alpha) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         const size_t per = (size_t)g_.nObj * nOS_;
         const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
         std::vector<EvalReq> rq(n);
@@ -2460,20 +2460,20 @@ public:
     }
 
     bool restore_ready() override {
-        const hipError_t e = hipEventQuery(lanes_[2].done);
+        const hipError_t e = hipEventQuery(lanes_[kSyncLane].done);
         if (e == hipErrorNotReady) return false;
         HIPCHK(e);
         return true;
     }
 
     void restore_end(double* P0, double* Q0) override {
-        use_lane(2);
+        use_lane(kSyncLane);
         const size_t per = (size_t)g_.nObj * nOS_;
         HIPCHK(hipEventSynchronize(sc_->done));
         std::memcpy(P0, sc_->hG.p, 2 * (size_t)restoreN_ * per * sizeof(double));
         std::memcpy(Q0, sc_->hL.p, (size_t)restoreN_ * per * sizeof(double));
         restoreN_ = 0;
-        collect(2);
+        collect(kSyncLane);
     }
 
     void debug_get(const std::string& what, int slot, std::vector<double>& out) override {
@@ -2481,7 +2481,7 @@ public:
             out = {(double)lr_, (double)lc_};
             return;
         }
-        use_lane(2);
+        use_lane(kSyncLane);
         auto getc = [&](const C* p, size_t n) {
             std::vector<C> h(n);
             HIPCHK(hipMemcpy(h.data(), p, n * sizeof(C), hipMemcpyDeviceToHost));
@@ -3052,7 +3052,7 @@ private:
     Geometry g_;
     int maxSlots_, maxBatch_;
     int otfGrid_ = 1;                        // workgroups of the persistent FP64 OTF kernel
-    int restoreN_ = 0;                       // slots of the restoration in flight on lane 2
+    int restoreN_ = 0;                       // slots of the restoration in flight on the synchronous lane
     hipStream_t st_;
     hipEvent_t ev0_, ev1_;
     bool timing_ = false;
@@ -3088,8 +3088,8 @@ private:
         invPs_;
     DBuf<float> rawF_;
     bool winSet_ = false;
-    // per-round work buffers: lanes 0 and 1 take alternate halves of a batch (evaluate_begin
-    // / evaluate_end), lane 2 serves the synchronous calls
+    // per-round work buffers: the evaluation lanes take contiguous shares of a batch
+    // (evaluate_begin / evaluate_end), kSyncLane serves the synchronous calls
     struct Scratch {
         DBuf<C> tiles, S, Sfd;
         DBuf<T> phi, phiFd, v, greal, fdStep, power;
@@ -3106,10 +3106,10 @@ private:
         hipStream_t stream = nullptr;
         int nReq = 0;
     };
-    Scratch lanes_[3];
-    Scratch* sc_ = &lanes_[2];
+    Scratch lanes_[kSyncLane + 1];
+    Scratch* sc_ = &lanes_[kSyncLane];
     int lane_ = 2;
-    int laneFirst_[3] = {0, 0, 0};
+    int laneFirst_[kSyncLane + 1] = {};
 };
 
 std::unique_ptr<DeviceEvaluator> make_device_evaluator(const Geometry& g, int precision, int maxSlots, int maxBatch,

@@ -449,7 +449,7 @@ void reset_init(mfbd_ctx* c, const std::vector<int>& patches) {
         c->gamma[p].assign(gam, gam + nObj);
     }
     c->dev->set_gamma((int)patches.size(), patches.data(), gamAll.data());
-    c->dev->init_images(imgs, c->window, 2);
+    c->dev->init_images(imgs, c->window, kSyncLane);
 }
 
 // SubImage::adjustShifts (subimage.cpp:444-480) for one image; returns true if shifted
@@ -1144,6 +1144,15 @@ void shift_inits(mfbd_ctx* c, std::vector<PatchRun*>& rs, int lane) {
     c->dev->init_images(imgs, c->window, lane);
 }
 
+// Evaluation lanes: each a contiguous share of the batch on a stream of its own; while the
+// device evaluates one share, the host resumes the others' patches.  MFBD_EVAL_LANES
+// (1..kMaxEvalLanes) overrides the default.
+int eval_lanes() {
+    const char* e = getenv("MFBD_EVAL_LANES");
+    const int v = e ? atoi(e) : 2;
+    return std::max(1, std::min(v, kMaxEvalLanes));
+}
+
 int check_ready(mfbd_ctx* c) {
     if (!c) return MFBD_EINVAL;
     if (!c->haveStructure || !c->haveBasis || !c->haveCons || c->nPatch == 0)
@@ -1508,16 +1517,16 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             if (!c->pool) c->pool = std::make_unique<Pool>(
                 (int)std::min<unsigned>(std::max(2u, std::thread::hardware_concurrency()), 16u) - 1);
             c->pool->run((int)tasks.size(), [&](int k) { tasks[k].h.resume(); });
-            // Two lanes take the two contiguous halves of the batch.  While the device
-            // evaluates one half, the host resumes the other half's patches (their CG /
-            // line-search steps) and enqueues their next requests, so neither side waits
-            // for the other.  Each patch still sees exactly its own sequence of requests.
-            const int nLanes = nb >= 2 ? 2 : 1;
-            const int split = (nb + 1) / 2;
-            auto laneOf = [&](int k) { return nLanes == 1 ? 0 : (k < split ? 0 : 1); };
-            std::vector<EvalReq> rq[2];
-            std::vector<PatchRun*> pend[2];
-            bool inflight[2] = {false, false};
+            // The evaluation lanes take contiguous shares of the batch.  While the device
+            // evaluates one share (and, on their own streams, the lanes' kernels overlap),
+            // the host resumes another share's patches (their CG / line-search steps) and
+            // enqueues their next requests.  Each patch still sees exactly its own sequence
+            // of requests.
+            const int nLanes = std::max(1, std::min(eval_lanes(), nb));
+            auto laneOf = [&](int k) { return (int)((long long)k * nLanes / nb); };
+            std::vector<EvalReq> rq[kMaxEvalLanes];
+            std::vector<PatchRun*> pend[kMaxEvalLanes];
+            bool inflight[kMaxEvalLanes] = {};
             auto do_inits = [&](int e) {         // batched re-initialisations after a stage
                 while (true) {
                     std::vector<PatchRun*> inits;
@@ -1574,7 +1583,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 inflight[e] = false;
             };
             // Restoration (Object::restorePatch) of the patches that have finished, while the
-            // others still iterate: their P0/Q0 sums go to the device on lane 2 behind the
+            // others still iterate: their P0/Q0 sums go to the device on the synchronous lane behind the
             // lanes' evaluations, in groups of RESTORE_MIN..RESTORE_MAX patches, and the
             // filter and inverse transform run on a host thread.  A patch's restoration
             // depends on its own final alpha only, so grouping leaves it unchanged.

@@ -118,3 +118,30 @@ def test_batched_solve_equals_one_at_a_time(prec):
         assert solo["nF"] == b["nF"] and solo["nDF"] == b["nDF"], (prec, k, solo["nF"], b["nF"])
         assert np.array_equal(solo["alpha"], b["alpha"]), (prec, k)
         assert np.array_equal(solo["img"], b["img"]), (prec, k)
+
+
+def test_solve_is_deterministic_across_runs_and_lane_counts():
+    """The same context solved again, and with 1, 2 or 3 evaluation lanes (contiguous
+    shares of the batch on streams of their own, overlapping on the device), gives the
+    same results bit for bit: every patch sees its own sequence of requests, and the
+    set-up on the synchronous lane completes before any lane evaluates."""
+    import os
+    from redux_amd import synth
+    from redux_amd.momfbd import Solver
+    print("determinism across runs and lanes", flush=True)
+    _, pbs = synth.build_field_batch(synth.CFG2_PATCH, 48, seed=77)
+    sv = Solver(pbs[0], precision="fp64", max_batch=len(pbs))
+    sv.load(pbs)
+    runs = []
+    try:
+        for lanes in ("2", "2", "1", "3"):
+            os.environ["MFBD_EVAL_LANES"] = lanes
+            runs.append(sv.run())
+    finally:
+        os.environ.pop("MFBD_EVAL_LANES", None)
+        sv.close()
+    for k, res in enumerate(runs[1:], 1):
+        for a, b in zip(res, runs[0]):
+            assert a["initialMetric"] == b["initialMetric"] and a["finalMetric"] == b["finalMetric"], k
+            np.testing.assert_array_equal(a["alpha"], b["alpha"])
+            np.testing.assert_array_equal(a["img"], b["img"])

@@ -181,6 +181,18 @@ int mfbd_mosaic_size(int nPatches, int patchSize, const int* posY, const int* po
 int mfbd_mosaic(int nPatches, int patchSize, const float* patches, const int* posY, const int* posX, int clip,
                 int device, float* img, int* imgRows, int* imgCols, int* firstRow, int* firstCol);
 
+/* ---- output files (Object::writeAna / writeFits, object.cpp:1212-1334) ---- */
+/* ANA (.f0) file, uncompressed (Ana::write, fileana.cpp:150-225, 498-560): datyp 0..4 =
+ * int8, int16, int32, float32, float64; dims fastest first (columns, rows, ...); text is
+ * the header text (NULL for none; more than 255 characters take further 512-byte blocks,
+ * up to 7935).  The reference compresses 2-D int16 data when that is smaller; this writer
+ * stores every type uncompressed, which ANA readers accept alike. */
+int mfbd_write_ana(const char* path, int datyp, int ndim, const int* dims, const void* data, const char* text);
+/* FITS primary image (Fits::write, filefits.cpp:1756-1818): BITPIX 8, 16, 32, 64, -32 or
+ * -64; dims fastest first (NAXIS1 = columns); data in host order, written big-endian. */
+int mfbd_write_fits(const char* path, int bitpix, int ndim, const int* dims, const void* data);
+const char* mfbd_file_last_error(void);
+
 /* ---- per-job setup (host; GlobalData/Object::initObject equivalents) ---- */
 const char* mfbd_setup_last_error(void);
 /* Pupil::calculatePupilSize (pupil.cpp:78-100) */

@@ -59,6 +59,9 @@ SIGNATURES = {
     "mfbd_set_patch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_ubyte), C.c_int, C.c_int, ip, ip, dp]),
     "mfbd_solve_group": (C.c_int, [C.c_int, C.POINTER(C.c_void_p), C.POINTER(dp), C.POINTER(C.c_void_p),
                                    C.POINTER(dp), C.POINTER(fp)]),
+    "mfbd_write_ana": (C.c_int, [C.c_char_p, C.c_int, C.c_int, ip, C.c_void_p, C.c_char_p]),
+    "mfbd_write_fits": (C.c_int, [C.c_char_p, C.c_int, C.c_int, ip, C.c_void_p]),
+    "mfbd_file_last_error": (C.c_char_p, []),
     "mfbd_mosaic_size": (C.c_int, [C.c_int, C.c_int, ip, ip, ip, ip]),
     "mfbd_mosaic": (C.c_int, [C.c_int, C.c_int, fp, ip, ip, C.c_int, C.c_int, fp, ip, ip, ip, ip]),
     "mfbd_setup_last_error": (C.c_char_p, []),

@@ -44,7 +44,7 @@ def build(verbose: bool = False) -> str:
         obj = os.path.join(BUILD, name + ".o")
         jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, name), "-o", obj])
         objs.append(obj)
-    for name in ("solver.cpp", "setup.cpp"):
+    for name in ("solver.cpp", "setup.cpp", "writers.cpp"):
         obj = os.path.join(BUILD, name + ".o")
         jobs.append(["g++", *CXX_FLAGS, "-c", os.path.join(CSRC, name), "-o", obj])
         objs.append(obj)

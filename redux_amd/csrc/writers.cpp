@@ -1,0 +1,130 @@
+// Output files of a solved job: the ANA (.f0) and FITS images that Object::writeAna /
+// Object::writeFits produce from the mosaic and the alphas (src/lib/momfbd/object.cpp:
+// 1212-1334), written without CFITSIO or the ANA library.
+//   ANA:  Ana::write (src/lib/file/fileana.cpp:150-225, 498-560; header layout
+//         include/redux/file/fileana.hpp:70-82): a 512-byte first block, little-endian,
+//         the header text in (nhb - 1) * 512 + 256 bytes, then the data, uncompressed.
+//   FITS: Fits::write (src/lib/file/filefits.cpp:1756-1818) through fits_create_img /
+//         fits_write_img: a primary HDU of 80-character cards in 2880-byte records, the
+//         data big-endian and padded to a whole record.
+#include "../../include/momfbd_hip.h"
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_file_err;
+
+int file_fail(const std::string& m) {
+    g_file_err = m;
+    return MFBD_EINVAL;
+}
+
+size_t elem_bytes_ana(int datyp) {
+    static const size_t sz[] = {1, 2, 4, 4, 8};
+    return datyp >= 0 && datyp <= 4 ? sz[datyp] : 0;
+}
+
+// one FITS header card: keyword padded to 8, "= ", value right-aligned to column 30
+std::string card(const std::string& key, const std::string& value, const std::string& comment = "") {
+    char buf[81];
+    std::string v = value;
+    if (v.size() < 20) v = std::string(20 - v.size(), ' ') + v;
+    std::snprintf(buf, sizeof buf, "%-8.8s= %s%s%s", key.c_str(), v.c_str(), comment.empty() ? "" : " / ",
+                  comment.c_str());
+    std::string c(buf);
+    c.resize(80, ' ');
+    return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mfbd_file_last_error(void) { return g_file_err.c_str(); }
+
+int mfbd_write_ana(const char* path, int datyp, int ndim, const int* dims, const void* data, const char* text) {
+    if (!path || !dims || !data) return file_fail("mfbd_write_ana: null argument");
+    const size_t eb = elem_bytes_ana(datyp);
+    if (!eb) return file_fail("mfbd_write_ana: datyp must be 0..4 (int8, int16, int32, float32, float64)");
+    if (ndim < 1 || ndim > 16) return file_fail("mfbd_write_ana: 1..16 dimensions");
+    size_t n = 1;
+    for (int i = 0; i < ndim; ++i) {
+        if (dims[i] < 1) return file_fail("mfbd_write_ana: dimension < 1");
+        n *= (size_t)dims[i];
+    }
+    std::string txt = text ? text : "";
+    // header text beyond the first block's 256 bytes takes further 512-byte blocks (at most 16)
+    uint8_t nhb = 1;
+    if (txt.size() > 255) {
+        nhb = (uint8_t)std::min<size_t>(16, 1 + (txt.size() + 256) / 512);
+        if (nhb == 16) txt.resize(std::min<size_t>(txt.size(), 7935));
+    }
+    unsigned char head[256] = {};
+    const uint32_t magic = 0x5555aaaaU;
+    std::memcpy(head, &magic, 4);              // little-endian host
+    head[4] = 0;                               // subf: uncompressed
+    head[5] = 0;                               // source
+    head[6] = nhb;
+    head[7] = (unsigned char)datyp;
+    head[8] = (unsigned char)ndim;
+    for (int i = 0; i < ndim; ++i) {
+        const uint32_t d = (uint32_t)dims[i];
+        std::memcpy(head + 192 + 4 * i, &d, 4);
+    }
+    std::vector<char> textBlock((size_t)(nhb - 1) * 512 + 256, 0);
+    std::memcpy(textBlock.data(), txt.data(), std::min(textBlock.size(), txt.size()));
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return file_fail(std::string("mfbd_write_ana: cannot open ") + path);
+    bool ok = std::fwrite(head, 1, 256, f) == 256 && std::fwrite(textBlock.data(), 1, textBlock.size(), f) == textBlock.size() &&
+              std::fwrite(data, eb, n, f) == n;
+    ok &= std::fclose(f) == 0;
+    return ok ? MFBD_OK : file_fail(std::string("mfbd_write_ana: write failed: ") + path);
+}
+
+int mfbd_write_fits(const char* path, int bitpix, int ndim, const int* dims, const void* data) {
+    if (!path || !dims || !data) return file_fail("mfbd_write_fits: null argument");
+    size_t eb = 0;
+    switch (bitpix) {
+        case 8: eb = 1; break;
+        case 16: eb = 2; break;
+        case 32: case -32: eb = 4; break;
+        case 64: case -64: eb = 8; break;
+        default: return file_fail("mfbd_write_fits: BITPIX must be 8, 16, 32, 64, -32 or -64");
+    }
+    if (ndim < 1 || ndim > 9) return file_fail("mfbd_write_fits: 1..9 dimensions");
+    size_t n = 1;
+    for (int i = 0; i < ndim; ++i) {
+        if (dims[i] < 1) return file_fail("mfbd_write_fits: dimension < 1");
+        n *= (size_t)dims[i];
+    }
+    // the cards fits_create_img writes for a primary image
+    std::string hdr = card("SIMPLE", "T", "file does conform to FITS standard") +
+                      card("BITPIX", std::to_string(bitpix), "number of bits per data pixel") +
+                      card("NAXIS", std::to_string(ndim), "number of data axes");
+    for (int i = 0; i < ndim; ++i)
+        hdr += card("NAXIS" + std::to_string(i + 1), std::to_string(dims[i]), "length of data axis " + std::to_string(i + 1));
+    hdr += card("EXTEND", "T", "FITS dataset may contain extensions");
+    std::string end = "END";
+    end.resize(80, ' ');
+    hdr += end;
+    hdr.resize((hdr.size() + 2879) / 2880 * 2880, ' ');
+    // data big-endian, padded with zeros to a whole record
+    std::vector<unsigned char> body(n * eb);
+    const unsigned char* src = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i)
+        for (size_t b = 0; b < eb; ++b) body[i * eb + b] = src[i * eb + (eb - 1 - b)];
+    body.resize((body.size() + 2879) / 2880 * 2880, 0);
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return file_fail(std::string("mfbd_write_fits: cannot open ") + path);
+    bool ok = std::fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size() && std::fwrite(body.data(), 1, body.size(), f) == body.size();
+    ok &= std::fclose(f) == 0;
+    return ok ? MFBD_OK : file_fail(std::string("mfbd_write_fits: write failed: ") + path);
+}
+
+}  // extern "C"

@@ -241,3 +241,48 @@ def mosaic(patches, pos_y, pos_x, clip=True, device=0):
     if rc:
         raise SolverError(f"mosaic failed ({rc})")
     return img[: int(rows[0]) * int(cols[0])].reshape(int(rows[0]), int(cols[0])), (int(r0[0]), int(c0[0]))
+
+
+_ANA_TYPES = {np.dtype(np.int8): 0, np.dtype(np.int16): 1, np.dtype(np.int32): 2, np.dtype(np.float32): 3,
+              np.dtype(np.float64): 4}
+_FITS_BITPIX = {np.dtype(np.uint8): 8, np.dtype(np.int16): 16, np.dtype(np.int32): 32, np.dtype(np.int64): 64,
+                np.dtype(np.float32): -32, np.dtype(np.float64): -64}
+
+
+def write_ana(path, array, text=""):
+    """Ana::write of an array (row-major, last axis fastest), uncompressed."""
+    a = np.ascontiguousarray(array)
+    if a.dtype not in _ANA_TYPES:
+        raise SolverError(f"ANA has no type for {a.dtype}")
+    dims = np.ascontiguousarray(a.shape[::-1], dtype=np.int32)
+    rc = L.lib().mfbd_write_ana(str(path).encode(), _ANA_TYPES[a.dtype], a.ndim, ptr(dims, ip),
+                                a.ctypes.data_as(C.c_void_p), text.encode() if text else None)
+    if rc:
+        raise SolverError(f"write_ana failed ({rc}): {L.lib().mfbd_file_last_error().decode()}")
+
+
+def write_fits(path, array):
+    """Fits::write of an array as the primary image (NAXIS1 = last axis)."""
+    a = np.ascontiguousarray(array)
+    if a.dtype not in _FITS_BITPIX:
+        raise SolverError(f"FITS has no BITPIX for {a.dtype}")
+    dims = np.ascontiguousarray(a.shape[::-1], dtype=np.int32)
+    rc = L.lib().mfbd_write_fits(str(path).encode(), _FITS_BITPIX[a.dtype], a.ndim, ptr(dims, ip),
+                                 a.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise SolverError(f"write_fits failed ({rc}): {L.lib().mfbd_file_last_error().decode()}")
+
+
+def write_object(base, mosaic_img, fmt="fits", as_int16=False, alpha=None, text=""):
+    """Object::writeFits / writeAna (object.cpp:1212-1334): the mosaic to `base`.fits or
+    `base`.f0 (float32, or int16 as the reference's default output type, converted by
+    truncation), and with `alpha` ([patchesY][patchesX][images][modes]) the alphas as
+    float32 to `base`.alpha.fits / .alpha.f0."""
+    img = np.asarray(mosaic_img, dtype=np.float32)
+    if as_int16:
+        img = img.astype(np.int16)
+    ext = ".fits" if fmt == "fits" else ".f0"
+    (write_fits if fmt == "fits" else lambda p, a: write_ana(p, a, text))(str(base) + ext, img)
+    if alpha is not None:
+        a = np.asarray(alpha, dtype=np.float32)
+        (write_fits if fmt == "fits" else lambda p, x: write_ana(p, x))(str(base) + ".alpha" + ext, a)

@@ -2481,6 +2481,10 @@ public:
             out = {(double)lr_, (double)lc_};
             return;
         }
+        if (what == "otf_lines") {      // ... of the four OTF passes (0 before the first OTF)
+            out = {(double)stageL_[0], (double)stageL_[1], (double)stageL_[2], (double)stageL_[3]};
+            return;
+        }
         use_lane(kSyncLane);
         auto getc = [&](const C* p, size_t n) {
             std::vector<C> h(n);
@@ -2608,34 +2612,91 @@ private:
         fft_pass<DIR, LINES>(tiles, nTiles, rows);
     }
 
-    // The four passes of the multi-pass OTF (see k_field_rows) at the tuned tile shapes
+    // The four passes of the multi-pass OTF (see k_field_rows) at L lines per workgroup
     template <int L>
-    void otf_stage(int stage, C* tiles, int nQ, const T* phi, const int* qImg, C* Sout) {
+    void otf_stage(int stage, C* tiles, T* power, int nQ, const T* phi, const int* qImg, const T* scale, C* Sout) {
         const dim3 bs(L * N / 16);
-        T* power = sc_->power.p;
         if (stage == 0)
             hipLaunchKernelGGL((k_field_rows<T, N, L>), dim3(nQ * (N / L)), bs, 0, st_, tiles, phi, invPio_.p,
-                               pupil_.p, qImg, imgScale_.p, rowNZ_.p, nPS_, tw_.p);
+                               pupil_.p, qImg, scale, rowNZ_.p, nPS_, tw_.p);
         else if (stage == 1)
             hipLaunchKernelGGL((k_cols_power<T, N, L>), dim3(nQ * (N / L)), bs, 0, st_, power, tiles, rowNZ_.p, qImg,
-                               imgScale_.p, tw_.p);
+                               scale, tw_.p);
         else if (stage == 2)
             hipLaunchKernelGGL((k_rows_inv_real<T, N, L>), dim3(nQ * (N / (2 * L))), bs, 0, st_, tiles, power, qImg,
-                               imgScale_.p, tw_.p);
+                               scale, tw_.p);
         else {
             const int chunks = (N / 2) / L + 1;
             hipLaunchKernelGGL((k_cols_extract<T, N, L>), dim3(nQ * chunks), bs, 0, st_, Sout, tiles, invOs_.p, nOS_,
-                               (T)(1.0 / N2), qImg, imgScale_.p, chunks, tw_.p);
+                               (T)(1.0 / N2), qImg, scale, chunks, tw_.p);
         }
     }
 
-    void otf_stage_any(int stage, C* tiles, int nQ, const T* phi, const int* qImg, C* Sout) {
-        const int l = (stage == 0 || stage == 2) ? lr_ : lc_;
+    void otf_stage_l(int stage, int l, C* tiles, T* power, int nQ, const T* phi, const int* qImg, const T* scale,
+                     C* Sout) {
         if constexpr (N == 256) {
-            if (l == LINES / 4) return otf_stage<LINES / 4>(stage, tiles, nQ, phi, qImg, Sout);
-            if (l == LINES / 2) return otf_stage<LINES / 2>(stage, tiles, nQ, phi, qImg, Sout);
+            if (l == LINES / 4) return otf_stage<LINES / 4>(stage, tiles, power, nQ, phi, qImg, scale, Sout);
+            if (l == LINES / 2) return otf_stage<LINES / 2>(stage, tiles, power, nQ, phi, qImg, scale, Sout);
         }
-        otf_stage<LINES>(stage, tiles, nQ, phi, qImg, Sout);
+        otf_stage<LINES>(stage, tiles, power, nQ, phi, qImg, scale, Sout);
+    }
+
+    // Tile shape of each of the four OTF passes at N = 256, picked by timing every
+    // candidate on 32 scratch images at the first OTF of the context (the geometry is known
+    // by then), once per precision and device; with MFBD_FFT_LINES set, its rows / columns
+    // shapes are used.  The shape does not change any result.
+    void tune_otf_stages() {
+        for (int st = 0; st < 4; ++st) stageL_[st] = (st == 0 || st == 2) ? lr_ : lc_;
+        if constexpr (N == 256) {
+            if (getenv("MFBD_FFT_LINES")) return;
+            static std::mutex mu;
+            static std::map<std::pair<int, int>, std::vector<int>> chosen;
+            std::lock_guard<std::mutex> lock(mu);
+            const auto key = std::make_pair((int)sizeof(T), device_);
+            auto it = chosen.find(key);
+            if (it == chosen.end()) {
+                const int nT = 32;
+                DBuf<C> tiles, S;
+                DBuf<T> power, phi, one;
+                DBuf<int> qz;
+                tiles.alloc((size_t)nT * N2);
+                power.alloc((size_t)nT * N2);
+                phi.alloc((size_t)nT * nPS_);
+                S.alloc((size_t)nT * nOS_);
+                qz.alloc(nT);
+                upload(one, std::vector<T>(1, T(1)), st_);
+                HIPCHK(hipMemsetAsync(tiles.p, 0, (size_t)nT * N2 * sizeof(C), st_));
+                HIPCHK(hipMemsetAsync(power.p, 0, (size_t)nT * N2 * sizeof(T), st_));
+                HIPCHK(hipMemsetAsync(phi.p, 0, (size_t)nT * nPS_ * sizeof(T), st_));
+                HIPCHK(hipMemsetAsync(qz.p, 0, nT * sizeof(int), st_));
+                const int cand[3] = {LINES / 4, LINES / 2, LINES};
+                std::vector<int> best(4, LINES);
+                for (int st = 0; st < 4; ++st) {
+                    float bestMs = 1e30f;
+                    for (int l : cand) {
+                        otf_stage_l(st, l, tiles.p, power.p, nT, phi.p, qz.p, one.p, S.p);      // warm-up
+                        float ms = 1e30f;
+                        for (int rep = 0; rep < 3; ++rep) {
+                            HIPCHK(hipEventRecord(ev0_, st_));
+                            otf_stage_l(st, l, tiles.p, power.p, nT, phi.p, qz.p, one.p, S.p);
+                            HIPCHK(hipEventRecord(ev1_, st_));
+                            HIPCHK(hipEventSynchronize(ev1_));
+                            float t = 0;
+                            HIPCHK(hipEventElapsedTime(&t, ev0_, ev1_));
+                            ms = std::min(ms, t);
+                        }
+                        HIPCHK(hipGetLastError());
+                        if (ms < bestMs) {
+                            bestMs = ms;
+                            best[st] = l;
+                        }
+                    }
+                }
+                HIPCHK(hipStreamSynchronize(st_));
+                it = chosen.emplace(key, best).first;
+            }
+            for (int st = 0; st < 4; ++st) stageL_[st] = it->second[st];
+        }
     }
 
     void fft2(C* tiles, int nTiles, int dir) {
@@ -2721,11 +2782,19 @@ private:
             });
             return;
         }
+        if (!stageL_[0]) {
+            const hipStream_t keep = st_;
+            st_ = lanes_[kSyncLane].stream;
+            tune_otf_stages();
+            st_ = keep;
+        }
         sc_->tiles.alloc((size_t)nQ * N2);
         sc_->power.alloc((size_t)nQ * N2);
         static const char* names[4] = {"k_field_rows", "k_cols_power", "k_rows_inv_real", "k_cols_extract"};
         for (int stage = 0; stage < 4; ++stage)
-            launch(names[stage], nQ, [&] { otf_stage_any(stage, sc_->tiles.p, nQ, phi, qImg, Sout); });
+            launch(names[stage], nQ, [&] {
+                otf_stage_l(stage, stageL_[stage], sc_->tiles.p, sc_->power.p, nQ, phi, qImg, imgScale_.p, Sout);
+            });
     }
 
     // phases, OTFs, P/Q and the metric for n requests (virtual images q = r*nImg + j).
@@ -3075,6 +3144,7 @@ private:
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
     int device_ = 0;
     int lr_ = LINES, lc_ = LINES;           // lines per workgroup of the transform passes
+    int stageL_[4] = {0, 0, 0, 0};          // ... of the four OTF passes at N = 256 (0: not yet tuned)
     std::vector<T> hScale_;                 // host copy of imgScale_ [maxSlots][nImg]
     std::vector<unsigned char> imgOn_;      // images each slot's patch has [maxSlots][nImg]
     DBuf<__bf16> Bt_;                 // BF16 basis, transposed, for the mixed-precision form

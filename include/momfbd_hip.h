@@ -69,13 +69,15 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
     int gradient;       /* MFBD_GRAD_SCHEDULE_REFERENCE: finite differences for the first
                            gradient, Vogel after (solver.cpp:385,478); or MFBD_GRAD_ADJOINT
                            for the exact dL/dalpha throughout                              */
-    int getstep;        /* GETSTEP: MFBD_GETSTEP_CONJUGATE or MFBD_GETSTEP_BFGS_INV
-                           (the stepper switch of Solver::run, solver.cpp:352-382)         */
+    int getstep;        /* GETSTEP: MFBD_GETSTEP_STEEPEST, _CONJUGATE, _BFGS or _BFGS_INV
+                           (GlobalCfg::getstepMethod); 0 is the conjugate gradient */
 } mfbd_params;
 
 #define MFBD_GRAD_SCHEDULE_REFERENCE 0
 /* steppers, numbered as the reference's getstepMethod (config.cpp:37, 102-114) */
+#define MFBD_GETSTEP_STEEPEST 1    /* GSM_SDSC: GSL steepest_descent */
 #define MFBD_GETSTEP_CONJUGATE 2   /* GSM_CNJG: conjugate_rdx (conjugate_gradient.cpp:128-256)   */
+#define MFBD_GETSTEP_BFGS 3        /* GSM_BFGS: GSL vector_bfgs (init_step 1e8, tol 0.01) */
 #define MFBD_GETSTEP_BFGS_INV 4    /* GSM_BFGS_inv, the reference default: GSL vector_bfgs2      */
 
 typedef struct {

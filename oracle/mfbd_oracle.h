@@ -61,7 +61,7 @@ typedef struct {
     int noFilter;
     const double* alphaInit;      /* [nImg][K] or NULL */
     int maxMetrics;
-    int getstep;                  /* 0: conjugate_rdx (GETSTEP=conjugate), 4: GSL vector_bfgs2 (BFGS_inv) */
+    int getstep;                  /* 1: steepest_descent, 0/2: conjugate_rdx, 3: vector_bfgs, 4: vector_bfgs2 (BFGS_inv) */
 } OProblem;
 
 typedef struct {
@@ -109,6 +109,12 @@ OBfgs2* o_bfgs2_new(int n);
 void o_bfgs2_free(OBfgs2* w);
 void o_bfgs2_set(OBfgs2* w, const OFdf* fn, const double* x, double* f, double* g, double stepSize, double tol);
 int o_bfgs2_iterate(OBfgs2* w, double* x, double* f, double* g, double* dx);
+/* GSL 1.16 steepest_descent (kind 1) and vector_bfgs (kind 3) restatements (solver.c) */
+typedef struct OStep OStep;
+OStep* o_step_new(int n, int kind);
+void o_step_free(OStep* w);
+void o_step_set(OStep* w, const OFdf* fn, const double* x, double* f, double* g, double stepSize, double tol);
+int o_step_iterate(OStep* w, double* x, double* f, double* g, double* dx);
 
 /* ---- mosaic of restored patches (mosaic.c; object.cpp:1160-1207) ---- */
 void o_blend_window(double* w, int nRows, int nCols, int blend, int margin);

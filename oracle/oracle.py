@@ -39,7 +39,7 @@ class OProblem(C.Structure):
     ]
 
 
-GETSTEP = {"conjugate": 0, "conjugate_gradient": 0, "bfgs_inv": 4}    # getstepMap (config.cpp:102-114)
+GETSTEP = {"steepest": 1, "steepest_descent": 1, "bfgs": 3, "conjugate": 0, "conjugate_gradient": 0, "bfgs_inv": 4}    # getstepMap (config.cpp:102-114)
 
 
 class OFdf(C.Structure):
@@ -79,6 +79,12 @@ def lib():
         L.o_bfgs2_set.argtypes = [C.c_void_p, C.POINTER(OFdf), dp, dp, dp, C.c_double, C.c_double]
         L.o_bfgs2_iterate.argtypes = [C.c_void_p, dp, dp, dp, dp]
         L.o_bfgs2_iterate.restype = C.c_int
+        L.o_step_new.argtypes = [C.c_int, C.c_int]
+        L.o_step_new.restype = C.c_void_p
+        L.o_step_free.argtypes = [C.c_void_p]
+        L.o_step_set.argtypes = [C.c_void_p, C.POINTER(OFdf), dp, dp, dp, C.c_double, C.c_double]
+        L.o_step_iterate.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.o_step_iterate.restype = C.c_int
         L.o_state_new.restype = C.c_void_p
         L.o_state_free.argtypes = [C.c_void_p]
         L.o_state_init.argtypes = [C.c_void_p]
@@ -301,6 +307,48 @@ class State:
 
     def scalar(self, what, idx=0):
         return lib().o_state_scalar(self.h, what.encode(), idx)
+
+
+def step_minimize(kind, f, grad, x0, step=1.0, tol=1e-6, iters=100):
+    """Drive the steepest_descent (kind 1) or vector_bfgs (kind 3) restatement on a Python
+    function (tests); returns the iterates as bfgs2_minimize does."""
+    L = lib()
+    n = len(x0)
+    calls = {"f": 0, "df": 0}
+
+    def _f(ctx, x):
+        calls["f"] += 1
+        return float(f(np.ctypeslib.as_array(x, (n,)).copy()))
+
+    def _df(ctx, x, g):
+        calls["df"] += 1
+        np.ctypeslib.as_array(g, (n,))[:] = grad(np.ctypeslib.as_array(x, (n,)).copy())
+
+    def _fdf(ctx, x, fo, g):
+        calls["f"] += 1
+        calls["df"] += 1
+        xv = np.ctypeslib.as_array(x, (n,)).copy()
+        fo[0] = float(f(xv))
+        np.ctypeslib.as_array(g, (n,))[:] = grad(xv)
+
+    cf, cdf, cfdf = F_T(_f), DF_T(_df), FDF_T(_fdf)
+    fn = OFdf(None, C.cast(cf, C.c_void_p), C.cast(cdf, C.c_void_p), C.cast(cfdf, C.c_void_p))
+    w = L.o_step_new(n, kind)
+    x = np.array(x0, dtype=np.float64)
+    g = np.zeros(n)
+    dx = np.zeros(n)
+    fv = np.zeros(1)
+    try:
+        L.o_step_set(w, C.byref(fn), _p(x), _p(fv), _p(g), step, tol)
+        out = [(x.copy(), float(fv[0]), g.copy(), 0, dict(calls))]
+        for _ in range(iters):
+            st = L.o_step_iterate(w, _p(x), _p(fv), _p(g), _p(dx))
+            out.append((x.copy(), float(fv[0]), g.copy(), st, dict(calls)))
+            if st != 0:
+                break
+    finally:
+        L.o_step_free(w)
+    return out
 
 
 def bfgs2_minimize(f, grad, x0, step=1.0, tol=1e-6, iters=100):

@@ -1187,6 +1187,219 @@ static void restore_objects(OState* st, double* imgOut)
     free(ao); free(dn); free(tmp);
 }
 
+/* ---------- GETSTEP = steepest_descent and BFGS: GSL 1.16 steepest_descent and
+   vector_bfgs (solver.cpp:355-361, 369-372; GSL is not in the reference tree, its
+   published algorithms restated: multimin/steepest_descent.c, vector_bfgs.c and the
+   directional minimiser they share, directional_minimize.c) ----------
+   Steepest descent: from x, trial x - step g/|g|; a trial that raises f shrinks the step
+   by tol and is retried; on success the step doubles (or, after a shrink, shrinks once
+   more).  vector_bfgs: along -+p (downhill), a trial at the current step; if it lowers f
+   the step doubles, else a parabolic guess from (0, fa, slope) and (stepc, fc) finds a
+   point below fa (halving towards 0), then a safeguarded parabolic / golden-section
+   search (at most 10 trials) stops when |p.g lambda/|g|| < tol; the new direction is the
+   BFGS update p' = g1 - A dx - B dg, restarted to g every n iterations.
+   Parity with GSL's arithmetic is unpinned; the tests pin the mathematics. */
+
+struct OStep {
+    const OFdf* fn;
+    int n, kind, iter;                 /* kind 1: steepest descent, 3: vector_bfgs */
+    double step, maxStep, tol, pnorm, g0norm;
+    double *x1, *g1, *dx1, *x2, *p, *x0, *g0, *dx0, *dg0;
+};
+
+OStep* o_step_new(int n, int kind)
+{
+    OStep* w = calloc(1, sizeof(OStep));
+    w->n = n;
+    w->kind = kind;
+    double** v[] = {&w->x1, &w->g1, &w->dx1, &w->x2, &w->p, &w->x0, &w->g0, &w->dx0, &w->dg0};
+    for (size_t i = 0; i < sizeof v / sizeof v[0]; ++i) *v[i] = calloc(n ? n : 1, sizeof(double));
+    return w;
+}
+
+void o_step_free(OStep* w)
+{
+    if (!w) return;
+    free(w->x1); free(w->g1); free(w->dx1); free(w->x2); free(w->p); free(w->x0); free(w->g0);
+    free(w->dx0); free(w->dg0); free(w);
+}
+
+static int o_vec_equal(const double* a, const double* b, int n)
+{
+    for (int k = 0; k < n; ++k) if (a[k] != b[k]) return 0;
+    return 1;
+}
+
+/* x1 = x + dx, dx = -step lambda p */
+static void o_take_step(const double* x, const double* p, double step, double lambda, double* x1, double* dx, int n)
+{
+    for (int k = 0; k < n; ++k) dx[k] = 0.0 + (-step * lambda) * p[k];
+    for (int k = 0; k < n; ++k) x1[k] = x[k] + 1.0 * dx[k];
+}
+
+void o_step_set(OStep* w, const OFdf* fn, const double* x, double* f, double* g, double stepSize, double tol)
+{
+    const int n = w->n;
+    w->fn = fn;
+    w->iter = 0;
+    w->step = stepSize;
+    w->maxStep = stepSize;
+    w->tol = tol;
+    fn->fdf(fn->ctx, x, f, g);
+    if (w->kind == 3) {                 /* the first direction is the gradient */
+        memcpy(w->x0, x, sizeof(double) * n);
+        memcpy(w->p, g, sizeof(double) * n);
+        memcpy(w->g0, g, sizeof(double) * n);
+        w->pnorm = w->g0norm = dnrm2(g, n);
+    }
+}
+
+static int o_sd_iterate(OStep* w, double* x, double* f, double* g, double* dx)
+{
+    const int n = w->n;
+    const double f0 = *f, gnorm = dnrm2(g, n);
+    double step = w->step, f1;
+    int shrunk = 0;
+    if (gnorm == 0.0) { memset(dx, 0, sizeof(double) * n); return GSL_ENOPROG; }
+    for (;;) {
+        for (int k = 0; k < n; ++k) dx[k] = 0.0 + (-step / gnorm) * g[k];
+        for (int k = 0; k < n; ++k) w->x1[k] = x[k] + 1.0 * dx[k];
+        if (o_vec_equal(x, w->x1, n)) return GSL_ENOPROG;
+        w->fn->fdf(w->fn->ctx, w->x1, &f1, w->g1);
+        if (f1 > f0) { shrunk = 1; step *= w->tol; continue; }
+        break;
+    }
+    step = shrunk ? step * w->tol : step * 2.0;
+    w->step = step;
+    memcpy(x, w->x1, sizeof(double) * n);
+    memcpy(g, w->g1, sizeof(double) * n);
+    *f = f1;
+    return GSL_SUCCESS;
+}
+
+/* a point between 0 and stepc with f below fa: parabolic guess from the slope pg at 0,
+   moving towards 0 while f does not fall (directional_minimize.c: intermediate_point) */
+static void o_intermediate(OStep* w, const double* x, double lambda, double pg, double stepc, double fa,
+                           double fc, double* g, double* stepOut, double* fOut)
+{
+    const int n = w->n;
+    double stepb, fb;
+    for (;;) {
+        const double u = fabs(pg * lambda * stepc);
+        stepb = 0.5 * stepc * u / ((fc - fa) + u);
+        o_take_step(x, w->p, stepb, lambda, w->x1, w->dx1, n);
+        if (o_vec_equal(x, w->x1, n)) {            /* the trial does not move: give up */
+            *stepOut = 0;
+            *fOut = fa;
+            w->fn->df(w->fn->ctx, w->x1, g);
+            return;
+        }
+        fb = w->fn->f(w->fn->ctx, w->x1);
+        if (fb >= fa && stepb > 0.0) { fc = fb; stepc = stepb; continue; }
+        break;
+    }
+    *stepOut = stepb;
+    *fOut = fb;
+    w->fn->df(w->fn->ctx, w->x1, g);
+}
+
+/* safeguarded parabolic / golden-section search in (stepa, stepc) around stepb; the
+   best point goes to x2 (directional_minimize.c: minimize) */
+static void o_dirmin(OStep* w, const double* x, double lambda, double stepa, double stepb, double stepc,
+                     double fa, double fb, double fc, double* dx2, double* g, double* stepOut, double* fOut,
+                     double* gnormOut)
+{
+    const int n = w->n;
+    double u = stepb, v = stepa, wv = stepc, fu = fb, fv = fa, fw = fc;
+    double old2 = fabs(wv - v), old1 = fabs(v - u);
+    memcpy(w->x2, w->x1, sizeof(double) * n);
+    memcpy(dx2, w->dx1, sizeof(double) * n);
+    *fOut = fb;
+    *stepOut = stepb;
+    *gnormOut = dnrm2(g, n);
+    for (int iter = 1; iter <= 10; ++iter) {
+        const double dw = wv - u, dv = v - u;
+        const double e1 = (fv - fu) * dw * dw + (fu - fw) * dv * dv;
+        const double e2 = 2.0 * ((fv - fu) * dw + (fu - fw) * dv);
+        const double du = e2 != 0.0 ? e1 / e2 : 0.0;
+        double stepm;
+        if (du > 0.0 && du < stepc - stepb && fabs(du) < 0.5 * old2) stepm = u + du;
+        else if (du < 0.0 && du > stepa - stepb && fabs(du) < 0.5 * old2) stepm = u + du;
+        else if (stepc - stepb > stepb - stepa) stepm = 0.38 * (stepc - stepb) + stepb;
+        else stepm = stepb - 0.38 * (stepb - stepa);
+        o_take_step(x, w->p, stepm, lambda, w->x1, w->dx1, n);
+        const double fm = w->fn->f(w->fn->ctx, w->x1);
+        if (fm > fb) {
+            if (fm < fv) { wv = v; v = stepm; fw = fv; fv = fm; }
+            else if (fm < fw) { wv = stepm; fw = fm; }
+            if (stepm < stepb) { stepa = stepm; fa = fm; }
+            else { stepc = stepm; fc = fm; }
+            continue;
+        }
+        old2 = old1;
+        old1 = fabs(u - stepm);
+        wv = v; v = u; u = stepm;
+        fw = fv; fv = fu; fu = fm;
+        memcpy(w->x2, w->x1, sizeof(double) * n);
+        memcpy(dx2, w->dx1, sizeof(double) * n);
+        w->fn->df(w->fn->ctx, w->x1, g);
+        const double pg = o_dot(w->p, g, n), gnorm1 = dnrm2(g, n);
+        *fOut = fm;
+        *stepOut = stepm;
+        *gnormOut = gnorm1;
+        if (fabs(pg * lambda / gnorm1) < w->tol) return;
+        if (stepm < stepb) { stepc = stepb; fc = fb; stepb = stepm; fb = fm; }
+        else { stepa = stepb; fa = fb; stepb = stepm; fb = fm; }
+    }
+}
+
+static int o_bfgs1_iterate(OStep* w, double* x, double* f, double* g, double* dx)
+{
+    const int n = w->n;
+    const double fa = *f, stepc = w->step;
+    double stepb, fb, g1norm;
+    if (w->pnorm == 0.0 || w->g0norm == 0.0) { memset(dx, 0, sizeof(double) * n); return GSL_ENOPROG; }
+    const double pg = o_dot(w->p, g, n), dir = pg >= 0.0 ? +1.0 : -1.0;
+    o_take_step(x, w->p, stepc, dir / w->pnorm, w->x1, dx, n);
+    const double fc = w->fn->f(w->fn->ctx, w->x1);
+    if (fc < fa) {                                     /* the trial step is downhill: take it */
+        w->step = stepc * 2.0;
+        *f = fc;
+        memcpy(x, w->x1, sizeof(double) * n);
+        w->fn->df(w->fn->ctx, w->x1, g);
+        return GSL_SUCCESS;
+    }
+    o_intermediate(w, x, dir / w->pnorm, pg, stepc, fa, fc, g, &stepb, &fb);
+    if (stepb == 0.0) return GSL_ENOPROG;
+    o_dirmin(w, x, dir / w->pnorm, 0.0, stepb, stepc, fa, fb, fc, dx, g, &w->step, f, &g1norm);
+    memcpy(x, w->x2, sizeof(double) * n);
+    w->iter = (w->iter + 1) % n;
+    if (w->iter == 0) {                                 /* restart along the gradient */
+        memcpy(w->p, g, sizeof(double) * n);
+        w->pnorm = g1norm;
+    } else {
+        for (int k = 0; k < n; ++k) { w->dx0[k] = x[k] + (-1.0) * w->x0[k]; w->dg0[k] = g[k] + (-1.0) * w->g0[k]; }
+        const double dxg = o_dot(w->dx0, g, n), dgg = o_dot(w->dg0, g, n), dxdg = o_dot(w->dx0, w->dg0, n);
+        const double dgnorm = dnrm2(w->dg0, n);
+        double A = 0, B = 0;
+        if (dxdg != 0) {
+            B = dxg / dxdg;
+            A = -(1.0 + dgnorm * dgnorm / dxdg) * B + dgg / dxdg;
+        }
+        for (int k = 0; k < n; ++k) w->p[k] = g[k] + (-A) * w->dx0[k] + (-B) * w->dg0[k];
+        w->pnorm = dnrm2(w->p, n);
+    }
+    memcpy(w->g0, g, sizeof(double) * n);
+    memcpy(w->x0, x, sizeof(double) * n);
+    w->g0norm = dnrm2(w->g0, n);
+    return GSL_SUCCESS;
+}
+
+int o_step_iterate(OStep* w, double* x, double* f, double* g, double* dx)
+{
+    return w->kind == 1 ? o_sd_iterate(w, x, f, g, dx) : o_bfgs1_iterate(w, x, f, g, dx);
+}
+
 /* ---------- the driver: Solver::run (solver.cpp:330-580) ---------- */
 
 int o_solve(const OProblem* pr, OResult* res)
@@ -1205,6 +1418,8 @@ int o_solve(const OProblem* pr, OResult* res)
 
     double init_step = 1E-4 * maxWl, init_tol = 1E-6;
     OBfgs2* bfgs = pr->getstep == 4 ? o_bfgs2_new(nFree) : NULL;
+    OStep* other = (pr->getstep == 1 || pr->getstep == 3) ? o_step_new(nFree, pr->getstep) : NULL;
+    if (pr->getstep == 3) { init_step = 1E8; init_tol = 0.01; }   /* solver.cpp:359-360 */
     OFdf fn = { st, cb_f, cb_df, cb_fdf };
     st->gradMethod = 1;
     double previousMetric = 0, thisMetric = 0, gradNorm = 0, tol = 1.0E-1;
@@ -1230,6 +1445,8 @@ int o_solve(const OProblem* pr, OResult* res)
         memset(dx, 0, sizeof(double) * nFree);
         if (bfgs) {
             o_bfgs2_set(bfgs, &fn, x, &f, g, init_step, init_tol);
+        } else if (other) {
+            o_step_set(other, &fn, x, &f, g, init_step, init_tol);
         } else {
             cs.iter = 0; cs.step = init_step; cs.tol = init_tol;
             f = my_f(st, x);
@@ -1237,7 +1454,8 @@ int o_solve(const OProblem* pr, OResult* res)
         int iter = 0, successCount = 0, done = 0;
         while ((!done || iter < pr->minIter) && iter < maxIterations) {
             iter++;
-            status = bfgs ? o_bfgs2_iterate(bfgs, x, &f, g, dx) : cg_iterate(st, &cs, x, &f, g, dx, nFree);
+            status = bfgs ? o_bfgs2_iterate(bfgs, x, &f, g, dx)
+                   : other ? o_step_iterate(other, x, &f, g, dx) : cg_iterate(st, &cs, x, &f, g, dx, nFree);
             if (failCount > maxFails) { status = GSL_FAILURE; exitLoop = 1; done = 1; break; }
             st->gradMethod = 2;
             if (status == GSL_ENOPROG) { failCount++; break; }
@@ -1273,6 +1491,7 @@ int o_solve(const OProblem* pr, OResult* res)
     if (res->img) restore_objects(st, res->img);
     free(x); free(g); free(dx); free(beta0); free(cs.x1); free(cs.p); free(cs.prevG);
     o_bfgs2_free(bfgs);
+    o_step_free(other);
     o_state_free(st);
     return 0;
 }

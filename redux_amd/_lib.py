@@ -15,7 +15,7 @@ MFBD_OK, MFBD_EINVAL, MFBD_ENOMEM, MFBD_EDEVICE, MFBD_ENONFINITE, MFBD_ESTATE = 
 FP64, FP32, MIXED = 0, 1, 2
 GRAD_FD, GRAD_VOGEL, GRAD_ADJOINT = 1, 2, 3
 GRAD_SCHEDULE_REFERENCE = 0
-GETSTEP_CONJUGATE, GETSTEP_BFGS_INV = 2, 4
+GETSTEP_STEEPEST, GETSTEP_CONJUGATE, GETSTEP_BFGS, GETSTEP_BFGS_INV = 1, 2, 3, 4
 BASIS_ZERNIKE, BASIS_KL = 0, 1
 
 

@@ -570,6 +570,12 @@ struct PatchRun {
         double fa = 0, dfa = 0, xKey = 0, fKey = 0, gKey = 0, dfKey = 0;
         double step = 0, g0norm = 0, pnorm = 0, deltaF = 0, fp0 = 0, sigma = 0;
     } bf;
+    // steepest_descent / vector_bfgs (GETSTEP = steepest_descent / BFGS)
+    struct Step1 {
+        int iter = 0;
+        double step = 0, tol = 0, pnorm = 0, g0norm = 0;
+        std::vector<double> x1, g1, dx1, x2, p, x0, g0, dx0, dg0;
+    } sd;
 };
 
 struct EvalAwait {
@@ -1050,6 +1056,193 @@ Task<int> bfgs2_iterate(PatchRun& r, double& f) {
     co_return ST_SUCCESS;
 }
 
+// ---- GETSTEP = steepest_descent and BFGS: GSL 1.16 steepest_descent and vector_bfgs
+// (solver.cpp:355-361, 369-372), their published algorithms restated (multimin/
+// steepest_descent.c, vector_bfgs.c, directional_minimize.c).  Steepest descent tries
+// x - step g/|g|, shrinking the step by tol while f rises, then doubles it (or, after a
+// shrink, shrinks it once more).  vector_bfgs tries the current step along the downhill
+// direction; if f does not fall it finds a lower point by a parabolic guess from the slope
+// (halving towards 0), then refines it by a safeguarded parabolic / golden-section search
+// of at most 10 trials that stops when |p.g lambda/|g|| < tol; the next direction is the
+// BFGS update p' = g1 - A dx - B dg, restarted along g every n iterations.
+Task<int> step1_set(PatchRun& r, double& f, double stepSize, double tol, bool bfgs) {
+    auto& w = r.sd;
+    const size_t n = r.x.size();
+    for (auto* v : {&w.x1, &w.g1, &w.dx1, &w.x2, &w.p, &w.x0, &w.g0, &w.dx0, &w.dg0}) v->assign(n, 0.0);
+    w.iter = 0;
+    w.step = stepSize;
+    w.tol = tol;
+    f = co_await my_fdf(r, r.x.data(), r.g.data());
+    if (bfgs) {
+        w.x0 = r.x;
+        w.p = r.g;
+        w.g0 = r.g;
+        w.pnorm = w.g0norm = dnrm2(r.g);
+    }
+    co_return ST_SUCCESS;
+}
+
+Task<int> sd_iterate(PatchRun& r, double& f) {
+    auto& w = r.sd;
+    const size_t n = r.x.size();
+    const double f0 = f, gnorm = dnrm2(r.g);
+    double step = w.step, f1 = 0;
+    bool shrunk = false;
+    if (gnorm == 0.0) {
+        std::fill(r.dx.begin(), r.dx.end(), 0.0);
+        co_return ST_ENOPROG;
+    }
+    for (;;) {
+        for (size_t k = 0; k < n; ++k) r.dx[k] = 0.0 + (-step / gnorm) * r.g[k];
+        for (size_t k = 0; k < n; ++k) w.x1[k] = r.x[k] + 1.0 * r.dx[k];
+        if (w.x1 == r.x) co_return ST_ENOPROG;
+        f1 = co_await my_fdf(r, w.x1.data(), w.g1.data());
+        if (f1 > f0) {
+            shrunk = true;
+            step *= w.tol;
+            continue;
+        }
+        break;
+    }
+    w.step = shrunk ? step * w.tol : step * 2.0;
+    r.x = w.x1;
+    r.g = w.g1;
+    f = f1;
+    co_return ST_SUCCESS;
+}
+
+void take_step(const std::vector<double>& x, const std::vector<double>& p, double step, double lambda,
+               std::vector<double>& x1, std::vector<double>& dx) {
+    for (size_t k = 0; k < x.size(); ++k) dx[k] = 0.0 + (-step * lambda) * p[k];
+    for (size_t k = 0; k < x.size(); ++k) x1[k] = x[k] + 1.0 * dx[k];
+}
+
+// a point in (0, stepc) below fa (directional_minimize.c: intermediate_point); the
+// gradient at it goes to r.g.  Returns (step, f), step 0 if the trial does not move x.
+Task<std::pair<double, double>> intermediate_point(PatchRun& r, double lambda, double pg, double stepc, double fa,
+                                                   double fc) {
+    auto& w = r.sd;
+    double stepb, fb;
+    for (;;) {
+        const double u = std::fabs(pg * lambda * stepc);
+        stepb = 0.5 * stepc * u / ((fc - fa) + u);
+        take_step(r.x, w.p, stepb, lambda, w.x1, w.dx1);
+        if (w.x1 == r.x) {
+            co_await my_df(r, w.x1.data(), r.g.data());
+            co_return std::make_pair(0.0, fa);
+        }
+        fb = co_await my_f(r, w.x1.data());
+        if (fb >= fa && stepb > 0.0) {
+            fc = fb;
+            stepc = stepb;
+            continue;
+        }
+        break;
+    }
+    co_await my_df(r, w.x1.data(), r.g.data());
+    co_return std::make_pair(stepb, fb);
+}
+
+// directional_minimize.c: minimize; the best point in w.x2 (its step in r.dx), f and the
+// gradient norm there out, w.step set to its step
+Task<std::pair<double, double>> dir_minimize(PatchRun& r, double lambda, double stepa, double stepb, double stepc,
+                                             double fa, double fb, double fc) {
+    auto& w = r.sd;
+    const size_t n = r.x.size();
+    double u = stepb, v = stepa, ww = stepc, fu = fb, fv = fa, fw = fc;
+    double old2 = std::fabs(ww - v), old1 = std::fabs(v - u);
+    w.x2 = w.x1;
+    r.dx = w.dx1;
+    double fOut = fb, gnorm = dnrm2(r.g);
+    w.step = stepb;
+    for (int iter = 1; iter <= 10; ++iter) {
+        const double dw = ww - u, dv = v - u;
+        const double e1 = (fv - fu) * dw * dw + (fu - fw) * dv * dv;
+        const double e2 = 2.0 * ((fv - fu) * dw + (fu - fw) * dv);
+        const double du = e2 != 0.0 ? e1 / e2 : 0.0;
+        double stepm;
+        if (du > 0.0 && du < stepc - stepb && std::fabs(du) < 0.5 * old2) stepm = u + du;
+        else if (du < 0.0 && du > stepa - stepb && std::fabs(du) < 0.5 * old2) stepm = u + du;
+        else if (stepc - stepb > stepb - stepa) stepm = 0.38 * (stepc - stepb) + stepb;
+        else stepm = stepb - 0.38 * (stepb - stepa);
+        take_step(r.x, w.p, stepm, lambda, w.x1, w.dx1);
+        const double fm = co_await my_f(r, w.x1.data());
+        if (fm > fb) {
+            if (fm < fv) { ww = v; v = stepm; fw = fv; fv = fm; }
+            else if (fm < fw) { ww = stepm; fw = fm; }
+            if (stepm < stepb) { stepa = stepm; fa = fm; }
+            else { stepc = stepm; fc = fm; }
+            continue;
+        }
+        old2 = old1;
+        old1 = std::fabs(u - stepm);
+        ww = v; v = u; u = stepm;
+        fw = fv; fv = fu; fu = fm;
+        w.x2 = w.x1;
+        r.dx = w.dx1;
+        co_await my_df(r, w.x1.data(), r.g.data());
+        double pg = 0;
+        for (size_t k = 0; k < n; ++k) pg += w.p[k] * r.g[k];
+        const double gnorm1 = dnrm2(r.g);
+        fOut = fm;
+        w.step = stepm;
+        gnorm = gnorm1;
+        if (std::fabs(pg * lambda / gnorm1) < w.tol) break;
+        if (stepm < stepb) { stepc = stepb; fc = fb; stepb = stepm; fb = fm; }
+        else { stepa = stepb; fa = fb; stepb = stepm; fb = fm; }
+    }
+    co_return std::make_pair(fOut, gnorm);
+}
+
+Task<int> bfgs1_iterate(PatchRun& r, double& f) {
+    auto& w = r.sd;
+    const size_t n = r.x.size();
+    const double fa = f, stepc = w.step;
+    if (w.pnorm == 0.0 || w.g0norm == 0.0) {
+        std::fill(r.dx.begin(), r.dx.end(), 0.0);
+        co_return ST_ENOPROG;
+    }
+    double pg = 0;
+    for (size_t k = 0; k < n; ++k) pg += w.p[k] * r.g[k];
+    const double dir = pg >= 0.0 ? +1.0 : -1.0, lambda = dir / w.pnorm;
+    take_step(r.x, w.p, stepc, lambda, w.x1, r.dx);
+    const double fc = co_await my_f(r, w.x1.data());
+    if (fc < fa) {                                       // the trial step is downhill: take it
+        w.step = stepc * 2.0;
+        f = fc;
+        r.x = w.x1;
+        co_await my_df(r, w.x1.data(), r.g.data());
+        co_return ST_SUCCESS;
+    }
+    const auto [stepb, fb] = co_await intermediate_point(r, lambda, pg, stepc, fa, fc);
+    if (stepb == 0.0) co_return ST_ENOPROG;
+    const auto [fm, g1norm] = co_await dir_minimize(r, lambda, 0.0, stepb, stepc, fa, fb, fc);
+    f = fm;
+    r.x = w.x2;
+    w.iter = (w.iter + 1) % (int)n;
+    if (w.iter == 0) {                                   // restart along the gradient
+        w.p = r.g;
+        w.pnorm = g1norm;
+    } else {
+        for (size_t k = 0; k < n; ++k) {
+            w.dx0[k] = r.x[k] + (-1.0) * w.x0[k];
+            w.dg0[k] = r.g[k] + (-1.0) * w.g0[k];
+        }
+        const double dxg = dotv(w.dx0, r.g), dgg = dotv(w.dg0, r.g), dxdg = dotv(w.dx0, w.dg0), dgnorm = dnrm2(w.dg0);
+        double A = 0, B = 0;
+        if (dxdg != 0) {
+            B = dxg / dxdg;
+            A = -(1.0 + dgnorm * dgnorm / dxdg) * B + dgg / dxdg;
+        }
+        for (size_t k = 0; k < n; ++k) w.p[k] = r.g[k] + (-A) * w.dx0[k] + (-B) * w.dg0[k];
+        w.pnorm = dnrm2(w.p);
+    }
+    w.g0 = r.g;
+    w.x0 = r.x;
+    w.g0norm = dnrm2(w.g0);
+    co_return ST_SUCCESS;
+}
+
 // Solver::run (solver.cpp:330-580), starting after the reset initialisation
 Task<int> run_patch(PatchRun& r) {
     mfbd_ctx* c = r.c;
@@ -1057,6 +1250,7 @@ Task<int> run_patch(PatchRun& r) {
     const int K = c->g.K, nModes = r.K;
     const bool adjoint = pr.gradient == MFBD_GRAD_ADJOINT;
     const bool bfgs = pr.getstep == MFBD_GETSTEP_BFGS_INV;
+    const bool sdesc = pr.getstep == MFBD_GETSTEP_STEEPEST, bfgs1 = pr.getstep == MFBD_GETSTEP_BFGS;
     double maxWl = 0;                                   // init_step = 1e-4 lambda_max (solver.cpp:351)
     for (int o = 0; o < c->g.nObj; ++o) maxWl = std::max(maxWl, c->st.wavelength[o]);
     const double initStep = 1E-4 * maxWl;
@@ -1088,12 +1282,16 @@ Task<int> run_patch(PatchRun& r) {
         r.iterCG = 0;
         double f = 0;
         if (bfgs) co_await bfgs2_set(r, f, initStep, 1E-6);
+        else if (sdesc) co_await step1_set(r, f, initStep, 1E-6, false);
+        else if (bfgs1) co_await step1_set(r, f, 1E8, 0.01, true);      // solver.cpp:359-360
         else f = co_await my_f(r, r.x.data());
         int iter = 0, successCount = 0;
         bool done = false;
         while ((!done || iter < pr.minIter) && iter < maxIterations) {
             iter++;
-            status = bfgs ? co_await bfgs2_iterate(r, f) : co_await cg_iterate(r, f);
+            status = bfgs ? co_await bfgs2_iterate(r, f)
+                     : sdesc ? co_await sd_iterate(r, f)
+                     : bfgs1 ? co_await bfgs1_iterate(r, f) : co_await cg_iterate(r, f);
             if (failCount > maxFails) { status = ST_FAILURE; exitLoop = true; done = true; break; }
             if (!adjoint) r.gradMethod = MFBD_GRAD_VOGEL;   // Vogel from here on (solver.cpp:478)
             if (status == ST_ENOPROG) { failCount++; break; }
@@ -1196,7 +1394,7 @@ int mfbd_ctx_create(const mfbd_geometry* geo, const mfbd_params* prm, int device
     }
     // 0 (unset) falls to the conjugate gradient, as the reference's switch default does
     if (c->prm.getstep == 0) c->prm.getstep = MFBD_GETSTEP_CONJUGATE;
-    if (c->prm.getstep != MFBD_GETSTEP_CONJUGATE && c->prm.getstep != MFBD_GETSTEP_BFGS_INV) {
+    if (c->prm.getstep < MFBD_GETSTEP_STEEPEST || c->prm.getstep > MFBD_GETSTEP_BFGS_INV) {
         delete c;
         return MFBD_EINVAL;
     }

@@ -30,8 +30,11 @@ def default_params(**kw):
 
 
 # getstepMap (config.cpp:102-114), the steppers built here
-GETSTEP = {"conjugate": L.GETSTEP_CONJUGATE, "conjugate_gradient": L.GETSTEP_CONJUGATE,
-           "getstep_conjugate_gradient": L.GETSTEP_CONJUGATE, "bfgs_inv": L.GETSTEP_BFGS_INV,
+GETSTEP = {"steepest": L.GETSTEP_STEEPEST, "steepest_descent": L.GETSTEP_STEEPEST,
+           "getstep_steepest_descent": L.GETSTEP_STEEPEST,
+           "conjugate": L.GETSTEP_CONJUGATE, "conjugate_gradient": L.GETSTEP_CONJUGATE,
+           "getstep_conjugate_gradient": L.GETSTEP_CONJUGATE, "bfgs": L.GETSTEP_BFGS,
+           "getstep_bfgs": L.GETSTEP_BFGS, "bfgs_inv": L.GETSTEP_BFGS_INV,
            "bfgsinv": L.GETSTEP_BFGS_INV, "getstep_bfgs_inv": L.GETSTEP_BFGS_INV}
 
 

@@ -60,6 +60,8 @@ def test_header_constants_match_binding():
              "MFBD_FP64": L.FP64, "MFBD_FP32": L.FP32, "MFBD_MIXED": L.MIXED, "MFBD_GRAD_FD": L.GRAD_FD,
              "MFBD_GRAD_VOGEL": L.GRAD_VOGEL, "MFBD_GRAD_ADJOINT": L.GRAD_ADJOINT,
              "MFBD_GRAD_SCHEDULE_REFERENCE": L.GRAD_SCHEDULE_REFERENCE, "MFBD_BASIS_ZERNIKE": L.BASIS_ZERNIKE,
-             "MFBD_BASIS_KL": L.BASIS_KL}
+             "MFBD_BASIS_KL": L.BASIS_KL, "MFBD_GETSTEP_STEEPEST": L.GETSTEP_STEEPEST,
+             "MFBD_GETSTEP_CONJUGATE": L.GETSTEP_CONJUGATE, "MFBD_GETSTEP_BFGS": L.GETSTEP_BFGS,
+             "MFBD_GETSTEP_BFGS_INV": L.GETSTEP_BFGS_INV}
     for name, val in pairs.items():
         assert int(defs[name]) == val, name

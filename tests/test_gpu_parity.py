@@ -202,3 +202,35 @@ def test_bfgs_inv_with_the_adjoint_gradient_converges(golden):
     assert abs(r["finalMetric"] - rc["finalMetric"]) <= 2e-2 * rc["finalMetric"], (r["finalMetric"], rc["finalMetric"])
     for b in out[3]:
         assert np.array_equal(b["alpha"], r["alpha"]) and np.array_equal(b["img"], r["img"])
+
+
+@pytest.mark.parametrize("getstep", ["steepest_descent", "bfgs"])
+def test_other_steppers_against_the_oracle(golden, getstep):
+    """GETSTEP = steepest_descent and BFGS (GSL steepest_descent and vector_bfgs, restated in
+    both the driver and the oracle) with the reference's gradient schedule: the same initial
+    metric, both converged or both not, iteration counts within one and final metrics
+    within 1e-6 of each other (measured: 2e-13 for steepest descent, 1e-9 for BFGS, whose
+    last stage ends one trial apart; rounding of the metric decides between close trials);
+    batched and one-at-a-time solves are bit-identical."""
+    pb, z = golden
+    from oracle import oracle as O
+    from redux_amd.momfbd import Solver
+    p = dict(pb)
+    p["settings"] = dict(pb["settings"], GETSTEP=getstep)
+    ro = O.solve(p, restore=False)
+    out = {}
+    for batch in (1, 2):
+        sv = Solver(p, precision="fp64", max_batch=batch)
+        sv.load([p] * batch)
+        out[batch] = sv.run()
+        sv.close()
+    r = out[1][0]
+    print(getstep, "gpu", r["status"], r["iterations"], r["finalMetric"], "oracle", ro["status"],
+          ro["iterations"], ro["finalMetric"], "initial", ro["initialMetric"], flush=True)
+    assert abs(r["initialMetric"] - ro["initialMetric"]) <= 1e-12 * ro["initialMetric"]
+    assert (r["status"] == 0) == (ro["status"] == 0)
+    assert r["finalMetric"] < r["initialMetric"] and ro["finalMetric"] < ro["initialMetric"]
+    assert abs(r["iterations"] - ro["iterations"]) <= 1
+    assert abs(r["finalMetric"] - ro["finalMetric"]) <= 1e-6 * ro["finalMetric"]
+    for b in out[2]:
+        assert np.array_equal(b["alpha"], r["alpha"]) and np.array_equal(b["img"], r["img"])

@@ -1159,31 +1159,32 @@ __device__ __forceinline__ void tile_fft2(cx<T>* t, const cx<T>* __restrict__ tw
 //   5. inverse column FFTs of the N/2+1 half-spectrum columns; the 2-D result is
 //      Hermitian, so these columns hold every stored support point (hread: index into
 //      the half array, -1-index for a point read as the conjugate of its mirror).
+// The per-image body of k_otf_fused: S of image q from its phases `ph` (this thread's
+// pupil points); a separate function, so that the persistent loop around it does not keep
+// loop-invariant loads in registers
+template <int IP>
+struct PhaseSet {               // one thread's pupil-point phases of an image, by value
+    float v[IP];
+};
+
 template <int N, int NT>
-__global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, const float* __restrict__ phi,
-                                                  const int* __restrict__ invPs, const float* __restrict__ pupil,
-                                                  const int* __restrict__ qImg, const float* __restrict__ imgScale,
-                                                  const int* __restrict__ hread, const cx<float>* __restrict__ tw,
-                                                  int nPS, int nOS) {
+__device__ __noinline__ void otf_image(cx<float>* __restrict__ S, PhaseSet<(N / 2 * N / 2 + NT - 1) / NT> ph,
+                                       const int* __restrict__ invPs,
+                                       const float* __restrict__ pupil, float sc, const int* __restrict__ hread,
+                                       const cx<float>* tw, cx<float>* t, cx<float>* hs, int nOS, int q) {
     constexpr int P = N / 2, LD = Tile<N>::LD, HC = N / 2 + 1;
     constexpr int RF = N == 64 ? 8 : 16;                     // radix of the first pass (fft_lines)
     constexpr int RL = N == 256 ? 16 : 8;                    // radix of the last pass
-    extern __shared__ cx<float> lds_dyn[];
-    cx<float>* t = lds_dyn;                                  // N x LD tile (natural layout)
-    cx<float>* hs = lds_dyn + (N / 2) * LD;                  // N x HC half spectra, element (y, k) at y*HC + k
-    const int q = blockIdx.x;
-    const float sc = imgScale[qImg[q]];
-    if (sc == 0.f) {            // an image the patch lacks: its OTF is zero
-        for (int u = threadIdx.x; u < nOS; u += NT) S[(size_t)q * nOS + u] = {0.f, 0.f};
-        return;
-    }
-    const float* ph = phi + (size_t)q * nPS;
-    for (int e = threadIdx.x; e < P * P; e += NT) {
+    constexpr int IP = (P * P + NT - 1) / NT;
+#pragma unroll
+    for (int i = 0; i < IP; ++i) {
+        const int e = threadIdx.x + i * NT;
+        if (e >= P * P) continue;
         const int k = invPs[e];
         cx<float> val = {0.f, 0.f};
         if (k >= 0) {
             float sn, cs;
-            sincosf(ph[k], &sn, &cs);
+            sincosf(ph.v[i], &sn, &cs);
             const float m = pupil[k] * sc;
             val = {m * cs, m * sn};
         }
@@ -1213,6 +1214,45 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
         const int code = hread[u];
         const cx<float> v = hs[code >= 0 ? code : -1 - code];
         Sq[u] = {v.x * inv, (code >= 0 ? v.y : -v.y) * inv};
+    }
+}
+
+template <int N, int NT>
+__global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, const float* __restrict__ phi,
+                                                  const int* __restrict__ invPs, const float* __restrict__ pupil,
+                                                  const int* __restrict__ qImg, const float* __restrict__ imgScale,
+                                                  const int* __restrict__ hread, const cx<float>* __restrict__ twg,
+                                                  int nPS, int nOS, int nQ) {
+    constexpr int P = N / 2, LD = Tile<N>::LD, HC = N / 2 + 1;
+    constexpr int IP = (P * P + NT - 1) / NT;                // pupil-grid points per thread
+    extern __shared__ cx<float> lds_dyn[];
+    cx<float>* t = lds_dyn;                                  // N x LD tile (natural layout)
+    cx<float>* hs = lds_dyn + (N / 2) * LD;                  // N x HC half spectra, element (y, k) at y*HC + k
+    cx<float>* tw = hs + N * HC;                             // twiddles
+    for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
+    // persistent: the next image's phases load while the current one is transformed
+    float ph[IP];
+    auto fetch = [&](int qq) {
+#pragma unroll
+        for (int i = 0; i < IP; ++i) {
+            const int e = threadIdx.x + i * NT;
+            const int k = e < P * P ? invPs[e] : -1;
+            ph[i] = (qq < nQ && k >= 0) ? phi[(size_t)qq * nPS + k] : 0.f;
+        }
+    };
+    fetch(blockIdx.x);
+    for (int q = blockIdx.x; q < nQ; q += gridDim.x) {
+        const float sc = imgScale[qImg[q]];
+        PhaseSet<IP> cur;
+#pragma unroll
+        for (int i = 0; i < IP; ++i) cur.v[i] = ph[i];
+        fetch(q + gridDim.x);
+        if (sc == 0.f) {        // an image the patch lacks: its OTF is zero
+            for (int u = threadIdx.x; u < nOS; u += NT) S[(size_t)q * nOS + u] = {0.f, 0.f};
+            continue;
+        }
+        otf_image<N, NT>(S, cur, invPs, pupil, sc, hread, tw, t, hs, nOS, q);
+        __syncthreads();        // the next image's field overwrites the half spectra
     }
 }
 
@@ -1987,7 +2027,7 @@ public:
     static constexpr bool FUSED = std::is_same_v<T, float> && N <= 128;
     static constexpr int FNT = N == 128 ? 1024 : 256;
     static constexpr size_t FLDS = (size_t)Tile<N>::SIZE * sizeof(cx<float>);
-    static constexpr size_t OLDS = ((size_t)(N / 2) * Tile<N>::LD + (size_t)N * (N / 2 + 1)) * sizeof(cx<float>);
+    static constexpr size_t OLDS = ((size_t)(N / 2) * Tile<N>::LD + (size_t)N * (N / 2 + 1) + N) * sizeof(cx<float>);
     // FP64 OTF in one kernel with an N/2 x (N+1) complex-double LDS buffer per image
     static constexpr bool FUSED64 = std::is_same_v<T, double> && N <= 128;
     static constexpr int F64NT = N == 128 ? 512 : 256;
@@ -2173,6 +2213,11 @@ public:
         if constexpr (FUSED) {
             HIPCHK(hipFuncSetAttribute((const void*)k_otf_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)OLDS));
+            int nCU = 0, dev = 0, perCU = 0;
+            HIPCHK(hipGetDevice(&dev));
+            HIPCHK(hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
+            HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k_otf_fused<N, FNT>, FNT, OLDS));
+            otfGrid_ = std::max(nCU * std::max(perCU, 1), 1);   // persistent workgroups: all that fit at once
             HIPCHK(hipFuncSetAttribute((const void*)k_vogel_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)FLDS));
         }
@@ -2770,8 +2815,8 @@ private:
     void otf_chain(const T* phi, const int* qImg, int nQ, C* Sout) {
         if constexpr (FUSED) {
             launch("k_otf_fused", nQ, [&] {
-                hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(nQ), dim3(FNT), OLDS, st_, Sout, phi, invPs_.p,
-                                   pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_);
+                hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(std::min(nQ, otfGrid_)), dim3(FNT), OLDS, st_, Sout, phi,
+                                   invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_, nQ);
             });
             return;
         }

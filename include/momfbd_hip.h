@@ -187,9 +187,13 @@ int mfbd_mosaic(int nPatches, int patchSize, const float* patches, const int* po
 /* ANA (.f0) file, uncompressed (Ana::write, fileana.cpp:150-225, 498-560): datyp 0..4 =
  * int8, int16, int32, float32, float64; dims fastest first (columns, rows, ...); text is
  * the header text (NULL for none; more than 255 characters take further 512-byte blocks,
- * up to 7935).  The reference compresses 2-D int16 data when that is smaller; this writer
- * stores every type uncompressed, which ANA readers accept alike. */
+ * up to 7935).  mfbd_write_ana_compressed writes the compressed form. */
 int mfbd_write_ana(const char* path, int datyp, int ndim, const int* dims, const void* data, const char* text);
+/* ANA file of 2-D int16 (datyp 1) or int32 (datyp 2) data [ny][nx] with the crunch
+ * compression at bit slice `slice` (Ana::write with compression, fileana.cpp:498-560;
+ * anacrunch / anacrunch32, anacompress.cpp), stored raw when that is not smaller. */
+int mfbd_write_ana_compressed(const char* path, int datyp, int nx, int ny, const void* data, const char* text,
+                              int slice);
 /* FITS primary image (Fits::write, filefits.cpp:1756-1818): BITPIX 8, 16, 32, 64, -32 or
  * -64; dims fastest first (NAXIS1 = columns); data in host order, written big-endian. */
 int mfbd_write_fits(const char* path, int bitpix, int ndim, const int* dims, const void* data);

@@ -61,6 +61,7 @@ SIGNATURES = {
                                    C.POINTER(dp), C.POINTER(fp)]),
     "mfbd_write_ana": (C.c_int, [C.c_char_p, C.c_int, C.c_int, ip, C.c_void_p, C.c_char_p]),
     "mfbd_write_fits": (C.c_int, [C.c_char_p, C.c_int, C.c_int, ip, C.c_void_p]),
+    "mfbd_write_ana_compressed": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_char_p, C.c_int]),
     "mfbd_file_last_error": (C.c_char_p, []),
     "mfbd_mosaic_size": (C.c_int, [C.c_int, C.c_int, ip, ip, ip, ip]),
     "mfbd_mosaic": (C.c_int, [C.c_int, C.c_int, fp, ip, ip, C.c_int, C.c_int, fp, ip, ip, ip, ip]),

@@ -3,7 +3,9 @@
 // 1212-1334), written without CFITSIO or the ANA library.
 //   ANA:  Ana::write (src/lib/file/fileana.cpp:150-225, 498-560; header layout
 //         include/redux/file/fileana.hpp:70-82): a 512-byte first block, little-endian,
-//         the header text in (nhb - 1) * 512 + 256 bytes, then the data, uncompressed.
+//         the header text in (nhb - 1) * 512 + 256 bytes, then the data: raw, or for 2-D
+//         int16 / int32 data the "crunch" code (anacrunch / anacrunch32,
+//         src/lib/file/anacompress.cpp:757-920 and after) behind a 14-byte block header.
 //   FITS: Fits::write (src/lib/file/filefits.cpp:1756-1818) through fits_create_img /
 //         fits_write_img: a primary HDU of 80-character cards in 2880-byte records, the
 //         data big-endian and padded to a whole record.
@@ -40,6 +42,61 @@ std::string card(const std::string& key, const std::string& value, const std::st
     std::string c(buf);
     c.resize(80, ' ');
     return c;
+}
+
+// ANA crunch code, one row (block) at a time: the first value raw (16 or 32 bits, little
+// endian), then per element the difference d from its left neighbour as the low `slice`
+// bits of d followed by r = d >> slice in a unary code: "1" for r = 0, else v zero bits
+// and a one for v = 2r (r > 0) or -2r - 1 (r < 0); v >= 31 escapes as 31 zeros, a one and
+// d itself in 17 (int16) or 33 (int32) bits.  Bits fill bytes from the least significant
+// end; every row ends on a byte boundary.
+class BitSink {
+public:
+    std::vector<uint8_t> b;
+    size_t bits = 0;
+    void put(uint64_t v, int n) {            // the n low bits of v
+        for (int i = 0; i < n; ++i, ++bits) {
+            if ((bits >> 3) >= b.size()) b.push_back(0);
+            if ((v >> i) & 1) b[bits >> 3] |= (uint8_t)(1u << (bits & 7));
+        }
+    }
+    void zeros(int n) { put(0, n); }
+    void align() {
+        bits = (bits + 7) & ~(size_t)7;
+        b.resize(bits >> 3, 0);
+    }
+};
+
+template <typename I>
+std::vector<uint8_t> ana_crunch(const I* a, int nx, int ny, int slice) {
+    constexpr bool wide = sizeof(I) == 4;
+    BitSink s;
+    const int64_t mask = ((int64_t)1 << slice) - 1;
+    for (int y = 0; y < ny; ++y) {
+        const I* row = a + (size_t)y * nx;
+        s.put((uint64_t)(wide ? (uint32_t)row[0] : (uint16_t)row[0]), wide ? 32 : 16);
+        for (int x = 1; x < nx; ++x) {
+            const int64_t d = (int64_t)row[x] - (int64_t)row[x - 1];
+            s.put((uint64_t)(d & mask), slice);
+            int64_t r = d >> slice;            // arithmetic shift: floor(d / 2^slice)
+            if (r == 0) {
+                s.put(1, 1);
+                continue;
+            }
+            r = 2 * r;
+            if (r < 0) r = -r - 1;
+            if (r < 31) {
+                s.zeros((int)r);
+                s.put(1, 1);
+            } else {                           // escape: the whole difference follows
+                s.zeros(31);
+                s.put(1, 1);
+                s.put((uint64_t)d & (wide ? 0x1ffffffffULL : 0x1ffffULL), wide ? 33 : 17);
+            }
+        }
+        s.align();
+    }
+    return s.b;
 }
 
 }  // namespace
@@ -85,6 +142,53 @@ int mfbd_write_ana(const char* path, int datyp, int ndim, const int* dims, const
               std::fwrite(data, eb, n, f) == n;
     ok &= std::fclose(f) == 0;
     return ok ? MFBD_OK : file_fail(std::string("mfbd_write_ana: write failed: ") + path);
+}
+
+int mfbd_write_ana_compressed(const char* path, int datyp, int nx, int ny, const void* data, const char* text,
+                              int slice) {
+    if (!path || !data) return file_fail("mfbd_write_ana_compressed: null argument");
+    if (datyp != 1 && datyp != 2) return file_fail("mfbd_write_ana_compressed: int16 or int32 data only");
+    if (nx < 1 || ny < 1) return file_fail("mfbd_write_ana_compressed: dimension < 1");
+    if (slice < 0 || slice >= (datyp == 1 ? 16 : 32)) return file_fail("mfbd_write_ana_compressed: slice out of range");
+    const std::vector<uint8_t> c = datyp == 1 ? ana_crunch(static_cast<const int16_t*>(data), nx, ny, slice)
+                                              : ana_crunch(static_cast<const int32_t*>(data), nx, ny, slice);
+    const int dims[2] = {nx, ny};
+    const size_t raw = (size_t)nx * ny * (datyp == 1 ? 2 : 4);
+    if (c.size() + 14 >= raw) return mfbd_write_ana(path, datyp, 2, dims, data, text);   // stored raw, as Ana::write
+    // header as mfbd_write_ana, with subf bit 0 and cbytes = the compressed size
+    std::string txt = text ? text : "";
+    uint8_t nhb = 1;
+    if (txt.size() > 255) {
+        nhb = (uint8_t)std::min<size_t>(16, 1 + (txt.size() + 256) / 512);
+        if (nhb == 16) txt.resize(std::min<size_t>(txt.size(), 7935));
+    }
+    unsigned char head[256] = {};
+    const uint32_t magic = 0x5555aaaaU, tsize = (uint32_t)(c.size() + 14);
+    std::memcpy(head, &magic, 4);
+    head[4] = 1;
+    head[6] = nhb;
+    head[7] = (unsigned char)datyp;
+    head[8] = 2;
+    std::memcpy(head + 10, &tsize, 4);
+    for (int i = 0; i < 2; ++i) {
+        const uint32_t d = (uint32_t)dims[i];
+        std::memcpy(head + 192 + 4 * i, &d, 4);
+    }
+    std::vector<char> textBlock((size_t)(nhb - 1) * 512 + 256, 0);
+    std::memcpy(textBlock.data(), txt.data(), std::min(textBlock.size(), txt.size()));
+    unsigned char ch[14] = {};
+    const uint32_t nblocks = (uint32_t)ny, bsize = (uint32_t)nx;
+    std::memcpy(ch, &tsize, 4);
+    std::memcpy(ch + 4, &nblocks, 4);
+    std::memcpy(ch + 8, &bsize, 4);
+    ch[12] = (unsigned char)slice;
+    ch[13] = datyp == 1 ? 0 : 4;                 // crunch type: 16-bit 0, 32-bit 4
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return file_fail(std::string("mfbd_write_ana_compressed: cannot open ") + path);
+    bool ok = std::fwrite(head, 1, 256, f) == 256 && std::fwrite(textBlock.data(), 1, textBlock.size(), f) == textBlock.size() &&
+              std::fwrite(ch, 1, 14, f) == 14 && std::fwrite(c.data(), 1, c.size(), f) == c.size();
+    ok &= std::fclose(f) == 0;
+    return ok ? MFBD_OK : file_fail(std::string("mfbd_write_ana_compressed: write failed: ") + path);
 }
 
 int mfbd_write_fits(const char* path, int bitpix, int ndim, const int* dims, const void* data) {

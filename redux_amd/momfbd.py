@@ -252,11 +252,21 @@ _FITS_BITPIX = {np.dtype(np.uint8): 8, np.dtype(np.int16): 16, np.dtype(np.int32
                 np.dtype(np.float32): -32, np.dtype(np.float64): -64}
 
 
-def write_ana(path, array, text=""):
-    """Ana::write of an array (row-major, last axis fastest), uncompressed."""
+def write_ana(path, array, text="", slice=None):
+    """Ana::write of an array (row-major, last axis fastest): uncompressed, or with `slice`
+    (2-D int16 / int32) the crunch compression at that bit slice."""
     a = np.ascontiguousarray(array)
     if a.dtype not in _ANA_TYPES:
         raise SolverError(f"ANA has no type for {a.dtype}")
+    if slice is not None:
+        if a.ndim != 2 or a.dtype not in (np.dtype(np.int16), np.dtype(np.int32)):
+            raise SolverError("ANA compression takes 2-D int16 or int32 data")
+        rc = L.lib().mfbd_write_ana_compressed(str(path).encode(), _ANA_TYPES[a.dtype], a.shape[1], a.shape[0],
+                                               a.ctypes.data_as(C.c_void_p), text.encode() if text else None,
+                                               int(slice))
+        if rc:
+            raise SolverError(f"write_ana failed ({rc}): {L.lib().mfbd_file_last_error().decode()}")
+        return
     dims = np.ascontiguousarray(a.shape[::-1], dtype=np.int32)
     rc = L.lib().mfbd_write_ana(str(path).encode(), _ANA_TYPES[a.dtype], a.ndim, ptr(dims, ip),
                                 a.ctypes.data_as(C.c_void_p), text.encode() if text else None)
@@ -279,13 +289,17 @@ def write_fits(path, array):
 def write_object(base, mosaic_img, fmt="fits", as_int16=False, alpha=None, text=""):
     """Object::writeFits / writeAna (object.cpp:1212-1334): the mosaic to `base`.fits or
     `base`.f0 (float32, or int16 as the reference's default output type, converted by
-    truncation), and with `alpha` ([patchesY][patchesX][images][modes]) the alphas as
-    float32 to `base`.alpha.fits / .alpha.f0."""
+    truncation; int16 ANA output crunch-compressed at slice 5 as the reference writes it),
+    and with `alpha` ([patchesY][patchesX][images][modes]) the alphas as float32 to
+    `base`.alpha.fits / .alpha.f0."""
     img = np.asarray(mosaic_img, dtype=np.float32)
     if as_int16:
         img = img.astype(np.int16)
     ext = ".fits" if fmt == "fits" else ".f0"
-    (write_fits if fmt == "fits" else lambda p, a: write_ana(p, a, text))(str(base) + ext, img)
+    if fmt == "fits":
+        write_fits(str(base) + ext, img)
+    else:
+        write_ana(str(base) + ext, img, text, slice=5 if as_int16 else None)
     if alpha is not None:
         a = np.asarray(alpha, dtype=np.float32)
         (write_fits if fmt == "fits" else lambda p, x: write_ana(p, x))(str(base) + ".alpha" + ext, a)

@@ -97,9 +97,10 @@ def test_write_object_files(tmp_path):
     _, ab, _ = read_fits(tmp_path / "obj.alpha.fits")
     np.testing.assert_array_equal(ab, alpha.astype(np.float32))
     write_object(tmp_path / "obj", img, fmt="ana", as_int16=True, text="DATE=2026-01-01")
-    r = read_ana(tmp_path / "obj.f0")
-    np.testing.assert_array_equal(r["data"], img.astype(np.int16))
-    assert r["text"] == "DATE=2026-01-01"
+    raw = open(tmp_path / "obj.f0", "rb").read()
+    assert raw[4] & 1 and raw[512 + 12] == 5          # int16 output crunched at slice 5, as the reference
+    np.testing.assert_array_equal(decrunch(raw), img.astype(np.int16))
+    assert raw[256:512].split(b"\0")[0] == b"DATE=2026-01-01"
 
 
 def test_bad_arguments(tmp_path):
@@ -111,3 +112,121 @@ def test_bad_arguments(tmp_path):
     assert L.lib().mfbd_write_fits(str(tmp_path / "z.fits").encode(), -32, 2, dims, buf) == L.MFBD_EINVAL
     assert L.lib().mfbd_write_ana(str(tmp_path / "z.f0").encode(), 9, 1, dims, buf, None) == L.MFBD_EINVAL
     assert b"datyp" in L.lib().mfbd_file_last_error()
+
+
+# ---- against the reference's own test files (testsuite/testdata, copied by
+# tests/golden/copy_file_fixtures.py): arrays with element (j, k) = j + k
+import os
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "files")
+
+
+def _gradient(rows, cols, dtype):
+    j, k = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+    return (j + k).astype(dtype)
+
+
+@pytest.mark.parametrize("name,dtype", [("gradient_8u_4x5.f0", np.int8), ("gradient_16s_4x5_le.f0", np.int16),
+                                        ("gradient_32s_4x5_le.f0", np.int32), ("gradient_32f_4x5_le.f0", np.float32),
+                                        ("gradient_64f_4x5_le.f0", np.float64),
+                                        ("gradient_32s_40x50_le.f0", np.int32)])
+def test_ana_files_equal_the_reference_files(tmp_path, name, dtype):
+    """Uncompressed ANA files written here are byte for byte the reference's."""
+    _lib_or_skip()
+    from redux_amd.momfbd import write_ana
+    shape = (40, 50) if "40x50" in name else (4, 5)
+    write_ana(tmp_path / "g.f0", _gradient(*shape, dtype))
+    assert open(tmp_path / "g.f0", "rb").read() == open(os.path.join(FIX, name), "rb").read()
+
+
+def test_ana_crunch_equals_the_reference_compressed_file(tmp_path):
+    """The crunch code of the 40 x 50 int32 gradient at the fixture's slice (5) is the
+    reference's compressed file byte for byte, and decodes back (independent decoder)."""
+    _lib_or_skip()
+    from redux_amd.momfbd import write_ana
+    ref = open(os.path.join(FIX, "gradient_32s_40x50_le.fz"), "rb").read()
+    assert ref[4] & 1 and ref[524] == 5
+    a = _gradient(40, 50, np.int32)
+    write_ana(tmp_path / "g.fz", a, slice=5)
+    got = open(tmp_path / "g.fz", "rb").read()
+    assert got == ref
+    np.testing.assert_array_equal(decrunch(got), a)
+
+
+def decrunch(raw):
+    """An independent reader of ANA's crunch code (LSB-first bit stream per row)."""
+    datyp, ndim = raw[7], raw[8]
+    nx, ny = np.frombuffer(raw[192:200], "<u4")
+    nhb = raw[6]
+    off = 256 + (nhb - 1) * 512 + 256
+    tsize, nblocks, bsize = np.frombuffer(raw[off:off + 12], "<u4")
+    slice_, wide = raw[off + 12], raw[off + 13] == 4
+    bits = np.unpackbits(np.frombuffer(raw[off + 14:], np.uint8), bitorder="little")
+    pos, out = 0, np.zeros((ny, nx), dtype=np.int64)
+
+    def take(n):
+        nonlocal pos
+        v = int(sum(int(b) << i for i, b in enumerate(bits[pos:pos + n])))
+        pos += n
+        return v
+
+    first_bits = 32 if wide else 16
+    for y in range(ny):
+        v = take(first_bits)
+        v = v - (1 << first_bits) if v >= 1 << (first_bits - 1) else v
+        out[y, 0] = v
+        for x in range(1, nx):
+            low = take(slice_)
+            z = 0
+            while bits[pos] == 0:
+                z += 1
+                pos += 1
+            pos += 1
+            if z == 31:
+                nb = 33 if wide else 17
+                d = take(nb)
+                d = d - (1 << nb) if d >= 1 << (nb - 1) else d
+            else:
+                r = z // 2 if z % 2 == 0 else -(z + 1) // 2
+                d = (r << slice_) + low
+            out[y, x] = out[y, x - 1] + d
+        pos = (pos + 7) // 8 * 8
+    return out
+
+
+@pytest.mark.parametrize("dtype,slice_", [(np.int16, 1), (np.int16, 3), (np.int16, 7), (np.int16, 12),
+                                          (np.int32, 2), (np.int32, 9), (np.int32, 20)])
+def test_ana_crunch_round_trip(tmp_path, dtype, slice_):
+    """Large and small differences (the escape of the unary code included) decode back."""
+    _lib_or_skip()
+    from redux_amd.momfbd import write_ana
+    rng = np.random.default_rng(slice_)
+    info = np.iinfo(dtype)
+    a = np.cumsum(rng.integers(-40, 40, (30, 64)), axis=1)
+    a[:, ::7] += rng.integers(-3000, 3000, a[:, ::7].shape)          # occasional big jumps
+    a = np.clip(a, info.min // 2, info.max // 2).astype(dtype)
+    write_ana(tmp_path / "r.fz", a, slice=slice_)
+    raw = open(tmp_path / "r.fz", "rb").read()
+    if raw[4] & 1:
+        np.testing.assert_array_equal(decrunch(raw), a)
+    else:                                      # not smaller compressed: stored raw
+        assert raw[256 + 256:] == a.tobytes()
+
+
+@pytest.mark.parametrize("name,dtype", [("gradient_16s_4x5.fits", np.int16), ("gradient_32s_4x5.fits", np.int32),
+                                        ("gradient_32f_4x5.fits", np.float32), ("gradient_64f_4x5.fits", np.float64),
+                                        ("gradient_32s_40x50.fits", np.int32)])
+def test_fits_files_match_the_reference_files(tmp_path, name, dtype):
+    """The same keywords and the same data records as the reference's FITS files."""
+    _lib_or_skip()
+    from redux_amd.momfbd import write_fits
+    shape = (40, 50) if "40x50" in name else (4, 5)
+    a = _gradient(*shape, dtype)
+    write_fits(tmp_path / "g.fits", a)
+    cg, dg, sg = read_fits(tmp_path / "g.fits")
+    cr, dr, sr = read_fits(os.path.join(FIX, name))
+    for key in ("SIMPLE", "BITPIX", "NAXIS", "NAXIS1", "NAXIS2"):
+        assert cg[key] == cr[key], key
+    np.testing.assert_array_equal(dg, dr)
+    np.testing.assert_array_equal(dg, a)
+    assert sg == sr

@@ -91,6 +91,9 @@ void o_state_grad_vogel(OState* st, const double* alpha, int nEnabled, double* g
 void o_state_grad_fd(OState* st, const double* alpha, int nEnabled, double* g);
 /* restored objects at alpha (object.cpp:278-443), img [nObj][s][s] */
 void o_state_restore(OState* st, const double* alpha, double* img);
+/* optional restoration outputs at alpha (object.cpp:332-407): mask bits 2 convolved objects,
+   32 PSF per image, 64 PSF averaged per object, 128 residuals; each [.][s][s] */
+void o_state_restore_outputs(OState* st, const double* alpha, int mask, double* psf, double* cobj, double* res);
 /* export per-image/per-object arrays after the last evaluation */
 void o_state_get(OState* st, const char* what, double* out);
 double o_state_scalar(OState* st, const char* what, int idx);

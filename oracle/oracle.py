@@ -93,6 +93,7 @@ def lib():
         for f in (L.o_state_grad_vogel, L.o_state_grad_fd):
             f.argtypes = [C.c_void_p, dp, C.c_int, dp]
         L.o_state_restore.argtypes = [C.c_void_p, dp, dp]
+        L.o_state_restore_outputs.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, dp]
         L.o_state_get.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.o_state_scalar.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
         L.o_state_scalar.restype = C.c_double
@@ -292,6 +293,19 @@ class State:
         img = np.zeros((self.nObj, self.s, self.s))
         lib().o_state_restore(self.h, _p(a), _p(img))
         return img
+
+    def restore_outputs(self, alpha, mask):
+        """Optional restoration outputs (object.cpp:332-407): mask bits 2 (convolved objects),
+        32 (PSF per image), 64 (PSF averaged per object; wins over 32), 128 (residuals).
+        Returns {"psf", "cobj", "res"}: [nImg or nObj][s][s] float values, None if not asked."""
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        s = self.s
+        psf = np.zeros((self.nObj if mask & 64 else self.nImg, s, s))
+        cobj = np.zeros((self.nImg, s, s))
+        res = np.zeros((self.nImg, s, s))
+        lib().o_state_restore_outputs(self.h, _p(a), int(mask), _p(psf), _p(cobj), _p(res))
+        return {"psf": psf if mask & 96 else None, "cobj": cobj if mask & 2 else None,
+                "res": res if mask & 128 else None}
 
     def get(self, what):
         shapes = {"otf": (self.nImg, self.N, self.N, 2), "imgft": (self.nImg, self.N, self.N, 2),

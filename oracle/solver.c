@@ -1153,38 +1153,129 @@ static void scharmer_filter(ocplx* data, const double* q2, int ny, int nx, doubl
     free(pt); free(rr); free(tmp);
 }
 
+/* The restored object o on the full N x N grid, before the centred crop (object.cpp:298-331):
+   sums of SubImage::restore, conj, safeDivide, Scharmer filter, normalised inverse
+   transform (avgObjFT's flags: de-centre, no image reorder) */
+static void restore_full(OState* st, int o, double* full, int normalise)
+{
+    const OProblem* pr = st->pr;
+    int N = st->N, N2 = st->N2;
+    ocplx* ao = calloc(N2, sizeof(ocplx)); double* dn = calloc(N2, sizeof(double));
+    ocplx* tmp = malloc(sizeof(ocplx) * N2);
+    double avgNoiseVar = 0; int cnt = 0;
+    for (int i = 0; i < pr->nImg; ++i) {
+        if (img_obj(st, i) != o) continue;
+        const ocplx* otf = st->OTF + (long)i * N2; const ocplx* ft = st->imgFT + (long)i * N2;
+        for (int k = 0; k < st->nOS[o]; ++k) {       /* SubImage::restore (subimage.cpp:242-258) */
+            int ind = st->os[o][k];
+            dn[ind] += cnorm(otf[ind]);
+            ao[ind] += conj(ft[ind]) * otf[ind];
+        }
+        avgNoiseVar += sqr(st->noise[i]);
+        cnt++;
+    }
+    for (int k = 0; k < N2; ++k) ao[k] = conj(ao[k]);
+    avgNoiseVar /= cnt;
+    for (int k = 0; k < N2; ++k) ao[k] = (fabs(dn[k]) > 0) ? ao[k] / dn[k] : 0;   /* safeDivide, eps 0 */
+    if (!pr->noFilter) scharmer_filter(ao, dn, N, N, avgNoiseVar, 0.90 * pr->freqCutoff[o]);
+    o_reorder_into_c(ao, N, N, tmp, 0, 0, 0);
+    o_fft2(tmp, N, N, +1);
+    for (int k = 0; k < N2; ++k) full[k] = normalise ? creal(tmp[k]) / N2 : creal(tmp[k]);
+    free(ao); free(dn); free(tmp);
+}
+
 static void restore_objects(OState* st, double* imgOut)
 {
     const OProblem* pr = st->pr;
-    int N = st->N, N2 = st->N2, s = st->s;
-    ocplx* ao = malloc(sizeof(ocplx) * N2); double* dn = malloc(sizeof(double) * N2);
-    ocplx* tmp = malloc(sizeof(ocplx) * N2);
+    int N = st->N, s = st->s, off = (N - s) / 2;
+    double* full = malloc(sizeof(double) * st->N2);
     for (int o = 0; o < pr->nObj; ++o) {
-        memset(ao, 0, sizeof(ocplx) * N2); memset(dn, 0, sizeof(double) * N2);
-        double avgNoiseVar = 0; int cnt = 0;
-        for (int i = 0; i < pr->nImg; ++i) {
-            if (img_obj(st, i) != o) continue;
-            const ocplx* otf = st->OTF + (long)i * N2; const ocplx* ft = st->imgFT + (long)i * N2;
-            for (int k = 0; k < st->nOS[o]; ++k) {       /* SubImage::restore (subimage.cpp:242-258) */
-                int ind = st->os[o][k];
-                dn[ind] += cnorm(otf[ind]);
-                ao[ind] += conj(ft[ind]) * otf[ind];
-            }
-            avgNoiseVar += sqr(st->noise[i]);
-            cnt++;
-        }
-        for (int k = 0; k < N2; ++k) ao[k] = conj(ao[k]);
-        avgNoiseVar /= cnt;
-        for (int k = 0; k < N2; ++k) ao[k] = (fabs(dn[k]) > 0) ? ao[k] / dn[k] : 0;   /* safeDivide, eps 0 */
-        if (!pr->noFilter) scharmer_filter(ao, dn, N, N, avgNoiseVar, 0.90 * pr->freqCutoff[o]);
-        o_reorder_into_c(ao, N, N, tmp, 0, 0, 0);
-        o_fft2(tmp, N, N, +1);
-        int off = (N - s) / 2;
+        restore_full(st, o, full, 1);
         double* out = imgOut + (long)o * s * s;
         for (int y = 0; y < s; ++y)
-            for (int x = 0; x < s; ++x) out[y * s + x] = creal(tmp[(long)(y + off) * N + x + off]) / N2;
+            for (int x = 0; x < s; ++x) out[y * s + x] = full[(long)(y + off) * N + x + off];
     }
-    free(ao); free(dn); free(tmp);
+    free(full);
+}
+
+/* Object::restorePatch's optional outputs (object.cpp:332-407), each cropped to the centred
+   s x s (object.cpp:393-408) and rounded to float as ObjectData stores them (data.hpp:109-112):
+   - PSF of image i (SubImage::getPSF, subimage.cpp:615-634): the OTF's own inverse transform
+     (its flags, subimage.cpp:89-90: de-centre, backward, normalise by 1/N^2, re-centre the
+     image), real part;
+   - average PSF of object o: the per-image PSFs summed in double, rounded to float, then
+     scaled by 1/imgCount;
+   - convolved object (SubImage::convolveImage, subimage.hpp:110-115, and
+     FourierTransform::convolve, fouriertransform.hpp:309-324): the float restored object
+     (full grid, before the crop and the fitted plane) forward transformed, centred and
+     normalised, times the centred OTF, de-centred, backward transformed, real part;
+   - residual (subimage.hpp:118-130): the float convolved object minus the windowed, padded
+     image (the inverse of imgFT), rounded to float; the same arithmetic with and without the
+     convolved objects saved.
+   psf: [nImg][s][s] (mask & 32) or [nObj][s][s] (mask & 64, which wins when both are set,
+   config.cpp:497-500); cobj (mask & 2) and res (mask & 128): [nImg][s][s]. */
+void o_state_restore_outputs(OState* st, const double* alpha, int mask, double* psf, double* cobj, double* res)
+{
+    const OProblem* pr = st->pr;
+    const int N = st->N, N2 = st->N2, s = st->s, off = (N - s) / 2;
+    const double nrm = 1.0 / N2;
+    if ((mask & 32) && (mask & 64)) mask &= ~32;
+    o_state_metric(st, alpha);
+    double* full = malloc(sizeof(double) * N2);
+    double* acc = malloc(sizeof(double) * N2);
+    double* conv = malloc(sizeof(double) * N2);
+    ocplx* t = malloc(sizeof(ocplx) * N2); ocplx* u = malloc(sizeof(ocplx) * N2);
+    ocplx* fo = malloc(sizeof(ocplx) * N2);
+    for (int o = 0; o < pr->nObj; ++o) {
+        restore_full(st, o, full, 0);
+        for (int k = 0; k < N2; ++k) full[k] = (float)((double)(float)full[k] * nrm);   /* od.img: float, ift<float> */
+        for (int k = 0; k < N2; ++k) fo[k] = full[k];
+        o_fft2(fo, N, N, -1);
+        o_reorder_c(fo, N, N);
+        for (int k = 0; k < N2; ++k) fo[k] *= nrm;
+        memset(acc, 0, sizeof(double) * N2);
+        int cnt = 0;
+        for (int i = 0; i < pr->nImg; ++i) {
+            if (img_obj(st, i) != o) continue;
+            cnt++;
+            const ocplx* otf = st->OTF + (long)i * N2;
+            if (mask & (32 | 64)) {
+                o_reorder_into_c(otf, N, N, t, 0, 0, 0);
+                o_fft2(t, N, N, +1);
+                for (int k = 0; k < N2; ++k) conv[k] = creal(t[k] * nrm);
+                o_reorder_d(conv, N, N);
+                if (mask & 64) for (int k = 0; k < N2; ++k) acc[k] += conv[k];
+                else
+                    for (int y = 0; y < s; ++y)
+                        for (int x = 0; x < s; ++x)
+                            psf[((long)i * s + y) * s + x] = (float)conv[(long)(y + off) * N + x + off];
+            }
+            if (mask & (2 | 128)) {
+                for (int k = 0; k < N2; ++k) u[k] = fo[k] * otf[k];
+                o_reorder_into_c(u, N, N, t, 0, 0, 0);
+                o_fft2(t, N, N, +1);
+                for (int k = 0; k < N2; ++k) conv[k] = (float)creal(t[k]);
+                if (mask & 2)
+                    for (int y = 0; y < s; ++y)
+                        for (int x = 0; x < s; ++x)
+                            cobj[((long)i * s + y) * s + x] = conv[(long)(y + off) * N + x + off];
+                if (mask & 128) {
+                    o_reorder_into_c(st->imgFT + (long)i * N2, N, N, t, 0, 0, 0);
+                    o_fft2(t, N, N, +1);
+                    for (int y = 0; y < s; ++y)
+                        for (int x = 0; x < s; ++x) {
+                            long k = (long)(y + off) * N + x + off;
+                            res[((long)i * s + y) * s + x] = (float)(conv[k] - creal(t[k]) * nrm);
+                        }
+                }
+            }
+        }
+        if (mask & 64)
+            for (int y = 0; y < s; ++y)
+                for (int x = 0; x < s; ++x)
+                    psf[((long)o * s + y) * s + x] = (float)((double)(float)acc[(long)(y + off) * N + x + off] * (1.0 / cnt));
+    }
+    free(full); free(acc); free(conv); free(t); free(u); free(fo);
 }
 
 /* ---------- GETSTEP = steepest_descent and BFGS: GSL 1.16 steepest_descent and

@@ -926,6 +926,52 @@ __global__ void __launch_bounds__(256) k_window(cx<T>* __restrict__ tiles, const
     }
 }
 
+// ----------------------------------------- optional restoration outputs ----
+// tiles[q][p] = OTF of image img[q] at natural index p: the stored value (invOs code h),
+// the conjugate of the stored mirror (-2-h), or 0 off the support -- the de-centred OTF
+// that SubImage::getPSF and FourierTransform::convolve transform (subimage.cpp:615-634,
+// fouriertransform.hpp:309-324)
+template <typename T>
+__global__ void k_otf_scatter(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ S, const int* __restrict__ invOs,
+                              const int* __restrict__ img, int nQ, int nOS, int N2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * N2) return;
+    const int q = e / N2, code = invOs[e % N2];
+    cx<T> v = {T(0), T(0)};
+    if (code != -1) {
+        v = S[(size_t)img[q] * nOS + (code >= 0 ? code : -2 - code)];
+        if (code < 0) v.y = -v.y;
+    }
+    tiles[e] = v;
+}
+
+// tiles[q][p] = (F[obj[q]][p] * nrm) * tiles[q][p]: the normalised transform of the
+// object times the OTF (FourierTransform::convolve's product)
+template <typename T>
+__global__ void k_mul_obj(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ F, const int* __restrict__ obj,
+                          int nQ, int N2, T nrm) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nQ * N2) return;
+    const int q = e / N2;
+    const cx<T> f = F[(size_t)obj[q] * N2 + e % N2];
+    const T fr = f.x * nrm, fi = f.y * nrm;
+    const cx<T> t = tiles[e];
+    tiles[e] = {fr * t.x - fi * t.y, fr * t.y + fi * t.x};
+}
+
+// out[q][y][x] = Re tiles[q][(y + off + sh) % N][(x + off + sh) % N] * scale: the centred
+// s x s of each tile, re-centred by sh = N/2 for a PSF (the OTF's REORDER_IMG flag)
+template <typename T>
+__global__ void k_crop_re(double* __restrict__ out, const cx<T>* __restrict__ tiles, int nQ, int N, int s, int sh,
+                          double scale) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t s2 = (size_t)s * s;
+    if (e >= (size_t)nQ * s2) return;
+    const int q = e / s2, r = e % s2, off = (N - s) / 2;
+    const int y = (r / s + off + sh) % N, x = (r % s + off + sh) % N;
+    out[e] = (double)tiles[((size_t)q * N + y) * N + x].x * scale;
+}
+
 // D[slot][img][u] = tile[q][osNat[u]]
 template <typename T>
 __global__ void k_store_d(cx<T>* __restrict__ D, const cx<T>* __restrict__ tiles, const int* __restrict__ osNat,
@@ -2504,6 +2550,80 @@ public:
         restoreN_ = n;
     }
 
+    void restore_extra(int slot, const double* alpha, const std::vector<ImgInit>& imgs,
+                       const std::vector<float>& objFull, int want, double* psf, double* conv,
+                       double* win) override {
+        use_lane(kSyncLane);
+        const int nI = (int)imgs.size();
+        if (!nI || !want) return;
+        const size_t s2 = (size_t)g_.s * g_.s, nT = (size_t)nI * N2;
+        EvalReq rq;
+        rq.slot = slot;
+        rq.alpha = alpha;
+        forward(&rq, 1, /*useGamma=*/0);             // S of request 0: [nImg][nOS_]
+        std::vector<int> ints(5 * (size_t)nI);
+        for (int q = 0; q < nI; ++q) {
+            ints[q] = imgs[q].slot;
+            ints[nI + q] = imgs[q].img;
+            ints[2 * nI + q] = imgs[q].shiftX;
+            ints[3 * nI + q] = imgs[q].shiftY;
+            ints[4 * nI + q] = st.imgObj[imgs[q].img];
+        }
+        upload(xInts_, ints, st_);
+        const int* dSlot = xInts_.p;
+        const int* dImg = xInts_.p + nI;
+        xT_.alloc(nT);
+        xOut_.alloc((size_t)nI * s2);
+        auto crop = [&](int sh, double scale) {
+            launch("k_crop_re", nI, [&] {
+                hipLaunchKernelGGL(k_crop_re<T>, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, xOut_.p, xT_.p, nI,
+                                   N, g_.s, sh, scale);
+            });
+        };
+        auto fetch = [&](double* dst) {
+            HIPCHK(hipStreamSynchronize(st_));
+            HIPCHK(hipMemcpy(dst, xOut_.p, (size_t)nI * s2 * sizeof(double), hipMemcpyDeviceToHost));
+        };
+        auto scatter = [&] {
+            launch("k_otf_scatter", nI, [&] {
+                hipLaunchKernelGGL(k_otf_scatter<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, sc_->S.p, invOs_.p, dImg,
+                                   nI, nOS_, N2);
+            });
+        };
+        if (want & 1) {            // PSF: normalised inverse transform of the OTF, re-centred
+            scatter();
+            fft2(xT_.p, nI, +1);
+            crop(N / 2, 1.0 / N2);
+            fetch(psf);
+        }
+        if (want & 2) {            // convolved objects
+            const int nO = g_.nObj;
+            std::vector<C> h((size_t)nO * N2);
+            for (size_t k = 0; k < h.size(); ++k) h[k] = {(T)objFull[k], T(0)};
+            upload(xO_, h, st_);
+            fft2(xO_.p, nO, -1);
+            scatter();
+            launch("k_mul_obj", nI, [&] {
+                hipLaunchKernelGGL(k_mul_obj<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, xO_.p, xInts_.p + 4 * nI,
+                                   nI, N2, (T)(1.0 / N2));
+            });
+            fft2(xT_.p, nI, +1);
+            crop(0, 1.0);
+            fetch(conv);
+        }
+        if (want & 4) {            // windowed, padded frames (the inverse of imgFT)
+            if (!winSet_) throw std::runtime_error("restore_extra: frames not initialised");
+            launch("k_window", nI, [&] {
+                hipLaunchKernelGGL(k_window<T>, dim3(nI), dim3(256), 0, st_, xT_.p, rawF_.p, win_.p, dSlot, dImg,
+                                   xInts_.p + 2 * nI, xInts_.p + 3 * nI, g_.nImg, g_.dataY, g_.dataX, g_.startY,
+                                   g_.startX, g_.s, N);
+            });
+            crop(0, 1.0);
+            fetch(win);
+        }
+        collect(kSyncLane);
+    }
+
     bool restore_ready() override {
         const hipError_t e = hipEventQuery(lanes_[kSyncLane].done);
         if (e == hipErrorNotReady) return false;
@@ -3186,6 +3306,9 @@ private:
     std::vector<double> otfNorm_;
     // resident: basis, supports, per-patch frames, D, ftSum, gamma, channel phases
     DBuf<C> tw_, D_;
+    DBuf<C> xT_, xO_;                  // optional restoration outputs: image and object tiles
+    DBuf<double> xOut_;
+    DBuf<int> xInts_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
     int device_ = 0;
     int lr_ = LINES, lc_ = LINES;           // lines per workgroup of the transform passes

@@ -495,32 +495,40 @@ void effective_alpha(const mfbd_ctx* c, int p, const double* alpha, double* out)
     }
 }
 
-// Object::restorePatch after the sums (object.cpp:278-443): conj, safeDivide, Scharmer
-// filter, inverse transform, centred crop
+// Object::restorePatch after the sums (object.cpp:298-326): conj, safeDivide, Scharmer
+// filter; returns object o's spectrum de-centred for the inverse transform.
 // P0, Q0: the patch's sums on the stored half of the support ([nObj][nH]); `map` takes
 // each full support point to its stored value (code h) or that value's conjugate (-1-h)
+std::vector<cd> restore_spectrum(const mfbd_ctx* c, int p, int o, const std::vector<int>& os,
+                                 const std::vector<int>& map, int nH, const double* P0, const double* Q0) {
+    const int N = 2 * c->g.P, nOS = (int)os.size();
+    std::vector<cd> ao((size_t)N * N, 0.0);
+    std::vector<double> dn((size_t)N * N, 0.0);
+    for (int u = 0; u < nOS; ++u) {
+        const int code = map[u], h = code >= 0 ? code : -1 - code;
+        const size_t k = (size_t)o * nH + h;
+        const cd pv(P0[2 * k], code >= 0 ? P0[2 * k + 1] : -P0[2 * k + 1]);   // P(-u) = P(u)*
+        ao[os[u]] = std::conj(pv);
+        dn[os[u]] = Q0[k];
+    }
+    for (size_t k = 0; k < ao.size(); ++k) ao[k] = (std::fabs(dn[k]) > 0) ? ao[k] / dn[k] : cd(0.0);
+    double nv = 0;
+    int cnt = 0;
+    for (int j = 0; j < c->g.nImg; ++j)
+        if (c->st.imgObj[j] == o && img_on(c, p, j)) { nv += c->noise[p][j] * c->noise[p][j]; cnt++; }
+    nv /= cnt;
+    if (!c->prm.noFilter) scharmer(ao.data(), dn.data(), N, N, nv, 0.90 * c->st.freqCutoff[o]);
+    std::vector<cd> t((size_t)N * N);
+    reorder_into(ao.data(), N, N, t.data());
+    return t;
+}
+
+// ... then the inverse transform and the centred s x s crop (object.cpp:329-331, 393-397)
 void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const std::vector<int>& map, int nH,
                     const double* P0, const double* Q0, float* imgOut) {
-    const int N = 2 * c->g.P, s = c->g.s, nOS = (int)os.size();
+    const int N = 2 * c->g.P, s = c->g.s;
     for (int o = 0; o < c->g.nObj; ++o) {
-        std::vector<cd> ao((size_t)N * N, 0.0);
-        std::vector<double> dn((size_t)N * N, 0.0);
-        for (int u = 0; u < nOS; ++u) {
-            const int code = map[u], h = code >= 0 ? code : -1 - code;
-            const size_t k = (size_t)o * nH + h;
-            const cd pv(P0[2 * k], code >= 0 ? P0[2 * k + 1] : -P0[2 * k + 1]);   // P(-u) = P(u)*
-            ao[os[u]] = std::conj(pv);
-            dn[os[u]] = Q0[k];
-        }
-        for (size_t k = 0; k < ao.size(); ++k) ao[k] = (std::fabs(dn[k]) > 0) ? ao[k] / dn[k] : cd(0.0);
-        double nv = 0;
-        int cnt = 0;
-        for (int j = 0; j < c->g.nImg; ++j)
-            if (c->st.imgObj[j] == o && img_on(c, p, j)) { nv += c->noise[p][j] * c->noise[p][j]; cnt++; }
-        nv /= cnt;
-        if (!c->prm.noFilter) scharmer(ao.data(), dn.data(), N, N, nv, 0.90 * c->st.freqCutoff[o]);
-        std::vector<cd> t((size_t)N * N);
-        reorder_into(ao.data(), N, N, t.data());
+        std::vector<cd> t = restore_spectrum(c, p, o, os, map, nH, P0, Q0);
         // inverse 2-D transform; only the columns of the centred s x s crop are needed
         const int off = (N - s) / 2;
         {
@@ -533,6 +541,24 @@ void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const 
             for (int x = 0; x < s; ++x)
                 out[(size_t)y * s + x] = (float)(t[(size_t)(y + off) * N + x + off].real() / ((double)N * N));
     }
+}
+
+// The restored objects on the full N x N grid before the crop, as the float array od.img
+// holds them (object.cpp:328-329; ift<float> rounds the real part, then normalises):
+// [nObj][N][N], natural layout
+std::vector<float> restore_full(const mfbd_ctx* c, int p, const std::vector<int>& os, const std::vector<int>& map,
+                                int nH, const double* P0, const double* Q0) {
+    const int N = 2 * c->g.P;
+    const size_t N2 = (size_t)N * N;
+    std::vector<float> full(c->g.nObj * N2);
+    for (int o = 0; o < c->g.nObj; ++o) {
+        std::vector<cd> t = restore_spectrum(c, p, o, os, map, nH, P0, Q0);
+        HostFFT f(N);
+        for (int y = 0; y < N; ++y) f.run(t.data() + (size_t)y * N, 1, +1);
+        for (int x = 0; x < N; ++x) f.run(t.data() + x, N, +1);
+        for (size_t k = 0; k < N2; ++k) full[o * N2 + k] = (float)((double)(float)t[k].real() * (1.0 / (double)N2));
+    }
+    return full;
 }
 
 // --------------------------------------------------------- patch solver ----
@@ -1650,6 +1676,76 @@ int mfbd_restore(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, f
             finish_restore(c, patchIdx[k], c->dev->otf_support(), c->dev->support_map(), c->dev->nOSStored(),
                            &P0[2 * po * k], &Q0[po * k], img + (size_t)k * c->g.nObj * c->g.s * c->g.s);
         });
+    } catch (const std::exception& e) {
+        return set_err(c, MFBD_EDEVICE, e.what());
+    }
+    return MFBD_OK;
+}
+
+int mfbd_restore_outputs(mfbd_ctx* c, int patch, const double* alpha, int saveMask, float* psf, float* cobj,
+                         float* res) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!alpha || patch < 0 || patch >= c->nPatch) return set_err(c, MFBD_EINVAL, "bad arguments");
+    int mask = saveMask & (MFBD_SAVE_COBJ | MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG | MFBD_SAVE_RESIDUAL);
+    if ((mask & MFBD_SAVE_PSF) && (mask & MFBD_SAVE_PSF_AVG)) mask &= ~MFBD_SAVE_PSF;   // config.cpp:497-500
+    if (((mask & (MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG)) && !psf) || ((mask & MFBD_SAVE_COBJ) && !cobj) ||
+        ((mask & MFBD_SAVE_RESIDUAL) && !res))
+        return set_err(c, MFBD_EINVAL, "an output asked for by saveMask is null");
+    if (!mask) return MFBD_OK;
+    try {
+        ensure_init(c);
+        const int nImg = c->g.nImg, nObj = c->g.nObj;
+        const size_t s2 = (size_t)c->g.s * c->g.s, po = (size_t)nObj * c->dev->nOSStored();
+        std::vector<double> ae((size_t)nImg * c->g.K), P0(2 * po), Q0(po);
+        effective_alpha(c, patch, alpha, ae.data());
+        std::vector<ImgInit> imgs;
+        for (int j = 0; j < nImg; ++j)
+            if (img_on(c, patch, j)) imgs.push_back({patch, j, c->shifts[patch][2 * j], c->shifts[patch][2 * j + 1]});
+        const int nI = (int)imgs.size();
+        std::vector<float> objFull;
+        if (mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) {
+            const double* ap = ae.data();
+            c->dev->restore_sums(1, &patch, &ap, P0.data(), Q0.data());
+            objFull = restore_full(c, patch, c->dev->otf_support(), c->dev->support_map(), c->dev->nOSStored(),
+                                   P0.data(), Q0.data());
+        }
+        const int want = ((mask & (MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG)) ? 1 : 0) |
+                         ((mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) ? 2 : 0) |
+                         ((mask & MFBD_SAVE_RESIDUAL) ? 4 : 0);
+        std::vector<double> dPsf((want & 1) ? nI * s2 : 0), dConv((want & 2) ? nI * s2 : 0),
+            dWin((want & 4) ? nI * s2 : 0);
+        c->dev->restore_extra(patch, ae.data(), imgs, objFull, want, dPsf.data(), dConv.data(), dWin.data());
+        if (mask & MFBD_SAVE_PSF) {                 // one PSF per image (SubImage::getPSF)
+            std::fill(psf, psf + nImg * s2, 0.0f);
+            for (int q = 0; q < nI; ++q)
+                for (size_t k = 0; k < s2; ++k) psf[imgs[q].img * s2 + k] = (float)dPsf[q * s2 + k];
+        }
+        if (mask & MFBD_SAVE_PSF_AVG) {             // summed in double, rounded, then / imgCount
+            std::fill(psf, psf + nObj * s2, 0.0f);
+            for (int o = 0; o < nObj; ++o) {
+                std::vector<double> acc(s2, 0.0);
+                int cnt = 0;
+                for (int q = 0; q < nI; ++q)
+                    if (c->st.imgObj[imgs[q].img] == o) {
+                        for (size_t k = 0; k < s2; ++k) acc[k] += dPsf[q * s2 + k];
+                        cnt++;
+                    }
+                if (cnt)
+                    for (size_t k = 0; k < s2; ++k) psf[o * s2 + k] = (float)((double)(float)acc[k] * (1.0 / cnt));
+            }
+        }
+        if (mask & MFBD_SAVE_COBJ) {
+            std::fill(cobj, cobj + nImg * s2, 0.0f);
+            for (int q = 0; q < nI; ++q)
+                for (size_t k = 0; k < s2; ++k) cobj[imgs[q].img * s2 + k] = (float)dConv[q * s2 + k];
+        }
+        if (mask & MFBD_SAVE_RESIDUAL) {            // float convolved object minus the frame
+            std::fill(res, res + nImg * s2, 0.0f);
+            for (int q = 0; q < nI; ++q)
+                for (size_t k = 0; k < s2; ++k)
+                    res[imgs[q].img * s2 + k] = (float)((double)(float)dConv[q * s2 + k] - dWin[q * s2 + k]);
+        }
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
     }

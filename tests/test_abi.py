@@ -62,6 +62,7 @@ def test_header_constants_match_binding():
              "MFBD_GRAD_SCHEDULE_REFERENCE": L.GRAD_SCHEDULE_REFERENCE, "MFBD_BASIS_ZERNIKE": L.BASIS_ZERNIKE,
              "MFBD_BASIS_KL": L.BASIS_KL, "MFBD_GETSTEP_STEEPEST": L.GETSTEP_STEEPEST,
              "MFBD_GETSTEP_CONJUGATE": L.GETSTEP_CONJUGATE, "MFBD_GETSTEP_BFGS": L.GETSTEP_BFGS,
-             "MFBD_GETSTEP_BFGS_INV": L.GETSTEP_BFGS_INV}
+             "MFBD_GETSTEP_BFGS_INV": L.GETSTEP_BFGS_INV, "MFBD_SAVE_COBJ": L.SAVE_COBJ,
+             "MFBD_SAVE_PSF": L.SAVE_PSF, "MFBD_SAVE_PSF_AVG": L.SAVE_PSF_AVG, "MFBD_SAVE_RESIDUAL": L.SAVE_RESIDUAL}
     for name, val in pairs.items():
         assert int(defs[name]) == val, name

@@ -129,3 +129,20 @@ def test_transform_tile_shape_outside_the_candidates_is_refused(patch):
     from redux_amd.momfbd import SolverError
     with pytest.raises(SolverError, match="MFBD_FFT_LINES"):
         _with_lines(patch, "fp64", "12,16")
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-6), ("fp32", 1e-5)])
+def test_restore_outputs_on_the_256_grid(patch, ref, prec, tol):
+    """PSFs, convolved objects and residuals (mfbd_restore_outputs) through the multi-pass
+    transforms of this grid, against the oracle; tolerances relative to the largest PSF /
+    convolved-object value, as on the smaller grids."""
+    sv = _solver(patch, prec)
+    a = ref["alphas"][2]
+    got = sv.restore_outputs(a, 2 | 32 | 128)
+    want = ref["state"].restore_outputs(a, 2 | 32 | 128)
+    sc = np.abs(want["cobj"]).max()
+    for key, scale in (("psf", np.abs(want["psf"]).max()), ("cobj", sc), ("res", sc)):
+        err = np.abs(got[key] - want[key]).max() / scale
+        print(prec, key, f"{err:.2e}", flush=True)
+        assert err <= tol, (prec, key, err)
+    sv.close()

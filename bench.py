@@ -294,6 +294,8 @@ def uneven(args):
         for g, sv, sr, ra, rr in zip(groups, solvers, s64, res, ref):
             a64 = np.stack([r["alpha"] for r in rr])
             idx = np.arange(len(rr), dtype=np.int32)
+            sv.init()          # both contexts back to the frames' first initialisation, so the
+            sr.init()          # two evaluate on the same frame state
             L, Lr = sv.metric(a64, patches=idx), sr.metric(a64, patches=idx)
             gv, gr = sv.grad(a64, patches=idx), sr.grad(a64, patches=idx)
             acc["metric_eval"] += list(np.abs(L - Lr) / np.abs(Lr))

@@ -150,7 +150,11 @@ int mfbd_solve(mfbd_ctx* ctx, const double* alpha0, mfbd_result* results, double
 int mfbd_solve_group(int n, mfbd_ctx* const* ctxs, const double* const* alpha0, mfbd_result* const* results,
                      double* const* alphaOut, float* const* imgOut);
 
-/* ---- stage-wise entry points (parity checks, autograd harness) ---- */
+/* ---- stage-wise entry points (parity checks, autograd harness) ----
+ * They evaluate on the frames' current state: the first initialisation at zero shifts
+ * (made on first use, or again by mfbd_init_patches), or, after mfbd_solve, the state the
+ * solve left (each image's final integer shift and tilt offset), so that the restoration
+ * of a patch's alphaOut reproduces its imgOut. */
 /* First SubImage::initialize (doReset) of every loaded patch at zero alpha
  * (Solver::shiftAndInit(alpha, true), solver.cpp:583-605). */
 int mfbd_init_patches(mfbd_ctx* ctx);

@@ -424,7 +424,7 @@ struct MfmaVec<double> {
 template <typename T>
 struct PhaseChunk { static constexpr int value = 13; };        // k-steps whose fragments are loaded together
 template <>
-struct PhaseChunk<double> { static constexpr int value = 5; };
+struct PhaseChunk<double> { static constexpr int value = 3; };
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T* __restrict__ B,
@@ -2598,6 +2598,7 @@ public:
         }
         if (want & 2) {            // convolved objects
             const int nO = g_.nObj;
+            if (objFull.size() < (size_t)nO * N2) throw std::runtime_error("restore_extra: objFull too small");
             std::vector<C> h((size_t)nO * N2);
             for (size_t k = 0; k < h.size(); ++k) h[k] = {(T)objFull[k], T(0)};
             upload(xO_, h, st_);

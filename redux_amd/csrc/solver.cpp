@@ -341,6 +341,9 @@ struct mfbd_ctx {
     std::vector<std::vector<double>> noise;     // [patch][nImg]
     std::vector<std::vector<double>> gamma;     // [patch][nObj]
     bool initDone = false;
+    // the patches hold the state the last mfbd_solve left (each image's final shift and tilt
+    // offset); stage-wise calls evaluate on it until mfbd_init_patches or a new load
+    bool solved = false;
     std::unique_ptr<Pool> pool;
     // host-side timing of the batch driver: evaluate / resume / re-init phases
     std::vector<KernelStat> hostStats = std::vector<KernelStat>(6);
@@ -1385,7 +1388,7 @@ int check_ready(mfbd_ctx* c) {
 }
 
 void ensure_init(mfbd_ctx* c) {
-    if (c->initDone) return;
+    if (c->initDone || c->solved) return;
     std::vector<int> all(c->nPatch);
     for (int p = 0; p < c->nPatch; ++p) {
         all[p] = p;
@@ -1523,6 +1526,7 @@ int mfbd_load_patches(mfbd_ctx* c, int n, const float* data, const double* phiFi
     c->imgOn.clear();
     c->nsP.clear();
     c->initDone = false;
+    c->solved = false;
     try {
         // device slots: the evaluator keeps every loaded patch resident
         std::string err;
@@ -1586,6 +1590,7 @@ int mfbd_set_patch_shape(mfbd_ctx* c, int patch, int nModes, const unsigned char
         c->dev->load(patch, 1, blk, phi.data());
         c->dev->set_images_enabled(patch, c->imgOn[patch].data());
         c->initDone = false;
+        c->solved = false;
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
     }
@@ -1597,6 +1602,7 @@ int mfbd_init_patches(mfbd_ctx* c) {
     if (rc) return rc;
     try {
         c->initDone = false;
+        c->solved = false;
         ensure_init(c);
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
@@ -1987,7 +1993,8 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 if (alphaOut) std::copy(r.alpha.begin(), r.alpha.end(), alphaOut + (size_t)r.p * nPar);
             }
         }
-        c->initDone = false;    // shifts moved during the solve
+        c->initDone = false;    // shifts moved during the solve: its final state is kept
+        c->solved = true;
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
     }

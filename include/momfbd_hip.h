@@ -167,18 +167,19 @@ int mfbd_eval_grad(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alph
                    double* gAlpha);
 /* Restored objects at alpha: Object::restorePatch.  img [n][nObj][s][s]. */
 int mfbd_restore(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, float* img);
-/* Optional restoration outputs of one patch at alpha: the PSF, convolved-object and
- * residual blocks of Object::restorePatch (object.cpp:332-407), selected by saveMask bits
- * (the reference's SaveFlags, config.hpp:31-33; other bits are ignored, and PSF_AVG wins
- * over PSF as in config.cpp:497-500).  Each image is cropped to the centred s x s:
- *   MFBD_SAVE_PSF       psf  [nImg][s][s]  PSF of each image (SubImage::getPSF)
- *   MFBD_SAVE_PSF_AVG   psf  [nObj][s][s]  mean PSF of each object's images
- *   MFBD_SAVE_COBJ      cobj [nImg][s][s]  restored object convolved with the image's OTF
- *   MFBD_SAVE_RESIDUAL  res  [nImg][s][s]  convolved object minus the windowed frame
+/* Optional restoration outputs of n patches at their alphas ([n][nImg][K], as for
+ * mfbd_restore): the PSF, convolved-object and residual blocks of Object::restorePatch
+ * (object.cpp:332-407), selected by saveMask bits (the reference's SaveFlags,
+ * config.hpp:31-33; other bits are ignored, and PSF_AVG wins over PSF as in
+ * config.cpp:497-500).  Each image is cropped to the centred s x s; per patch:
+ *   MFBD_SAVE_PSF       psf  [n][nImg][s][s]  PSF of each image (SubImage::getPSF)
+ *   MFBD_SAVE_PSF_AVG   psf  [n][nObj][s][s]  mean PSF of each object's images
+ *   MFBD_SAVE_COBJ      cobj [n][nImg][s][s]  restored object convolved with the image's OTF
+ *   MFBD_SAVE_RESIDUAL  res  [n][nImg][s][s]  convolved object minus the windowed frame
  * Images a patch lacks (mfbd_set_patch_shape) are zero.  Called after mfbd_solve with the
- * patch's alphaOut row, it gives what restorePatch saves with the solution. */
-int mfbd_restore_outputs(mfbd_ctx* ctx, int patch, const double* alpha, int saveMask, float* psf, float* cobj,
-                         float* res);
+ * patches' alphaOut rows, it gives what restorePatch saves with the solution. */
+int mfbd_restore_outputs(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, int saveMask, float* psf,
+                         float* cobj, float* res);
 /* Diagnostics: copy a device array for patch p ("S" OTF on support, "D", "P", "Q",
  * "ftsum", "phi") or host state ("gamma_host", "noise_host", "otfnorm", "window",
  * "shifts" [nImg][2] integer cut-out shifts (x, y), "tiltoff" [nImg][2], "sizes" = pupil

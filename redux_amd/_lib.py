@@ -55,7 +55,7 @@ SIGNATURES = {
     "mfbd_eval_metric": (C.c_int, [C.c_void_p, C.c_int, ip, dp, dp]),
     "mfbd_eval_grad": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_int, C.c_int, dp]),
     "mfbd_restore": (C.c_int, [C.c_void_p, C.c_int, ip, dp, fp]),
-    "mfbd_restore_outputs": (C.c_int, [C.c_void_p, C.c_int, dp, C.c_int, fp, fp, fp]),
+    "mfbd_restore_outputs": (C.c_int, [C.c_void_p, C.c_int, ip, dp, C.c_int, fp, fp, fp]),
     "mfbd_debug_get": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, dp, ip]),
     "mfbd_kernel_stats": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, dp, ip, C.POINTER(C.c_longlong)]),
     "mfbd_set_patch_shape": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_ubyte), C.c_int, C.c_int, ip, ip, dp]),

@@ -85,15 +85,15 @@ public:
     virtual void restore_begin(int n, const int* slots, const double* const* alpha) = 0;
     virtual bool restore_ready() = 0;
     virtual void restore_end(double* P0, double* Q0) = 0;
-    // Optional restoration outputs of one slot at alpha (object.cpp:332-407), for the images
-    // `imgs` (their current shifts), each cropped to the centred s x s of the N x N grid, in
-    // double: want bit 0 -> psf [nI][s][s], the PSF (normalised inverse transform of the OTF,
-    // re-centred); bit 1 -> conv [nI][s][s], the object of the image (objFull: [nObj][N][N],
-    // natural layout) convolved with its OTF; bit 2 -> win [nI][s][s], the windowed, padded
-    // frame.  On the synchronous lane.
-    virtual void restore_extra(int slot, const double* alpha, const std::vector<ImgInit>& imgs,
-                               const std::vector<float>& objFull, int want, double* psf, double* conv,
-                               double* win) = 0;
+    // Optional restoration outputs of n slots at their alphas (object.cpp:332-407), for the
+    // images imgs[k] of slot k (their current shifts), each cropped to the centred s x s of
+    // the N x N grid, in double, rows in the order of the images over all slots: want bit
+    // 0 -> psf, the PSF (normalised inverse transform of the OTF, re-centred); bit 1 -> conv,
+    // the object of the image (objFull: [n][nObj][N][N], natural layout) convolved with its
+    // OTF; bit 2 -> win, the windowed, padded frame.  On the synchronous lane.
+    virtual void restore_extra(int n, const int* slots, const double* const* alpha,
+                               const std::vector<std::vector<ImgInit>>& imgs, const std::vector<float>& objFull,
+                               int want, double* psf, double* conv, double* win) = 0;
     virtual int nOS() const = 0;            // OTF support points (full)
     virtual int nOSStored() const = 0;      // points stored on the device (Hermitian half)
     // full support point u -> stored point h (value as stored), or -1-h (its conjugate)

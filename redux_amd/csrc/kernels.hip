@@ -2550,77 +2550,95 @@ public:
         restoreN_ = n;
     }
 
-    void restore_extra(int slot, const double* alpha, const std::vector<ImgInit>& imgs,
-                       const std::vector<float>& objFull, int want, double* psf, double* conv,
-                       double* win) override {
+    void restore_extra(int n, const int* slots, const double* const* alpha,
+                       const std::vector<std::vector<ImgInit>>& imgs, const std::vector<float>& objFull, int want,
+                       double* psf, double* conv, double* win) override {
         use_lane(kSyncLane);
-        const int nI = (int)imgs.size();
-        if (!nI || !want) return;
-        const size_t s2 = (size_t)g_.s * g_.s, nT = (size_t)nI * N2;
-        EvalReq rq;
-        rq.slot = slot;
-        rq.alpha = alpha;
-        forward(&rq, 1, /*useGamma=*/0);             // S of request 0: [nImg][nOS_]
-        std::vector<int> ints(5 * (size_t)nI);
-        for (int q = 0; q < nI; ++q) {
-            ints[q] = imgs[q].slot;
-            ints[nI + q] = imgs[q].img;
-            ints[2 * nI + q] = imgs[q].shiftX;
-            ints[3 * nI + q] = imgs[q].shiftY;
-            ints[4 * nI + q] = st.imgObj[imgs[q].img];
-        }
-        upload(xInts_, ints, st_);
-        const int* dSlot = xInts_.p;
-        const int* dImg = xInts_.p + nI;
-        xT_.alloc(nT);
-        xOut_.alloc((size_t)nI * s2);
-        auto crop = [&](int sh, double scale) {
-            launch("k_crop_re", nI, [&] {
-                hipLaunchKernelGGL(k_crop_re<T>, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, xOut_.p, xT_.p, nI,
-                                   N, g_.s, sh, scale);
-            });
-        };
-        auto fetch = [&](double* dst) {
-            HIPCHK(hipStreamSynchronize(st_));
-            HIPCHK(hipMemcpy(dst, xOut_.p, (size_t)nI * s2 * sizeof(double), hipMemcpyDeviceToHost));
-        };
-        auto scatter = [&] {
-            launch("k_otf_scatter", nI, [&] {
-                hipLaunchKernelGGL(k_otf_scatter<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, sc_->S.p, invOs_.p, dImg,
-                                   nI, nOS_, N2);
-            });
-        };
-        if (want & 1) {            // PSF: normalised inverse transform of the OTF, re-centred
-            scatter();
-            fft2(xT_.p, nI, +1);
-            crop(N / 2, 1.0 / N2);
-            fetch(psf);
-        }
-        if (want & 2) {            // convolved objects
-            const int nO = g_.nObj;
-            if (objFull.size() < (size_t)nO * N2) throw std::runtime_error("restore_extra: objFull too small");
-            std::vector<C> h((size_t)nO * N2);
-            for (size_t k = 0; k < h.size(); ++k) h[k] = {(T)objFull[k], T(0)};
-            upload(xO_, h, st_);
-            fft2(xO_.p, nO, -1);
-            scatter();
-            launch("k_mul_obj", nI, [&] {
-                hipLaunchKernelGGL(k_mul_obj<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, xO_.p, xInts_.p + 4 * nI,
-                                   nI, N2, (T)(1.0 / N2));
-            });
-            fft2(xT_.p, nI, +1);
-            crop(0, 1.0);
-            fetch(conv);
-        }
-        if (want & 4) {            // windowed, padded frames (the inverse of imgFT)
-            if (!winSet_) throw std::runtime_error("restore_extra: frames not initialised");
-            launch("k_window", nI, [&] {
-                hipLaunchKernelGGL(k_window<T>, dim3(nI), dim3(256), 0, st_, xT_.p, rawF_.p, win_.p, dSlot, dImg,
-                                   xInts_.p + 2 * nI, xInts_.p + 3 * nI, g_.nImg, g_.dataY, g_.dataX, g_.startY,
-                                   g_.startX, g_.s, N);
-            });
-            crop(0, 1.0);
-            fetch(win);
+        if (n < 1 || !want) return;
+        const int nO = g_.nObj;
+        const size_t s2 = (size_t)g_.s * g_.s;
+        if ((want & 2) && objFull.size() < (size_t)n * nO * N2)
+            throw std::runtime_error("restore_extra: objFull too small");
+        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
+        size_t outQ = 0;                                   // output row of the chunk's first image
+        for (int b = 0; b < n; b += perChunk) {
+            const int m = std::min(perChunk, n - b);
+            std::vector<EvalReq> rq(m);
+            int nI = 0;
+            for (int r = 0; r < m; ++r) {
+                rq[r].slot = slots[b + r];
+                rq[r].alpha = alpha[b + r];
+                nI += (int)imgs[b + r].size();
+            }
+            if (!nI) continue;
+            forward(rq.data(), m, /*useGamma=*/0);        // S of request r: [r][nImg][nOS_]
+            // per image q: slot, image, shifts, object tile (request * nObj + object), S row
+            std::vector<int> ints(6 * (size_t)nI);
+            int q = 0;
+            for (int r = 0; r < m; ++r)
+                for (const ImgInit& im : imgs[b + r]) {
+                    ints[q] = im.slot;
+                    ints[nI + q] = im.img;
+                    ints[2 * nI + q] = im.shiftX;
+                    ints[3 * nI + q] = im.shiftY;
+                    ints[4 * nI + q] = r * nO + st.imgObj[im.img];
+                    ints[5 * nI + q] = r * g_.nImg + im.img;
+                    ++q;
+                }
+            upload(xInts_, ints, st_);
+            const int* dSlot = xInts_.p;
+            const int* dImg = xInts_.p + nI;
+            const size_t nT = (size_t)nI * N2;
+            xT_.alloc(nT);
+            xOut_.alloc((size_t)nI * s2);
+            auto crop = [&](int sh, double scale) {
+                launch("k_crop_re", nI, [&] {
+                    hipLaunchKernelGGL(k_crop_re<T>, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, xOut_.p, xT_.p,
+                                       nI, N, g_.s, sh, scale);
+                });
+            };
+            auto fetch = [&](double* dst) {
+                HIPCHK(hipStreamSynchronize(st_));
+                HIPCHK(hipMemcpy(dst + outQ * s2, xOut_.p, (size_t)nI * s2 * sizeof(double), hipMemcpyDeviceToHost));
+            };
+            auto scatter = [&] {
+                launch("k_otf_scatter", nI, [&] {
+                    hipLaunchKernelGGL(k_otf_scatter<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, sc_->S.p, invOs_.p,
+                                       xInts_.p + 5 * nI, nI, nOS_, N2);
+                });
+            };
+            if (want & 1) {            // PSF: normalised inverse transform of the OTF, re-centred
+                scatter();
+                fft2(xT_.p, nI, +1);
+                crop(N / 2, 1.0 / N2);
+                fetch(psf);
+            }
+            if (want & 2) {            // convolved objects
+                std::vector<C> h((size_t)m * nO * N2);
+                const float* src = objFull.data() + (size_t)b * nO * N2;
+                for (size_t k = 0; k < h.size(); ++k) h[k] = {(T)src[k], T(0)};
+                upload(xO_, h, st_);
+                fft2(xO_.p, m * nO, -1);
+                scatter();
+                launch("k_mul_obj", nI, [&] {
+                    hipLaunchKernelGGL(k_mul_obj<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, xO_.p,
+                                       xInts_.p + 4 * nI, nI, N2, (T)(1.0 / N2));
+                });
+                fft2(xT_.p, nI, +1);
+                crop(0, 1.0);
+                fetch(conv);
+            }
+            if (want & 4) {            // windowed, padded frames (the inverse of imgFT)
+                if (!winSet_) throw std::runtime_error("restore_extra: frames not initialised");
+                launch("k_window", nI, [&] {
+                    hipLaunchKernelGGL(k_window<T>, dim3(nI), dim3(256), 0, st_, xT_.p, rawF_.p, win_.p, dSlot, dImg,
+                                       xInts_.p + 2 * nI, xInts_.p + 3 * nI, g_.nImg, g_.dataY, g_.dataX, g_.startY,
+                                       g_.startX, g_.s, N);
+                });
+                crop(0, 1.0);
+                fetch(win);
+            }
+            outQ += nI;
         }
         collect(kSyncLane);
     }

@@ -1690,11 +1690,13 @@ int mfbd_restore(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, f
     return MFBD_OK;
 }
 
-int mfbd_restore_outputs(mfbd_ctx* c, int patch, const double* alpha, int saveMask, float* psf, float* cobj,
-                         float* res) {
+int mfbd_restore_outputs(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, int saveMask, float* psf,
+                         float* cobj, float* res) {
     int rc = check_ready(c);
     if (rc) return rc;
-    if (!alpha || patch < 0 || patch >= c->nPatch) return set_err(c, MFBD_EINVAL, "bad arguments");
+    if (n < 1 || !patchIdx || !alpha) return set_err(c, MFBD_EINVAL, "bad arguments");
+    for (int k = 0; k < n; ++k)
+        if (patchIdx[k] < 0 || patchIdx[k] >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
     int mask = saveMask & (MFBD_SAVE_COBJ | MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG | MFBD_SAVE_RESIDUAL);
     if ((mask & MFBD_SAVE_PSF) && (mask & MFBD_SAVE_PSF_AVG)) mask &= ~MFBD_SAVE_PSF;   // config.cpp:497-500
     if (((mask & (MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG)) && !psf) || ((mask & MFBD_SAVE_COBJ) && !cobj) ||
@@ -1704,55 +1706,76 @@ int mfbd_restore_outputs(mfbd_ctx* c, int patch, const double* alpha, int saveMa
     try {
         ensure_init(c);
         const int nImg = c->g.nImg, nObj = c->g.nObj;
-        const size_t s2 = (size_t)c->g.s * c->g.s, po = (size_t)nObj * c->dev->nOSStored();
-        std::vector<double> ae((size_t)nImg * c->g.K), P0(2 * po), Q0(po);
-        effective_alpha(c, patch, alpha, ae.data());
-        std::vector<ImgInit> imgs;
-        for (int j = 0; j < nImg; ++j)
-            if (img_on(c, patch, j)) imgs.push_back({patch, j, c->shifts[patch][2 * j], c->shifts[patch][2 * j + 1]});
-        const int nI = (int)imgs.size();
+        const size_t s2 = (size_t)c->g.s * c->g.s, per = (size_t)nImg * c->g.K;
+        const size_t po = (size_t)nObj * c->dev->nOSStored(), N2 = (size_t)4 * c->g.P * c->g.P;
+        std::vector<double> ae(per * n);
+        std::vector<const double*> ap(n);
+        std::vector<std::vector<ImgInit>> imgs(n);
+        std::vector<size_t> row0(n + 1, 0);           // first output row of each patch's images
+        for (int k = 0; k < n; ++k) {
+            const int p = patchIdx[k];
+            effective_alpha(c, p, alpha + per * k, &ae[per * k]);
+            ap[k] = &ae[per * k];
+            for (int j = 0; j < nImg; ++j)
+                if (img_on(c, p, j)) imgs[k].push_back({p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1]});
+            row0[k + 1] = row0[k] + imgs[k].size();
+        }
         std::vector<float> objFull;
         if (mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) {
-            const double* ap = ae.data();
-            c->dev->restore_sums(1, &patch, &ap, P0.data(), Q0.data());
-            objFull = restore_full(c, patch, c->dev->otf_support(), c->dev->support_map(), c->dev->nOSStored(),
-                                   P0.data(), Q0.data());
+            std::vector<double> P0(2 * po * n), Q0(po * n);
+            c->dev->restore_sums(n, patchIdx, ap.data(), P0.data(), Q0.data());
+            objFull.resize((size_t)n * nObj * N2);
+            parallel_for(n, [&](int k) {
+                std::vector<float> f = restore_full(c, patchIdx[k], c->dev->otf_support(), c->dev->support_map(),
+                                                    c->dev->nOSStored(), &P0[2 * po * k], &Q0[po * k]);
+                std::copy(f.begin(), f.end(), objFull.begin() + (size_t)k * nObj * N2);
+            });
         }
         const int want = ((mask & (MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG)) ? 1 : 0) |
                          ((mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) ? 2 : 0) |
                          ((mask & MFBD_SAVE_RESIDUAL) ? 4 : 0);
-        std::vector<double> dPsf((want & 1) ? nI * s2 : 0), dConv((want & 2) ? nI * s2 : 0),
-            dWin((want & 4) ? nI * s2 : 0);
-        c->dev->restore_extra(patch, ae.data(), imgs, objFull, want, dPsf.data(), dConv.data(), dWin.data());
-        if (mask & MFBD_SAVE_PSF) {                 // one PSF per image (SubImage::getPSF)
-            std::fill(psf, psf + nImg * s2, 0.0f);
-            for (int q = 0; q < nI; ++q)
-                for (size_t k = 0; k < s2; ++k) psf[imgs[q].img * s2 + k] = (float)dPsf[q * s2 + k];
-        }
-        if (mask & MFBD_SAVE_PSF_AVG) {             // summed in double, rounded, then / imgCount
-            std::fill(psf, psf + nObj * s2, 0.0f);
-            for (int o = 0; o < nObj; ++o) {
-                std::vector<double> acc(s2, 0.0);
-                int cnt = 0;
-                for (int q = 0; q < nI; ++q)
-                    if (c->st.imgObj[imgs[q].img] == o) {
-                        for (size_t k = 0; k < s2; ++k) acc[k] += dPsf[q * s2 + k];
-                        cnt++;
-                    }
-                if (cnt)
-                    for (size_t k = 0; k < s2; ++k) psf[o * s2 + k] = (float)((double)(float)acc[k] * (1.0 / cnt));
+        const size_t nQ = row0[n];
+        std::vector<double> dPsf((want & 1) ? nQ * s2 : 0), dConv((want & 2) ? nQ * s2 : 0),
+            dWin((want & 4) ? nQ * s2 : 0);
+        c->dev->restore_extra(n, patchIdx, ap.data(), imgs, objFull, want, dPsf.data(), dConv.data(), dWin.data());
+        for (int k = 0; k < n; ++k) {
+            const auto& im = imgs[k];
+            const size_t q0 = row0[k];
+            if (mask & MFBD_SAVE_PSF) {                 // one PSF per image (SubImage::getPSF)
+                float* out = psf + (size_t)k * nImg * s2;
+                std::fill(out, out + nImg * s2, 0.0f);
+                for (size_t q = 0; q < im.size(); ++q)
+                    for (size_t e = 0; e < s2; ++e) out[im[q].img * s2 + e] = (float)dPsf[(q0 + q) * s2 + e];
             }
-        }
-        if (mask & MFBD_SAVE_COBJ) {
-            std::fill(cobj, cobj + nImg * s2, 0.0f);
-            for (int q = 0; q < nI; ++q)
-                for (size_t k = 0; k < s2; ++k) cobj[imgs[q].img * s2 + k] = (float)dConv[q * s2 + k];
-        }
-        if (mask & MFBD_SAVE_RESIDUAL) {            // float convolved object minus the frame
-            std::fill(res, res + nImg * s2, 0.0f);
-            for (int q = 0; q < nI; ++q)
-                for (size_t k = 0; k < s2; ++k)
-                    res[imgs[q].img * s2 + k] = (float)((double)(float)dConv[q * s2 + k] - dWin[q * s2 + k]);
+            if (mask & MFBD_SAVE_PSF_AVG) {             // summed in double, rounded, then / imgCount
+                float* out = psf + (size_t)k * nObj * s2;
+                std::fill(out, out + nObj * s2, 0.0f);
+                for (int o = 0; o < nObj; ++o) {
+                    std::vector<double> acc(s2, 0.0);
+                    int cnt = 0;
+                    for (size_t q = 0; q < im.size(); ++q)
+                        if (c->st.imgObj[im[q].img] == o) {
+                            for (size_t e = 0; e < s2; ++e) acc[e] += dPsf[(q0 + q) * s2 + e];
+                            cnt++;
+                        }
+                    if (cnt)
+                        for (size_t e = 0; e < s2; ++e) out[o * s2 + e] = (float)((double)(float)acc[e] * (1.0 / cnt));
+                }
+            }
+            if (mask & MFBD_SAVE_COBJ) {
+                float* out = cobj + (size_t)k * nImg * s2;
+                std::fill(out, out + nImg * s2, 0.0f);
+                for (size_t q = 0; q < im.size(); ++q)
+                    for (size_t e = 0; e < s2; ++e) out[im[q].img * s2 + e] = (float)dConv[(q0 + q) * s2 + e];
+            }
+            if (mask & MFBD_SAVE_RESIDUAL) {            // float convolved object minus the frame
+                float* out = res + (size_t)k * nImg * s2;
+                std::fill(out, out + nImg * s2, 0.0f);
+                for (size_t q = 0; q < im.size(); ++q)
+                    for (size_t e = 0; e < s2; ++e)
+                        out[im[q].img * s2 + e] =
+                            (float)((double)(float)dConv[(q0 + q) * s2 + e] - dWin[(q0 + q) * s2 + e]);
+            }
         }
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());

@@ -169,21 +169,27 @@ class Solver:
         self._check(L.lib().mfbd_restore(self.h, len(idx), ptr(idx, ip), ptr(a), ptr(img, fp)), "restore")
         return img
 
-    def restore_outputs(self, alpha, mask, patch=0):
-        """Optional restoration outputs of one patch (Object::restorePatch, object.cpp:332-407):
-        mask bits SAVE_COBJ, SAVE_PSF, SAVE_PSF_AVG (wins over SAVE_PSF), SAVE_RESIDUAL.
-        Returns {"psf", "cobj", "res"}: float32 [nImg or nObj][s][s], None where not asked."""
-        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(self.nImg, self.K)
+    def restore_outputs(self, alpha, mask, patch=0, patches=None):
+        """Optional restoration outputs (Object::restorePatch, object.cpp:332-407): mask bits
+        SAVE_COBJ, SAVE_PSF, SAVE_PSF_AVG (wins over SAVE_PSF), SAVE_RESIDUAL.  One patch
+        (alpha [nImg][K], index `patch`): {"psf", "cobj", "res"} float32 [nImg or nObj][s][s],
+        None where not asked.  With `patches` (alpha [n][nImg][K]) one call for all of them,
+        each array with a leading [n]."""
+        many = patches is not None
+        idx = np.ascontiguousarray(patches if many else [patch], dtype=np.int32)
+        n = len(idx)
+        a = np.ascontiguousarray(alpha, dtype=np.float64).reshape(n, self.nImg, self.K)
         if mask & L.SAVE_PSF_AVG:
             mask &= ~L.SAVE_PSF
         s = self.s
-        psf = np.zeros((self.nObj if mask & L.SAVE_PSF_AVG else self.nImg, s, s), dtype=np.float32)
-        cobj = np.zeros((self.nImg, s, s), dtype=np.float32)
-        res = np.zeros((self.nImg, s, s), dtype=np.float32)
-        self._check(L.lib().mfbd_restore_outputs(self.h, int(patch), ptr(a), int(mask), ptr(psf, fp), ptr(cobj, fp),
-                                                 ptr(res, fp)), "restore_outputs")
-        return {"psf": psf if mask & (L.SAVE_PSF | L.SAVE_PSF_AVG) else None,
-                "cobj": cobj if mask & L.SAVE_COBJ else None, "res": res if mask & L.SAVE_RESIDUAL else None}
+        psf = np.zeros((n, self.nObj if mask & L.SAVE_PSF_AVG else self.nImg, s, s), dtype=np.float32)
+        cobj = np.zeros((n, self.nImg, s, s), dtype=np.float32)
+        res = np.zeros((n, self.nImg, s, s), dtype=np.float32)
+        self._check(L.lib().mfbd_restore_outputs(self.h, n, ptr(idx, ip), ptr(a), int(mask), ptr(psf, fp),
+                                                 ptr(cobj, fp), ptr(res, fp)), "restore_outputs")
+        out = {"psf": psf if mask & (L.SAVE_PSF | L.SAVE_PSF_AVG) else None,
+               "cobj": cobj if mask & L.SAVE_COBJ else None, "res": res if mask & L.SAVE_RESIDUAL else None}
+        return out if many else {k: (None if v is None else v[0]) for k, v in out.items()}
 
     def debug(self, what, p=0):
         cnt = np.zeros(1, dtype=np.int32)

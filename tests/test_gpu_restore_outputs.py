@@ -65,10 +65,12 @@ def test_padded_two_object_patch_against_the_oracle(oracle_lib, prec, tol):
     want = st.restore_outputs(a, 64)["psf"]
     assert avg.shape == (2, pb["s"], pb["s"])
     assert np.abs(avg - want).max() <= tol * np.abs(want).max()
-    # the same patch in either slot gives the same outputs
+    # the same patch in either slot gives the same outputs, one at a time or in one call
     again = sv.restore_outputs(a, ALL, patch=0)
+    both = sv.restore_outputs(np.stack([a, a]), ALL, patches=[0, 1])
     for key in ("psf", "cobj", "res"):
         assert np.array_equal(again[key], got[key]), key
+        assert np.array_equal(both[key][0], got[key]) and np.array_equal(both[key][1], got[key]), key
     sv.close()
 
 
@@ -114,7 +116,10 @@ def test_argument_errors():
     a = np.ascontiguousarray(z["alphas"][0])
     out = np.zeros((len(pb["imgCh"]), pb["s"], pb["s"]), dtype=np.float32)
     lib = L.lib()
-    assert lib.mfbd_restore_outputs(sv.h, 1, L.ptr(a), 32, L.ptr(out, L.fp), None, None) == L.MFBD_EINVAL
-    assert lib.mfbd_restore_outputs(sv.h, 0, L.ptr(a), 2, L.ptr(out, L.fp), None, None) == L.MFBD_EINVAL
-    assert lib.mfbd_restore_outputs(sv.h, 0, L.ptr(a), 1 | 4, None, None, None) == 0     # nothing asked
+    one, bad = np.zeros(1, np.int32), np.ones(1, np.int32)
+    assert lib.mfbd_restore_outputs(sv.h, 1, L.ptr(bad, L.ip), L.ptr(a), 32, L.ptr(out, L.fp), None, None) == \
+        L.MFBD_EINVAL
+    assert lib.mfbd_restore_outputs(sv.h, 1, L.ptr(one, L.ip), L.ptr(a), 2, L.ptr(out, L.fp), None, None) == \
+        L.MFBD_EINVAL
+    assert lib.mfbd_restore_outputs(sv.h, 1, L.ptr(one, L.ip), L.ptr(a), 1 | 4, None, None, None) == 0   # nothing asked
     sv.close()

@@ -62,6 +62,12 @@ typedef struct {
     const double* alphaInit;      /* [nImg][K] or NULL */
     int maxMetrics;
     int getstep;                  /* 1: steepest_descent, 0/2: conjugate_rdx, 3: vector_bfgs, 4: vector_bfgs2 (BFGS_inv) */
+    /* Sensitivity experiments only (0 in every parity use): each metric value the solver's
+       minimiser sees is multiplied by 1 + metricNoise*u, u uniform in [-1, 1) from a
+       splitmix64 stream seeded by noiseSeed.  It models an implementation whose metric
+       differs from this one at that relative level (another summation order, FP32). */
+    double metricNoise;
+    unsigned long long noiseSeed;
 } OProblem;
 
 typedef struct {
@@ -118,6 +124,21 @@ OStep* o_step_new(int n, int kind);
 void o_step_free(OStep* w);
 void o_step_set(OStep* w, const OFdf* fn, const double* x, double* f, double* g, double stepSize, double tol);
 int o_step_iterate(OStep* w, double* x, double* f, double* g, double* dx);
+
+/* ---- conjugate_rdx / bracket / brent restatements (solver.c) on caller functions ---- */
+typedef struct OCg OCg;
+OCg* o_cg_new(int n);
+void o_cg_free(OCg* w);
+void o_cg_set(OCg* w, const OFdf* fn, const double* x, double* f, double stepSize, double tol);
+int o_cg_iterate(OCg* w, double* x, double* f, double* g, double* dx);
+void o_cg_direction(const OCg* w, double* p);
+void o_bracket(const OFdf* fn, double* a, double* b, double* mid, double* fa, double* fb, double* fmid, int limit);
+void o_brent(const OFdf* fn, double ax, double bx, double* x, double* fx, double tol, int limit);
+/* Object::restorePatch pieces (solver.c): the ScharmerFilter restatement on a centred grid,
+   and object o's restored object on the full N x N grid before the crop (normalised
+   inverse transform) at alpha */
+void o_scharmer_filter(ocplx* data, const double* q2, int ny, int nx, double noisePower, double cutoff);
+void o_state_restore_full(OState* st, const double* alpha, int o, double* full);
 
 /* ---- mosaic of restored patches (mosaic.c; object.cpp:1160-1207) ---- */
 void o_blend_window(double* w, int nRows, int nCols, int blend, int margin);

@@ -36,6 +36,7 @@ class OProblem(C.Structure):
         ("minIter", C.c_int), ("maxIter", C.c_int), ("nDoneIter", C.c_int), ("modeStart", C.c_int),
         ("modeStep", C.c_int), ("apodSize", C.c_int), ("noFilter", C.c_int),
         ("alphaInit", dp), ("maxMetrics", C.c_int), ("getstep", C.c_int),
+        ("metricNoise", C.c_double), ("noiseSeed", C.c_ulonglong),
     ]
 
 
@@ -93,6 +94,17 @@ def lib():
         for f in (L.o_state_grad_vogel, L.o_state_grad_fd):
             f.argtypes = [C.c_void_p, dp, C.c_int, dp]
         L.o_state_restore.argtypes = [C.c_void_p, dp, dp]
+        L.o_state_restore_full.argtypes = [C.c_void_p, dp, C.c_int, dp]
+        L.o_scharmer_filter.argtypes = [C.c_void_p, dp, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.o_cg_new.argtypes = [C.c_int]
+        L.o_cg_new.restype = C.c_void_p
+        L.o_cg_free.argtypes = [C.c_void_p]
+        L.o_cg_set.argtypes = [C.c_void_p, C.POINTER(OFdf), dp, dp, C.c_double, C.c_double]
+        L.o_cg_iterate.argtypes = [C.c_void_p, dp, dp, dp, dp]
+        L.o_cg_iterate.restype = C.c_int
+        L.o_cg_direction.argtypes = [C.c_void_p, dp]
+        L.o_bracket.argtypes = [C.POINTER(OFdf), dp, dp, dp, dp, dp, dp, C.c_int]
+        L.o_brent.argtypes = [C.POINTER(OFdf), C.c_double, C.c_double, dp, dp, C.c_double, C.c_int]
         L.o_state_restore_outputs.argtypes = [C.c_void_p, dp, C.c_int, dp, dp, dp]
         L.o_state_get.argtypes = [C.c_void_p, C.c_char_p, dp]
         L.o_state_scalar.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
@@ -170,7 +182,7 @@ def noise(ft, mask=-1, limit=-1.0):
 class Problem:
     """Holds numpy arrays alive and exposes them as an OProblem."""
 
-    def __init__(self, pb: dict, alpha_init=None):
+    def __init__(self, pb: dict, alpha_init=None, metric_noise=0.0, noise_seed=0):
         objs, chans = pb["objects"], pb["channels"]
         K, P = pb["K"], pb["P"]
         self.keep = {}
@@ -217,6 +229,7 @@ class Problem:
             o.alphaInit = dp()
         o.maxMetrics = 100000
         o.getstep = GETSTEP[str(st.get("GETSTEP", "conjugate")).lower()]
+        o.metricNoise, o.noiseSeed = float(metric_noise), int(noise_seed)
         self.o = o
         self.nObj, self.nImg, self.K, self.s = o.nObj, o.nImg, K, o.s
         self.nPar = o.nPar
@@ -227,8 +240,12 @@ def solve(pb: dict, alpha_init=None, restore=True):
     return _solve_many([prob], restore)[0]
 
 
-def solve_many(pbs, restore=True, threads=0):
-    probs = [Problem(pb) for pb in pbs]
+def solve_many(pbs, restore=True, threads=0, metric_noise=0.0, noise_seeds=None):
+    """Oracle solves of independent patches on host threads.  metric_noise > 0 perturbs every
+    metric value the minimiser sees (relative, uniform; sensitivity experiments only), with
+    noise_seeds[k] seeding patch k's stream."""
+    seeds = noise_seeds if noise_seeds is not None else [0] * len(pbs)
+    probs = [Problem(pb, None, metric_noise, sd) for pb, sd in zip(pbs, seeds)]
     return _solve_many(probs, restore, threads)
 
 
@@ -293,6 +310,15 @@ class State:
         img = np.zeros((self.nObj, self.s, self.s))
         lib().o_state_restore(self.h, _p(a), _p(img))
         return img
+
+    def restore_full(self, alpha, obj=0):
+        """Object obj's restored object on the full N x N grid before the centred crop
+        (object.cpp:298-331: conj, safeDivide, Scharmer filter unless NO_FILTER, normalised
+        inverse transform)."""
+        a = np.ascontiguousarray(alpha, dtype=np.float64)
+        full = np.zeros((self.N, self.N))
+        lib().o_state_restore_full(self.h, _p(a), int(obj), _p(full))
+        return full
 
     def restore_outputs(self, alpha, mask):
         """Optional restoration outputs (object.cpp:332-407): mask bits 2 (convolved objects),
@@ -405,4 +431,78 @@ def bfgs2_minimize(f, grad, x0, step=1.0, tol=1e-6, iters=100):
                 break
     finally:
         L.o_bfgs2_free(w)
+    return out
+
+
+def scharmer_filter(data, q2, noise_power, cutoff):
+    """ScharmerFilter (src/lib/image/utils.cpp:334-392) applied to the centred complex grid
+    data with the summed |S|^2 q2; returns the filtered copy."""
+    d = np.ascontiguousarray(data, dtype=np.complex128).copy()
+    q = np.ascontiguousarray(q2, dtype=np.float64)
+    lib().o_scharmer_filter(d.ctypes.data, _p(q), d.shape[0], d.shape[1], float(noise_power), float(cutoff))
+    return d
+
+
+def _fdf1(f):
+    """OFdf over a function of one variable (x[0]); keeps the callbacks alive."""
+    def _f(ctx, x):
+        return float(f(x[0]))
+    cf = F_T(_f)
+    fn = OFdf(None, C.cast(cf, C.c_void_p), None, None)
+    return fn, cf
+
+
+def bracket(f, a, b, limit=100):
+    """math::bracket (src/lib/math/helpers.cpp:58-129) from the points a, b; returns
+    (a, mid, b, fa, fmid, fb) with a <= mid <= b as the reference sorts them."""
+    fn, keep = _fdf1(f)
+    v = [np.array([float(t)]) for t in (a, b, 0.0, f(a), f(b), 0.0)]
+    lib().o_bracket(C.byref(fn), *[_p(t) for t in v], int(limit))
+    a, b, mid, fa, fb, fmid = [float(t[0]) for t in v]
+    return a, mid, b, fa, fmid, fb
+
+
+def brent(f, ax, bx, x, tol, limit=100):
+    """math::brent (src/lib/math/helpers.cpp:133-216) in [ax, bx] from x; returns (x, f(x))."""
+    fn, keep = _fdf1(f)
+    xv, fv = np.array([float(x)]), np.zeros(1)
+    lib().o_brent(C.byref(fn), float(ax), float(bx), _p(xv), _p(fv), float(tol), int(limit))
+    return float(xv[0]), float(fv[0])
+
+
+def cg_minimize(f, grad, x0, step=1.0, tol=1e-6, iters=100):
+    """Drive the conjugate_rdx restatement (conjugate_gradient.cpp:72-256, with bracket and
+    brent) on a Python function.  Returns the iterates [(x, f, g, status, evals, p)] after
+    set and after each iteration; p is the search direction that iteration used."""
+    L = lib()
+    n = len(x0)
+    calls = {"f": 0, "df": 0}
+
+    def _f(ctx, x):
+        calls["f"] += 1
+        return float(f(np.ctypeslib.as_array(x, (n,)).copy()))
+
+    def _df(ctx, x, g):
+        calls["df"] += 1
+        np.ctypeslib.as_array(g, (n,))[:] = grad(np.ctypeslib.as_array(x, (n,)).copy())
+
+    cf, cdf = F_T(_f), DF_T(_df)
+    fn = OFdf(None, C.cast(cf, C.c_void_p), C.cast(cdf, C.c_void_p), None)
+    w = L.o_cg_new(n)
+    x = np.array(x0, dtype=np.float64)
+    g = np.zeros(n)
+    dx = np.zeros(n)
+    p = np.zeros(n)
+    fv = np.zeros(1)
+    try:
+        L.o_cg_set(w, C.byref(fn), _p(x), _p(fv), step, tol)
+        out = [(x.copy(), float(fv[0]), g.copy(), 0, dict(calls), p.copy())]
+        for _ in range(iters):
+            st = L.o_cg_iterate(w, _p(x), _p(fv), _p(g), _p(dx))
+            L.o_cg_direction(w, _p(p))
+            out.append((x.copy(), float(fv[0]), g.copy(), st, dict(calls), p.copy()))
+            if st != 0:
+                break
+    finally:
+        L.o_cg_free(w)
     return out

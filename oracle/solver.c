@@ -48,6 +48,7 @@ struct OState {
     double* alpha; double* alphaOffset; double* gradAlpha; double* regW;
     int* enabled; int gradMethod; /* 1 = finite difference, 2 = Vogel */
     int nF, nDF;
+    unsigned long long noiseState;   /* metricNoise stream */
     /* scratch */
     ocplx* tc1; ocplx* tc2; ocplx* tc3; double* td1; double* td2;
 };
@@ -117,6 +118,7 @@ OState* o_state_new(const OProblem* pr)
 {
     OState* st = (OState*)calloc(1, sizeof(OState));
     st->pr = pr;
+    st->noiseState = pr->noiseSeed;
     st->s = pr->s; st->P = pr->P; st->N = pr->N; st->K = pr->K;
     st->P2 = pr->P * pr->P; st->N2 = pr->N * pr->N; st->s2 = pr->s * pr->s;
     int nO = pr->nObj, nC = pr->nCh, nI = pr->nImg, P = pr->P, N = pr->N, P2 = st->P2, N2 = st->N2;
@@ -591,12 +593,24 @@ static void ns_reverse_add(const OProblem* pr, const double* in, const double* o
     for (int e = 0; e < pr->nnz; ++e) out[pr->nsRow[e]] += pr->nsVal[e] * in[pr->nsCol[e]];
 }
 
+/* the metricNoise factor (sensitivity experiments; 1 when metricNoise is 0) */
+static double noise_factor(OState* st)
+{
+    if (st->pr->metricNoise == 0.0) return 1.0;
+    unsigned long long z = (st->noiseState += 0x9E3779B97F4A7C15ULL);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z ^= z >> 31;
+    double u = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    return 1.0 + st->pr->metricNoise * u;
+}
+
 /* Solver::my_f / my_df (solver.cpp:219-234) */
 static double my_f(OState* st, const double* beta)
 {
     ns_reverse_add(st->pr, beta, st->alphaOffset, st->alpha);
     apply_alpha(st, st->alpha);
-    return metric_now(st);
+    return metric_now(st) * noise_factor(st);
 }
 
 static void my_df(OState* st, const double* beta, double* gb)
@@ -614,7 +628,7 @@ static void my_fdf(OState* st, const double* beta, double* f, double* gb)
 {
     ns_reverse_add(st->pr, beta, st->alphaOffset, st->alpha);
     apply_alpha(st, st->alpha);
-    *f = metric_now(st);
+    *f = metric_now(st) * noise_factor(st);
     calc_helpers(st);
     gradient_now(st);
     ns_apply(st->pr, st->gradAlpha, gb);
@@ -627,7 +641,7 @@ static void cb_fdf(void* ctx, const double* x, double* f, double* g) { my_fdf((O
 /* ---------- conjugate_rdx + bracket + brent ---------- */
 
 typedef struct {
-    OState* st; const double* x; const double* p; double lambda; double* x1; double* dx; int n;
+    const OFdf* fn; const double* x; const double* p; double lambda; double* x1; double* dx; int n;
 } LineCtx;
 
 /* take_step / value_at (conjugate_gradient.cpp:110-125) */
@@ -640,7 +654,7 @@ static void take_step(const double* x, const double* p, double step, double lamb
 static double value_at(LineCtx* c, double step)
 {
     take_step(c->x, c->p, step, c->lambda, c->x1, c->dx, c->n);
-    return my_f(c->st, c->x1);
+    return c->fn->f(c->fn->ctx, c->x1);
 }
 
 #define SHFT(a, b, c, d) (a) = (b); (b) = (c); (c) = (d);
@@ -749,9 +763,9 @@ typedef struct {
 } CGState;
 
 /* conjugate_rdx_iterate (conjugate_gradient.cpp:128-256) */
-static int cg_iterate(OState* st, CGState* cs, double* x, double* f, double* g, double* dx, int n)
+static int cg_iterate(const OFdf* fn, CGState* cs, double* x, double* f, double* g, double* dx, int n)
 {
-    my_df(st, x, g);
+    fn->df(fn->ctx, x, g);
     double pnorm = cs->pnorm, g0norm = cs->g0norm;
     double g1norm = dnrm2(g, n);
     double pg;
@@ -772,7 +786,7 @@ static int cg_iterate(OState* st, CGState* cs, double* x, double* f, double* g, 
     if (pnorm == 0.0) { memset(dx, 0, sizeof(double) * n); return GSL_ENOPROG; }
     double fa = *f, fb, fc;
     double stepa = 0.0, stepb = 0, stepc = 1E-21 / lambda;
-    LineCtx lc = { st, x, cs->p, lambda, cs->x1, dx, n };
+    LineCtx lc = { fn, x, cs->p, lambda, cs->x1, dx, n };
     fc = value_at(&lc, stepc);
     bracket(&lc, &stepa, &stepc, &stepb, &fa, &fc, &fb, 100);
     brent(&lc, stepa, stepc, &stepb, &fb, 0.1, 100);
@@ -784,6 +798,68 @@ static int cg_iterate(OState* st, CGState* cs, double* x, double* f, double* g, 
     memcpy(cs->prevG, g, sizeof(double) * n);
     cs->iter = cs->iter % n;
     return GSL_SUCCESS;
+}
+
+/* conjugate_rdx, bracket and brent on caller-supplied functions, so the tests can check
+   them on functions whose minimisers are known in closed form.  The solve above runs the
+   same cg_iterate, bracket and brent. */
+struct OCg {
+    const OFdf* fn;
+    CGState cs;
+    int n;
+};
+
+OCg* o_cg_new(int n)
+{
+    OCg* w = calloc(1, sizeof(OCg));
+    w->n = n;
+    w->cs.x1 = calloc(n ? n : 1, sizeof(double));
+    w->cs.p = calloc(n ? n : 1, sizeof(double));
+    w->cs.prevG = calloc(n ? n : 1, sizeof(double));
+    return w;
+}
+
+void o_cg_free(OCg* w)
+{
+    if (!w) return;
+    free(w->cs.x1); free(w->cs.p); free(w->cs.prevG); free(w);
+}
+
+/* conjugate_rdx_set (conjugate_gradient.cpp:72-86), f only (see the file header) */
+void o_cg_set(OCg* w, const OFdf* fn, const double* x, double* f, double stepSize, double tol)
+{
+    w->fn = fn;
+    w->cs.iter = 0;
+    w->cs.step = stepSize;
+    w->cs.tol = tol;
+    *f = fn->f(fn->ctx, x);
+}
+
+int o_cg_iterate(OCg* w, double* x, double* f, double* g, double* dx)
+{
+    return cg_iterate(w->fn, &w->cs, x, f, g, dx, w->n);
+}
+
+/* the search direction of the last iteration (PR update of conjugate_gradient.cpp:165-182) */
+void o_cg_direction(const OCg* w, double* p)
+{
+    memcpy(p, w->cs.p, sizeof(double) * w->n);
+}
+
+/* bracket / brent on a function of one variable: fn->f receives a one-element x.  The line
+   runs from x = 0 along p = 1 with lambda = -1, so every probe is exactly at the step. */
+void o_bracket(const OFdf* fn, double* a, double* b, double* mid, double* fa, double* fb, double* fmid, int limit)
+{
+    double x0 = 0.0, p = 1.0, x1 = 0.0, dx = 0.0;
+    LineCtx lc = { fn, &x0, &p, -1.0, &x1, &dx, 1 };
+    bracket(&lc, a, b, mid, fa, fb, fmid, limit);
+}
+
+void o_brent(const OFdf* fn, double ax, double bx, double* x, double* fx, double tol, int limit)
+{
+    double x0 = 0.0, p = 1.0, x1 = 0.0, dx = 0.0;
+    LineCtx lc = { fn, &x0, &p, -1.0, &x1, &dx, 1 };
+    brent(&lc, ax, bx, x, fx, tol, limit);
 }
 
 /* ---------- GETSTEP = BFGS_inv: GSL 1.16 vector_bfgs2 ----------
@@ -1114,6 +1190,7 @@ static void connected_region(double* d, int ny, int nx, int y0, int x0)
     free(mask); free(stack);
 }
 
+/* ScharmerFilter (utils.cpp:334-392) on a centred ny x nx grid */
 static void scharmer_filter(ocplx* data, const double* q2, int ny, int nx, double noise_power, double cutoff)
 {
     double noiseFactor = noise_power * ny * nx;
@@ -1182,6 +1259,11 @@ static void restore_full(OState* st, int o, double* full, int normalise)
     o_fft2(tmp, N, N, +1);
     for (int k = 0; k < N2; ++k) full[k] = normalise ? creal(tmp[k]) / N2 : creal(tmp[k]);
     free(ao); free(dn); free(tmp);
+}
+
+void o_scharmer_filter(ocplx* data, const double* q2, int ny, int nx, double noisePower, double cutoff)
+{
+    scharmer_filter(data, q2, ny, nx, noisePower, cutoff);
 }
 
 static void restore_objects(OState* st, double* imgOut)
@@ -1546,7 +1628,7 @@ int o_solve(const OProblem* pr, OResult* res)
         while ((!done || iter < pr->minIter) && iter < maxIterations) {
             iter++;
             status = bfgs ? o_bfgs2_iterate(bfgs, x, &f, g, dx)
-                   : other ? o_step_iterate(other, x, &f, g, dx) : cg_iterate(st, &cs, x, &f, g, dx, nFree);
+                   : other ? o_step_iterate(other, x, &f, g, dx) : cg_iterate(&fn, &cs, x, &f, g, dx, nFree);
             if (failCount > maxFails) { status = GSL_FAILURE; exitLoop = 1; done = 1; break; }
             st->gradMethod = 2;
             if (status == GSL_ENOPROG) { failCount++; break; }
@@ -1616,6 +1698,12 @@ static void grad_common(OState* st, const double* alpha, int nEnabled, double* g
 
 void o_state_grad_vogel(OState* st, const double* alpha, int nEnabled, double* g) { grad_common(st, alpha, nEnabled, g, 2); }
 void o_state_grad_fd(OState* st, const double* alpha, int nEnabled, double* g) { grad_common(st, alpha, nEnabled, g, 1); }
+
+void o_state_restore_full(OState* st, const double* alpha, int o, double* full)
+{
+    o_state_metric(st, alpha);
+    restore_full(st, o, full, 1);
+}
 
 void o_state_restore(OState* st, const double* alpha, double* img)
 {

@@ -2090,6 +2090,9 @@ public:
         // synchronous calls, restoration sums) is the pipeline's main stream.  With kernel
         // timing on, every lane uses the main stream, so each launch's duration is its own.
         timing_ = getenv("MFBD_KERNEL_TIMING") != nullptr;
+        // MFBD_MAX_TILES caps the tile workspace (images per chunk of an evaluation or a
+        // restoration), so that tests can run the multi-chunk paths on small problems
+        if (const char* e = getenv("MFBD_MAX_TILES")) tileCap_ = std::max(1, atoi(e));
         HIPCHK(hipStreamCreate(&lanes_[kSyncLane].stream));
         for (int l = 0; l < kMaxEvalLanes; ++l) {
             if (timing_) lanes_[l].stream = lanes_[kSyncLane].stream;
@@ -2540,6 +2543,9 @@ public:
         sc_->hL.alloc((size_t)n * per);
         for (int b = 0; b < n; b += perChunk) {
             const int m = std::min(perChunk, n - b);
+            // forward() rewrites the lane's pinned table and alpha staging, which the previous
+            // chunk's queued uploads still read: let them finish first
+            if (b) HIPCHK(hipStreamSynchronize(st_));
             forward(&rq[b], m, /*useGamma=*/0);
             HIPCHK(hipMemcpyAsync(sc_->hG.p + 2 * (size_t)b * per, sc_->Pp.p, 2 * (size_t)m * per * sizeof(double),
                                   hipMemcpyDeviceToHost, st_));
@@ -2723,7 +2729,8 @@ private:
         // tile workspace (per lane, plus a real array of half its size) bounded to 4 GiB
         const size_t cap = (size_t)4 << 30;
         const size_t per = (size_t)N2 * sizeof(C);
-        return (int)std::max<size_t>(16, std::min<size_t>(cap / per, (size_t)maxBatch_ * g_.nImg));
+        const size_t t = std::max<size_t>(16, std::min<size_t>(cap / per, (size_t)maxBatch_ * g_.nImg));
+        return (int)(tileCap_ > 0 ? std::min<size_t>(t, (size_t)tileCap_) : t);
     }
 
     // Per-kernel timing (MFBD_KERNEL_TIMING set): an event pair around each launch on
@@ -3309,6 +3316,7 @@ private:
     hipStream_t st_;
     hipEvent_t ev0_, ev1_;
     bool timing_ = false;
+    int tileCap_ = 0;            // MFBD_MAX_TILES (0: no cap beyond the 4 GiB bound)
     std::map<std::string, KernelStat> stats_;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> evPool_;
     struct PendingEv {

@@ -27,6 +27,7 @@ extern "C" {
 #define MFBD_EDEVICE 3       /* HIP runtime error                          */
 #define MFBD_ENONFINITE 4    /* non-finite metric or gradient              */
 #define MFBD_ESTATE 5        /* call out of order (e.g. solve before load) */
+#define MFBD_ESCRATCH 6      /* larger than the fixed scratch of an allocation-free call      */
 
 /* mode bases (GlobalCfg::modeBasis, config.cpp:629) */
 #define MFBD_BASIS_ZERNIKE 0
@@ -204,6 +205,19 @@ int mfbd_mosaic_size(int nPatches, int patchSize, const int* posY, const int* po
  * imgRows x imgCols, the window starting at (firstRow, firstCol) of the untrimmed mosaic. */
 int mfbd_mosaic(int nPatches, int patchSize, const float* patches, const int* posY, const int* posX, int clip,
                 int device, float* img, int* imgRows, int* imgCols, int* firstRow, int* firstCol);
+
+/* mfbd_mosaic without any device or host allocation, for callers that cannot allocate (code
+ * under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No counterpart
+ * in the reference.  Same arguments and results as mfbd_mosaic (bit for bit); the patches,
+ * window, positions and the untrimmed mosaic are staged in a fixed 32 MiB device array that
+ * each GPU holds until the process ends.  Limits (MFBD_ESCRATCH beyond them): patchSize at
+ * most 2048, and 8 (2 patchSize + patchSize^2) + 8 nPatches + 4 nPatches patchSize^2
+ * + 4 rows cols bytes (each of the six regions rounded up to 256 bytes; rows x cols the
+ * untrimmed extent of mfbd_mosaic_size) at most 32 MiB: e.g. 256 patches of 88^2 into a
+ * 1024^2 mosaic take 12.2 MiB.  All calls on one GPU share the array, so calls that
+ * overlap on one GPU (from other host threads) race. */
+int mfbd_mosaic_noalloc(int nPatches, int patchSize, const float* patches, const int* posY, const int* posX, int clip,
+                        int device, float* img, int* imgRows, int* imgCols, int* firstRow, int* firstCol);
 
 /* ---- output files (Object::writeAna / writeFits, object.cpp:1212-1334) ---- */
 /* ANA (.f0) file, uncompressed (Ana::write, fileana.cpp:150-225, 498-560): datyp 0..4 =

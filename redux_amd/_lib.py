@@ -11,7 +11,7 @@ dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
 fp = C.POINTER(C.c_float)
 
-MFBD_OK, MFBD_EINVAL, MFBD_ENOMEM, MFBD_EDEVICE, MFBD_ENONFINITE, MFBD_ESTATE = 0, 1, 2, 3, 4, 5
+MFBD_OK, MFBD_EINVAL, MFBD_ENOMEM, MFBD_EDEVICE, MFBD_ENONFINITE, MFBD_ESTATE, MFBD_ESCRATCH = 0, 1, 2, 3, 4, 5, 6
 FP64, FP32, MIXED = 0, 1, 2
 GRAD_FD, GRAD_VOGEL, GRAD_ADJOINT = 1, 2, 3
 GRAD_SCHEDULE_REFERENCE = 0
@@ -67,6 +67,7 @@ SIGNATURES = {
     "mfbd_file_last_error": (C.c_char_p, []),
     "mfbd_mosaic_size": (C.c_int, [C.c_int, C.c_int, ip, ip, ip, ip]),
     "mfbd_mosaic": (C.c_int, [C.c_int, C.c_int, fp, ip, ip, C.c_int, C.c_int, fp, ip, ip, ip, ip]),
+    "mfbd_mosaic_noalloc": (C.c_int, [C.c_int, C.c_int, fp, ip, ip, C.c_int, C.c_int, fp, ip, ip, ip, ip]),
     "mfbd_setup_last_error": (C.c_char_p, []),
     "mfbd_pupil_size": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double, dp, dp, ip]),
     "mfbd_make_pupil": (C.c_int, [C.c_int, C.c_double, C.c_double, dp, dp]),

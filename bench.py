@@ -3,29 +3,28 @@
 Workload (default): 256 overlapping 88x88 patches from a seeded 1024x1024 k^-3
 field, 2 objects (wideband + narrowband with a 1.0-wave peak-to-valley focus
 diversity) x 24 frames sharing wavefronts, 50 KL modes (Kolmogorov, D/r0 = 20),
-Poisson noise at 2000 mean counts, pupil 64 px on a 128 x 128 OTF grid.
+Poisson noise at 2000 mean counts, pupil 64 px on a 128 x 128 OTF grid; the
+reference's solver defaults (FTOL 1e-3, conjugate gradient, MODE_START/STEP 5).
 
 One step = solving every patch of the batch to convergence (Solver::run per
-patch: mode stepping, CG + Brent, restoration) with all patches advancing in
-lockstep; the batch's metric/gradient evaluations go to the device in two alternating
-passes (halves of the batch) so the host steps one half while the device evaluates the
-other.
-A patch counts as converged when its solve exits normally and its restored
-objects match the FP64 solve of the same patch to relative RMS <= 1e-4.
+patch: mode stepping, CG + Brent, re-cuts, restoration) with all patches advancing in
+lockstep; the batch's metric/gradient evaluations go to the device in alternating
+lanes (contiguous shares of the batch) so the host steps one share while the device
+evaluates the others.
 
-`value` is the batched FP64 form: the reference's arithmetic, bit-identical to the
-plain drop-in (one patch at a time; checked live on a slice of the batch), so every
-normally converged patch meets the criterion.  The FP32 and mixed (BF16 mode GEMM)
-forms are reported beside it with their converged counts and object errors against
-the FP64 objects: their line searches take other integer re-cut decisions, so their
-objects differ by ~1e-3 and few or none meet 1e-4.
+The timed form is the batched FP32 form (FP32 phase GEMM and transforms, FP64 P/Q,
+metric and CG).  A patch counts as converged when its solve exits normally and its
+restored objects are within 1e-4 (relative RMS) of the FP64 oracle's solve of the same
+patch; the oracle solves a fixed sample of the batch (its timing is the CPU baseline),
+and `value` is the FP32 rate times the sample's converged fraction.  Beside it: the
+plain FP64 drop-in (one patch at a time, on the sample patches), the batched FP64 and
+mixed (BF16 mode GEMM) forms, each with the same check against the oracle.
 
 Each form gets a context with its patches loaded (frames resident in HBM); a step is
 one mfbd_solve of every loaded patch from zero alphas, so warmup and timed steps repeat
 the whole computation on the same context.  Per-kernel HIP-event timings (and the
-roofline of the dominant kernel, FP64 and FP32) come from one step on a separate
-context that brackets every kernel with events; the timed steps run without them.
-Also reported: the CPU FP64 restatement (oracle) timed on the host cores as a baseline.
+rooflines) come from one step per form on a separate context that brackets every kernel
+with events; the timed steps run without them.
 """
 from __future__ import annotations
 
@@ -170,52 +169,6 @@ def roofline(stats, N, nPS, nH, form):
             out.update(fft_tflops=round(tf, 2), fft_peak_tflops=VECTOR_PEAK_TFLOPS,
                        fft_frac=round(tf / VECTOR_PEAK_TFLOPS, 4))
     return out
-
-
-def cpu_baseline(pbs, evals_f, evals_g, seconds, threads):
-    """The FP64 oracle (C restatement) on host threads, one patch per thread, timing
-    metric and Vogel-gradient evaluations for about `seconds`; converted to patches/s
-    with the mean f/gradient evaluation counts per converged patch of the FP64 solve."""
-    from concurrent.futures import ThreadPoolExecutor
-
-    from oracle import oracle as O
-    O.build()
-    n = min(threads, len(pbs))
-    states = [O.State(pbs[k]) for k in range(n)]
-    alpha = np.zeros_like(pbs[0]["truth"]["alpha"])
-
-    def work(k):
-        st = states[k]
-        t0 = time.perf_counter()
-        nf = ng = 0
-        tf = tg = 0.0
-        while time.perf_counter() - t0 < seconds:
-            t1 = time.perf_counter()
-            st.metric(alpha)
-            tf += time.perf_counter() - t1
-            nf += 1
-            if nf % 8 == 0:
-                t1 = time.perf_counter()
-                st.grad(alpha, method="vogel")
-                tg += time.perf_counter() - t1
-                ng += 1
-        return nf, ng, tf, tg
-
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=n) as ex:
-        parts = list(ex.map(work, range(n)))
-    wall = time.perf_counter() - t0
-    nf = sum(p[0] for p in parts)
-    ng = sum(p[1] for p in parts)
-    tf = sum(p[2] for p in parts) / max(nf, 1)
-    tg = sum(p[3] for p in parts) / max(ng, 1)
-    per_patch = evals_f * tf + evals_g * tg            # CPU seconds per converged patch on one core
-    rate = n / per_patch                               # n patches in parallel on n cores
-    return dict(value=round(rate, 5), unit="patches/s", cores=n, kind="port",
-                sample=(f"oracle FP64 on {n} host threads for {wall:.0f}s: {nf} metric evals "
-                        f"({tf * 1e3:.1f} ms each), {ng} Vogel gradients ({tg * 1e3:.1f} ms each) on config-2 "
-                        f"patches; rate = threads / (mean per-patch evals {evals_f:.0f} f + {evals_g:.0f} grad "
-                        f"of the FP64 solve)"))
 
 
 COPY_PROBE_BYTES = 256 << 20
@@ -422,9 +375,11 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--patches", type=int, default=256)
-    ap.add_argument("--plain-patches", type=int, default=4)
+    ap.add_argument("--plain-patches", type=int, default=16)
+    ap.add_argument("--oracle-sample", type=int, default=16,
+                    help="patches of the batch solved by the FP64 oracle for the convergence check and the "
+                         "CPU baseline")
     ap.add_argument("--gradient", choices=["reference", "adjoint"], default="reference")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--workload", choices=["cfg2", "uneven", "cfg5"], default="cfg2",
                     help="cfg2: the headline (256 patches of the benchmark geometry); uneven: the uneven batch "
@@ -432,7 +387,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="fp64",
                     help="form of the uneven workload (mixed: BF16 mode GEMM, FP32 transforms; errors reported "
                          "against the FP64 form)")
-    ap.add_argument("--pmc-probe", choices=["fp32", "fp64"], default=None,
+    ap.add_argument("--pmc-probe", choices=["fp32", "fp64", "mixed"], default=None,
                     help="run the short workload profiled for HBM traffic (FETCH_SIZE / WRITE_SIZE) in one "
                          "precision form and exit")
     args = ap.parse_args()
@@ -449,78 +404,132 @@ def main():
     if args.workload == "cfg5":
         cfg5(args)
         return
+    headline(args)
+
+
+def oracle_sample(pbs, n_sample, threads):
+    """Oracle (FP64 C restatement) solves of a fixed sample of the batch, every
+    (len/n_sample)-th patch, on `threads` host threads, one patch per thread at a time:
+    the reference objects and alphas of the convergence check, and, timed, the CPU
+    baseline."""
+    from oracle import oracle as O
+    O.build()
+    idx = list(range(0, len(pbs), max(1, len(pbs) // n_sample)))[:n_sample]
+    t0 = time.perf_counter()
+    res = O.solve_many([pbs[k] for k in idx], threads=threads)
+    wall = time.perf_counter() - t0
+    return idx, res, wall
+
+
+def against_oracle(res, idx, ores):
+    """Relative RMS errors of a form's restored objects and alphas against the oracle's solves
+    of the sample patches; a patch converges when its solve exits normally and its objects
+    are within 1e-4."""
+    obj, alp, ok = [], [], 0
+    for k, o in zip(idx, ores):
+        r = res[k]
+        e = float(np.linalg.norm(r["img"].astype(np.float64) - o["img"]) / np.linalg.norm(o["img"]))
+        a = float(np.linalg.norm(r["alpha"] - o["alpha"]) / np.linalg.norm(o["alpha"]))
+        obj.append(e)
+        alp.append(a)
+        ok += int(r["status"] == 0 and o["status"] == 0 and e <= 1e-4)
+    return {"sample": len(idx), "within_1e-4": ok, "fraction": round(ok / len(idx), 4),
+            "obj_rel_err_median": float(np.median(obj)), "obj_rel_err_max": float(np.max(obj)),
+            "alpha_rel_err_median": float(np.median(alp)), "alpha_rel_err_max": float(np.max(alp))}
+
+
+def stage_roofline(stats, N, P, form):
+    """The OTF + P/Q + metric stage (pupil field, both transforms, P/Q sums, metric) against the
+    stage bytes per image-evaluation of the path's roofline table: read phi on the P x P
+    pupil grid and D_j on the N x N grid, r P^2 + c N^2 (c, r the complex and real sizes)."""
+    c, r = (16, 8) if form == "fp64" else (8, 4)
+    names = [k for k in stats if k in ("k_otf64_fused", "k_otf_fused", "k_pq", "k_metric", "k_field_rows",
+                                       "k_cols_power", "k_rows_inv_real", "k_cols_extract")]
+    ms = sum(stats[k]["ms"] for k in names)
+    imgs = stats.get("k_otf64_fused", stats.get("k_otf_fused", {"units": 0}))["units"]
+    per = r * P * P + c * N * N
+    ach = per * imgs / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return dict(kernels=names, ms_per_step=round(ms, 3), image_evaluations=int(imgs), bytes_per_image=per,
+                achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4))
+
+
+def headline(args):
+    """The headline workload: converged patches/s of the batched FP32 form, where a patch
+    converges when its solve exits normally and its restored objects are within 1e-4
+    (relative RMS) of the FP64 oracle's solve of the same patch, checked on a fixed sample
+    of the batch; beside it the plain FP64 drop-in (one patch at a time), the batched FP64
+    and mixed forms, all with the same check."""
     workers = min(16, os.cpu_count() or 1)
     t0 = time.perf_counter()
     su, pbs = make_batch(args.patches, args.seed, workers)
     log(f"generated {len(pbs)} patches in {time.perf_counter() - t0:.1f}s")
+    settings = pbs[0]["settings"]
 
-    # Kernel statistics: one step of batched FP64 on a context that brackets every
-    # kernel with HIP events (which costs a few percent of a step); its results are the
-    # FP64 reference objects.
-    sv = make_solver(pbs, "fp64", len(pbs), args.gradient, timing=True)
-    ref, _ = run_step(sv)
-    stats = sv.kernel_stats()
-    nps, nos, nh = (int(x) for x in sv.debug("sizes"))
-    sv.close()
-    ev_f = float(np.mean([r["nF"] for r in ref]))
-    ev_g = float(np.mean([r["nDF"] for r in ref]))
+    # Oracle solves of the sample (reference objects and the timed CPU baseline)
+    threads = min(16, os.cpu_count() or 1)
+    idx, ores, owall = oracle_sample(pbs, args.oracle_sample, threads)
+    log(f"oracle: {len(idx)} patches on {threads} threads in {owall:.1f}s")
+    cpu = dict(value=round(len(idx) / owall, 5), unit="patches/s", cores=threads, kind="port",
+               sample=f"oracle FP64 solves (C restatement of Solver::run, to convergence, with restoration) of "
+                      f"{len(idx)} of the {len(pbs)} patches (every {max(1, len(pbs) // args.oracle_sample)}th), "
+                      f"{threads} host threads, one patch per thread: {owall:.1f}s wall")
 
-    # Timed form: batched FP64 on a context without kernel events (warmup steps, then the
-    # timed steps).  It computes the reference's arithmetic and is bit-identical to the
-    # plain drop-in (checked below on the plain slice), so every normally converged patch
-    # meets the 1e-4 criterion; no FP32-FFT form can (the FP32 and mixed forms are
-    # reported beside it).
-    sv = make_solver(pbs, "fp64", len(pbs), args.gradient)
+    # Per-kernel statistics: one step of each form on a context that brackets every kernel
+    # with HIP events and keeps its evaluation lanes on one stream
+    def stats_step(prec):
+        sv = make_solver(pbs, prec, len(pbs), args.gradient, timing=True)
+        res, _ = run_step(sv)
+        st = sv.kernel_stats()
+        sizes = [int(x) for x in sv.debug("sizes")]
+        sv.close()
+        return res, st, sizes
+
+    ref64, st64, (nps, nos, nh) = stats_step("fp64")
+    ref32, st32, _ = stats_step("fp32")
+
+    # Timed form: batched FP32 (warmup steps, then the timed steps) on a context without events
+    sv = make_solver(pbs, "fp32", len(pbs), args.gradient)
     for _ in range(args.warmup):
         _, dt = run_step(sv)
-        log(f"fp64 batched warmup: {dt:.2f}s")
+        log(f"fp32 batched warmup: {dt:.2f}s")
     times = []
     for _ in range(args.steps):
-        res64, dt = run_step(sv)
+        res32, dt = run_step(sv)
         times.append(dt)
-        log(f"fp64 batched step: {dt:.2f}s")
+        log(f"fp32 batched step: {dt:.2f}s")
     sv.close()
-    t64b = max(times)
-    same_runs = all(np.array_equal(a["img"], b["img"]) and np.array_equal(a["alpha"], b["alpha"])
-                    for a, b in zip(res64, ref))
-    conv64 = int(sum(1 for r in res64 if r["status"] == 0))
-    value = conv64 / t64b
+    t32 = max(times)
+    same32 = all(np.array_equal(a["img"], b["img"]) for a, b in zip(res32, ref32))
+    chk32 = against_oracle(res32, idx, ores)
+    rate32 = len(pbs) / t32
+    value = rate32 * chk32["fraction"]
 
-    # plain FP64 drop-in: one patch at a time; its objects must equal the batched ones
-    plain = pbs[: args.plain_patches]
-    sv = make_solver(plain, "fp64", 1, args.gradient)
-    resp, t64p = run_step(sv)
-    sv.close()
-    plain_rate = len(plain) / t64p
-    plain_identical = all(np.array_equal(a["img"], b["img"]) and np.array_equal(a["alpha"], b["alpha"])
-                          for a, b in zip(resp, ref))
-    log(f"fp64 plain: {len(plain)} patches in {t64p:.2f}s, identical to batched: {plain_identical}")
-
-    def form(prec, stats=False):
-        st = None
-        if stats:
-            sv = make_solver(pbs, prec, len(pbs), args.gradient, timing=True)
-            run_step(sv)
-            st = sv.kernel_stats()
-            sv.close()
+    def one_step(prec):
         sv = make_solver(pbs, prec, len(pbs), args.gradient)
         res, dt = run_step(sv)
         sv.close()
-        errs = np.array([np.linalg.norm(a["img"].astype(np.float64) - b["img"]) / np.linalg.norm(b["img"])
-                         if a["status"] == 0 else np.inf for a, b in zip(res, ref)])
-        fin = errs[np.isfinite(errs)]
         log(f"{prec} batched: {dt:.2f}s")
-        return dt, st, dict(patches_per_s=round(len(pbs) / dt, 4), converged=int(np.sum(errs <= 1e-4)),
-                            converged_patches_per_s=round(float(np.sum(errs <= 1e-4)) / dt, 4),
-                            obj_rel_err_median=float(np.median(fin)) if len(fin) else None,
-                            obj_rel_err_max=float(np.max(errs)))
+        return res, dt
 
-    t32, st32, f32 = form("fp32", stats=True)
-    tmx, _, fmx = form("mixed")
+    res64, t64 = one_step("fp64")
+    resmx, tmx = one_step("mixed")
+    chk64, chkmx = against_oracle(res64, idx, ores), against_oracle(resmx, idx, ores)
+    same64 = all(np.array_equal(a["img"], b["img"]) and np.array_equal(a["alpha"], b["alpha"])
+                 for a, b in zip(res64, ref64))
 
-    roof = roofline(stats, su["N"], nps, nh, "fp64")
-    roof32 = roofline(st32, su["N"], nps, nh, "fp32")
-    cpu = cpu_baseline(pbs, ev_f, ev_g, args.cpu_seconds, min(16, os.cpu_count() or 1))
+    # plain FP64 drop-in: the sample patches one at a time; their results must equal the batch's
+    plain = [pbs[k] for k in idx[: args.plain_patches]]
+    sv = make_solver(plain, "fp64", 1, args.gradient)
+    resp, tp = run_step(sv)
+    sv.close()
+    plain_rate = len(plain) / tp
+    plain_identical = all(np.array_equal(a["img"], res64[k]["img"]) and np.array_equal(a["alpha"], res64[k]["alpha"])
+                          for a, k in zip(resp, idx))
+    log(f"fp64 plain: {len(plain)} patches in {tp:.2f}s, identical to batched: {plain_identical}")
+    chk_plain = against_oracle({k: r for k, r in zip(idx, resp)}, idx[: len(plain)], ores[: len(plain)])
+
+    ev32 = {"f": float(np.mean([r["nF"] for r in res32])), "grad": float(np.mean([r["nDF"] for r in res32]))}
+    ev64 = {"f": float(np.mean([r["nF"] for r in res64])), "grad": float(np.mean([r["nDF"] for r in res64]))}
     line = {
         "metric": "converged patches/s (rel. obj RMS \u22641e-4 vs FP64) batched FP32 vs plain FP64",
         "value": round(value, 4),
@@ -528,32 +537,44 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(t64b * 1e3, 1),
+        "ms_per_step": round(t32 * 1e3, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"{len(pbs)} patches 88x88 from a 1024^2 k^-3 field, 2 objects x 24 frames, "
                                f"50 KL modes, N=128 OTF grid, pupil 64 px, D/r0=20, 2000 counts",
-                   "form": "batched FP64 (the only form meeting the 1e-4 criterion; bit-identical to the plain "
-                           "drop-in)",
-                   "patches": len(pbs), "gradient": args.gradient, "parallelism": "batch"},
-        "converged": conv64,
-        "timed_steps_identical_to_stats_step": bool(same_runs),
-        "fp64_plain_patches_per_s": round(plain_rate, 4),
-        "fp64_plain_identical_to_batched": bool(plain_identical),
-        "speedup_fp64_batched_vs_fp64_plain": round((len(pbs) / t64b) / plain_rate, 2),
-        "fp32_batched": f32,
-        "speedup_fp32_batched_vs_fp64_plain": round((len(pbs) / t32) / plain_rate, 2),
-        "mixed_bf16_batched": fmx,
-        "evals_per_patch": {"f": ev_f, "grad": ev_g},
-        "kernels_ms_per_step": {k: round(v["ms"], 3) for k, v in sorted(stats.items(), key=lambda kv: -kv[1]["ms"])},
-        "kernel_stats_note": "one step on a context with HIP events around every kernel and its two evaluation "
+                   "form": "batched FP32 (FP32 phase GEMM and transforms, FP64 P/Q/metric reductions and CG)",
+                   "solver": {k: settings[k] for k in ("FTOL", "EPS", "MIN_ITER", "MAX_ITER", "N_DONE_ITER",
+                                                       "MODE_START", "MODE_STEP", "GRADDIFF_STEP")}
+                   | {"GETSTEP": "conjugate", "gradient": args.gradient},
+                   "converged": "solve exits normally and restored objects within 1e-4 (relative RMS) of the "
+                                f"FP64 oracle's solve, on a sample of {len(idx)} patches; value = patches/s x "
+                                "sampled fraction",
+                   "patches": len(pbs), "parallelism": "batch"},
+        "fp32_batched": dict(patches_per_s=round(rate32, 4), vs_oracle=chk32, evals_per_patch=ev32,
+                             timed_steps_identical_to_stats_step=bool(same32)),
+        "fp64_batched": dict(patches_per_s=round(len(pbs) / t64, 4), vs_oracle=chk64, evals_per_patch=ev64,
+                             converged_patches_per_s=round(len(pbs) / t64 * chk64["fraction"], 4),
+                             identical_to_stats_step=bool(same64)),
+        "mixed_bf16_batched": dict(patches_per_s=round(len(pbs) / tmx, 4), vs_oracle=chkmx,
+                                   converged_patches_per_s=round(len(pbs) / tmx * chkmx["fraction"], 4)),
+        "fp64_plain": dict(patches_per_s=round(plain_rate, 4), patches=len(plain),
+                           identical_to_fp64_batched=bool(plain_identical), vs_oracle=chk_plain),
+        "ratio_fp32_batched_vs_fp64_plain": {"raw": round(rate32 / plain_rate, 2),
+                                             "converged": round(value / (plain_rate * chk_plain["fraction"]), 2)
+                                             if chk_plain["fraction"] > 0 else None},
+        "ratio_fp64_batched_vs_fp64_plain": round((len(pbs) / t64) / plain_rate, 2),
+        "kernels_ms_per_step": {"fp32": {k: round(v["ms"], 3) for k, v in sorted(st32.items(), key=lambda kv: -kv[1]["ms"])},
+                                "fp64": {k: round(v["ms"], 3) for k, v in sorted(st64.items(), key=lambda kv: -kv[1]["ms"])}},
+        "kernel_stats_note": "one step per form on a context with HIP events around every kernel and its evaluation "
                              "lanes on one stream, so each launch's duration is its own; the timed steps run "
-                             "without events and with the lanes' kernels overlapping on two streams",
-        "roofline": roof,
-        "fp32_roofline": roof32,
+                             "without events and with the lanes' kernels overlapping on their own streams",
+        "roofline": roofline(st32, su["N"], nps, nh, "fp32"),
+        "stage_roofline": {"fp32": stage_roofline(st32, su["N"], su["P"], "fp32"),
+                           "fp64": stage_roofline(st64, su["N"], su["P"], "fp64")},
+        "fp64_roofline": roofline(st64, su["N"], nps, nh, "fp64"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

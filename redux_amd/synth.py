@@ -159,9 +159,10 @@ def build_problem(cfg: dict, seed: int = 1, setup=None, field=None, field_offset
     return build_patch(build_setup(cfg, setup), seed, field, field_offset)
 
 
-def build_field_batch(cfg: dict, n_patches: int, seed: int = 2024, field_size: int = 1024, setup=None):
+def build_field_batch(cfg: dict, n_patches: int, seed: int = 2024, field_size: int = 1024, setup=None, picks=None):
     """n_patches overlapping patches on a square grid over one seeded k^-3 field per
-    object (the benchmark's config-2 layout: 256 patches from a 1024 x 1024 field)."""
+    object (the benchmark's config-2 layout: 256 patches from a 1024 x 1024 field); with
+    `picks`, only those patches of the layout (same seeds and positions)."""
     su = build_setup(cfg, setup)
     nO = cfg.get("n_objects", 1)
     fields = [grf_field(field_size, seed + 17 * o, mean=cfg.get("mean_counts", 1000.0)) for o in range(nO)]
@@ -170,7 +171,7 @@ def build_field_batch(cfg: dict, n_patches: int, seed: int = 2024, field_size: i
     G = N if N >= s + 2 * margin else int(2 ** math.ceil(math.log2(s + 2 * margin)))
     stride = (field_size - G) // max(side - 1, 1)
     out = []
-    for k in range(n_patches):
+    for k in (range(n_patches) if picks is None else picks):
         iy, ix = divmod(k, side)
         out.append(build_patch(su, seed * 1000 + k, field=fields, field_offset=(iy * stride, ix * stride)))
     return su, out

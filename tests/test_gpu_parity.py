@@ -8,8 +8,10 @@ decides on metric comparisons, and the GPU's and the oracle's metrics differ by
 ~1e-13 relative (FFT algorithm and summation order), which is enough to change Brent
 probes and the iteration count.  Full solves are therefore checked on converged
 quantities: converged status, a final metric no worse than the oracle's (1e-5) and
-within 1e-2 of it, and the restored object within 1e-2.  Bit-exactness is checked
-where it holds by construction: batched == one patch at a time.
+within 1e-4 of it, the restored object within 1e-3 and the alphas within 5e-2 (measured
+on this patch: -6e-6, 2.1e-4; the spread of the oracle's own solve under 1e-12 relative
+metric perturbations is up to 5e-3, tests/test_oracle_sensitivity.py).  Bit-exactness is
+checked where it holds by construction: batched == one patch at a time.
 """
 import numpy as np
 import pytest
@@ -99,9 +101,14 @@ def test_full_solve_fp64_converges_like_the_oracle(golden):
     assert r["nDF"] > 0 and r["nF"] > r["nDF"]
     assert abs(r["initialMetric"] - z["metric"][0]) <= 1e-12 * z["metric"][0]
     rel = (r["finalMetric"] - ref_metric) / ref_metric
-    assert -1e-2 < rel < 1e-5, rel
     err = np.linalg.norm(r["img"] - ref_img) / np.linalg.norm(ref_img)
-    assert err < 1e-2, err
+    aerr = np.linalg.norm(r["alpha"] - z["solve_alpha"]) / np.linalg.norm(z["solve_alpha"])
+    print(f"golden solve: final metric {rel:+.2e}, object {err:.2e}, alpha {aerr:.2e}", flush=True)
+    # measured: final metric -6.0e-6, object 2.1e-4 (on this patch, metric perturbations of
+    # 1e-12 relative move the oracle's own object by up to 5e-3: test_oracle_sensitivity.py)
+    assert -1e-4 < rel < 1e-5, rel
+    assert err < 1e-3, err
+    assert aerr < 5e-2, aerr
     sv.close()
 
 

@@ -48,6 +48,9 @@ extern "C" {
 #define MFBD_PATCH_CONVERGED 0   /* relative-change or gradient-norm criterion met        */
 #define MFBD_PATCH_MAXITER 1     /* last stage stopped at MAX_ITER                        */
 #define MFBD_PATCH_FAILED 2      /* more than 3 failures (GSL_FAILURE path)               */
+#define MFBD_PATCH_NO_PROGRESS 3 /* last stage ended by GSL_ENOPROG (solver.cpp:480-483)  */
+#define MFBD_PATCH_NONFINITE 4   /* last stage ended by a non-finite metric or gradient
+                                    (solver.cpp:488-499)                                  */
 
 typedef struct {
     int s;              /* patch side, NUM_POINTS (Object::patchSize)                   */

@@ -13,6 +13,7 @@ fp = C.POINTER(C.c_float)
 
 MFBD_OK, MFBD_EINVAL, MFBD_ENOMEM, MFBD_EDEVICE, MFBD_ENONFINITE, MFBD_ESTATE, MFBD_ESCRATCH = 0, 1, 2, 3, 4, 5, 6
 FP64, FP32, MIXED = 0, 1, 2
+PATCH_CONVERGED, PATCH_MAXITER, PATCH_FAILED, PATCH_NO_PROGRESS, PATCH_NONFINITE = 0, 1, 2, 3, 4
 GRAD_FD, GRAD_VOGEL, GRAD_ADJOINT = 1, 2, 3
 GRAD_SCHEDULE_REFERENCE = 0
 GETSTEP_STEEPEST, GETSTEP_CONJUGATE, GETSTEP_BFGS, GETSTEP_BFGS_INV = 1, 2, 3, 4

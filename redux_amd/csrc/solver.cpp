@@ -466,6 +466,7 @@ bool adjust_shift(mfbd_ctx* c, int p, int j, const double* a) {
     int* sh = &c->shifts[p][2 * j];
     const double ax = a[tx] + off[0], ay = a[ty] + off[1];
     const double sx = ax * J[0] + ay * J[1], sy = ax * J[2] + ay * J[3];
+    if (!std::isfinite(sx) || !std::isfinite(sy)) return false;     // a failed stage's alphas: no re-cut
     const int thx = -(int)std::lround(sx), thy = -(int)std::lround(sy);
     auto shift1 = [&](int first, int size, int n) {           // Array::shift (array.hpp:1119-1140)
         if (n == 0) return 0;
@@ -1288,6 +1289,7 @@ Task<int> run_patch(PatchRun& r) {
     int failCount = 0, totalIter = 0, maxIterations = 10, status = 0;
     const int maxFails = 3;
     bool lastDone = false;
+    int lastBreak = 0;          // how the last stage's loop ended early: 1 no progress, 2 non-finite
     double initial = co_await my_f(r, r.beta0.data());
     r.metrics.push_back(initial);
     int nImgP = 0;
@@ -1316,6 +1318,7 @@ Task<int> run_patch(PatchRun& r) {
         else f = co_await my_f(r, r.x.data());
         int iter = 0, successCount = 0;
         bool done = false;
+        lastBreak = 0;
         while ((!done || iter < pr.minIter) && iter < maxIterations) {
             iter++;
             status = bfgs ? co_await bfgs2_iterate(r, f)
@@ -1323,12 +1326,12 @@ Task<int> run_patch(PatchRun& r) {
                      : bfgs1 ? co_await bfgs1_iterate(r, f) : co_await cg_iterate(r, f);
             if (failCount > maxFails) { status = ST_FAILURE; exitLoop = true; done = true; break; }
             if (!adjoint) r.gradMethod = MFBD_GRAD_VOGEL;   // Vogel from here on (solver.cpp:478)
-            if (status == ST_ENOPROG) { failCount++; break; }
+            if (status == ST_ENOPROG) { failCount++; lastBreak = 1; break; }
             thisMetric = f;
-            if (!std::isfinite(thisMetric)) { failCount++; break; }
+            if (!std::isfinite(thisMetric)) { failCount++; lastBreak = 2; break; }
             r.metrics.push_back(thisMetric);
             gradNorm = dnrm2(r.g) * gradScale;
-            if (!std::isfinite(gradNorm)) { failCount++; break; }
+            if (!std::isfinite(gradNorm)) { failCount++; lastBreak = 2; break; }
             if (gradNorm < pr.FTOL) { exitLoop = true; done = true; break; }
             if (iter > 1) {
                 double rel = 2.0 * std::fabs(thisMetric - prev) / (std::fabs(thisMetric) + std::fabs(prev) + pr.EPS);
@@ -1356,7 +1359,12 @@ Task<int> run_patch(PatchRun& r) {
     r.res.finalMetric = thisMetric;
     r.res.iterations = totalIter;
     r.res.failCount = failCount;
+    // the exits of solver.cpp:470-516, told apart: more than maxFails failures; the last
+    // stage ended by GSL_ENOPROG or by a non-finite metric or gradient; converged by the
+    // relative-change or gradient-norm test; or stopped at its iteration limit
     r.res.status = (status == ST_FAILURE) ? MFBD_PATCH_FAILED
+                   : lastBreak == 1 ? MFBD_PATCH_NO_PROGRESS
+                   : lastBreak == 2 ? MFBD_PATCH_NONFINITE
                    : (lastDone || maxIterations == 0) ? MFBD_PATCH_CONVERGED : MFBD_PATCH_MAXITER;
     co_return 0;
 }

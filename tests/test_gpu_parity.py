@@ -181,6 +181,7 @@ def test_bfgs_inv_reproduces_the_oracle_outcome(golden):
     r = sv.run()[0]
     sv.close()
     assert ro["status"] == 27 and ro["finalMetric"] == ro["initialMetric"]
+    assert r["status"] == 3           # MFBD_PATCH_NO_PROGRESS: the last stage ended by GSL_ENOPROG
     assert r["failCount"] == ro["failCount"], (r["failCount"], ro["failCount"])
     assert r["finalMetric"] == r["initialMetric"]
     assert abs(r["initialMetric"] - ro["initialMetric"]) <= 1e-12 * ro["initialMetric"]

@@ -236,6 +236,47 @@ int mfbd_write_ana_compressed(const char* path, int datyp, int nx, int ny, const
 /* FITS primary image (Fits::write, filefits.cpp:1756-1818): BITPIX 8, 16, 32, 64, -32 or
  * -64; dims fastest first (NAXIS1 = columns); data in host order, written big-endian. */
 int mfbd_write_fits(const char* path, int bitpix, int ndim, const int* dims, const void* data);
+/* The .momfbd result container (FileMomfbd::write, filemomfbd.cpp:391-564 with
+ * PatchInfo::write, filled as Object::writeMomfbd does, object.cpp:1336-1565): every
+ * number in host byte order behind a one-byte endian flag.  Layout written, in order:
+ *   int8 big-endian flag; int32 length + version string (with its NUL); int32 length +
+ *   modified-time string (ISO "YYYY-MM-DDTHH:MM:SS", UTC); int32 length + time string;
+ *   int32 length + date string; int32 region[4] (1-based first x, last x, first y, last y,
+ *   the union of the patches'); int8 hasModes [float pix2cf, cf2pix; int32 nPH, nModes;
+ *   float pupil[nPH][nPH]; per mode: uint8 type 1 and float mode[nPH][nPH]]; int32 nCh;
+ *   int16 clipStartX[nCh], clipEndX[nCh], clipStartY[nCh], clipEndY[nCh]; int32 nPatchesY,
+ *   nPatchesX, nPoints; per patch (y slow): int32 region[4], offx, offy (0), nCh, nim[nCh],
+ *   dx[nCh], dy[nCh]; int8 1 and float img[ny][nx]; int32 npsf + psf; int32 nobj + convolved
+ *   objects; int32 nres + residuals; int32 nalpha [+ int32 nm + float alpha[nalpha][nm]];
+ *   int32 ndiv [+ int32 nphx, nphy + per diversity: uint8 type 0 and float div[nphy][nphx]];
+ *   then, with names, int32 nFileNames + per name int32 length + the name with its NUL.
+ * The version must be 20190408 or later (the layout above, which the reference writes and
+ * reads from that version on).  Arrays absent (NULL or zero count) are written as counts 0. */
+typedef struct {
+    const char* version;        /* e.g. "20240612.1" (reduxCommitTime's "%4d%02d%02d.%01d")        */
+    const char* modified;       /* ISO time, or NULL for the current UTC time                        */
+    const char* time;           /* observation time of day ("N/A" when unknown)                      */
+    const char* date;           /* observation date                                                  */
+    int nCh;                    /* channels of the object                                            */
+    const int16_t* clip;        /* [4][nCh]: clipStartX, clipEndX, clipStartY, clipEndY (1-based)    */
+    int nPH, nModes;            /* pupil grid side and modes; nPH = 0 or pupil NULL: no modes block   */
+    float pix2cf, cf2pix;       /* ModeSet::shiftToAlpha / alphaToShift of (1, 1), averaged          */
+    const float* pupil;         /* [nPH][nPH]                                                        */
+    const float* modes;         /* [nModes][nPH][nPH]                                                */
+    int nPatchesY, nPatchesX, nPoints;   /* patch grid (y slow) and patch side                        */
+    const int32_t* region;      /* [nPatchesY * nPatchesX][4] 1-based first x, last x, first y, last y */
+    const int32_t* nim;         /* [nCh] images per channel                                          */
+    const int32_t* dx;          /* [patches][nCh] channel offsets (ChannelData::channelOffset)       */
+    const int32_t* dy;          /* [patches][nCh]                                                    */
+    const float* img;           /* [patches][nPoints][nPoints] restored objects                      */
+    int npsf; const float* psf; /* [patches][npsf][nPoints][nPoints]                                 */
+    int nobj; const float* cobj;/* [patches][nobj][nPoints][nPoints] convolved objects               */
+    int nres; const float* res; /* [patches][nres][nPoints][nPoints] residuals                       */
+    int nalpha, nm; const float* alpha;  /* [patches][nalpha][nm]                                   */
+    int ndiv; const float* div; /* [ndiv][nPH][nPH] diversity phases (the same in every patch)       */
+    int nFileNames; const char* const* fileNames;
+} mfbd_momfbd;
+int mfbd_write_momfbd(const char* path, const mfbd_momfbd* m);
 const char* mfbd_file_last_error(void);
 
 /* ---- per-job setup (host; GlobalData/Object::initObject equivalents) ---- */

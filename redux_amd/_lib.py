@@ -40,6 +40,17 @@ class Result(C.Structure):
                 ("failCount", C.c_int), ("initialMetric", C.c_double), ("finalMetric", C.c_double)]
 
 
+class Momfbd(C.Structure):
+    _fields_ = [("version", C.c_char_p), ("modified", C.c_char_p), ("time", C.c_char_p), ("date", C.c_char_p),
+                ("nCh", C.c_int), ("clip", C.POINTER(C.c_int16)), ("nPH", C.c_int), ("nModes", C.c_int),
+                ("pix2cf", C.c_float), ("cf2pix", C.c_float), ("pupil", fp), ("modes", fp),
+                ("nPatchesY", C.c_int), ("nPatchesX", C.c_int), ("nPoints", C.c_int),
+                ("region", ip), ("nim", ip), ("dx", ip), ("dy", ip), ("img", fp),
+                ("npsf", C.c_int), ("psf", fp), ("nobj", C.c_int), ("cobj", fp), ("nres", C.c_int), ("res", fp),
+                ("nalpha", C.c_int), ("nm", C.c_int), ("alpha", fp), ("ndiv", C.c_int), ("div", fp),
+                ("nFileNames", C.c_int), ("fileNames", C.POINTER(C.c_char_p))]
+
+
 _lib = None
 
 # every exported symbol and its (restype, argtypes); include/momfbd_hip.h declares the same set
@@ -66,6 +77,7 @@ SIGNATURES = {
     "mfbd_write_fits": (C.c_int, [C.c_char_p, C.c_int, C.c_int, ip, C.c_void_p]),
     "mfbd_write_ana_compressed": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_char_p, C.c_int]),
     "mfbd_file_last_error": (C.c_char_p, []),
+    "mfbd_write_momfbd": (C.c_int, [C.c_char_p, C.POINTER(Momfbd)]),
     "mfbd_mosaic_size": (C.c_int, [C.c_int, C.c_int, ip, ip, ip, ip]),
     "mfbd_mosaic": (C.c_int, [C.c_int, C.c_int, fp, ip, ip, C.c_int, C.c_int, fp, ip, ip, ip, ip]),
     "mfbd_mosaic_noalloc": (C.c_int, [C.c_int, C.c_int, fp, ip, ip, C.c_int, C.c_int, fp, ip, ip, ip, ip]),

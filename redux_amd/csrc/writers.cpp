@@ -14,7 +14,9 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <ctime>
 #include <string>
 #include <vector>
 
@@ -229,6 +231,136 @@ int mfbd_write_fits(const char* path, int bitpix, int ndim, const int* dims, con
     bool ok = std::fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size() && std::fwrite(body.data(), 1, body.size(), f) == body.size();
     ok &= std::fclose(f) == 0;
     return ok ? MFBD_OK : file_fail(std::string("mfbd_write_fits: write failed: ") + path);
+}
+
+int mfbd_write_momfbd(const char* path, const mfbd_momfbd* m) {
+    if (!path || !m || !m->version || !m->time || !m->date) return file_fail("mfbd_write_momfbd: null argument");
+    const double version = std::atof(m->version);
+    if (version < 20190408.0) return file_fail("mfbd_write_momfbd: version before 20190408 has another layout");
+    const int nP = m->nPatchesY * m->nPatchesX, nCh = m->nCh, pts = m->nPoints;
+    if (m->nPatchesY < 1 || m->nPatchesX < 1 || pts < 1 || nCh < 1 || !m->region || !m->nim || !m->clip || !m->img)
+        return file_fail("mfbd_write_momfbd: patches, channels, regions, clips, image counts and images are required");
+    const bool modes = m->nPH > 0 && m->pupil;
+    if (modes && m->nModes > 0 && !m->modes) return file_fail("mfbd_write_momfbd: modes missing");
+    if ((m->npsf > 0 && !m->psf) || (m->nobj > 0 && !m->cobj) || (m->nres > 0 && !m->res) ||
+        (m->nalpha > 0 && m->nm > 0 && !m->alpha) || (m->ndiv > 0 && (!m->div || m->nPH < 1)) ||
+        (m->nFileNames > 0 && !m->fileNames))
+        return file_fail("mfbd_write_momfbd: a block with a count has no data");
+    for (int k = 0; k < nP; ++k) {
+        const int32_t* r = m->region + 4 * k;
+        if (std::abs(r[1] - r[0]) + 1 != pts || std::abs(r[3] - r[2]) + 1 != pts)
+            return file_fail("mfbd_write_momfbd: a patch region is not nPoints wide");
+    }
+    std::vector<char> out;
+    auto raw = [&](const void* p, size_t n) {
+        const char* c = static_cast<const char*>(p);
+        out.insert(out.end(), c, c + n);
+    };
+    auto i32 = [&](int32_t v) { raw(&v, 4); };
+    auto str = [&](const std::string& t) {      // length with the NUL, then the characters and the NUL
+        i32((int32_t)t.size() + 1);
+        raw(t.c_str(), t.size() + 1);
+    };
+    const uint16_t probe = 1;
+    const char bigEndian = *reinterpret_cast<const char*>(&probe) == 0;
+    raw(&bigEndian, 1);
+    str(m->version);
+    std::string modified;
+    if (m->modified) modified = m->modified;
+    else {                                       // to_iso_extended_string(universal_time())
+        const std::time_t now = std::time(nullptr);
+        std::tm tmv{};
+        gmtime_r(&now, &tmv);
+        char buf[32];
+        std::strftime(buf, sizeof buf, "%Y-%m-%dT%H:%M:%S", &tmv);
+        modified = buf;
+    }
+    str(modified);
+    str(m->time);
+    str(m->date);
+    int32_t reg[4] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
+    for (int k = 0; k < nP; ++k) {
+        int32_t r[4];
+        std::memcpy(r, m->region + 4 * k, sizeof r);
+        if (r[0] > r[1]) std::swap(r[0], r[1]);
+        if (r[2] > r[3]) std::swap(r[2], r[3]);
+        reg[0] = std::min(reg[0], r[0]);
+        reg[1] = std::max(reg[1], r[1]);
+        reg[2] = std::min(reg[2], r[2]);
+        reg[3] = std::max(reg[3], r[3]);
+    }
+    raw(reg, sizeof reg);
+    const char hasModes = modes ? 1 : 0;
+    raw(&hasModes, 1);
+    const size_t ph2 = (size_t)std::max(m->nPH, 0) * std::max(m->nPH, 0);
+    if (modes) {
+        raw(&m->pix2cf, 4);
+        raw(&m->cf2pix, 4);
+        i32(m->nPH);
+        i32(std::max(m->nModes, 0));
+        raw(m->pupil, ph2 * 4);
+        const uint8_t pupilMode = 1;                // mode type 1 (a pupil mode), from 20120201
+        for (int i = 0; i < m->nModes; ++i) {
+            raw(&pupilMode, 1);
+            raw(m->modes + (size_t)i * ph2, ph2 * 4);
+        }
+    }
+    i32(nCh);
+    raw(m->clip, (size_t)4 * nCh * sizeof(int16_t));
+    i32(m->nPatchesY);
+    i32(m->nPatchesX);
+    i32(pts);
+    const size_t img2 = (size_t)pts * pts;
+    const int npsf = m->psf ? std::max(m->npsf, 0) : 0, nobj = m->cobj ? std::max(m->nobj, 0) : 0;
+    const int nres = m->res ? std::max(m->nres, 0) : 0, nalpha = std::max(m->nalpha, 0), nm = std::max(m->nm, 0);
+    const int ndiv = m->div ? std::max(m->ndiv, 0) : 0;
+    for (int k = 0; k < nP; ++k) {
+        int32_t r[4];
+        std::memcpy(r, m->region + 4 * k, sizeof r);
+        if (r[0] > r[1]) std::swap(r[0], r[1]);
+        if (r[2] > r[3]) std::swap(r[2], r[3]);
+        raw(r, sizeof r);
+        i32(0);                                   // offx, offy: never set by Object::writeMomfbd
+        i32(0);
+        i32(nCh);
+        raw(m->nim, (size_t)nCh * 4);
+        const int32_t zeros[1] = {0};
+        for (int c = 0; c < nCh; ++c) raw(m->dx ? m->dx + (size_t)k * nCh + c : zeros, 4);
+        for (int c = 0; c < nCh; ++c) raw(m->dy ? m->dy + (size_t)k * nCh + c : zeros, 4);
+        const char one = 1;
+        raw(&one, 1);
+        raw(m->img + (size_t)k * img2, img2 * 4);
+        i32(npsf);
+        if (npsf) raw(m->psf + (size_t)k * npsf * img2, (size_t)npsf * img2 * 4);
+        i32(nobj);
+        if (nobj) raw(m->cobj + (size_t)k * nobj * img2, (size_t)nobj * img2 * 4);
+        i32(nres);
+        if (nres) raw(m->res + (size_t)k * nres * img2, (size_t)nres * img2 * 4);
+        i32(nalpha);
+        if (nalpha) {
+            i32(nm);
+            if (nm) raw(m->alpha + (size_t)k * nalpha * nm, (size_t)nalpha * nm * 4);
+        }
+        i32(ndiv);
+        if (ndiv) {
+            i32(m->nPH);
+            i32(m->nPH);
+            const uint8_t divType = 0;
+            for (int d = 0; d < ndiv; ++d) {
+                raw(&divType, 1);
+                raw(m->div + (size_t)d * ph2, ph2 * 4);
+            }
+        }
+    }
+    if (m->nFileNames > 0) {
+        i32(m->nFileNames);
+        for (int i = 0; i < m->nFileNames; ++i) str(m->fileNames[i] ? m->fileNames[i] : "");
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return file_fail(std::string("mfbd_write_momfbd: cannot open ") + path);
+    bool ok = std::fwrite(out.data(), 1, out.size(), f) == out.size();
+    ok &= std::fclose(f) == 0;
+    return ok ? MFBD_OK : file_fail(std::string("mfbd_write_momfbd: write failed: ") + path);
 }
 
 }  // extern "C"

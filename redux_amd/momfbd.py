@@ -325,3 +325,69 @@ def write_object(base, mosaic_img, fmt="fits", as_int16=False, alpha=None, text=
     if alpha is not None:
         a = np.asarray(alpha, dtype=np.float32)
         (write_fits if fmt == "fits" else lambda p, x: write_ana(p, x))(str(base) + ".alpha" + ext, a)
+
+
+def write_momfbd(path, img, region, nim, clip, version="20240612.1", date="", time="N/A", modified=None,
+                 dx=None, dy=None, psf=None, cobj=None, res=None, alpha=None, pupil=None, modes=None,
+                 pix2cf=0.0, cf2pix=0.0, div=None, file_names=()):
+    """Object::writeMomfbd (object.cpp:1336-1565) through mfbd_write_momfbd: one object's
+    patches on a [nPatchesY][nPatchesX] grid.
+    img [py][px][s][s] restored objects; region [py][px][4] 1-based first x, last x, first y,
+    last y of each patch; nim [nCh] images per channel; clip [4][nCh] (clipStartX, clipEndX,
+    clipStartY, clipEndY, 1-based); dx, dy [py][px][nCh] channel offsets; psf / cobj / res
+    [py][px][n][s][s]; alpha [py][px][images][modes]; pupil [nPH][nPH] and modes
+    [nModes][nPH][nPH] (the reference writes the modes divided by the wavelength);
+    div [nCh][nPH][nPH]; file_names the frames' file names."""
+    img = np.ascontiguousarray(img, dtype=np.float32)
+    py, px, s = img.shape[0], img.shape[1], img.shape[2]
+    keep = []
+
+    def arr(a, dt, t):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(t))
+
+    m = L.Momfbd()
+    m.version, m.time, m.date = version.encode(), time.encode(), date.encode()
+    m.modified = modified.encode() if modified else None
+    nim = np.asarray(nim, dtype=np.int32)
+    m.nCh = len(nim)
+    m.clip = arr(np.asarray(clip).reshape(4, m.nCh), np.int16, C.c_int16)
+    if pupil is not None:
+        pup = np.asarray(pupil, dtype=np.float32)
+        m.nPH = pup.shape[0]
+        m.pupil = arr(pup, np.float32, C.c_float)
+        if modes is not None:
+            md = np.asarray(modes, dtype=np.float32)
+            m.nModes = md.shape[0]
+            m.modes = arr(md, np.float32, C.c_float)
+        m.pix2cf, m.cf2pix = float(pix2cf), float(cf2pix)
+    m.nPatchesY, m.nPatchesX, m.nPoints = py, px, s
+    m.region = arr(region, np.int32, C.c_int)
+    m.nim = arr(nim, np.int32, C.c_int)
+    m.dx = arr(dx, np.int32, C.c_int)
+    m.dy = arr(dy, np.int32, C.c_int)
+    m.img = arr(img, np.float32, C.c_float)
+    for name, block in (("psf", psf), ("cobj", cobj), ("res", res)):
+        if block is not None:
+            b = np.asarray(block, dtype=np.float32)
+            setattr(m, {"psf": "npsf", "cobj": "nobj", "res": "nres"}[name], b.shape[2])
+            setattr(m, name, arr(b, np.float32, C.c_float))
+    if alpha is not None:
+        a = np.asarray(alpha, dtype=np.float32)
+        m.nalpha, m.nm = a.shape[2], a.shape[3]
+        m.alpha = arr(a, np.float32, C.c_float)
+    if div is not None:
+        d = np.asarray(div, dtype=np.float32)
+        if pupil is None:
+            m.nPH = d.shape[1]
+        m.ndiv = d.shape[0]
+        m.div = arr(d, np.float32, C.c_float)
+    names = (C.c_char_p * max(1, len(file_names)))(*[f.encode() for f in file_names])
+    m.nFileNames = len(file_names)
+    m.fileNames = names if file_names else None
+    rc = L.lib().mfbd_write_momfbd(str(path).encode(), C.byref(m))
+    if rc:
+        raise SolverError(f"write_momfbd failed ({rc}): {L.lib().mfbd_file_last_error().decode()}")

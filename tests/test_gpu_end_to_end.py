@@ -1,11 +1,12 @@
 """The product chain after preprocessing, end to end on one GPU, as a job runs it: patches
 cut from one field, solved as one batch (Solver::run per patch), the optional restoration
 outputs of a patch (Object::restorePatch), the mosaic of the restored patches
-(Object::doMozaic) and the output files (Object::writeFits / writeAna).
+(Object::doMozaic) and the output files (Object::writeFits / writeAna / writeMomfbd).
 
 Checked: every patch converges; each patch's restored object equals the restoration at
 its returned alpha (mfbd_restore); the mosaic equals the oracle's mosaic of the same
-patches bit for bit; the files read back to the mosaic (FITS float32, ANA int16 crunched);
+patches bit for bit; the files read back to the mosaic (FITS float32, ANA int16 crunched), the .momfbd
+container to the patches and their alphas;
 the PSFs at the solution sum to the OTF's central value over the full grid less what falls
 outside the crop."""
 import math
@@ -60,3 +61,19 @@ def test_field_to_files(tmp_path):
     np.testing.assert_array_equal(back, mos)
     momfbd.write_object(tmp_path / "obj", mos, fmt="ana", as_int16=True)
     np.testing.assert_array_equal(decrunch(open(tmp_path / "obj.f0", "rb").read()), mos.astype(np.int16))
+    # the .momfbd container of the same patches (a 2 x 2 grid, y slow): restored objects,
+    # alphas, modes and pupil, read back field by field
+    from tests.test_momfbd_file import Reader
+    ob = pbs[0]["objects"][0]
+    region = np.array([[px[k] + 1, px[k] + s, py[k] + 1, py[k] + s] for k in range(n)], np.int32).reshape(2, 2, 4)
+    alpha = np.stack([r["alpha"] for r in res]).reshape(2, 2, len(pbs[0]["imgCh"]), -1)
+    momfbd.write_momfbd(tmp_path / "obj.momfbd", img=imgs.reshape(2, 2, s, s), region=region,
+                        nim=[len(pbs[0]["imgCh"])], clip=np.array([[1], [field], [1], [field]]), alpha=alpha,
+                        pupil=ob["pupil"], modes=ob["modes"] / ob["wavelength"], date="2024-06-12",
+                        modified="2024-06-12T09:00:00")
+    f = Reader(open(tmp_path / "obj.momfbd", "rb").read()).read()
+    for k in range(n):
+        p = f["patches"][k // 2][k % 2]
+        np.testing.assert_array_equal(p["img"], imgs[k])
+        np.testing.assert_array_equal(p["alpha"], res[k]["alpha"].astype(np.float32))
+    np.testing.assert_array_equal(f["modes"], (ob["modes"] / ob["wavelength"]).astype(np.float32))

@@ -548,7 +548,8 @@ def headline(args):
                    "form": "batched FP32 (FP32 phase GEMM and transforms, FP64 P/Q/metric reductions and CG)",
                    "solver": {k: settings[k] for k in ("FTOL", "EPS", "MIN_ITER", "MAX_ITER", "N_DONE_ITER",
                                                        "MODE_START", "MODE_STEP", "GRADDIFF_STEP")}
-                   | {"GETSTEP": "conjugate", "gradient": args.gradient},
+                   | {"GETSTEP": "conjugate", "gradient": args.gradient,
+                      "line_phases": "FP32 and mixed: probes from base + step * direction (metricAt); FP64: the GEMM per probe"},
                    "converged": "solve exits normally and restored objects within 1e-4 (relative RMS) of the "
                                 f"FP64 oracle's solve, on a sample of {len(idx)} patches; value = patches/s x "
                                 "sampled fraction",

@@ -75,6 +75,11 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
                            for the exact dL/dalpha throughout                              */
     int getstep;        /* GETSTEP: MFBD_GETSTEP_STEEPEST, _CONJUGATE, _BFGS or _BFGS_INV
                            (GlobalCfg::getstepMethod); 0 is the conjugate gradient */
+    int linePhases;     /* 1: the conjugate-gradient line search's probes take their phases as
+                           base + step * direction, both computed once per line
+                           (Solver::my_precalc / metricAt, solver.cpp:246-327, which the
+                           reference leaves disabled), instead of the mode GEMM per probe;
+                           equal in exact arithmetic, the phases differ by rounding          */
 } mfbd_params;
 
 #define MFBD_GRAD_SCHEDULE_REFERENCE 0

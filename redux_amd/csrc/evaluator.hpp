@@ -33,6 +33,15 @@ struct EvalReq {
     int gradMethod = 2;     // 1 finite difference, 2 Vogel, 3 adjoint
     int nEnabled = 0;       // gradient for modes [0, nEnabled)
     const double* alpha = nullptr;   // [nImg][K], tilt offsets already added
+    // line-search probes without the mode GEMM (Solver::my_precalc / metricAt,
+    // solver.cpp:246-327): line 1 starts a line at this slot (base phases from alpha, the
+    // direction's phases B dAlpha) and probes it; line 2 probes the slot's current line.
+    // The probe's phases are base + lineS * direction.  Never with wantGrad.
+    int line = 0;
+    double lineS = 0;
+    bool saveBase = false;           // line 0: its phases also become the slot's line base (a
+                                     // gradient that starts a conjugate-gradient iteration)
+    const double* dAlpha = nullptr;  // line 1: [nImg][K], no tilt offsets
     double f = 0;                    // out: metric
     double* g = nullptr;             // out: gradient [nImg][K] (when wantGrad)
 };

@@ -426,11 +426,22 @@ struct PhaseChunk { static constexpr int value = 13; };        // k-steps whose 
 template <>
 struct PhaseChunk<double> { static constexpr int value = 3; };
 
+// Row q of a phase GEMM goes to phi + q nPS, or with an output table to phi + qOut[q] nPS
+// (qOut[q] >= 0) or phi1 + (-1 - qOut[q]) nPS; a channel row qPhRow[q] < 0 adds nothing.
+template <typename T>
+__device__ __forceinline__ T* phase_out_row(T* phi, T* phi1, const int* qOut, int q, int nPS) {
+    if (!qOut) return phi + (size_t)q * nPS;
+    const int r = qOut[q];
+    return r >= 0 ? phi + (size_t)r * nPS : phi1 + (size_t)(-1 - r) * nPS;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T* __restrict__ B,
                                                     const T* __restrict__ phiCh, const double* __restrict__ alpha,
                                                     const int* __restrict__ qArow, const int* __restrict__ qPhRow,
-                                                    const int* __restrict__ qObj, int nQ, int K, int nPS) {
+                                                    const int* __restrict__ qObj, int nQ, int K, int nPS,
+                                                    T* __restrict__ phi1 = nullptr, const int* __restrict__ qOut = nullptr,
+                                                    const int* __restrict__ qOut2 = nullptr) {
     using V = typename MfmaVec<T>::v4;
     constexpr int KS = PhaseChunk<T>::value;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -445,11 +456,12 @@ __global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int q = min(q0 + MfmaVec<T>::row(lk, v), nQ - 1);
-        const T* pc = phiCh + (size_t)qPhRow[q] * nPS;
+        const int pr = qPhRow[q];
+        const T* pc = phiCh + (size_t)(pr >= 0 ? pr : 0) * nPS;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int col = c0 + 16 * c + li;
-            acc[c][v] = col < nPS ? pc[col] : T(0);
+            acc[c][v] = (pr >= 0 && col < nPS) ? pc[col] : T(0);
         }
     }
     // distinct objects among the wave's 16 rows, ascending (wave-uniform)
@@ -487,7 +499,10 @@ __global__ void __launch_bounds__(256) k_phase_mfma(T* __restrict__ phi, const T
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int q = q0 + MfmaVec<T>::row(lk, v), col = c0 + 16 * c + li;
-            if (q < nQ && col < nPS) phi[(size_t)q * nPS + col] = acc[c][v];
+            if (q < nQ && col < nPS) {
+                phase_out_row(phi, phi1, qOut, q, nPS)[col] = acc[c][v];
+                if (qOut2 && qOut2[q] >= 0) phi1[(size_t)qOut2[q] * nPS + col] = acc[c][v];
+            }
         }
     }
 }
@@ -505,7 +520,10 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 __global__ void __launch_bounds__(256) k_phase_bf16(float* __restrict__ phi, const bf16x8* __restrict__ Bt,
                                                     const float* __restrict__ phiCh, const double* __restrict__ alpha,
                                                     const int* __restrict__ qArow, const int* __restrict__ qPhRow,
-                                                    const int* __restrict__ qObj, int nQ, int K, int Kp, int nPS) {
+                                                    const int* __restrict__ qObj, int nQ, int K, int Kp, int nPS,
+                                                    float* __restrict__ phi1 = nullptr,
+                                                    const int* __restrict__ qOut = nullptr,
+                                                    const int* __restrict__ qOut2 = nullptr) {
     using V = MfmaVec<float>::v4;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
@@ -518,11 +536,12 @@ __global__ void __launch_bounds__(256) k_phase_bf16(float* __restrict__ phi, con
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int q = min(q0 + MfmaVec<float>::row(lk, v), nQ - 1);
-        const float* pc = phiCh + (size_t)qPhRow[q] * nPS;
+        const int pr = qPhRow[q];
+        const float* pc = phiCh + (size_t)(pr >= 0 ? pr : 0) * nPS;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
             const int col = c0 + 16 * c + li;
-            acc[c][v] = col < nPS ? pc[col] : 0.f;
+            acc[c][v] = (pr >= 0 && col < nPS) ? pc[col] : 0.f;
         }
     }
     int o = INT_MAX;
@@ -561,9 +580,26 @@ __global__ void __launch_bounds__(256) k_phase_bf16(float* __restrict__ phi, con
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int q = q0 + MfmaVec<float>::row(lk, v), col = c0 + 16 * c + li;
-            if (q < nQ && col < nPS) phi[(size_t)q * nPS + col] = acc[c][v];
+            if (q < nQ && col < nPS) {
+                phase_out_row(phi, phi1, qOut, q, nPS)[col] = acc[c][v];
+                if (qOut2 && qOut2[q] >= 0) phi1[(size_t)qOut2[q] * nPS + col] = acc[c][v];
+            }
         }
     }
+}
+
+// Phases of line-search probes without the mode GEMM (Solver::my_precalc / metricAt,
+// solver.cpp:246-327): phi[q] = base[row] + s * dir[row] for the line rows i < nL (q = lq[i],
+// row = lrow[i], the slot's line: base phases at the line's start, dir = B dAlpha).
+template <typename T>
+__global__ void k_line_phase(T* __restrict__ phi, const T* __restrict__ line, const int* __restrict__ lq,
+                             const int* __restrict__ lrow, const double* __restrict__ qS, int nL, int nPS,
+                             size_t dirOff) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nL * nPS) return;
+    const int i = (int)(e / nPS), k = (int)(e % nPS), q = lq[i];
+    const size_t src = (size_t)lrow[i] * nPS + k;
+    phi[(size_t)q * nPS + k] = line[src] + (T)qS[q] * line[dirOff + src];
 }
 
 // phiOut[q'] = phi[base(q')] + step * B[obj][mode(q')]   (finite-difference probes)
@@ -1263,12 +1299,17 @@ __device__ __noinline__ void otf_image(cx<float>* __restrict__ S, PhaseSet<(N / 
     }
 }
 
+// With line tables (qLine[q] >= 0: a line-search probe, Solver::metricAt, solver.cpp:294-327)
+// image q's phases are line[row] + s line[dirOff + row] (row = qLine[q], s = qLineS[q]),
+// the slot's line base and direction, read in place of phi.
 template <int N, int NT>
 __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, const float* __restrict__ phi,
                                                   const int* __restrict__ invPs, const float* __restrict__ pupil,
                                                   const int* __restrict__ qImg, const float* __restrict__ imgScale,
                                                   const int* __restrict__ hread, const cx<float>* __restrict__ twg,
-                                                  int nPS, int nOS, int nQ) {
+                                                  int nPS, int nOS, int nQ, const float* __restrict__ line = nullptr,
+                                                  const int* __restrict__ qLine = nullptr,
+                                                  const double* __restrict__ qLineS = nullptr, size_t dirOff = 0) {
     constexpr int P = N / 2, LD = Tile<N>::LD, HC = N / 2 + 1;
     constexpr int IP = (P * P + NT - 1) / NT;                // pupil-grid points per thread
     extern __shared__ cx<float> lds_dyn[];
@@ -1276,22 +1317,37 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
     cx<float>* hs = lds_dyn + (N / 2) * LD;                  // N x HC half spectra, element (y, k) at y*HC + k
     cx<float>* tw = hs + N * HC;                             // twiddles
     for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
-    // persistent: the next image's phases load while the current one is transformed
-    float ph[IP];
+    // persistent: the next image's phases load while the current one is transformed (a line
+    // probe's base and direction separately, combined when the image starts)
+    float ph[IP], pd[IP];
     auto fetch = [&](int qq) {
+        const int lr = (qq < nQ && qLine) ? qLine[qq] : -1;
 #pragma unroll
         for (int i = 0; i < IP; ++i) {
             const int e = threadIdx.x + i * NT;
             const int k = e < P * P ? invPs[e] : -1;
-            ph[i] = (qq < nQ && k >= 0) ? phi[(size_t)qq * nPS + k] : 0.f;
+            if (lr < 0) {
+                ph[i] = (qq < nQ && k >= 0) ? phi[(size_t)qq * nPS + k] : 0.f;
+                pd[i] = 0.f;
+            } else {
+                const size_t src = (size_t)lr * nPS + k;
+                ph[i] = k >= 0 ? line[src] : 0.f;
+                pd[i] = k >= 0 ? line[dirOff + src] : 0.f;
+            }
         }
     };
     fetch(blockIdx.x);
     for (int q = blockIdx.x; q < nQ; q += gridDim.x) {
         const float sc = imgScale[qImg[q]];
         PhaseSet<IP> cur;
+        if (qLine && qLine[q] >= 0) {
+            const float ls = (float)qLineS[q];
 #pragma unroll
-        for (int i = 0; i < IP; ++i) cur.v[i] = ph[i];
+            for (int i = 0; i < IP; ++i) cur.v[i] = ph[i] + ls * pd[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < IP; ++i) cur.v[i] = ph[i];
+        }
         fetch(q + gridDim.x);
         if (sc == 0.f) {        // an image the patch lacks: its OTF is zero
             for (int u = threadIdx.x; u < nOS; u += NT) S[(size_t)q * nOS + u] = {0.f, 0.f};
@@ -2958,11 +3014,21 @@ private:
     }
 
     // OTF chain for nQ virtual images whose phases are in `phi`: S = reorder(IFFT(|FFT(a)|^2))/N^2
-    void otf_chain(const T* phi, const int* qImg, int nQ, C* Sout) {
+    // line-search probes read by the fused FP32 transform: line buffers, per image the line
+    // row (-1: phases from phi) and the probe step, the direction's offset
+    struct LineIn {
+        const T* line;
+        const int* qLine;
+        const double* qS;
+        size_t dirOff;
+    };
+
+    void otf_chain(const T* phi, const int* qImg, int nQ, C* Sout, LineIn lin = {}) {
         if constexpr (FUSED) {
             launch("k_otf_fused", nQ, [&] {
                 hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(std::min(nQ, otfGrid_)), dim3(FNT), OLDS, st_, Sout, phi,
-                                   invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_, nQ);
+                                   invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_, nQ, lin.line,
+                                   lin.qLine, lin.qS, lin.dirOff);
             });
             return;
         }
@@ -2996,19 +3062,37 @@ private:
         const int K = g_.K, nImg = g_.nImg, nQ = n * nImg;
         const size_t row = (size_t)nImg * K;
         bool staged = true;
-        int lo = maxSlots_, hi = -1;
+        int lo = maxSlots_, hi = -1, nStart = 0, nLine = 0, nSave = 0;
         for (int r = 0; r < n; ++r) {
             staged &= rq[r].alpha == hStage_.p + (size_t)rq[r].slot * row;
             lo = std::min(lo, rq[r].slot);
             hi = std::max(hi, rq[r].slot);
+            nStart += rq[r].line == 1;
+            nLine += rq[r].line != 0;
+            nSave += rq[r].line == 0 && rq[r].saveBase;
+            if (rq[r].line == 1 && !rq[r].dAlpha) throw std::runtime_error("forward: line start without a direction");
         }
-        // one pinned block and one copy per evaluation: the caller's doubles xd, then the
-        // request tables (6 nQ + n ints), then the caller's ints xi
+        const bool rowsMode = nLine || nSave;      // the GEMM through row tables (lines in the slots' buffers)
+        if (rowsMode) {      // a line lives in its slot's buffers: one request per slot
+            std::vector<char> seen(maxSlots_, 0);
+            for (int r = 0; r < n; ++r) {
+                if (seen[rq[r].slot]) throw std::runtime_error("forward: line requests need distinct slots");
+                seen[rq[r].slot] = 1;
+            }
+            lineBuf_.alloc((size_t)2 * maxSlots_ * nImg * nPS_);
+        }
+        // one pinned block and one copy per evaluation: the caller's doubles xd, with line
+        // rows each image's probe step (nQ), then the request tables (7 nQ + n ints), the
+        // caller's ints xi, and in rows mode the GEMM's row tables (5 per GEMM row), the line
+        // rows' (2) and each image's line row (nQ)
         const size_t nd = xd ? xd->size() : 0, nx = xi ? xi->size() : 0;
-        const size_t nInts = 7 * (size_t)nQ + n + nx, nBlk = nd + (nInts + 1) / 2;
+        const int nLR = nLine * nImg, nGemm = nQ - nLR + nStart * nImg;
+        const size_t nd2 = nd + (nLine ? (size_t)nQ : 0);
+        const size_t nInts = 7 * (size_t)nQ + n + nx + (rowsMode ? 5 * (size_t)nGemm + 2 * (size_t)nLR + nQ : 0);
+        const size_t nBlk = nd2 + (nInts + 1) / 2;
         sc_->hTab.alloc(nBlk);
         if (nd) std::memcpy(sc_->hTab.p, xd->data(), nd * sizeof(double));
-        int* ints = reinterpret_cast<int*>(sc_->hTab.p + nd);
+        int* ints = reinterpret_cast<int*>(sc_->hTab.p + nd2);
         for (int r = 0; r < n; ++r) {
             for (int j = 0; j < nImg; ++j) {
                 const int q = r * nImg + j;
@@ -3023,20 +3107,72 @@ private:
             ints[4 * nQ + r] = rq[r].slot;
         }
         if (nx) std::memcpy(ints + 7 * (size_t)nQ + n, xi->data(), nx * sizeof(int));
+        const size_t aRows = staged ? (size_t)(hi - lo + 1) * nImg : (size_t)nQ;   // alpha rows before the directions
+        const int LS = maxSlots_ * nImg;
+        if (rowsMode) {
+            // GEMM rows: phases of the requests without a line to the work array (row q), and
+            // of a gradient request that starts a conjugate-gradient iteration also to its
+            // slot's line base; for each line start the direction B dAlpha (no channel phase)
+            // to the slot's line direction.  Line rows: the probes' phases from the line.
+            int* gA = ints + 7 * (size_t)nQ + n + nx;
+            int* gP = gA + nGemm;
+            int* gO = gP + nGemm;
+            int* gOut = gO + nGemm;
+            int* gOut2 = gOut + nGemm;
+            int* lq = gOut2 + nGemm;
+            int* lrow = lq + nLR;
+            int* qLine = lrow + nLR;
+            double* qS = sc_->hTab.p + nd;
+            int g = 0, d = 0, l = 0;
+            for (int r = 0; r < n; ++r)
+                for (int j = 0; j < nImg; ++j) {
+                    const int q = r * nImg + j, srow = rq[r].slot * nImg + j;
+                    if (rq[r].line == 0) {
+                        gA[g] = ints[2 * nQ + q];
+                        gP[g] = ints[4 * nQ + n + q];
+                        gO[g] = st.imgObj[j];
+                        gOut[g] = q;
+                        gOut2[g] = rq[r].saveBase ? srow : -1;
+                        ++g;
+                    } else if (rq[r].line == 1) {
+                        gA[g] = (int)aRows + d * nImg + j;
+                        gP[g] = -1;
+                        gO[g] = st.imgObj[j];
+                        gOut[g] = -1 - (LS + srow);
+                        gOut2[g] = -1;
+                        ++g;
+                    }
+                    qLine[q] = rq[r].line ? srow : -1;
+                    if (nLine) qS[q] = rq[r].line ? rq[r].lineS : 0.0;
+                    if (rq[r].line) {
+                        lq[l] = q;
+                        lrow[l] = srow;
+                        ++l;
+                    }
+                    if (rq[r].line == 1 && j == nImg - 1) ++d;
+                }
+            if (g != nGemm || l != nLR) throw std::runtime_error("forward: line tables");
+            sc_->hDir.alloc(std::max<size_t>((size_t)nStart * row, 1));
+            d = 0;
+            for (int r = 0; r < n; ++r)
+                if (rq[r].line == 1) std::memcpy(sc_->hDir.p + (size_t)d++ * row, rq[r].dAlpha, sizeof(double) * row);
+        }
+        const size_t aAll = aRows + (size_t)nStart * nImg;
+        sc_->alpha.alloc(std::max(aAll * K, sc_->alpha.n));
         if (staged) {
-            const size_t cnt = (size_t)(hi - lo + 1) * row;
-            sc_->alpha.alloc(std::max(cnt, sc_->alpha.n));
-            HIPCHK(hipMemcpyAsync(sc_->alpha.p, hStage_.p + (size_t)lo * row, cnt * sizeof(double),
+            HIPCHK(hipMemcpyAsync(sc_->alpha.p, hStage_.p + (size_t)lo * row, aRows * K * sizeof(double),
                                   hipMemcpyHostToDevice, st_));
         } else {
             sc_->hAlpha.alloc((size_t)nQ * K);
             for (int r = 0; r < n; ++r) std::memcpy(sc_->hAlpha.p + (size_t)r * row, rq[r].alpha, sizeof(double) * row);
-            sc_->alpha.alloc((size_t)nQ * K);
             HIPCHK(hipMemcpyAsync(sc_->alpha.p, sc_->hAlpha.p, (size_t)nQ * K * sizeof(double), hipMemcpyHostToDevice, st_));
         }
+        if (nStart)
+            HIPCHK(hipMemcpyAsync(sc_->alpha.p + aRows * K, sc_->hDir.p, (size_t)nStart * row * sizeof(double),
+                                  hipMemcpyHostToDevice, st_));
         sc_->tab.alloc(nBlk);
         HIPCHK(hipMemcpyAsync(sc_->tab.p, sc_->hTab.p, nBlk * sizeof(double), hipMemcpyHostToDevice, st_));
-        const int* ti = reinterpret_cast<const int*>(sc_->tab.p + nd);
+        const int* ti = reinterpret_cast<const int*>(sc_->tab.p + nd2);
         sc_->qm = {ti, ti + nQ, ti + 2 * nQ};
         sc_->qReq = ti + 3 * nQ;
         sc_->reqSlot = ti + 4 * nQ;
@@ -3047,21 +3183,46 @@ private:
         sc_->td = sc_->tab.p;
         sc_->phi.alloc((size_t)nQ * nPS_);
         sc_->S.alloc((size_t)nQ * nOS_);
-        if (bf16Phase_) {
-            if constexpr (std::is_same_v<T, float>) {
-                launch("k_phase_bf16", nQ, [&] {
-                    hipLaunchKernelGGL(k_phase_bf16, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_,
-                                       sc_->phi.p, reinterpret_cast<const bf16x8*>(Bt_.p), phiCh_.p, sc_->alpha.p,
-                                       sc_->qm.arow, qPhRow, qObj, nQ, K, (K + 31) & ~31, nPS_);
+        // the mode GEMM: every image of every request, or in rows mode the GEMM rows
+        const int* gA = rowsMode ? sc_->tx + nx : sc_->qm.arow;
+        const int* gP = rowsMode ? gA + nGemm : qPhRow;
+        const int* gO = rowsMode ? gP + nGemm : qObj;
+        const int* gOut = rowsMode ? gO + nGemm : nullptr;
+        const int* gOut2 = rowsMode ? gOut + nGemm : nullptr;
+        const int* lq = rowsMode ? gOut2 + nGemm : nullptr;
+        const int* qLineD = rowsMode ? lq + 2 * nLR : nullptr;
+        const int rows = rowsMode ? nGemm : nQ;
+        T* lineBuf = rowsMode ? lineBuf_.p : nullptr;
+        if (rows > 0) {
+            if (bf16Phase_) {
+                if constexpr (std::is_same_v<T, float>) {
+                    launch("k_phase_bf16", rows, [&] {
+                        hipLaunchKernelGGL(k_phase_bf16, dim3((nPS_ + 63) / 64, (rows + 63) / 64), dim3(256), 0, st_,
+                                           sc_->phi.p, reinterpret_cast<const bf16x8*>(Bt_.p), phiCh_.p, sc_->alpha.p,
+                                           gA, gP, gO, rows, K, (K + 31) & ~31, nPS_, lineBuf, gOut, gOut2);
+                    });
+                }
+            } else {
+                launch("k_phase_mfma", rows, [&] {
+                    hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (rows + 63) / 64), dim3(256), 0, st_,
+                                       sc_->phi.p, B_.p, phiCh_.p, sc_->alpha.p, gA, gP, gO, rows, K, nPS_, lineBuf,
+                                       gOut, gOut2);
                 });
             }
-        } else {
-            launch("k_phase_mfma", nQ, [&] {
-                hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (nQ + 63) / 64), dim3(256), 0, st_,
-                                   sc_->phi.p, B_.p, phiCh_.p, sc_->alpha.p, sc_->qm.arow, qPhRow, qObj, nQ, K, nPS_);
-            });
         }
-        otf_chain(sc_->phi.p, qScale, nQ, sc_->S.p);
+        const double* qSD = nLine ? sc_->tab.p + nd : nullptr;
+        LineIn lin{};
+        if (nLine) {
+            if constexpr (FUSED) {          // the fused FP32 transform reads the line in place of phi
+                lin = {lineBuf_.p, qLineD, qSD, (size_t)maxSlots_ * nImg * nPS_};
+            } else {
+                launch("k_line_phase", nLR, [&] {
+                    hipLaunchKernelGGL(k_line_phase<T>, dim3(nblk((size_t)nLR * nPS_)), dim3(256), 0, st_, sc_->phi.p,
+                                       lineBuf_.p, lq, lq + nLR, qSD, nLR, nPS_, (size_t)maxSlots_ * nImg * nPS_);
+                });
+            }
+        }
+        otf_chain(sc_->phi.p, qScale, nQ, sc_->S.p, lin);
         const int nChunk = (nOS_ + 255) / 256;
         sc_->Pp.alloc((size_t)n * g_.nObj * nOS_ * 2);
         sc_->Qp.alloc((size_t)n * g_.nObj * nOS_);
@@ -3337,6 +3498,7 @@ private:
     DBuf<double> xOut_;
     DBuf<int> xInts_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
+    DBuf<T> lineBuf_;                 // [2][maxSlots][nImg][nPS]: each slot's line base phases, then directions
     int device_ = 0;
     int lr_ = LINES, lc_ = LINES;           // lines per workgroup of the transform passes
     int stageL_[4] = {0, 0, 0, 0};          // ... of the four OTF passes at N = 256 (0: not yet tuned)
@@ -3361,6 +3523,7 @@ private:
         DBuf<double> Pp, Qp, part, alpha, fdOut, statOut, gpart, tab, out;
         DBuf<int> ints, fdInts, initInts;
         HBuf<double> hAlpha, hL, hG, hTab, hOut;   // hTab: the evaluation's tables, hOut: its results
+        HBuf<double> hDir;                         // line starts' directions in alpha
         HBuf<int> hInit;
         QMap qm{};                       // device tables of the current evaluation (in tab)
         const int* qReq = nullptr;

@@ -571,6 +571,10 @@ struct EvalSlot {
     bool wantGrad = false;
     int method = 2, nEnabled = 0;
     double* alphaEff = nullptr;        // this patch's row of the device's pinned alpha staging
+    int line = 0;                      // EvalReq::line: 1 start a line, 2 probe it
+    double lineS = 0;
+    bool saveBase = false;             // EvalReq::saveBase
+    std::vector<double> dAlpha;        // the line's direction in alpha ([nImg][K])
     double f = 0;
     std::vector<double> gAlpha;
     std::coroutine_handle<> h;
@@ -587,6 +591,7 @@ struct PatchRun {
     std::coroutine_handle<> initH;
     // solver state (solver.cpp:134-144, conjugate_gradient.cpp:29-38)
     std::vector<double> alpha, alphaOffset, x, g, dx, x1, p_, prevG, beta0, ga;
+    bool lineStart = false;         // the next line-search probe starts a line (linePhases)
     int iterCG = 0;
     double pnorm = 0, g0norm = 0, step = 0;
     int nEnabled = 0, gradMethod = 1;
@@ -635,15 +640,19 @@ void ns_apply(const NullSpace& ns, const double* in, double* out) {
 Task<double> my_f(PatchRun& r, const double* beta) {   // Solver::my_f (solver.cpp:219-225)
     ns_reverse_add(r.c, *r.ns, beta, r.alphaOffset.data(), r.alpha.data());
     r.ev.wantGrad = false;
+    r.ev.line = 0;
+    r.ev.saveBase = false;
     effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
     co_await EvalAwait{&r};
     r.nF++;
     co_return r.ev.f;
 }
 
-Task<int> my_df(PatchRun& r, const double* beta, double* gb) {   // Solver::my_df (solver.cpp:228-234)
+Task<int> my_df(PatchRun& r, const double* beta, double* gb, bool saveBase = false) {   // Solver::my_df (solver.cpp:228-234)
     ns_reverse_add(r.c, *r.ns, beta, r.alphaOffset.data(), r.alpha.data());
     r.ev.wantGrad = true;
+    r.ev.line = 0;
+    r.ev.saveBase = saveBase;
     r.ev.method = r.gradMethod;
     r.ev.nEnabled = r.nEnabled;
     effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
@@ -653,9 +662,22 @@ Task<int> my_df(PatchRun& r, const double* beta, double* gb) {   // Solver::my_d
     co_return 0;
 }
 
-// value_at / take_step (conjugate_gradient.cpp:110-125)
+// value_at / take_step (conjugate_gradient.cpp:110-125).  With linePhases the probe at
+// x - stp lambda p is evaluated as the line's phases base + s direction, s = -stp lambda
+// (the reference's metricAt, solver.cpp:294-327): the first probe of a line uploads the base
+// alpha (the gradient's point x) and the direction in alpha, N p; later probes send s only.
 Task<double> value_at(PatchRun& r, double stp, double lambda) {
     const int n = r.ns->nFree;
+    if (r.c->prm.linePhases) {
+        r.ev.wantGrad = false;
+        r.ev.saveBase = false;
+        r.ev.line = r.lineStart ? 1 : 2;
+        r.ev.lineS = -stp * lambda;
+        r.lineStart = false;
+        co_await EvalAwait{&r};
+        r.nF++;
+        co_return r.ev.f;
+    }
     for (int k = 0; k < n; ++k) r.dx[k] = 0.0 + (-stp * lambda) * r.p_[k];
     for (int k = 0; k < n; ++k) r.x1[k] = r.x[k] + 1.0 * r.dx[k];
     double f = co_await my_f(r, r.x1.data());
@@ -768,7 +790,7 @@ double dnrm2(const std::vector<double>& v) {
 // conjugate_rdx_iterate (conjugate_gradient.cpp:128-256)
 Task<int> cg_iterate(PatchRun& r, double& f) {
     const int n = r.ns->nFree;
-    co_await my_df(r, r.x.data(), r.g.data());
+    co_await my_df(r, r.x.data(), r.g.data(), r.c->prm.linePhases != 0);   // its phases: the next line's base
     double pnorm = r.pnorm, g0norm = r.g0norm;
     const double g1norm = dnrm2(r.g);
     double pg;
@@ -792,6 +814,14 @@ Task<int> cg_iterate(PatchRun& r, double& f) {
     }
     const double lambda = (pg >= 0.0) ? +1.0 / pnorm : -1.0 / pnorm;
     if (pnorm == 0.0) { std::fill(r.dx.begin(), r.dx.end(), 0.0); co_return ST_ENOPROG; }
+    if (r.c->prm.linePhases) {
+        // the line through x along p: alpha(x - stp lambda p) = alpha(x) + s N p; r.alpha
+        // holds alpha(x) from the gradient evaluation above
+        std::vector<double>& da = r.ev.dAlpha;
+        da.assign(r.alpha.size(), 0.0);
+        for (size_t e = 0; e < r.ns->row.size(); ++e) da[r.ns->row[e]] += r.ns->val[e] * r.p_[r.ns->col[e]];
+        r.lineStart = true;
+    }
     const double fa = f;
     const double stepc = 1E-21 / lambda;
     const double fc = co_await value_at(r, stepc, lambda);
@@ -812,6 +842,8 @@ Task<int> cg_iterate(PatchRun& r, double& f) {
 Task<double> my_fdf(PatchRun& r, const double* beta, double* gb) {
     ns_reverse_add(r.c, *r.ns, beta, r.alphaOffset.data(), r.alpha.data());
     r.ev.wantGrad = true;
+    r.ev.line = 0;
+    r.ev.saveBase = false;
     r.ev.method = r.gradMethod;
     r.ev.nEnabled = r.nEnabled;
     effective_alpha(r.c, r.p, r.alpha.data(), r.ev.alphaEff);
@@ -1892,6 +1924,10 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                     rq[e][k].gradMethod = ev.method;
                     rq[e][k].nEnabled = ev.nEnabled;
                     rq[e][k].g = ev.gAlpha.data();
+                    rq[e][k].line = ev.line;
+                    rq[e][k].lineS = ev.lineS;
+                    rq[e][k].saveBase = ev.saveBase;
+                    rq[e][k].dAlpha = ev.line == 1 ? ev.dAlpha.data() : nullptr;
                 }
                 auto t0 = clk::now();
                 c->dev->evaluate_begin(e, rq[e]);

@@ -42,7 +42,12 @@ class Solver:
     """One context: geometry + structure + basis + constraints; patches are loaded in bulk."""
 
     def __init__(self, pb: dict, precision: str = "fp64", max_batch: int = 1, device: int = 0, settings=None,
-                 gradient: str = "reference"):
+                 gradient: str = "reference", line_phases=None):
+        """line_phases: line-search probes from base + step * direction phases instead of the
+        mode GEMM per probe (mfbd_params.linePhases).  By default on for the FP32 and mixed
+        forms (their transform reads the line in place of the GEMM's output) and off for
+        FP64 (there the probe phases take a pass of their own, which costs what the GEMM
+        saves), so FP64 probes compute the reference's arithmetic."""
         self.pb = pb
         st = default_params(**pb.get("settings", {}))
         if settings:
@@ -58,7 +63,8 @@ class Solver:
                      st["N_DONE_ITER"], st["MODE_START"], st["MODE_STEP"], st["APODIZATION"], int(st["NO_FILTER"]),
                      {"fp64": L.FP64, "fp32": L.FP32, "mixed": L.MIXED}[precision], max_batch,
                      L.GRAD_ADJOINT if gradient == "adjoint" else L.GRAD_SCHEDULE_REFERENCE,
-                     GETSTEP[str(st["GETSTEP"]).lower()])
+                     GETSTEP[str(st["GETSTEP"]).lower()],
+                     int(precision != "fp64" if line_phases is None else bool(line_phases)))
         h = C.c_void_p()
         self._check(L.lib().mfbd_ctx_create(C.byref(g), C.byref(prm), device, C.byref(h)), "ctx_create", None)
         self.h = h

@@ -549,7 +549,9 @@ def headline(args):
                    "solver": {k: settings[k] for k in ("FTOL", "EPS", "MIN_ITER", "MAX_ITER", "N_DONE_ITER",
                                                        "MODE_START", "MODE_STEP", "GRADDIFF_STEP")}
                    | {"GETSTEP": "conjugate", "gradient": args.gradient,
-                      "line_phases": "FP32 and mixed: probes from base + step * direction (metricAt); FP64: the GEMM per probe"},
+                      "line_phases": "FP32 and mixed: probes from base + step * direction (metricAt); FP64: the GEMM per probe",
+                      "line_search": "FP32 and mixed: on the device (bracket + brent state machine, one HIP graph per "
+                                     "round of probes, read back every 4 rounds); FP64: stepped by the host"},
                    "converged": "solve exits normally and restored objects within 1e-4 (relative RMS) of the "
                                 f"FP64 oracle's solve, on a sample of {len(idx)} patches; value = patches/s x "
                                 "sampled fraction",

@@ -80,6 +80,13 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
                            (Solver::my_precalc / metricAt, solver.cpp:246-327, which the
                            reference leaves disabled), instead of the mode GEMM per probe;
                            equal in exact arithmetic, the phases differ by rounding          */
+    int lineRounds;     /* > 0 (with linePhases and the conjugate gradient): each patch's line
+                           search (math::bracket + math::brent as conjugate_rdx_iterate calls
+                           them, helpers.cpp:58-216) runs on the device, one captured HIP
+                           graph per round of probes, and the host reads the lines' states
+                           back every lineRounds rounds; the same probes and result as the
+                           host's line search.  0: the host steps the line search.  Forms
+                           without a fused transform (N = 256) keep the host's.            */
 } mfbd_params;
 
 #define MFBD_GRAD_SCHEDULE_REFERENCE 0

@@ -58,9 +58,35 @@ struct KernelStat {
     long long units = 0;   // images (tiles) processed, summed over launches
 };
 
+// A device line search (conjugate gradient with line phases): a new line of patch `idx` (its
+// index in the lane's list) from metric fa along the direction dAlpha ([nImg][K], no tilt
+// offsets) with conjugate_rdx's lambda; its base phases are those of the slot's last
+// gradient evaluation with saveBase.
+struct LsStart {
+    int idx;
+    double fa, lambda;
+    const double* dAlpha;
+};
+
+struct LsResult {
+    bool done;
+    double step, f;         // the line's end (brent's x and f(x))
+    int probes;             // metric evaluations of the line
+};
+
 class DeviceEvaluator {
 public:
     virtual ~DeviceEvaluator() = default;
+    // Device line search of a lane's patches (slots, fixed for the solve): ls_begin sets up
+    // the lane's state and captures one round (advance every line to its next probe,
+    // evaluate the probes, their metrics) as a hipGraph; ls_start enqueues new lines;
+    // ls_run replays the round `rounds` times; ls_end waits and reports each patch's line.
+    // Returns false from ls_begin when this form has no device line search.
+    virtual bool ls_begin(int lane, const std::vector<int>& slots) = 0;
+    virtual void ls_start(int lane, const std::vector<LsStart>& starts) = 0;
+    virtual void ls_run(int lane, int rounds) = 0;
+    virtual void ls_end(int lane, std::vector<LsResult>& out) = 0;
+    virtual void ls_clear() = 0;
     virtual void set_basis(const double* pupil, const double* modes) = 0;
     virtual void set_structure(const Structure& st, const std::vector<double>& otfNorm) = 0;
     // raw frames of `n` patches into slots [first, first+n); phiCh [n][nCh][P*P]

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "evaluator.hpp"
+#include "line_search.hpp"
 
 namespace mfbd {
 
@@ -588,16 +589,53 @@ __global__ void __launch_bounds__(256) k_phase_bf16(float* __restrict__ phi, con
     }
 }
 
+// One round of the device line search for the nL patches of a lane (one thread each): each
+// patch in a line search takes the metric of its last probe (L[r]) and runs to its next
+// probe, whose step goes to its images' qS (s = -stp lambda) with qLine = its line row; a
+// patch out of a line search marks its images -2 (skipped by every kernel of the round).
+__global__ void k_ls_step(LsState* __restrict__ st, const double* __restrict__ L, int* __restrict__ qLine,
+                          double* __restrict__ qS, int* __restrict__ active, const int* __restrict__ slot, int nL,
+                          int nImg) {
+#pragma clang fp contract(off)
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nL) return;
+    LsState S = st[r];
+    int on = 0;
+    double stp = 0.0;
+    if (S.pc >= 0) on = ls_advance(S, L[r], &stp);
+    S.nProbe += on;
+    st[r] = S;
+    active[r] = on;
+    const double sv = -stp * S.lambda;
+    for (int j = 0; j < nImg; ++j) {
+        qLine[r * nImg + j] = on ? slot[r] * nImg + j : -2;
+        qS[r * nImg + j] = on ? sv : 0.0;
+    }
+}
+
+// New lines: state i of the list starts at metric fa with lambda (a line's probes count from 0)
+__global__ void k_ls_init(LsState* __restrict__ st, const int* __restrict__ idx, const double* __restrict__ fl,
+                          int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    LsState S = {};
+    S.pc = 0;
+    S.fa0 = fl[2 * i];
+    S.lambda = fl[2 * i + 1];
+    st[idx[i]] = S;
+}
+
 // Phases of line-search probes without the mode GEMM (Solver::my_precalc / metricAt,
 // solver.cpp:246-327): phi[q] = base[row] + s * dir[row] for the line rows i < nL (q = lq[i],
 // row = lrow[i], the slot's line: base phases at the line's start, dir = B dAlpha).
 template <typename T>
 __global__ void k_line_phase(T* __restrict__ phi, const T* __restrict__ line, const int* __restrict__ lq,
                              const int* __restrict__ lrow, const double* __restrict__ qS, int nL, int nPS,
-                             size_t dirOff) {
+                             size_t dirOff, const int* __restrict__ qLine = nullptr) {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (size_t)nL * nPS) return;
     const int i = (int)(e / nPS), k = (int)(e % nPS), q = lq[i];
+    if (qLine && qLine[q] < 0) return;          // an image outside this round's probes
     const size_t src = (size_t)lrow[i] * nPS + k;
     phi[(size_t)q * nPS + k] = line[src] + (T)qS[q] * line[dirOff + src];
 }
@@ -677,9 +715,10 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
                                             const int* __restrict__ chObj, const int* __restrict__ chFirst,
                                             const int* __restrict__ chCount, const double* __restrict__ wt,
                                             const T* __restrict__ imgScale, int nCh, int nObj, int nImg, int nOS,
-                                            int useGamma) {
+                                            int useGamma, const int* __restrict__ active = nullptr) {
     __shared__ double sh[256];
     const int r = blockIdx.z, o = blockIdx.y;
+    if (active && !active[r]) return;            // a request not probed this round (the whole block)
     const int u = blockIdx.x * 256 + threadIdx.x;
     const int slot = reqSlot[r];
     double pr = 0, pi = 0, q = 0, part = 0;
@@ -1326,8 +1365,8 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
         for (int i = 0; i < IP; ++i) {
             const int e = threadIdx.x + i * NT;
             const int k = e < P * P ? invPs[e] : -1;
-            if (lr < 0) {
-                ph[i] = (qq < nQ && k >= 0) ? phi[(size_t)qq * nPS + k] : 0.f;
+            if (lr < 0) {                        // -1: phases from phi; -2: an image not probed
+                ph[i] = (qq < nQ && k >= 0 && lr == -1) ? phi[(size_t)qq * nPS + k] : 0.f;
                 pd[i] = 0.f;
             } else {
                 const size_t src = (size_t)lr * nPS + k;
@@ -1338,6 +1377,10 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
     };
     fetch(blockIdx.x);
     for (int q = blockIdx.x; q < nQ; q += gridDim.x) {
+        if (qLine && qLine[q] == -2) {           // not probed this round: no S
+            fetch(q + gridDim.x);
+            continue;
+        }
         const float sc = imgScale[qImg[q]];
         PhaseSet<IP> cur;
         if (qLine && qLine[q] >= 0) {
@@ -1427,7 +1470,7 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
                                                     const int* __restrict__ invPs, const double* __restrict__ pupil,
                                                     const int* __restrict__ qImg, const double* __restrict__ imgScale,
                                                     const int* __restrict__ hread, const cx<double>* __restrict__ twg,
-                                                    int nPS, int nOS, int nQ) {
+                                                    int nPS, int nOS, int nQ, const int* __restrict__ qSkip = nullptr) {
     constexpr int P = N / 2, LD = N + 1, HC = N / 2 + 1;
     constexpr int RF = N == 64 ? 8 : 16, RL = 8;             // radices of the two passes (N = RF RL)
     static_assert(RF * RL == N, "two-pass transforms only (N = 64, 128)");
@@ -1462,6 +1505,10 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
     fetch(blockIdx.x);
     auto at = [](int k, int y) { return y < P ? y * LD + k : (y - P) * LD + (k == 0 ? P : N - k); };
     for (int q = blockIdx.x; q < nQ; q += gridDim.x) {
+        if (qSkip && qSkip[q] == -2) {          // not probed this round (device line search): no S
+            fetch(q + gridDim.x);
+            continue;
+        }
         const double sc = imgScale[qImg[q]];
         if (sc == 0.0) {        // an image the patch lacks: its OTF is zero (t is not touched)
             for (int u = threadIdx.x; u < nOS; u += NT) S[(size_t)q * nOS + u] = {0.0, 0.0};
@@ -2169,6 +2216,9 @@ public:
 
     ~Pipeline() override {
         for (auto& l : lanes_) (void)hipStreamSynchronize(l.stream);
+        ls_clear();
+        for (auto& l : ls_)
+            if (l.done) (void)hipEventDestroy(l.done);
         (void)hipEventDestroy(ev0_);
         (void)hipEventDestroy(ev1_);
         for (auto& l : lanes_) (void)hipEventDestroy(l.done);
@@ -2545,6 +2595,192 @@ public:
     void evaluate(std::vector<EvalReq>& reqs) override {
         evaluate_begin(kSyncLane, reqs);
         evaluate_end(kSyncLane, reqs);
+    }
+
+    // ---- device line search: per lane a fixed list of patches and one captured round ----
+    struct LsLane {
+        int nL = 0;
+        std::vector<int> slots;
+        DBuf<LsState> st;
+        DBuf<double> L, qS, Pp, Qp, part, fl, dirA;
+        DBuf<int> tab, startIdx, dirTab;   // tab: slots [nL] | scale rows [nQ] | qLine [nQ] | active [nL] | lq [nQ] | lrow [nQ]
+        DBuf<C> S;
+        DBuf<T> phi;
+        HBuf<LsState> hSt;
+        HBuf<double> hFl, hDir;
+        HBuf<int> hIdx, hDirTab;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    LsLane ls_[kSyncLane + 1];
+
+    void ls_clear() override {
+        for (auto& l : ls_) {
+            if (l.exec) (void)hipGraphExecDestroy(l.exec);
+            if (l.graph) (void)hipGraphDestroy(l.graph);
+            l.exec = nullptr;
+            l.graph = nullptr;
+            l.nL = 0;
+        }
+    }
+
+    bool ls_begin(int lane, const std::vector<int>& slots) override {
+        if constexpr (!(FUSED || FUSED64)) {
+            return false;
+        } else {
+            use_lane(lane);
+            LsLane& l = ls_[lane];
+            if (l.exec) (void)hipGraphExecDestroy(l.exec);
+            if (l.graph) (void)hipGraphDestroy(l.graph);
+            l.exec = nullptr;
+            l.graph = nullptr;
+            const int nL = (int)slots.size(), nImg = g_.nImg, nQ = nL * nImg;
+            l.nL = nL;
+            l.slots = slots;
+            if (!nL) return true;
+            if (!l.done) HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
+            lineBuf_.alloc((size_t)2 * maxSlots_ * nImg * nPS_);
+            const int nChunk = (nOS_ + 255) / 256;
+            l.st.alloc(nL);
+            l.L.alloc(nL);
+            l.qS.alloc(nQ);
+            l.Pp.alloc((size_t)nL * g_.nObj * nOS_ * 2);
+            l.Qp.alloc((size_t)nL * g_.nObj * nOS_);
+            l.part.alloc((size_t)nL * g_.nObj * nChunk);
+            l.S.alloc((size_t)nQ * nOS_);
+            l.phi.alloc((size_t)nQ * nPS_);
+            std::vector<int> t((size_t)2 * nL + 4 * (size_t)nQ);
+            for (int r = 0; r < nL; ++r) {
+                t[r] = slots[r];
+                for (int j = 0; j < nImg; ++j) {
+                    const int q = r * nImg + j;
+                    t[nL + q] = slots[r] * nImg + j;               // pupil scale row
+                    t[nL + nQ + q] = -2;                           // qLine: no probe yet
+                    t[2 * nL + 2 * nQ + q] = q;                    // lq
+                    t[2 * nL + 3 * nQ + q] = slots[r] * nImg + j;  // lrow
+                }
+                t[nL + 2 * nQ + r] = 0;                            // active
+            }
+            upload(l.tab, t, st_);
+            std::vector<LsState> idle(nL);
+            for (auto& x : idle) { x = LsState{}; x.pc = -1; }
+            l.hSt.alloc(nL);
+            std::copy(idle.begin(), idle.end(), l.hSt.p);
+            HIPCHK(hipMemcpyAsync(l.st.p, l.hSt.p, nL * sizeof(LsState), hipMemcpyHostToDevice, st_));
+            HIPCHK(hipStreamSynchronize(st_));
+            const int* tSlot = l.tab.p;
+            const int* tScale = l.tab.p + nL;
+            int* tLine = l.tab.p + nL + nQ;
+            int* tActive = l.tab.p + nL + 2 * nQ;
+            const int* tLq = l.tab.p + 2 * nL + 2 * nQ;
+            const int* tLrow = tLq + nQ;
+            const size_t dirOff = (size_t)maxSlots_ * nImg * nPS_;
+            // one round: every line to its next probe, the probes' transforms, P / Q and metrics
+            HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+            hipLaunchKernelGGL(k_ls_step, dim3((nL + 63) / 64), dim3(64), 0, st_, l.st.p, l.L.p, tLine, l.qS.p, tActive,
+                               tSlot, nL, nImg);
+            if constexpr (FUSED) {
+                hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(std::min(nQ, otfGrid_)), dim3(FNT), OLDS, st_, l.S.p,
+                                   l.phi.p, invPs_.p, pupil_.p, tScale, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_, nQ,
+                                   lineBuf_.p, tLine, l.qS.p, dirOff);
+            } else {
+                hipLaunchKernelGGL(k_line_phase<T>, dim3(nblk((size_t)nQ * nPS_)), dim3(256), 0, st_, l.phi.p,
+                                   lineBuf_.p, tLq, tLrow, l.qS.p, nQ, nPS_, dirOff, tLine);
+                hipLaunchKernelGGL((k_otf64_fused<N, F64NT>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT), F64LDS, st_,
+                                   l.S.p, l.phi.p, invPs_.p, pupil_.p, tScale, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_,
+                                   nQ, tLine);
+            }
+            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, nL), dim3(256), 0, st_, l.Pp.p, l.Qp.p, l.part.p, l.S.p,
+                               D_.p, ftSum_.p, gamma_.p, tSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, imgScale_.p,
+                               g_.nCh, g_.nObj, nImg, nOS_, 1, tActive);
+            hipLaunchKernelGGL(k_metric, dim3(nblk(nL, 64)), dim3(64), 0, st_, l.L.p, l.part.p, w_.p, nL, g_.nObj,
+                               nChunk, 1.0 / ((double)g_.s * g_.s));
+            HIPCHK(hipStreamEndCapture(st_, &l.graph));
+            HIPCHK(hipGraphInstantiate(&l.exec, l.graph, nullptr, nullptr, 0));
+            return true;
+        }
+    }
+
+    // new lines: their states and the directions' phases B dAlpha (no channel phase) into
+    // the slots' line buffers, on the lane's stream before its next rounds
+    void ls_start(int lane, const std::vector<LsStart>& starts) override {
+        if (starts.empty()) return;
+        use_lane(lane);
+        LsLane& l = ls_[lane];
+        const int n = (int)starts.size(), nImg = g_.nImg, K = g_.K, rows = n * nImg;
+        const size_t row = (size_t)nImg * K;
+        l.hIdx.alloc(n);
+        l.hFl.alloc(2 * (size_t)n);
+        l.hDir.alloc((size_t)n * row);
+        l.hDirTab.alloc(4 * (size_t)rows);
+        const int LS = maxSlots_ * nImg;
+        for (int i = 0; i < n; ++i) {
+            const LsStart& a = starts[i];
+            if (a.idx < 0 || a.idx >= l.nL || !a.dAlpha) throw std::runtime_error("ls_start: bad line");
+            l.hIdx.p[i] = a.idx;
+            l.hFl.p[2 * i] = a.fa;
+            l.hFl.p[2 * i + 1] = a.lambda;
+            std::memcpy(l.hDir.p + (size_t)i * row, a.dAlpha, sizeof(double) * row);
+            for (int j = 0; j < nImg; ++j) {
+                const int g = i * nImg + j;
+                l.hDirTab.p[g] = g;                                        // alpha row
+                l.hDirTab.p[rows + g] = -1;                                // no channel phase
+                l.hDirTab.p[2 * rows + g] = st.imgObj[j];
+                l.hDirTab.p[3 * rows + g] = -1 - (LS + l.slots[a.idx] * nImg + j);   // line direction row
+            }
+        }
+        l.startIdx.alloc(n);
+        l.fl.alloc(2 * (size_t)n);
+        l.dirA.alloc((size_t)n * row);
+        l.dirTab.alloc(4 * (size_t)rows);
+        HIPCHK(hipMemcpyAsync(l.startIdx.p, l.hIdx.p, n * sizeof(int), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(l.fl.p, l.hFl.p, 2 * n * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(l.dirA.p, l.hDir.p, (size_t)n * row * sizeof(double), hipMemcpyHostToDevice, st_));
+        HIPCHK(hipMemcpyAsync(l.dirTab.p, l.hDirTab.p, 4 * (size_t)rows * sizeof(int), hipMemcpyHostToDevice, st_));
+        launch("k_ls_init", n, [&] {
+            hipLaunchKernelGGL(k_ls_init, dim3((n + 63) / 64), dim3(64), 0, st_, l.st.p, l.startIdx.p, l.fl.p, n);
+        });
+        const int* gA = l.dirTab.p;
+        if (bf16Phase_) {
+            if constexpr (std::is_same_v<T, float>) {
+                launch("k_phase_bf16", rows, [&] {
+                    hipLaunchKernelGGL(k_phase_bf16, dim3((nPS_ + 63) / 64, (rows + 63) / 64), dim3(256), 0, st_,
+                                       l.phi.p, reinterpret_cast<const bf16x8*>(Bt_.p), phiCh_.p, l.dirA.p, gA, gA + rows,
+                                       gA + 2 * rows, rows, K, (K + 31) & ~31, nPS_, lineBuf_.p, gA + 3 * rows,
+                                       (const int*)nullptr);
+                });
+            }
+        } else {
+            launch("k_phase_mfma", rows, [&] {
+                hipLaunchKernelGGL(k_phase_mfma<T>, dim3((nPS_ + 63) / 64, (rows + 63) / 64), dim3(256), 0, st_,
+                                   l.phi.p, B_.p, phiCh_.p, l.dirA.p, gA, gA + rows, gA + 2 * rows, rows, K, nPS_,
+                                   lineBuf_.p, gA + 3 * rows, (const int*)nullptr);
+            });
+        }
+    }
+
+    void ls_run(int lane, int rounds) override {
+        use_lane(lane);
+        LsLane& l = ls_[lane];
+        if (!l.exec) throw std::runtime_error("ls_run: no device line search on this lane");
+        launch("k_ls_round", (long long)rounds * l.nL * g_.nImg, [&] {
+            for (int k = 0; k < rounds; ++k) HIPCHK(hipGraphLaunch(l.exec, st_));
+        });
+        l.hSt.alloc(l.nL);
+        HIPCHK(hipMemcpyAsync(l.hSt.p, l.st.p, l.nL * sizeof(LsState), hipMemcpyDeviceToHost, st_));
+        HIPCHK(hipEventRecord(l.done, st_));
+    }
+
+    void ls_end(int lane, std::vector<LsResult>& out) override {
+        LsLane& l = ls_[lane];
+        HIPCHK(hipEventSynchronize(l.done));
+        out.resize(l.nL);
+        for (int r = 0; r < l.nL; ++r) {
+            const LsState& x = l.hSt.p[r];
+            out[r] = {x.pc == -1, x.stepOut, x.fOut, x.nProbe};
+        }
+        collect(lane);
     }
 
     // Enqueue the evaluation of `reqs` on an evaluation lane (kSyncLane: the synchronous calls) and

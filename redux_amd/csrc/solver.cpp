@@ -345,8 +345,9 @@ struct mfbd_ctx {
     // offset); stage-wise calls evaluate on it until mfbd_init_patches or a new load
     bool solved = false;
     std::unique_ptr<Pool> pool;
+    bool deviceLS = false;            // this solve's line searches run on the device (lineRounds)
     // host-side timing of the batch driver: evaluate / resume / re-init phases
-    std::vector<KernelStat> hostStats = std::vector<KernelStat>(6);
+    std::vector<KernelStat> hostStats = std::vector<KernelStat>(7);
 };
 
 namespace {
@@ -592,6 +593,13 @@ struct PatchRun {
     // solver state (solver.cpp:134-144, conjugate_gradient.cpp:29-38)
     std::vector<double> alpha, alphaOffset, x, g, dx, x1, p_, prevG, beta0, ga;
     bool lineStart = false;         // the next line-search probe starts a line (linePhases)
+    struct DeviceLine {             // a line search handed to the device (lineRounds)
+        bool pending = false, active = false;
+        double fa = 0, lambda = 0;  // the line's start: metric at x and 1 / |p| signed downhill
+        double step = 0, f = 0;     // its end: brent's x and f(x)
+        int probes = 0;
+        std::coroutine_handle<> h;
+    } dl;
     int iterCG = 0;
     double pnorm = 0, g0norm = 0, step = 0;
     int nEnabled = 0, gradMethod = 1;
@@ -617,6 +625,13 @@ struct EvalAwait {
     PatchRun* pr;
     bool await_ready() { return false; }
     void await_suspend(std::coroutine_handle<> h) { pr->ev.h = h; pr->ev.pending = true; }
+    void await_resume() {}
+};
+
+struct LineAwait {
+    PatchRun* pr;
+    bool await_ready() { return false; }
+    void await_suspend(std::coroutine_handle<> h) { pr->dl.h = h; pr->dl.pending = true; }
     void await_resume() {}
 };
 
@@ -823,10 +838,20 @@ Task<int> cg_iterate(PatchRun& r, double& f) {
         r.lineStart = true;
     }
     const double fa = f;
-    const double stepc = 1E-21 / lambda;
-    const double fc = co_await value_at(r, stepc, lambda);
-    Bracket B = co_await bracket(r, lambda, Bracket{0.0, stepc, 0.0, fa, fc, 0.0}, 100);
-    BrentOut bo = co_await brent(r, lambda, B.a, B.b, B.mid, 0.1, 100);
+    BrentOut bo;
+    if (r.c->deviceLS) {
+        // the same bracket and brent, stepped on the device from the line's phases
+        r.dl.fa = fa;
+        r.dl.lambda = lambda;
+        co_await LineAwait{&r};
+        r.nF += r.dl.probes;
+        bo = {r.dl.step, r.dl.f};
+    } else {
+        const double stepc = 1E-21 / lambda;
+        const double fc = co_await value_at(r, stepc, lambda);
+        Bracket B = co_await bracket(r, lambda, Bracket{0.0, stepc, 0.0, fa, fc, 0.0}, 100);
+        bo = co_await brent(r, lambda, B.a, B.b, B.mid, 0.1, 100);
+    }
     for (int k = 0; k < n; ++k) r.dx[k] = 0.0 + (-bo.x * lambda) * r.p_[k];
     for (int k = 0; k < n; ++k) r.x1[k] = r.x[k] + 1.0 * r.dx[k];
     r.x = r.x1;
@@ -1465,7 +1490,7 @@ int mfbd_ctx_create(const mfbd_geometry* geo, const mfbd_params* prm, int device
     }
     // 0 (unset) falls to the conjugate gradient, as the reference's switch default does
     if (c->prm.getstep == 0) c->prm.getstep = MFBD_GETSTEP_CONJUGATE;
-    if (c->prm.getstep < MFBD_GETSTEP_STEEPEST || c->prm.getstep > MFBD_GETSTEP_BFGS_INV) {
+    if (c->prm.getstep < MFBD_GETSTEP_STEEPEST || c->prm.getstep > MFBD_GETSTEP_BFGS_INV || c->prm.lineRounds < 0) {
         delete c;
         return MFBD_EINVAL;
     }
@@ -1870,12 +1895,31 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             c->hostStats[3].name = "host_reset_init";
             c->hostStats[4].name = "host_restore";
             c->hostStats[5].name = "host_restore_sums";
+            c->hostStats[6].name = "host_line_search";
             {
                 auto t0 = clk::now();
                 reset_init(c, ids);
                 c->hostStats[3].ms += since(t0);
                 c->hostStats[3].launches++;
                 c->hostStats[3].units += nb;
+            }
+            const int nLanes = std::max(1, std::min(eval_lanes(c->prm.precision), nb));
+            auto laneOf = [&](int k) { return (int)((long long)k * nLanes / nb); };
+            // device line searches: per lane one captured round over the lane's patches
+            std::vector<int> lsIdx(nb, -1);
+            c->deviceLS = false;
+            if (c->prm.lineRounds > 0 && c->prm.linePhases && c->prm.getstep == MFBD_GETSTEP_CONJUGATE) {
+                c->deviceLS = true;
+                for (int e = 0; e < nLanes && c->deviceLS; ++e) {
+                    std::vector<int> slots;
+                    for (int k = 0; k < nb; ++k)
+                        if (laneOf(k) == e) {
+                            lsIdx[k] = (int)slots.size();
+                            slots.push_back(runs[k]->p);
+                        }
+                    c->deviceLS = c->dev->ls_begin(e, slots);
+                }
+                if (!c->deviceLS) c->dev->ls_clear();
             }
             std::vector<Task<int>> tasks;
             for (auto& r : runs) tasks.push_back(run_patch(*r));
@@ -1887,8 +1931,6 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             // the host resumes another share's patches (their CG / line-search steps) and
             // enqueues their next requests.  Each patch still sees exactly its own sequence
             // of requests.
-            const int nLanes = std::max(1, std::min(eval_lanes(c->prm.precision), nb));
-            auto laneOf = [&](int k) { return (int)((long long)k * nLanes / nb); };
             std::vector<EvalReq> rq[kMaxEvalLanes];
             std::vector<PatchRun*> pend[kMaxEvalLanes];
             bool inflight[kMaxEvalLanes] = {};
@@ -1950,6 +1992,56 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 c->hostStats[1].ms += since(t0);
                 c->hostStats[1].launches++;
                 inflight[e] = false;
+            };
+            // Device line searches: new lines start (their states and directions) and every
+            // lane with lines in flight runs lineRounds rounds behind its evaluation; the lines
+            // that ended resume their patches when the lane's rounds are read back.
+            bool lsInflight[kMaxEvalLanes] = {};
+            std::vector<LsResult> lsOut;
+            auto launch_lines = [&](int e) {
+                if (!c->deviceLS) return false;
+                std::vector<LsStart> starts;
+                bool any = false;
+                for (int k = 0; k < nb; ++k) {
+                    if (laneOf(k) != e) continue;
+                    auto& d = runs[k]->dl;
+                    if (d.pending) {
+                        starts.push_back({lsIdx[k], d.fa, d.lambda, runs[k]->ev.dAlpha.data()});
+                        d.pending = false;
+                        d.active = true;
+                    }
+                    any |= d.active;
+                }
+                if (!any) return false;
+                auto t0 = clk::now();
+                c->dev->ls_start(e, starts);
+                c->dev->ls_run(e, c->prm.lineRounds);
+                c->hostStats[6].ms += since(t0);
+                c->hostStats[6].launches++;
+                c->hostStats[6].units += (long long)starts.size();
+                lsInflight[e] = true;
+                return true;
+            };
+            auto finish_lines = [&](int e) {
+                auto t0 = clk::now();
+                c->dev->ls_end(e, lsOut);
+                c->hostStats[6].ms += since(t0);
+                std::vector<PatchRun*> ended;
+                for (int k = 0; k < nb; ++k) {
+                    auto& d = runs[k]->dl;
+                    if (laneOf(k) != e || !d.active) continue;
+                    const LsResult& o = lsOut[lsIdx[k]];
+                    if (!o.done) continue;
+                    d.active = false;
+                    d.step = o.step;
+                    d.f = o.f;
+                    d.probes = o.probes;
+                    ended.push_back(runs[k].get());
+                }
+                t0 = clk::now();
+                c->pool->run((int)ended.size(), [&](int k) { ended[k]->dl.h.resume(); });
+                c->hostStats[1].ms += since(t0);
+                lsInflight[e] = false;
             };
             // Restoration (Object::restorePatch) of the patches that have finished, while the
             // others still iterate: their P0/Q0 sums go to the device on the synchronous lane behind the
@@ -2031,12 +2123,18 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                         finish_lane(e);
                         any = true;
                     }
+                    if (lsInflight[e]) {
+                        finish_lines(e);
+                        any = true;
+                    }
                     do_inits(e);
                     any |= launch_lane(e);
+                    any |= launch_lines(e);
                     poll_restores(false);
                 }
                 if (!any) break;
             }
+            if (c->deviceLS) c->dev->ls_clear();
             for (size_t k = 0; k < tasks.size(); ++k) {
                 if (!tasks[k].h.done()) throw std::runtime_error("patch state machine stalled");
                 if (tasks[k].h.promise().exc) std::rethrow_exception(tasks[k].h.promise().exc);

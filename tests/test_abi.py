@@ -45,6 +45,8 @@ def test_errors_without_gpu_work():
     bad = _lib.Geometry(64, 32, 20, 1, 1, 8, 60, 60, 5, 5)     # block smaller than patch + start
     h2 = C.c_void_p()
     assert L.mfbd_ctx_create(C.byref(bad), C.byref(prm), 0, C.byref(h2)) == _lib.MFBD_EINVAL
+    neg = _lib.Params(1e-3, 1e-10, 1e-2, 0, 5, 500, 3, 5, 5, -1, 0, 0, 1, 0, 2, 1, -1)   # lineRounds < 0
+    assert L.mfbd_ctx_create(C.byref(g), C.byref(neg), 0, C.byref(h2)) == _lib.MFBD_EINVAL
     assert L.mfbd_ctx_destroy(h) == 0
 
 

@@ -87,6 +87,9 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
                            back every lineRounds rounds; the same probes and result as the
                            host's line search.  0: the host steps the line search.  Forms
                            without a fused transform (N = 256) keep the host's.            */
+    int lineGraph;      /* with lineRounds: 1 replays one captured HIP graph per round of
+                           probes, 0 launches the round's kernels one by one (the same
+                           kernels and results)                                             */
 } mfbd_params;
 
 #define MFBD_GRAD_SCHEDULE_REFERENCE 0

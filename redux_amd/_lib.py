@@ -32,7 +32,8 @@ class Params(C.Structure):
                 ("regAlpha", C.c_double), ("minIter", C.c_int), ("maxIter", C.c_int), ("nDoneIter", C.c_int),
                 ("modeStart", C.c_int), ("modeStep", C.c_int), ("apodSize", C.c_int), ("noFilter", C.c_int),
                 ("precision", C.c_int), ("maxBatch", C.c_int), ("gradient", C.c_int),
-                ("getstep", C.c_int), ("linePhases", C.c_int), ("lineRounds", C.c_int)]
+                ("getstep", C.c_int), ("linePhases", C.c_int), ("lineRounds", C.c_int),
+                ("lineGraph", C.c_int)]
 
 
 class Result(C.Structure):

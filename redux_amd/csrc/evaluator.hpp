@@ -78,11 +78,12 @@ class DeviceEvaluator {
 public:
     virtual ~DeviceEvaluator() = default;
     // Device line search of a lane's patches (slots, fixed for the solve): ls_begin sets up
-    // the lane's state and captures one round (advance every line to its next probe,
-    // evaluate the probes, their metrics) as a hipGraph; ls_start enqueues new lines;
-    // ls_run replays the round `rounds` times; ls_end waits and reports each patch's line.
-    // Returns false from ls_begin when this form has no device line search.
-    virtual bool ls_begin(int lane, const std::vector<int>& slots) = 0;
+    // the lane's state and, with `graph`, captures one round (advance every line to its next
+    // probe, evaluate the probes, their metrics) as a hipGraph; ls_start enqueues new lines;
+    // ls_run runs the round `rounds` times (graph replays or direct launches); ls_end waits
+    // and reports each patch's line.  Returns false from ls_begin when this form has no
+    // device line search.
+    virtual bool ls_begin(int lane, const std::vector<int>& slots, bool graph) = 0;
     virtual void ls_start(int lane, const std::vector<LsStart>& starts) = 0;
     virtual void ls_run(int lane, int rounds) = 0;
     virtual void ls_end(int lane, std::vector<LsResult>& out) = 0;

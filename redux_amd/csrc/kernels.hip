@@ -2612,6 +2612,8 @@ public:
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         hipEvent_t done = nullptr;
+        bool useGraph = false;
+        std::vector<int> probes;           // each line's probes at the last read-back
     };
     LsLane ls_[kSyncLane + 1];
 
@@ -2625,7 +2627,7 @@ public:
         }
     }
 
-    bool ls_begin(int lane, const std::vector<int>& slots) override {
+    bool ls_begin(int lane, const std::vector<int>& slots, bool graph) override {
         if constexpr (!(FUSED || FUSED64)) {
             return false;
         } else {
@@ -2638,6 +2640,8 @@ public:
             const int nL = (int)slots.size(), nImg = g_.nImg, nQ = nL * nImg;
             l.nL = nL;
             l.slots = slots;
+            l.useGraph = graph;
+            l.probes.assign(nL, 0);
             if (!nL) return true;
             if (!l.done) HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
             lineBuf_.alloc((size_t)2 * maxSlots_ * nImg * nPS_);
@@ -2669,37 +2673,64 @@ public:
             std::copy(idle.begin(), idle.end(), l.hSt.p);
             HIPCHK(hipMemcpyAsync(l.st.p, l.hSt.p, nL * sizeof(LsState), hipMemcpyHostToDevice, st_));
             HIPCHK(hipStreamSynchronize(st_));
-            const int* tSlot = l.tab.p;
-            const int* tScale = l.tab.p + nL;
-            int* tLine = l.tab.p + nL + nQ;
-            int* tActive = l.tab.p + nL + 2 * nQ;
-            const int* tLq = l.tab.p + 2 * nL + 2 * nQ;
-            const int* tLrow = tLq + nQ;
-            const size_t dirOff = (size_t)maxSlots_ * nImg * nPS_;
-            // one round: every line to its next probe, the probes' transforms, P / Q and metrics
-            HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+            if (graph) {
+                HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
+                ls_round(l, false);
+                HIPCHK(hipStreamEndCapture(st_, &l.graph));
+                HIPCHK(hipGraphInstantiate(&l.exec, l.graph, nullptr, nullptr, 0));
+            }
+            return true;
+        }
+    }
+
+    // One round of a lane's line searches: every line to its next probe (k_ls_step), the
+    // probes' transforms from the lines' phases, P / Q and the metrics.  Raw launches when
+    // captured; with `named` each through launch() (its statistics, events when timing).
+    void ls_round(LsLane& l, bool named) {
+        const int nL = l.nL, nImg = g_.nImg, nQ = nL * nImg, nChunk = (nOS_ + 255) / 256;
+        const int* tSlot = l.tab.p;
+        const int* tScale = l.tab.p + nL;
+        int* tLine = l.tab.p + nL + nQ;
+        int* tActive = l.tab.p + nL + 2 * nQ;
+        const int* tLq = l.tab.p + 2 * nL + 2 * nQ;
+        const int* tLrow = tLq + nQ;
+        const size_t dirOff = (size_t)maxSlots_ * nImg * nPS_;
+        auto go = [&](const char* name, auto&& f) {
+            if (named)
+                launch(name, 0, f);
+            else
+                f();
+        };
+        go("k_ls_step", [&] {
             hipLaunchKernelGGL(k_ls_step, dim3((nL + 63) / 64), dim3(64), 0, st_, l.st.p, l.L.p, tLine, l.qS.p, tActive,
                                tSlot, nL, nImg);
-            if constexpr (FUSED) {
+        });
+        if constexpr (FUSED) {
+            go("k_otf_fused", [&] {
                 hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(std::min(nQ, otfGrid_)), dim3(FNT), OLDS, st_, l.S.p,
                                    l.phi.p, invPs_.p, pupil_.p, tScale, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_, nQ,
                                    lineBuf_.p, tLine, l.qS.p, dirOff);
-            } else {
+            });
+        } else if constexpr (FUSED64) {
+            go("k_line_phase", [&] {
                 hipLaunchKernelGGL(k_line_phase<T>, dim3(nblk((size_t)nQ * nPS_)), dim3(256), 0, st_, l.phi.p,
                                    lineBuf_.p, tLq, tLrow, l.qS.p, nQ, nPS_, dirOff, tLine);
+            });
+            go("k_otf64_fused", [&] {
                 hipLaunchKernelGGL((k_otf64_fused<N, F64NT>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT), F64LDS, st_,
                                    l.S.p, l.phi.p, invPs_.p, pupil_.p, tScale, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_,
                                    nQ, tLine);
-            }
+            });
+        }
+        go("k_pq", [&] {
             hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, nL), dim3(256), 0, st_, l.Pp.p, l.Qp.p, l.part.p, l.S.p,
                                D_.p, ftSum_.p, gamma_.p, tSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, imgScale_.p,
                                g_.nCh, g_.nObj, nImg, nOS_, 1, tActive);
+        });
+        go("k_metric", [&] {
             hipLaunchKernelGGL(k_metric, dim3(nblk(nL, 64)), dim3(64), 0, st_, l.L.p, l.part.p, w_.p, nL, g_.nObj,
                                nChunk, 1.0 / ((double)g_.s * g_.s));
-            HIPCHK(hipStreamEndCapture(st_, &l.graph));
-            HIPCHK(hipGraphInstantiate(&l.exec, l.graph, nullptr, nullptr, 0));
-            return true;
-        }
+        });
     }
 
     // new lines: their states and the directions' phases B dAlpha (no channel phase) into
@@ -2719,6 +2750,7 @@ public:
             const LsStart& a = starts[i];
             if (a.idx < 0 || a.idx >= l.nL || !a.dAlpha) throw std::runtime_error("ls_start: bad line");
             l.hIdx.p[i] = a.idx;
+            l.probes[a.idx] = 0;
             l.hFl.p[2 * i] = a.fa;
             l.hFl.p[2 * i + 1] = a.lambda;
             std::memcpy(l.hDir.p + (size_t)i * row, a.dAlpha, sizeof(double) * row);
@@ -2763,10 +2795,14 @@ public:
     void ls_run(int lane, int rounds) override {
         use_lane(lane);
         LsLane& l = ls_[lane];
-        if (!l.exec) throw std::runtime_error("ls_run: no device line search on this lane");
-        launch("k_ls_round", (long long)rounds * l.nL * g_.nImg, [&] {
-            for (int k = 0; k < rounds; ++k) HIPCHK(hipGraphLaunch(l.exec, st_));
-        });
+        if (!l.nL || (l.useGraph && !l.exec)) throw std::runtime_error("ls_run: no device line search on this lane");
+        if (l.useGraph) {
+            launch("k_ls_round", 0, [&] {
+                for (int k = 0; k < rounds; ++k) HIPCHK(hipGraphLaunch(l.exec, st_));
+            });
+        } else {
+            for (int k = 0; k < rounds; ++k) ls_round(l, true);
+        }
         l.hSt.alloc(l.nL);
         HIPCHK(hipMemcpyAsync(l.hSt.p, l.st.p, l.nL * sizeof(LsState), hipMemcpyDeviceToHost, st_));
         HIPCHK(hipEventRecord(l.done, st_));
@@ -2776,11 +2812,20 @@ public:
         LsLane& l = ls_[lane];
         HIPCHK(hipEventSynchronize(l.done));
         out.resize(l.nL);
+        long long probes = 0;
         for (int r = 0; r < l.nL; ++r) {
             const LsState& x = l.hSt.p[r];
             out[r] = {x.pc == -1, x.stepOut, x.fOut, x.nProbe};
+            probes += x.nProbe - l.probes[r];
+            l.probes[r] = x.nProbe;
         }
         collect(lane);
+        // the rounds' work in images evaluated (skipped images not counted)
+        const long long img = probes * g_.nImg;
+        for (const char* k : {"k_otf_fused", "k_line_phase", "k_otf64_fused", "k_pq", "k_ls_round"}) {
+            auto it = stats_.find(k);
+            if (it != stats_.end() && (std::strcmp(k, "k_ls_round") == 0 || !l.useGraph)) it->second.units += img;
+        }
     }
 
     // Enqueue the evaluation of `reqs` on an evaluation lane (kSyncLane: the synchronous calls) and

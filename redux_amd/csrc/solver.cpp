@@ -1917,7 +1917,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                             lsIdx[k] = (int)slots.size();
                             slots.push_back(runs[k]->p);
                         }
-                    c->deviceLS = c->dev->ls_begin(e, slots);
+                    c->deviceLS = c->dev->ls_begin(e, slots, c->prm.lineGraph != 0);
                 }
                 if (!c->deviceLS) c->dev->ls_clear();
             }

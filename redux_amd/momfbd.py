@@ -42,7 +42,7 @@ class Solver:
     """One context: geometry + structure + basis + constraints; patches are loaded in bulk."""
 
     def __init__(self, pb: dict, precision: str = "fp64", max_batch: int = 1, device: int = 0, settings=None,
-                 gradient: str = "reference", line_phases=None, line_rounds=None):
+                 gradient: str = "reference", line_phases=None, line_rounds=None, line_graph=True):
         """line_phases: line-search probes from base + step * direction phases instead of the
         mode GEMM per probe (mfbd_params.linePhases).  By default on for the FP32 and mixed
         forms (their transform reads the line in place of the GEMM's output) and off for
@@ -54,7 +54,9 @@ class Solver:
         (mfbd_params.lineRounds); the same probes and results as the host's line search.
         By default 4 where line phases are on (FP32 and mixed: 256 config-2 patches solve
         ~18 % faster, the host no longer between probes) and 0 for FP64, whose solve the
-        device's transforms bound (lines that end early would wait for the read-back)."""
+        device's transforms bound (lines that end early would wait for the read-back).
+        line_graph: each round of probes as one captured HIP graph (mfbd_params.lineGraph),
+        else its kernels launched one by one."""
         self.pb = pb
         st = default_params(**pb.get("settings", {}))
         if settings:
@@ -72,7 +74,8 @@ class Solver:
                      {"fp64": L.FP64, "fp32": L.FP32, "mixed": L.MIXED}[precision], max_batch,
                      L.GRAD_ADJOINT if gradient == "adjoint" else L.GRAD_SCHEDULE_REFERENCE,
                      GETSTEP[str(st["GETSTEP"]).lower()],
-                     lp, int((4 if lp and precision != "fp64" else 0) if line_rounds is None else line_rounds))
+                     lp, int((4 if lp and precision != "fp64" else 0) if line_rounds is None else line_rounds),
+                     int(bool(line_graph)))
         h = C.c_void_p()
         self._check(L.lib().mfbd_ctx_create(C.byref(g), C.byref(prm), device, C.byref(h)), "ctx_create", None)
         self.h = h

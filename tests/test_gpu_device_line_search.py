@@ -6,8 +6,9 @@ probes in the host's order (tests/test_line_search_machine.py pins it to the ora
 CPU) and each probe's metric comes from the same kernels on the same line phases, so a
 solve with the device line search equals the same solve with the host's line search bit
 for bit: alphas, objects, metric and evaluation counts, for every form with a fused
-transform, for any number of rounds between read-backs, and again on a second solve of
-the same context (graphs captured anew).  Between graph replays the lanes' own eager
+transform, for any number of rounds between read-backs, with the rounds replayed as graphs
+or launched directly, and again on a second solve of the same context (graphs captured
+anew).  Between graph replays the lanes' own eager
 evaluations run on the same streams, which is what a solve does."""
 import numpy as np
 import pytest
@@ -17,9 +18,9 @@ from tests.golden_problem import load
 pytestmark = pytest.mark.gpu
 
 
-def _solve(pbs, prec, rounds, runs=1):
+def _solve(pbs, prec, rounds, runs=1, graph=True):
     from redux_amd.momfbd import Solver
-    sv = Solver(pbs[0], precision=prec, max_batch=len(pbs), line_phases=True, line_rounds=rounds)
+    sv = Solver(pbs[0], precision=prec, max_batch=len(pbs), line_phases=True, line_rounds=rounds, line_graph=graph)
     sv.load(pbs)
     out = [sv.run() for _ in range(runs)]
     st = sv.kernel_stats()
@@ -42,11 +43,12 @@ def test_golden_device_line_search_equals_host(prec):
     print("device line search, golden", prec, flush=True)
     (host,), st0 = _solve([pb], prec, 0)
     assert "host_line_search" not in st0
-    for rounds in (1, 5):
-        (dev,), st = _solve([pb], prec, rounds)
+    for rounds, graph in ((1, True), (5, True), (3, False)):
+        (dev,), st = _solve([pb], prec, rounds, graph=graph)
         assert st["host_line_search"]["launches"] > 0          # the lines ran on the device
-        print(prec, "rounds", rounds, "nF", dev[0]["nF"], "read-backs", st["host_line_search"]["launches"], flush=True)
-        _same(dev, host, (prec, rounds))
+        print(prec, "rounds", rounds, "graph", graph, "nF", dev[0]["nF"], "read-backs",
+              st["host_line_search"]["launches"], flush=True)
+        _same(dev, host, (prec, rounds, graph))
 
 
 @pytest.mark.parametrize("prec", ["fp32", "fp64"])
@@ -63,6 +65,8 @@ def test_config2_batch_device_line_search_equals_host(prec):
     assert all(r["status"] == 0 for r in host)
     _same(dev, host, prec)
     _same(again, host, prec)
+    (eager,), _ = _solve(pbs, prec, 4, graph=False)
+    _same(eager, host, (prec, "eager"))
 
 
 def test_device_line_search_needs_line_phases():

@@ -111,16 +111,17 @@ public:
     // evaluates the other
     virtual void evaluate_begin(int lane, std::vector<EvalReq>& reqs) = 0;
     virtual void evaluate_end(int lane, std::vector<EvalReq>& reqs) = 0;
-    // restoration sums at alpha for n patches: P0 = sum conj(D) S, Q0 = sum |S|^2 (no
-    // gamma), per object on the stored half of the OTF support: P0 [n][nObj][nOSStored]
-    // complex, Q0 [n][nObj][nOSStored]; alpha[k] points at [nImg][K] for slots[k]
-    virtual void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) = 0;
-    // restore_sums in two halves on the synchronous lane, for use while the evaluation lanes run:
-    // begin enqueues the sums and their download and returns; ready says whether they are
-    // done; end waits and copies them out.  One restoration at a time.
-    virtual void restore_begin(int n, const int* slots, const double* const* alpha) = 0;
+    // The restored objects (Object::restorePatch, object.cpp:298-331) of n patches at their
+    // alphas (alpha[k] points at [nImg][K] for slots[k]), on the synchronous lane while the
+    // evaluation lanes run: begin enqueues the sums, the Scharmer filter (when filter; nf
+    // [n][nObj]: the noise power times N^2, c2 [nObj]: the squared cut-off radius), the
+    // inverse transform and the crop, and returns; ready says whether they are done; end
+    // waits and copies out [n][nObj][s][s] (full: the N x N grid, [n][nObj][N][N]).  One
+    // restoration at a time.
+    virtual void restore_objects_begin(int n, const int* slots, const double* const* alpha, const double* nf,
+                                       const double* c2, bool filter, bool full) = 0;
     virtual bool restore_ready() = 0;
-    virtual void restore_end(double* P0, double* Q0) = 0;
+    virtual void restore_objects_end(float* out) = 0;
     // Optional restoration outputs of n slots at their alphas (object.cpp:332-407), for the
     // images imgs[k] of slot k (their current shifts), each cropped to the centred s x s of
     // the N x N grid, in double, rows in the order of the images over all slots: want bit

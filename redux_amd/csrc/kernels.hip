@@ -1034,6 +1034,153 @@ __global__ void k_mul_obj(cx<T>* __restrict__ tiles, const cx<T>* __restrict__ F
     tiles[e] = {fr * t.x - fi * t.y, fr * t.y + fi * t.x};
 }
 
+// ---- the restored objects after the sums (Object::restorePatch, object.cpp:298-331) ----
+// R restorations (patch, object) at once on the N x N grid in the centred layout (DC at
+// (N/2, N/2)); code[k] of a grid point: 2h for the stored support point h, 2h + 1 for the
+// mirror of h (P(-u) = P(u)*), -1 off the support.  Every step keeps the reference's
+// operation order (no contraction), the connected region aside, which is a set.
+
+// ao = conj(P) / Q where Q != 0 (safeDivide), dn = Q, and the filter's first power
+// pt = |ao|^2 Q (ScharmerFilter, utils.cpp:341-343)
+__global__ void k_rs_build(cx<double>* __restrict__ ao, double* __restrict__ dn, double* __restrict__ pt,
+                           const double* __restrict__ P, const double* __restrict__ Q, const int* __restrict__ code,
+                           int R, int nOS, int N2) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)R * N2) return;
+    const int r = (int)(e / N2), c = code[e % N2];
+    cx<double> a{0.0, 0.0};
+    double q = 0.0;
+    if (c >= 0) {
+        const size_t i = (size_t)r * nOS + (c >> 1);
+        const double pr = P[2 * i], pi = (c & 1) ? -P[2 * i + 1] : P[2 * i + 1];
+        q = Q[i];
+        if (fabs(q) > 0) a = {pr / q, -pi / q};
+    }
+    ao[e] = a;
+    dn[e] = q;
+    pt[e] = (a.x * a.x + a.y * a.y) * q;
+}
+
+// image::smooth (utils.cpp:304-328): sums over the half-open window [x - n, x + n) clipped
+// to the grid along rows ...
+__global__ void k_rs_box_rows(double* __restrict__ out, const double* __restrict__ in, int R, int N, int n) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)R * N * N) return;
+    const int x = (int)(e % N);
+    const double* row = in + (e - x);
+    const int xl = max(x - n, 0), xh = min(x + n, N);
+    double acc = 0;
+    for (int xx = xl; xx < xh; ++xx) acc += row[xx];
+    out[e] = acc;
+}
+
+// ... then along columns, divided by the window's size; with nf the result v becomes
+// nf[r] / v (the noise-to-power ratio, utils.cpp:344-345)
+__global__ void k_rs_box_cols(double* __restrict__ out, const double* __restrict__ in, const double* __restrict__ nf,
+                              int R, int N, int n) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N2 = (size_t)N * N;
+    if (e >= (size_t)R * N2) return;
+    const int r = (int)(e / N2), k = (int)(e % N2), y = k / N, x = k % N;
+    const double* col = in + (size_t)r * N2 + x;
+    const int yl = max(y - n, 0), yh = min(y + n, N), xl = max(x - n, 0), xh = min(x + n, N);
+    double acc = 0;
+    for (int yy = yl; yy < yh; ++yy) acc += col[(size_t)yy * N];
+    double v = acc / ((yh - yl) * (xh - xl));
+    if (nf) v = nf[r] / v;
+    out[e] = v;
+}
+
+// The filter weights (utils.cpp:346-365): 1 - (v(-u) + v(u)) / 2, clipped at 0, set to 0
+// below 0.2 and to at most 1, the centre 1
+__global__ void k_rs_weights(double* __restrict__ rr, const double* __restrict__ v, int R, int N) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N2 = (size_t)N * N;
+    if (e >= (size_t)R * N2) return;
+    const int k = (int)(e % N2), y = k / N, x = k % N;
+    const double* g = v + (e - k);
+    double w = 1.0 - 0.5 * (g[(size_t)((N - y) % N) * N + (N - x) % N] + g[k]);
+    w = w < 0.0 ? 0.0 : w;
+    if (w < 0.2) w = 0.0;
+    if (w > 1.0) w = 1.0;
+    if (y == N / 2 && x == N / 2) w = 1.0;
+    rr[e] = w;
+}
+
+// connectedRegion from the centre (utils.cpp:49-55, 294-300): the 4-connected set of
+// points with a positive weight that holds the centre, one workgroup per restoration,
+// marks in LDS (0 outside, 1 a candidate, 2 in the region) grown in place until a sweep
+// adds nothing (each sweep either adds a point or ends, so at most N^2 + 1 sweeps); the
+// weights outside the region are multiplied by 0.
+template <int N>
+__global__ void __launch_bounds__(1024) k_rs_region(double* __restrict__ out, const double* __restrict__ rr) {
+    constexpr int N2 = N * N;
+    __shared__ unsigned char m[N2];
+    __shared__ int changed;
+    const double* w = rr + (size_t)blockIdx.x * N2;
+    for (int e = threadIdx.x; e < N2; e += 1024) m[e] = w[e] > 0.0 ? 1 : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) m[(N / 2) * N + N / 2] = 2;
+    while (true) {
+        __syncthreads();
+        if (threadIdx.x == 0) changed = 0;
+        __syncthreads();
+        int ch = 0;
+        for (int e = threadIdx.x; e < N2; e += 1024) {
+            if (m[e] != 1) continue;
+            const int y = e / N, x = e % N;
+            if ((y + 1 < N && m[e + N] == 2) || (y > 0 && m[e - N] == 2) || (x + 1 < N && m[e + 1] == 2) ||
+                (x > 0 && m[e - 1] == 2)) {
+                m[e] = 2;
+                ch = 1;
+            }
+        }
+        if (ch) changed = 1;
+        __syncthreads();
+        if (!changed) break;
+    }
+    double* o = out + (size_t)blockIdx.x * N2;
+    for (int e = threadIdx.x; e < N2; e += 1024) o[e] = w[e] * (double)(m[e] == 2 ? 1 : 0);
+}
+
+// data = ao rs inside the cut-off radius, 0 outside (utils.cpp:372-389), or ao unfiltered,
+// then the quadrant swap to the natural layout (reorder) for the inverse transform
+__global__ void k_rs_apply(cx<double>* __restrict__ t, const cx<double>* __restrict__ ao, const double* __restrict__ rs,
+                           const double* __restrict__ c2, int R, int N, int nObj) {
+#pragma clang fp contract(off)
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N2 = (size_t)N * N;
+    if (e >= (size_t)R * N2) return;
+    const int r = (int)(e / N2), k = (int)(e % N2), y = k / N, x = k % N, H = N / 2;
+    cx<double> a = ao[e];
+    if (rs) {
+        if ((double)((y - H) * (y - H) + (x - H) * (x - H)) > c2[r % nObj]) a = {0.0, 0.0};
+        else a = {a.x * rs[e], a.y * rs[e]};
+    }
+    t[(size_t)r * N2 + (size_t)((y + H) % N) * N + (x + H) % N] = a;
+}
+
+// the real part after the inverse transform, normalised: the centred s x s crop
+// (object.cpp:329-331, 393-397), or the whole grid as the float od.img holds it (the real
+// part rounded to float before the normalisation)
+__global__ void k_rs_crop(float* __restrict__ out, const cx<double>* __restrict__ t, int R, int N, int s, int full) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t N2 = (size_t)N * N;
+    if (full) {
+        if (e >= (size_t)R * N2) return;
+        out[e] = (float)((double)(float)t[e].x * (1.0 / (double)N2));
+        return;
+    }
+    const size_t s2 = (size_t)s * s;
+    if (e >= (size_t)R * s2) return;
+    const int r = (int)(e / s2), k = (int)(e % s2), off = (N - s) / 2;
+    out[e] = (float)(t[(size_t)r * N2 + (size_t)(k / s + off) * N + k % s + off].x / ((double)N * N));
+}
+
 // out[q][y][x] = Re tiles[q][(y + off + sh) % N][(x + off + sh) % N] * scale: the centred
 // s x s of each tile, re-centred by sh = N/2 for a PSF (the OTF's REORDER_IMG flag)
 template <typename T>
@@ -2289,6 +2436,12 @@ public:
             mirNat.push_back(paired ? natOf(m) : -1);
         }
         nOS_ = (int)rep.size();
+        {   // each grid point's place on the stored support, for the restoration (k_rs_build)
+            std::vector<int> code(N2, -1);
+            for (int u = 0; u < nOSfull_; ++u)
+                code[os_[u]] = fullMap_[u] >= 0 ? 2 * fullMap_[u] : 2 * (-1 - fullMap_[u]) + 1;
+            upload(rsCode_, code, st_);
+        }
         for (int c : rep)       // k_cols_extract transforms only the natural columns [0, N/2]
             if (natOf(c) % N > N / 2) throw std::runtime_error("OTF support is not centro-symmetric");
         {   // where each stored point sits in the fused kernel's (N) x (N/2+1) half array
@@ -2859,38 +3012,99 @@ public:
         collect(lane);
     }
 
-    void restore_sums(int n, const int* slots, const double* const* alpha, double* P0, double* Q0) override {
-        restore_begin(n, slots, alpha);
-        restore_end(P0, Q0);
-    }
-
-    // P0, Q0 (no gamma) of n slots at the given alphas, on the synchronous lane: the sums and their
-    // download are enqueued behind whatever the stream holds; that lane's pinned buffers
-    // receive [n][nObj][nH] P (complex) and Q
-    void restore_begin(int n, const int* slots, const double* const* alpha) override {
+    // The restored objects of n slots at their alphas on the synchronous lane, in chunks of
+    // patches: the forward chain and the sums P0 = sum conj(D) S, Q0 = sum |S|^2 (no gamma),
+    // then per (patch, object) the restoration proper on the device in FP64: conj(P0) / Q0,
+    // the Scharmer filter (its box smoothings, weights and connected region), the inverse
+    // transform and the crop, each chunk's floats downloaded to pinned memory.
+    void restore_objects_begin(int n, const int* slots, const double* const* alpha, const double* nf,
+                               const double* c2, bool filter, bool full) override {
         use_lane(kSyncLane);
-        const size_t per = (size_t)g_.nObj * nOS_;
-        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
+        const int nO = g_.nObj, perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
+        const size_t outPer = (size_t)nO * (full ? N2 : (size_t)g_.s * g_.s);
+        if (!twD_.p) {
+            std::vector<cx<double>> tw(N);
+            for (int k = 0; k < N; ++k) {
+                long double a = -2.0L * 3.14159265358979323846264338327950288L * k / N;
+                tw[k] = {(double)cosl(a), (double)sinl(a)};
+            }
+            upload(twD_, tw, st_);
+        }
         std::vector<EvalReq> rq(n);
         for (int k = 0; k < n; ++k) {
             rq[k].slot = slots[k];
             rq[k].alpha = alpha[k];
         }
-        sc_->hG.alloc(2 * (size_t)n * per);        // pinned: P then Q
-        sc_->hL.alloc((size_t)n * per);
+        hRs_.alloc((size_t)n * outPer);
+        std::vector<double> c2v(c2, c2 + nO);
+        upload(rsC2_, c2v, st_);
         for (int b = 0; b < n; b += perChunk) {
-            const int m = std::min(perChunk, n - b);
+            const int m = std::min(perChunk, n - b), R = m * nO;
             // forward() rewrites the lane's pinned table and alpha staging, which the previous
             // chunk's queued uploads still read: let them finish first
             if (b) HIPCHK(hipStreamSynchronize(st_));
             forward(&rq[b], m, /*useGamma=*/0);
-            HIPCHK(hipMemcpyAsync(sc_->hG.p + 2 * (size_t)b * per, sc_->Pp.p, 2 * (size_t)m * per * sizeof(double),
-                                  hipMemcpyDeviceToHost, st_));
-            HIPCHK(hipMemcpyAsync(sc_->hL.p + (size_t)b * per, sc_->Qp.p, (size_t)m * per * sizeof(double),
-                                  hipMemcpyDeviceToHost, st_));
+            const size_t RN = (size_t)R * N2;
+            rsAo_.alloc(RN);
+            rsT_.alloc(RN);
+            rsDn_.alloc(RN);
+            rsA_.alloc(RN);
+            rsB_.alloc(RN);
+            rsOut_.alloc((size_t)m * outPer);
+            std::vector<double> nfv(nf + (size_t)b * nO, nf + (size_t)(b + m) * nO);
+            upload(rsNf_, nfv, st_);
+            launch("k_rs_build", R, [&] {
+                hipLaunchKernelGGL(k_rs_build, dim3(nblk(RN)), dim3(256), 0, st_, rsAo_.p, rsDn_.p, rsA_.p, sc_->Pp.p,
+                                   sc_->Qp.p, rsCode_.p, R, nOS_, (int)N2);
+            });
+            if (filter) {
+                auto box = [&](const double* nfp, int w) {
+                    launch("k_rs_box", R, [&] {
+                        hipLaunchKernelGGL(k_rs_box_rows, dim3(nblk(RN)), dim3(256), 0, st_, rsB_.p, rsA_.p, R, N, w);
+                        hipLaunchKernelGGL(k_rs_box_cols, dim3(nblk(RN)), dim3(256), 0, st_, rsA_.p, rsB_.p, nfp, R, N,
+                                           w);
+                    });
+                };
+                box(rsNf_.p, 1);                       // v = nf / smooth(pt, 1)
+                launch("k_rs_weights", R, [&] {
+                    hipLaunchKernelGGL(k_rs_weights, dim3(nblk(RN)), dim3(256), 0, st_, rsB_.p, rsA_.p, R, N);
+                });
+                launch("k_rs_region", R, [&] {
+                    hipLaunchKernelGGL(k_rs_region<N>, dim3(R), dim3(1024), 0, st_, rsA_.p, rsB_.p);
+                });
+                box(nullptr, 4);                       // rs = smooth(weights in the region, 4)
+            }
+            launch("k_rs_apply", R, [&] {
+                hipLaunchKernelGGL(k_rs_apply, dim3(nblk(RN)), dim3(256), 0, st_, rsT_.p, rsAo_.p,
+                                   filter ? rsA_.p : (const double*)nullptr, rsC2_.p, R, N, nO);
+            });
+            launch("k_rs_ifft", R, [&] {
+                constexpr int L = 16;
+                hipLaunchKernelGGL((k_fft_rows<double, N, +1, L>), dim3(R * (N / L)), dim3(L * N / 16), 0, st_, rsT_.p,
+                                   twD_.p);
+                hipLaunchKernelGGL((k_fft_cols<double, N, +1, L>), dim3(R * (N / L)), dim3(L * N / 16), 0, st_, rsT_.p,
+                                   twD_.p);
+            });
+            const size_t nOut = (size_t)m * outPer;
+            launch("k_rs_crop", R, [&] {
+                hipLaunchKernelGGL(k_rs_crop, dim3(nblk(nOut)), dim3(256), 0, st_, rsOut_.p, rsT_.p, R, N, g_.s,
+                                   full ? 1 : 0);
+            });
+            HIPCHK(hipMemcpyAsync(hRs_.p + (size_t)b * outPer, rsOut_.p, nOut * sizeof(float), hipMemcpyDeviceToHost,
+                                  st_));
         }
         HIPCHK(hipEventRecord(sc_->done, st_));
         restoreN_ = n;
+        restoreOut_ = n * outPer;
+    }
+
+    void restore_objects_end(float* out) override {
+        use_lane(kSyncLane);
+        HIPCHK(hipEventSynchronize(sc_->done));
+        std::memcpy(out, hRs_.p, restoreOut_ * sizeof(float));
+        restoreN_ = 0;
+        restoreOut_ = 0;
+        collect(kSyncLane);
     }
 
     void restore_extra(int n, const int* slots, const double* const* alpha,
@@ -2991,16 +3205,6 @@ public:
         if (e == hipErrorNotReady) return false;
         HIPCHK(e);
         return true;
-    }
-
-    void restore_end(double* P0, double* Q0) override {
-        use_lane(kSyncLane);
-        const size_t per = (size_t)g_.nObj * nOS_;
-        HIPCHK(hipEventSynchronize(sc_->done));
-        std::memcpy(P0, sc_->hG.p, 2 * (size_t)restoreN_ * per * sizeof(double));
-        std::memcpy(Q0, sc_->hL.p, (size_t)restoreN_ * per * sizeof(double));
-        restoreN_ = 0;
-        collect(kSyncLane);
     }
 
     void debug_get(const std::string& what, int slot, std::vector<double>& out) override {
@@ -3755,6 +3959,7 @@ private:
     int maxSlots_, maxBatch_;
     int otfGrid_ = 1;                        // workgroups of the persistent FP64 OTF kernel
     int restoreN_ = 0;                       // slots of the restoration in flight on the synchronous lane
+    size_t restoreOut_ = 0;                  // its output floats
     hipStream_t st_;
     hipEvent_t ev0_, ev1_;
     bool timing_ = false;
@@ -3780,6 +3985,12 @@ private:
     DBuf<int> xInts_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
     DBuf<T> lineBuf_;                 // [2][maxSlots][nImg][nPS]: each slot's line base phases, then directions
+    // the restoration: grid codes, FP64 twiddles, per (patch, object) grids and outputs
+    DBuf<int> rsCode_;
+    DBuf<cx<double>> twD_, rsAo_, rsT_;
+    DBuf<double> rsDn_, rsA_, rsB_, rsNf_, rsC2_;
+    DBuf<float> rsOut_;
+    HBuf<float> hRs_;
     int device_ = 0;
     int lr_ = LINES, lc_ = LINES;           // lines per workgroup of the transform passes
     int stageL_[4] = {0, 0, 0, 0};          // ... of the four OTF passes at N = 256 (0: not yet tuned)

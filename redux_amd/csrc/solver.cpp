@@ -227,77 +227,6 @@ double noise_level(const cd* ft, int ny, int nx) {
     return cnt ? std::sqrt(np / ((double)cnt * ny * nx)) : np;
 }
 
-// image::smooth (utils.cpp:304-328): mean over the half-open window [y-n, y+n) x
-// [x-n, x+n) clipped to the array; summed separably (rows, then columns)
-void smooth2(double* d, int ny, int nx, int n) {
-    std::vector<double> h((size_t)ny * nx);
-    for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-            const int xl = std::max(x - n, 0), xh = std::min(x + n, nx);
-            double acc = 0;
-            for (int xx = xl; xx < xh; ++xx) acc += d[(size_t)y * nx + xx];
-            h[(size_t)y * nx + x] = acc;
-        }
-    for (int y = 0; y < ny; ++y) {
-        const int yl = std::max(y - n, 0), yh = std::min(y + n, ny);
-        for (int x = 0; x < nx; ++x) {
-            const int xl = std::max(x - n, 0), xh = std::min(x + n, nx);
-            double acc = 0;
-            for (int yy = yl; yy < yh; ++yy) acc += h[(size_t)yy * nx + x];
-            d[(size_t)y * nx + x] = acc / ((yh - yl) * (xh - xl));
-        }
-    }
-}
-
-// ScharmerFilter (utils.cpp:334-392) with connectedRegion (utils.cpp:49-55, 294-300)
-void scharmer(cd* data, const double* q2, int ny, int nx, double noise_power, double cutoff) {
-    const double nf = noise_power * ny * nx;
-    const size_t n = (size_t)ny * nx;
-    std::vector<double> pt(n), rr(n), col(ny);
-    for (size_t k = 0; k < n; ++k) pt[k] = std::norm(data[k]) * q2[k];
-    smooth2(pt.data(), ny, nx, 1);
-    for (auto& v : pt) v = nf / v;
-    for (int y = 0; y < ny; ++y) {
-        rr[(size_t)y * nx] = pt[(size_t)y * nx];
-        for (int x = 1; x < nx; ++x) rr[(size_t)y * nx + x] = pt[(size_t)y * nx + nx - x];
-    }
-    for (int x = 0; x < nx; ++x) {
-        double v = 1.0 - 0.5 * (rr[x] + pt[x]);
-        rr[x] = v < 0.0 ? 0.0 : v;
-        for (int y = 1; y < ny; ++y) col[y] = rr[(size_t)y * nx + x];
-        for (int y = 1; y < ny; ++y) {
-            double w = 1.0 - 0.5 * (col[ny - y] + pt[(size_t)y * nx + x]);
-            rr[(size_t)y * nx + x] = w < 0.0 ? 0.0 : w;
-        }
-    }
-    for (auto& v : rr) { if (v < 0.2) v = 0.0; if (v > 1.0) v = 1.0; }
-    const int yH = ny / 2, xH = nx / 2;
-    rr[(size_t)yH * nx + xH] = 1.0;
-    std::vector<unsigned char> mask(n, 0);
-    std::vector<int> stack{yH * nx + xH};
-    while (!stack.empty()) {
-        int idx = stack.back();
-        stack.pop_back();
-        int y = idx / nx, x = idx % nx;
-        if (mask[idx]) continue;
-        if (!((y == yH && x == xH) || rr[idx] > 0.0)) continue;
-        mask[idx] = 1;
-        if (y + 1 < ny) stack.push_back(idx + nx);
-        if (y - 1 >= 0) stack.push_back(idx - nx);
-        if (x + 1 < nx) stack.push_back(idx + 1);
-        if (x - 1 >= 0) stack.push_back(idx - 1);
-    }
-    for (size_t k = 0; k < n; ++k) rr[k] *= mask[k];
-    smooth2(rr.data(), ny, nx, 4);
-    const double c2 = cutoff * cutoff;
-    for (int y = 0; y < ny; ++y)
-        for (int x = 0; x < nx; ++x) {
-            size_t k = (size_t)y * nx + x;
-            if ((double)((y - yH) * (y - yH) + (x - xH) * (x - xH)) > c2) data[k] = 0.0;
-            else data[k] *= rr[k];
-        }
-}
-
 }  // namespace
 
 // ------------------------------------------------------------------ context ----
@@ -500,70 +429,35 @@ void effective_alpha(const mfbd_ctx* c, int p, const double* alpha, double* out)
     }
 }
 
-// Object::restorePatch after the sums (object.cpp:298-326): conj, safeDivide, Scharmer
-// filter; returns object o's spectrum de-centred for the inverse transform.
-// P0, Q0: the patch's sums on the stored half of the support ([nObj][nH]); `map` takes
-// each full support point to its stored value (code h) or that value's conjugate (-1-h)
-std::vector<cd> restore_spectrum(const mfbd_ctx* c, int p, int o, const std::vector<int>& os,
-                                 const std::vector<int>& map, int nH, const double* P0, const double* Q0) {
-    const int N = 2 * c->g.P, nOS = (int)os.size();
-    std::vector<cd> ao((size_t)N * N, 0.0);
-    std::vector<double> dn((size_t)N * N, 0.0);
-    for (int u = 0; u < nOS; ++u) {
-        const int code = map[u], h = code >= 0 ? code : -1 - code;
-        const size_t k = (size_t)o * nH + h;
-        const cd pv(P0[2 * k], code >= 0 ? P0[2 * k + 1] : -P0[2 * k + 1]);   // P(-u) = P(u)*
-        ao[os[u]] = std::conj(pv);
-        dn[os[u]] = Q0[k];
-    }
-    for (size_t k = 0; k < ao.size(); ++k) ao[k] = (std::fabs(dn[k]) > 0) ? ao[k] / dn[k] : cd(0.0);
-    double nv = 0;
-    int cnt = 0;
-    for (int j = 0; j < c->g.nImg; ++j)
-        if (c->st.imgObj[j] == o && img_on(c, p, j)) { nv += c->noise[p][j] * c->noise[p][j]; cnt++; }
-    nv /= cnt;
-    if (!c->prm.noFilter) scharmer(ao.data(), dn.data(), N, N, nv, 0.90 * c->st.freqCutoff[o]);
-    std::vector<cd> t((size_t)N * N);
-    reorder_into(ao.data(), N, N, t.data());
-    return t;
-}
-
-// ... then the inverse transform and the centred s x s crop (object.cpp:329-331, 393-397)
-void finish_restore(const mfbd_ctx* c, int p, const std::vector<int>& os, const std::vector<int>& map, int nH,
-                    const double* P0, const double* Q0, float* imgOut) {
-    const int N = 2 * c->g.P, s = c->g.s;
-    for (int o = 0; o < c->g.nObj; ++o) {
-        std::vector<cd> t = restore_spectrum(c, p, o, os, map, nH, P0, Q0);
-        // inverse 2-D transform; only the columns of the centred s x s crop are needed
-        const int off = (N - s) / 2;
-        {
-            HostFFT f(N);
-            for (int y = 0; y < N; ++y) f.run(t.data() + (size_t)y * N, 1, +1);
-            for (int x = off; x < off + s; ++x) f.run(t.data() + x, N, +1);
+// The restoration's per-(patch, object) noise term (Object::restorePatch, object.cpp:306-312):
+// the mean of noise^2 over the object's images the patch has, times N^2 as ScharmerFilter
+// scales it (utils.cpp:336); [n][nObj]
+std::vector<double> restore_noise(const mfbd_ctx* c, const int* patches, int n) {
+    const int N = 2 * c->g.P, nO = c->g.nObj;
+    std::vector<double> nf((size_t)n * nO);
+    for (int k = 0; k < n; ++k) {
+        const int p = patches[k];
+        for (int o = 0; o < nO; ++o) {
+            double nv = 0;
+            int cnt = 0;
+            for (int j = 0; j < c->g.nImg; ++j)
+                if (c->st.imgObj[j] == o && img_on(c, p, j)) { nv += c->noise[p][j] * c->noise[p][j]; cnt++; }
+            nv /= cnt;
+            nf[(size_t)k * nO + o] = nv * N * N;
         }
-        float* out = imgOut + (size_t)o * s * s;
-        for (int y = 0; y < s; ++y)
-            for (int x = 0; x < s; ++x)
-                out[(size_t)y * s + x] = (float)(t[(size_t)(y + off) * N + x + off].real() / ((double)N * N));
     }
+    return nf;
 }
 
-// The restored objects on the full N x N grid before the crop, as the float array od.img
-// holds them (object.cpp:328-329; ift<float> rounds the real part, then normalises):
-// [nObj][N][N], natural layout
-std::vector<float> restore_full(const mfbd_ctx* c, int p, const std::vector<int>& os, const std::vector<int>& map,
-                                int nH, const double* P0, const double* Q0) {
-    const int N = 2 * c->g.P;
-    const size_t N2 = (size_t)N * N;
-    std::vector<float> full(c->g.nObj * N2);
+// the filter's squared cut-off radius per object, 0.9 of the frequency cut-off
+// (object.cpp:313-315)
+std::vector<double> restore_cutoffs(const mfbd_ctx* c) {
+    std::vector<double> c2(c->g.nObj);
     for (int o = 0; o < c->g.nObj; ++o) {
-        std::vector<cd> t = restore_spectrum(c, p, o, os, map, nH, P0, Q0);
-        HostFFT f(N);
-        for (int y = 0; y < N; ++y) f.run(t.data() + (size_t)y * N, 1, +1);
-        for (int x = 0; x < N; ++x) f.run(t.data() + x, N, +1);
-        for (size_t k = 0; k < N2; ++k) full[o * N2 + k] = (float)((double)(float)t[k].real() * (1.0 / (double)N2));
+        const double cutoff = 0.90 * c->st.freqCutoff[o];
+        c2[o] = cutoff * cutoff;
     }
-    return full;
+    return c2;
 }
 
 // --------------------------------------------------------- patch solver ----
@@ -1735,20 +1629,17 @@ int mfbd_restore(mfbd_ctx* c, int n, const int* patchIdx, const double* alpha, f
     try {
         ensure_init(c);
         const size_t per = (size_t)c->g.nImg * c->g.K;
-        const size_t po = (size_t)c->g.nObj * c->dev->nOSStored();
         for (int k = 0; k < n; ++k)
             if (patchIdx[k] < 0 || patchIdx[k] >= c->nPatch) return set_err(c, MFBD_EINVAL, "patch index out of range");
-        std::vector<double> ae(per * n), P0(2 * po * n), Q0(po * n);
+        std::vector<double> ae(per * n);
         std::vector<const double*> ap(n);
         for (int k = 0; k < n; ++k) {
             effective_alpha(c, patchIdx[k], alpha + per * k, &ae[per * k]);
             ap[k] = &ae[per * k];
         }
-        c->dev->restore_sums(n, patchIdx, ap.data(), P0.data(), Q0.data());
-        parallel_for(n, [&](int k) {
-            finish_restore(c, patchIdx[k], c->dev->otf_support(), c->dev->support_map(), c->dev->nOSStored(),
-                           &P0[2 * po * k], &Q0[po * k], img + (size_t)k * c->g.nObj * c->g.s * c->g.s);
-        });
+        const std::vector<double> nf = restore_noise(c, patchIdx, n), c2 = restore_cutoffs(c);
+        c->dev->restore_objects_begin(n, patchIdx, ap.data(), nf.data(), c2.data(), !c->prm.noFilter, false);
+        c->dev->restore_objects_end(img);
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
     }
@@ -1772,7 +1663,7 @@ int mfbd_restore_outputs(mfbd_ctx* c, int n, const int* patchIdx, const double* 
         ensure_init(c);
         const int nImg = c->g.nImg, nObj = c->g.nObj;
         const size_t s2 = (size_t)c->g.s * c->g.s, per = (size_t)nImg * c->g.K;
-        const size_t po = (size_t)nObj * c->dev->nOSStored(), N2 = (size_t)4 * c->g.P * c->g.P;
+        const size_t N2 = (size_t)4 * c->g.P * c->g.P;
         std::vector<double> ae(per * n);
         std::vector<const double*> ap(n);
         std::vector<std::vector<ImgInit>> imgs(n);
@@ -1787,14 +1678,11 @@ int mfbd_restore_outputs(mfbd_ctx* c, int n, const int* patchIdx, const double* 
         }
         std::vector<float> objFull;
         if (mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) {
-            std::vector<double> P0(2 * po * n), Q0(po * n);
-            c->dev->restore_sums(n, patchIdx, ap.data(), P0.data(), Q0.data());
+            // the restored objects on the whole grid (od.img before the crop)
+            const std::vector<double> nf = restore_noise(c, patchIdx, n), c2 = restore_cutoffs(c);
             objFull.resize((size_t)n * nObj * N2);
-            parallel_for(n, [&](int k) {
-                std::vector<float> f = restore_full(c, patchIdx[k], c->dev->otf_support(), c->dev->support_map(),
-                                                    c->dev->nOSStored(), &P0[2 * po * k], &Q0[po * k]);
-                std::copy(f.begin(), f.end(), objFull.begin() + (size_t)k * nObj * N2);
-            });
+            c->dev->restore_objects_begin(n, patchIdx, ap.data(), nf.data(), c2.data(), !c->prm.noFilter, true);
+            c->dev->restore_objects_end(objFull.data());
         }
         const int want = ((mask & (MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG)) ? 1 : 0) |
                          ((mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) ? 2 : 0) |
@@ -1894,7 +1782,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             c->hostStats[2].name = "host_reinit";
             c->hostStats[3].name = "host_reset_init";
             c->hostStats[4].name = "host_restore";
-            c->hostStats[5].name = "host_restore_sums";
+            c->hostStats[5].name = "host_restore_objects";
             c->hostStats[6].name = "host_line_search";
             {
                 auto t0 = clk::now();
@@ -2049,46 +1937,25 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
             // filter and inverse transform run on a host thread.  A patch's restoration
             // depends on its own final alpha only, so grouping leaves it unchanged.
             constexpr int RESTORE_MIN = 16, RESTORE_MAX = 32;
-            const size_t per = (size_t)c->g.nObj * c->dev->nOSStored();
+            const size_t objPer = (size_t)c->g.nObj * c->g.s * c->g.s;
             struct Restore {
                 std::vector<int> ks;
-                std::vector<double> ae, P0, Q0;
-                std::thread th;
+                std::vector<double> ae, nf;
             };
-            std::vector<std::unique_ptr<Restore>> restores;
-            struct JoinAll {                     // no host restoration outlives the solve, errors included
-                std::vector<std::unique_ptr<Restore>>& v;
-                ~JoinAll() {
-                    for (auto& r : v)
-                        if (r->th.joinable()) r->th.join();
-                }
-            } joinAll{restores};
-            Restore* onDevice = nullptr;
+            std::unique_ptr<Restore> onDevice;
             std::vector<char> queued(nb, 0);
-            std::exception_ptr restoreExc;
-            std::mutex restoreMu;
+            std::vector<float> rsOut;
+            const std::vector<double> rsC2 = restore_cutoffs(c);
             auto device_done = [&](bool wait) {
                 if (!onDevice || (!wait && !c->dev->restore_ready())) return;
                 auto t0 = clk::now();
-                Restore* rs = onDevice;
-                rs->P0.resize(2 * rs->ks.size() * per);
-                rs->Q0.resize(rs->ks.size() * per);
-                c->dev->restore_end(rs->P0.data(), rs->Q0.data());
-                onDevice = nullptr;
+                rsOut.resize(onDevice->ks.size() * objPer);
+                c->dev->restore_objects_end(rsOut.data());
+                for (size_t i = 0; i < onDevice->ks.size(); ++i)
+                    std::copy(rsOut.begin() + i * objPer, rsOut.begin() + (i + 1) * objPer,
+                              imgOut + (size_t)runs[onDevice->ks[i]]->p * objPer);
+                onDevice.reset();
                 c->hostStats[5].ms += since(t0);
-                rs->th = std::thread([&, rs] {
-                    try {
-                        const auto& os = c->dev->otf_support();
-                        parallel_for((int)rs->ks.size(), [&](int i) {
-                            const int p = runs[rs->ks[i]]->p;
-                            finish_restore(c, p, os, c->dev->support_map(), c->dev->nOSStored(), &rs->P0[2 * i * per],
-                                           &rs->Q0[i * per], imgOut + (size_t)p * c->g.nObj * c->g.s * c->g.s);
-                        }, 4);
-                    } catch (...) {
-                        std::lock_guard<std::mutex> l(restoreMu);
-                        if (!restoreExc) restoreExc = std::current_exception();
-                    }
-                });
             };
             auto poll_restores = [&](bool flush) {
                 if (!imgOut) return;
@@ -2110,9 +1977,10 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                     effective_alpha(c, slots[i], runs[ks[i]]->alpha.data(), &rs->ae[i * nPar]);
                     ap[i] = &rs->ae[i * nPar];
                 }
-                c->dev->restore_begin((int)ks.size(), slots.data(), ap.data());
-                onDevice = rs.get();
-                restores.push_back(std::move(rs));
+                rs->nf = restore_noise(c, slots.data(), (int)slots.size());
+                c->dev->restore_objects_begin((int)ks.size(), slots.data(), ap.data(), rs->nf.data(), rsC2.data(),
+                                              !c->prm.noFilter, false);
+                onDevice = std::move(rs);
                 c->hostStats[5].ms += since(t0);
                 c->hostStats[5].launches++;
             };
@@ -2145,11 +2013,9 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                     poll_restores(true);
                     if (!onDevice) break;
                 }
-                for (auto& rs : restores) rs->th.join();
                 c->hostStats[4].ms += since(t0);
                 c->hostStats[4].launches++;
                 c->hostStats[4].units += nb;
-                if (restoreExc) std::rethrow_exception(restoreExc);
             }
             // results (Solver::run tail, solver.cpp:549-566); the objects were restored above
             for (int k = 0; k < nb; ++k) {

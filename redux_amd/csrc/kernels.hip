@@ -1062,33 +1062,21 @@ __global__ void k_rs_build(cx<double>* __restrict__ ao, double* __restrict__ dn,
     pt[e] = (a.x * a.x + a.y * a.y) * q;
 }
 
-// image::smooth (utils.cpp:304-328): sums over the half-open window [x - n, x + n) clipped
-// to the grid along rows ...
-__global__ void k_rs_box_rows(double* __restrict__ out, const double* __restrict__ in, int R, int N, int n) {
-#pragma clang fp contract(off)
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)R * N * N) return;
-    const int x = (int)(e % N);
-    const double* row = in + (e - x);
-    const int xl = max(x - n, 0), xh = min(x + n, N);
-    double acc = 0;
-    for (int xx = xl; xx < xh; ++xx) acc += row[xx];
-    out[e] = acc;
-}
-
-// ... then along columns, divided by the window's size; with nf the result v becomes
-// nf[r] / v (the noise-to-power ratio, utils.cpp:344-345)
-__global__ void k_rs_box_cols(double* __restrict__ out, const double* __restrict__ in, const double* __restrict__ nf,
-                              int R, int N, int n) {
+// image::smooth (utils.cpp:304-328): the mean over the half-open window [y - n, y + n) x
+// [x - n, x + n) clipped to the grid, summed row by row as the reference sums it; with nf
+// the result v becomes nf[r] / v (the noise-to-power ratio, utils.cpp:344-345)
+__global__ void k_rs_box(double* __restrict__ out, const double* __restrict__ in, const double* __restrict__ nf, int R,
+                         int N, int n) {
 #pragma clang fp contract(off)
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const size_t N2 = (size_t)N * N;
     if (e >= (size_t)R * N2) return;
     const int r = (int)(e / N2), k = (int)(e % N2), y = k / N, x = k % N;
-    const double* col = in + (size_t)r * N2 + x;
+    const double* g = in + (size_t)r * N2;
     const int yl = max(y - n, 0), yh = min(y + n, N), xl = max(x - n, 0), xh = min(x + n, N);
     double acc = 0;
-    for (int yy = yl; yy < yh; ++yy) acc += col[(size_t)yy * N];
+    for (int yy = yl; yy < yh; ++yy)
+        for (int xx = xl; xx < xh; ++xx) acc += g[(size_t)yy * N + xx];
     double v = acc / ((yh - yl) * (xh - xl));
     if (nf) v = nf[r] / v;
     out[e] = v;
@@ -3058,21 +3046,19 @@ public:
                                    sc_->Qp.p, rsCode_.p, R, nOS_, (int)N2);
             });
             if (filter) {
-                auto box = [&](const double* nfp, int w) {
+                auto box = [&](double* o, const double* i, const double* nfp, int w) {
                     launch("k_rs_box", R, [&] {
-                        hipLaunchKernelGGL(k_rs_box_rows, dim3(nblk(RN)), dim3(256), 0, st_, rsB_.p, rsA_.p, R, N, w);
-                        hipLaunchKernelGGL(k_rs_box_cols, dim3(nblk(RN)), dim3(256), 0, st_, rsA_.p, rsB_.p, nfp, R, N,
-                                           w);
+                        hipLaunchKernelGGL(k_rs_box, dim3(nblk(RN)), dim3(256), 0, st_, o, i, nfp, R, N, w);
                     });
                 };
-                box(rsNf_.p, 1);                       // v = nf / smooth(pt, 1)
+                box(rsB_.p, rsA_.p, rsNf_.p, 1);       // v = nf / smooth(pt, 1)
                 launch("k_rs_weights", R, [&] {
-                    hipLaunchKernelGGL(k_rs_weights, dim3(nblk(RN)), dim3(256), 0, st_, rsB_.p, rsA_.p, R, N);
+                    hipLaunchKernelGGL(k_rs_weights, dim3(nblk(RN)), dim3(256), 0, st_, rsA_.p, rsB_.p, R, N);
                 });
                 launch("k_rs_region", R, [&] {
-                    hipLaunchKernelGGL(k_rs_region<N>, dim3(R), dim3(1024), 0, st_, rsA_.p, rsB_.p);
+                    hipLaunchKernelGGL(k_rs_region<N>, dim3(R), dim3(1024), 0, st_, rsB_.p, rsA_.p);
                 });
-                box(nullptr, 4);                       // rs = smooth(weights in the region, 4)
+                box(rsA_.p, rsB_.p, nullptr, 4);       // rs = smooth(weights in the region, 4)
             }
             launch("k_rs_apply", R, [&] {
                 hipLaunchKernelGGL(k_rs_apply, dim3(nblk(RN)), dim3(256), 0, st_, rsT_.p, rsAo_.p,

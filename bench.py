@@ -90,7 +90,9 @@ def make_solver(pbs, precision, batch, gradient, timing=False):
     if timing:
         os.environ["MFBD_KERNEL_TIMING"] = "1"
     try:
-        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient)
+        # device line-search rounds launched directly: a graph replay of a round measured no
+        # faster, and rocprofv3's kernel trace of this command must see each kernel
+        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient, line_graph=False)
         sv.load(pbs)
     finally:
         os.environ.pop("MFBD_KERNEL_TIMING", None)
@@ -130,6 +132,7 @@ def roofline(stats, N, nPS, nH, form):
     complex and real element sizes of the form (16 and 8 for FP64, 8 and 4 for FP32).
     Algorithmic bytes per image:
       k_otf_fused (FP32): read phi, write S                       r nPS + c nH
+        (a line-search probe reads the line's base and direction:  2 r nPS + c nH)
       k_otf64_fused (FP64): read phi, write S                     r nPS + c nH
       k_pq: read S and D                                             2 c nH
       k_phase_mfma: write phi                                        r nPS
@@ -155,14 +158,17 @@ def roofline(stats, N, nPS, nH, form):
     if name in per_unit and s["units"] > 0:
         mean_ms = s["ms"] / s["launches"]
         units_per_launch = s["units"] / s["launches"]
-        bytes_per_launch = per_unit[name] * units_per_launch
+        # a line-search probe's transform reads the line's base and direction (2 r nPS) for phi
+        line = stats.get("_line_images", {}).get("units", 0) if name == "k_otf_fused" else 0
+        bytes_per_launch = (per_unit[name] * s["units"] + r * nPS * line) / s["launches"]
         ach = bytes_per_launch / (mean_ms * 1e-3) / 1e9
         tr = measured_traffic(form).get(name)
         out.update(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                    frac=round(ach / HBM_PEAK_GBS, 4),
                    traffic=None if tr is None else int(tr * units_per_launch), launches=s["launches"],
                    mean_launch_ms=round(mean_ms, 4), images_per_launch=round(units_per_launch, 1),
-                   bytes_per_image=per_unit[name], bytes_per_launch=int(bytes_per_launch))
+                   bytes_per_image=round(bytes_per_launch / units_per_launch, 1), bytes_per_launch=int(bytes_per_launch),
+                   line_probe_images=int(line))
         if name in ("k_otf_fused", "k_otf64_fused"):
             flops = 2 * 5 * N * N * math.log2(N * N) * units_per_launch
             tf = flops / (mean_ms * 1e-3) / 1e12
@@ -443,7 +449,7 @@ def stage_roofline(stats, N, P, form):
     stage bytes per image-evaluation of the path's roofline table: read phi on the P x P
     pupil grid and D_j on the N x N grid, r P^2 + c N^2 (c, r the complex and real sizes)."""
     c, r = (16, 8) if form == "fp64" else (8, 4)
-    names = [k for k in stats if k in ("k_otf64_fused", "k_otf_fused", "k_pq", "k_metric", "k_field_rows",
+    names = [k for k in stats if k in ("k_otf64_fused", "k_otf_fused", "k_pq", "k_field_rows",
                                        "k_cols_power", "k_rows_inv_real", "k_cols_extract")]
     ms = sum(stats[k]["ms"] for k in names)
     imgs = stats.get("k_otf64_fused", stats.get("k_otf_fused", {"units": 0}))["units"]
@@ -550,8 +556,8 @@ def headline(args):
                                                        "MODE_START", "MODE_STEP", "GRADDIFF_STEP")}
                    | {"GETSTEP": "conjugate", "gradient": args.gradient,
                       "line_phases": "FP32 and mixed: probes from base + step * direction (metricAt); FP64: the GEMM per probe",
-                      "line_search": "FP32 and mixed: on the device (bracket + brent state machine, one HIP graph per "
-                                     "round of probes, read back every 4 rounds); FP64: stepped by the host"},
+                      "line_search": "FP32 and mixed: on the device (bracket + brent state machine per patch, rounds "
+                                     "of probes launched directly, read back every 4 rounds); FP64: stepped by the host"},
                    "converged": "solve exits normally and restored objects within 1e-4 (relative RMS) of the "
                                 f"FP64 oracle's solve, on a sample of {len(idx)} patches; value = patches/s x "
                                 "sampled fraction",
@@ -569,8 +575,10 @@ def headline(args):
                                              "converged": round(value / (plain_rate * chk_plain["fraction"]), 2)
                                              if chk_plain["fraction"] > 0 else None},
         "ratio_fp64_batched_vs_fp64_plain": round((len(pbs) / t64) / plain_rate, 2),
-        "kernels_ms_per_step": {"fp32": {k: round(v["ms"], 3) for k, v in sorted(st32.items(), key=lambda kv: -kv[1]["ms"])},
-                                "fp64": {k: round(v["ms"], 3) for k, v in sorted(st64.items(), key=lambda kv: -kv[1]["ms"])}},
+        "kernels_ms_per_step": {"fp32": {k: round(v["ms"], 3) for k, v in sorted(st32.items(), key=lambda kv: -kv[1]["ms"])
+                                         if not k.startswith("_")},
+                                "fp64": {k: round(v["ms"], 3) for k, v in sorted(st64.items(), key=lambda kv: -kv[1]["ms"])
+                                         if not k.startswith("_")}},
         "kernel_stats_note": "one step per form on a context with HIP events around every kernel and its evaluation "
                              "lanes on one stream, so each launch's duration is its own; the timed steps run "
                              "without events and with the lanes' kernels overlapping on their own streams",

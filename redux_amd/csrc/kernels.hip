@@ -589,20 +589,16 @@ __global__ void __launch_bounds__(256) k_phase_bf16(float* __restrict__ phi, con
     }
 }
 
-// One round of the device line search for the nL patches of a lane (one thread each): each
-// patch in a line search takes the metric of its last probe (L[r]) and runs to its next
-// probe, whose step goes to its images' qS (s = -stp lambda) with qLine = its line row; a
-// patch out of a line search marks its images -2 (skipped by every kernel of the round).
-__global__ void k_ls_step(LsState* __restrict__ st, const double* __restrict__ L, int* __restrict__ qLine,
-                          double* __restrict__ qS, int* __restrict__ active, const int* __restrict__ slot, int nL,
-                          int nImg) {
-#pragma clang fp contract(off)
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nL) return;
+// The line of patch r takes the metric f of its last probe and runs to its next probe,
+// whose step goes to its images' qS (s = -stp lambda) with qLine = its line row; a line
+// that has ended (or an idle patch) marks its images -2, which every kernel of a round
+// skips, and clears active[r].
+__device__ void ls_next(LsState* __restrict__ st, int r, double f, int* __restrict__ qLine, double* __restrict__ qS,
+                        int* __restrict__ active, const int* __restrict__ slot, int nImg) {
     LsState S = st[r];
     int on = 0;
     double stp = 0.0;
-    if (S.pc >= 0) on = ls_advance(S, L[r], &stp);
+    if (S.pc >= 0) on = ls_advance(S, f, &stp);
     S.nProbe += on;
     st[r] = S;
     active[r] = on;
@@ -613,9 +609,11 @@ __global__ void k_ls_step(LsState* __restrict__ st, const double* __restrict__ L
     }
 }
 
-// New lines: state i of the list starts at metric fa with lambda (a line's probes count from 0)
-__global__ void k_ls_init(LsState* __restrict__ st, const int* __restrict__ idx, const double* __restrict__ fl,
-                          int n) {
+// New lines: state idx[i] starts at metric fa with lambda (its probes count from 0) and
+// takes its first step, the probe at 1e-21 / lambda
+__global__ void k_ls_init(LsState* __restrict__ st, const int* __restrict__ idx, const double* __restrict__ fl, int n,
+                          int* __restrict__ qLine, double* __restrict__ qS, int* __restrict__ active,
+                          const int* __restrict__ slot, int nImg) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     LsState S = {};
@@ -623,6 +621,7 @@ __global__ void k_ls_init(LsState* __restrict__ st, const int* __restrict__ idx,
     S.fa0 = fl[2 * i];
     S.lambda = fl[2 * i + 1];
     st[idx[i]] = S;
+    ls_next(st, idx[i], 0.0, qLine, qS, active, slot, nImg);
 }
 
 // Phases of line-search probes without the mode GEMM (Solver::my_precalc / metricAt,
@@ -703,6 +702,20 @@ __device__ double block_sum(double v, double* sh) {
     return r;
 }
 
+// The metric of request r from its chunks' partial sums, L = sum_o w_o (sum_chunks
+// partial) / s^2, in a fixed order; ld(i) reads partial i
+template <typename LD>
+__device__ __forceinline__ double metric_of(int r, const double* __restrict__ w, int nObj, int nChunk, double invS2,
+                                            LD ld) {
+    double sum = 0;
+    for (int o = 0; o < nObj; ++o) {
+        double m = 0;
+        for (int c = 0; c < nChunk; ++c) m += ld(((size_t)r * nObj + o) * nChunk + c);
+        if (w[o] != 0.0) sum += w[o] * (m * invS2);
+    }
+    return sum;
+}
+
 // P, Q on the stored half of the OTF support for (request r, object o, chunk of 256
 // points) plus the chunk's metric partial sum(wt (ftSum - |P|^2/Q)), wt = 2 for a point
 // standing for itself and its mirror.  Images are summed per
@@ -760,13 +773,20 @@ __global__ void k_metric(double* __restrict__ L, const double* __restrict__ part
                          int nReq, int nObj, int nChunk, double invS2) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nReq) return;
-    double sum = 0;
-    for (int o = 0; o < nObj; ++o) {
-        double m = 0;
-        for (int c = 0; c < nChunk; ++c) m += partial[((size_t)r * nObj + o) * nChunk + c];
-        if (w[o] != 0.0) sum += w[o] * (m * invS2);
-    }
-    L[r] = sum;
+    L[r] = metric_of(r, w, nObj, nChunk, invS2, [&](size_t i) { return partial[i]; });
+}
+
+// The end of a device line-search round: each probed patch's metric (as k_metric forms
+// it), and its line moved to the next probe (ls_next); patches not probed keep their state
+__global__ void k_ls_metric(LsState* __restrict__ st, double* __restrict__ L, const double* __restrict__ partial,
+                            const double* __restrict__ w, int nL, int nObj, int nChunk, double invS2,
+                            int* __restrict__ qLine, double* __restrict__ qS, int* __restrict__ active,
+                            const int* __restrict__ slot, int nImg) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nL || !active[r]) return;
+    const double f = metric_of(r, w, nObj, nChunk, invS2, [&](size_t i) { return partial[i]; });
+    L[r] = f;
+    ls_next(st, r, f, qLine, qS, active, slot, nImg);
 }
 
 // G' (de-centred): tile[q][e] = (PQ D - PS S)/QS on the support (the mirror half as
@@ -2824,9 +2844,10 @@ public:
         }
     }
 
-    // One round of a lane's line searches: every line to its next probe (k_ls_step), the
-    // probes' transforms from the lines' phases, P / Q and the metrics.  Raw launches when
-    // captured; with `named` each through launch() (its statistics, events when timing).
+    // One round of a lane's line searches: the probes' transforms from the lines' phases,
+    // P / Q, then each probed patch's metric and its line's next probe (k_ls_metric).  Raw
+    // launches when captured; with `named` each through launch() (its statistics, events
+    // when timing).
     void ls_round(LsLane& l, bool named) {
         const int nL = l.nL, nImg = g_.nImg, nQ = nL * nImg, nChunk = (nOS_ + 255) / 256;
         const int* tSlot = l.tab.p;
@@ -2842,10 +2863,6 @@ public:
             else
                 f();
         };
-        go("k_ls_step", [&] {
-            hipLaunchKernelGGL(k_ls_step, dim3((nL + 63) / 64), dim3(64), 0, st_, l.st.p, l.L.p, tLine, l.qS.p, tActive,
-                               tSlot, nL, nImg);
-        });
         if constexpr (FUSED) {
             go("k_otf_fused", [&] {
                 hipLaunchKernelGGL((k_otf_fused<N, FNT>), dim3(std::min(nQ, otfGrid_)), dim3(FNT), OLDS, st_, l.S.p,
@@ -2868,9 +2885,9 @@ public:
                                D_.p, ftSum_.p, gamma_.p, tSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, imgScale_.p,
                                g_.nCh, g_.nObj, nImg, nOS_, 1, tActive);
         });
-        go("k_metric", [&] {
-            hipLaunchKernelGGL(k_metric, dim3(nblk(nL, 64)), dim3(64), 0, st_, l.L.p, l.part.p, w_.p, nL, g_.nObj,
-                               nChunk, 1.0 / ((double)g_.s * g_.s));
+        go("k_ls_metric", [&] {
+            hipLaunchKernelGGL(k_ls_metric, dim3((nL + 63) / 64), dim3(64), 0, st_, l.st.p, l.L.p, l.part.p, w_.p, nL,
+                               g_.nObj, nChunk, 1.0 / ((double)g_.s * g_.s), tLine, l.qS.p, tActive, tSlot, nImg);
         });
     }
 
@@ -2912,7 +2929,9 @@ public:
         HIPCHK(hipMemcpyAsync(l.dirA.p, l.hDir.p, (size_t)n * row * sizeof(double), hipMemcpyHostToDevice, st_));
         HIPCHK(hipMemcpyAsync(l.dirTab.p, l.hDirTab.p, 4 * (size_t)rows * sizeof(int), hipMemcpyHostToDevice, st_));
         launch("k_ls_init", n, [&] {
-            hipLaunchKernelGGL(k_ls_init, dim3((n + 63) / 64), dim3(64), 0, st_, l.st.p, l.startIdx.p, l.fl.p, n);
+            const int nQ = l.nL * nImg;
+            hipLaunchKernelGGL(k_ls_init, dim3((n + 63) / 64), dim3(64), 0, st_, l.st.p, l.startIdx.p, l.fl.p, n,
+                               l.tab.p + l.nL + nQ, l.qS.p, l.tab.p + l.nL + 2 * nQ, l.tab.p, nImg);
         });
         const int* gA = l.dirTab.p;
         if (bf16Phase_) {
@@ -2961,11 +2980,17 @@ public:
             l.probes[r] = x.nProbe;
         }
         collect(lane);
-        // the rounds' work in images evaluated (skipped images not counted)
+        // the rounds' work in images evaluated (skipped images not counted); "_line_images"
+        // counts the images whose transform read the line (base and direction) for phi
         const long long img = probes * g_.nImg;
         for (const char* k : {"k_otf_fused", "k_line_phase", "k_otf64_fused", "k_pq", "k_ls_round"}) {
             auto it = stats_.find(k);
             if (it != stats_.end() && (std::strcmp(k, "k_ls_round") == 0 || !l.useGraph)) it->second.units += img;
+        }
+        if (timing_ && !l.useGraph && FUSED) {
+            auto& li = stats_["_line_images"];
+            li.name = "_line_images";
+            li.units += img;
         }
     }
 

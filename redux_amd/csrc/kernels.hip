@@ -759,9 +759,11 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
             }
         }
         const size_t k = ((size_t)r * nObj + o) * nOS + u;
-        Pout[2 * k] = pr;
-        Pout[2 * k + 1] = pi;
-        Qout[k] = q;
+        if (Pout) {             // a line-search round needs the metric partials only
+            Pout[2 * k] = pr;
+            Pout[2 * k + 1] = pi;
+            Qout[k] = q;
+        }
         part = wt[u] * (ftSum[((size_t)slot * nObj + o) * nOS + u] - (pr * pr + pi * pi) / q);
     }
     const double s = block_sum(part, sh);
@@ -776,17 +778,38 @@ __global__ void k_metric(double* __restrict__ L, const double* __restrict__ part
     L[r] = metric_of(r, w, nObj, nChunk, invS2, [&](size_t i) { return partial[i]; });
 }
 
-// The end of a device line-search round: each probed patch's metric (as k_metric forms
-// it), and its line moved to the next probe (ls_next); patches not probed keep their state
-__global__ void k_ls_metric(LsState* __restrict__ st, double* __restrict__ L, const double* __restrict__ partial,
-                            const double* __restrict__ w, int nL, int nObj, int nChunk, double invS2,
-                            int* __restrict__ qLine, double* __restrict__ qS, int* __restrict__ active,
-                            const int* __restrict__ slot, int nImg) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nL || !active[r]) return;
-    const double f = metric_of(r, w, nObj, nChunk, invS2, [&](size_t i) { return partial[i]; });
-    L[r] = f;
-    ls_next(st, r, f, qLine, qS, active, slot, nImg);
+// The end of a device line-search round, one 64-thread workgroup per patch: each probed
+// patch's metric (the sum k_metric forms, from the partials staged in LDS by the whole
+// workgroup) and its line moved to the next probe (ls_next on a copy of the state staged
+// the same way); patches not probed return at once
+__global__ void __launch_bounds__(64) k_ls_metric(LsState* __restrict__ st, double* __restrict__ L,
+                                                  const double* __restrict__ partial, const double* __restrict__ w,
+                                                  int nObj, int nChunk, double invS2, int* __restrict__ qLine,
+                                                  double* __restrict__ qS, int* __restrict__ active,
+                                                  const int* __restrict__ slot, int nImg) {
+    constexpr int kMaxPart = 512;
+    constexpr int kWords = (int)(sizeof(LsState) / sizeof(int));
+    static_assert(sizeof(LsState) % sizeof(int) == 0, "LsState copies as ints");
+    __shared__ double part[kMaxPart];
+    __shared__ LsState S;
+    const int r = blockIdx.x, np = nObj * nChunk;
+    if (!active[r]) return;
+    const bool staged = np <= kMaxPart;
+    if (staged)
+        for (int i = threadIdx.x; i < np; i += 64) part[i] = partial[(size_t)r * np + i];
+    int* sw = reinterpret_cast<int*>(&S);
+    const int* gw = reinterpret_cast<const int*>(st + r);
+    for (int i = threadIdx.x; i < kWords; i += 64) sw[i] = gw[i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double f = staged ? metric_of(r, w, nObj, nChunk, invS2, [&](size_t i) { return part[i - (size_t)r * np]; })
+                                : metric_of(r, w, nObj, nChunk, invS2, [&](size_t i) { return partial[i]; });
+        L[r] = f;
+        ls_next(&S, 0, f, qLine + (size_t)r * nImg, qS + (size_t)r * nImg, active + r, slot + r, nImg);
+    }
+    __syncthreads();
+    int* ow = reinterpret_cast<int*>(st + r);
+    for (int i = threadIdx.x; i < kWords; i += 64) ow[i] = sw[i];
 }
 
 // G' (de-centred): tile[q][e] = (PQ D - PS S)/QS on the support (the mirror half as
@@ -2351,11 +2374,15 @@ public:
         // MFBD_MAX_TILES caps the tile workspace (images per chunk of an evaluation or a
         // restoration), so that tests can run the multi-chunk paths on small problems
         if (const char* e = getenv("MFBD_MAX_TILES")) tileCap_ = std::max(1, atoi(e));
+        // the evaluation lanes' streams first: the runtime spreads streams over its hardware
+        // queues (4 by default) in creation order, and with the main stream created first
+        // two evaluation lanes could share a queue and serialise (measured: the process's
+        // first FP32 context 12 % slower than later ones)
+        for (int l = 0; l < kMaxEvalLanes; ++l)
+            if (!timing_) HIPCHK(hipStreamCreate(&lanes_[l].stream));
         HIPCHK(hipStreamCreate(&lanes_[kSyncLane].stream));
-        for (int l = 0; l < kMaxEvalLanes; ++l) {
-            if (timing_) lanes_[l].stream = lanes_[kSyncLane].stream;
-            else HIPCHK(hipStreamCreate(&lanes_[l].stream));
-        }
+        if (timing_)
+            for (int l = 0; l < kMaxEvalLanes; ++l) lanes_[l].stream = lanes_[kSyncLane].stream;
         st_ = lanes_[kSyncLane].stream;
         HIPCHK(hipEventCreate(&ev0_));
         HIPCHK(hipEventCreate(&ev1_));
@@ -2763,7 +2790,7 @@ public:
         int nL = 0;
         std::vector<int> slots;
         DBuf<LsState> st;
-        DBuf<double> L, qS, Pp, Qp, part, fl, dirA;
+        DBuf<double> L, qS, part, fl, dirA;
         DBuf<int> tab, startIdx, dirTab;   // tab: slots [nL] | scale rows [nQ] | qLine [nQ] | active [nL] | lq [nQ] | lrow [nQ]
         DBuf<C> S;
         DBuf<T> phi;
@@ -2810,8 +2837,6 @@ public:
             l.st.alloc(nL);
             l.L.alloc(nL);
             l.qS.alloc(nQ);
-            l.Pp.alloc((size_t)nL * g_.nObj * nOS_ * 2);
-            l.Qp.alloc((size_t)nL * g_.nObj * nOS_);
             l.part.alloc((size_t)nL * g_.nObj * nChunk);
             l.S.alloc((size_t)nQ * nOS_);
             l.phi.alloc((size_t)nQ * nPS_);
@@ -2881,13 +2906,14 @@ public:
             });
         }
         go("k_pq", [&] {
-            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, nL), dim3(256), 0, st_, l.Pp.p, l.Qp.p, l.part.p, l.S.p,
+            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, nL), dim3(256), 0, st_, (double*)nullptr,
+                               (double*)nullptr, l.part.p, l.S.p,
                                D_.p, ftSum_.p, gamma_.p, tSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, imgScale_.p,
                                g_.nCh, g_.nObj, nImg, nOS_, 1, tActive);
         });
         go("k_ls_metric", [&] {
-            hipLaunchKernelGGL(k_ls_metric, dim3((nL + 63) / 64), dim3(64), 0, st_, l.st.p, l.L.p, l.part.p, w_.p, nL,
-                               g_.nObj, nChunk, 1.0 / ((double)g_.s * g_.s), tLine, l.qS.p, tActive, tSlot, nImg);
+            hipLaunchKernelGGL(k_ls_metric, dim3(nL), dim3(64), 0, st_, l.st.p, l.L.p, l.part.p, w_.p, g_.nObj, nChunk,
+                               1.0 / ((double)g_.s * g_.s), tLine, l.qS.p, tActive, tSlot, nImg);
         });
     }
 

@@ -393,6 +393,10 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "fp32", "mixed"], default="fp64",
                     help="form of the uneven workload (mixed: BF16 mode GEMM, FP32 transforms; errors reported "
                          "against the FP64 form)")
+    ap.add_argument("--config3-patches", type=int, default=96,
+                    help="uneven patches of the configs 3 / 4 sample in the headline line (0: skip it)")
+    ap.add_argument("--config5-patches", type=int, default=8,
+                    help="config-5 patches of the wave sample in the headline line")
     ap.add_argument("--pmc-probe", choices=["fp32", "fp64", "mixed"], default=None,
                     help="run the short workload profiled for HBM traffic (FETCH_SIZE / WRITE_SIZE) in one "
                          "precision form and exit")
@@ -411,6 +415,76 @@ def main():
         cfg5(args)
         return
     headline(args)
+
+
+def other_configs(args):
+    """Configs 3, 4 and 5 at their stated sizes on bounded samples, for the headline line:
+    an uneven batch (grids 64 / 128 / 256, 36-100 KL modes and 12-48 frames per object drawn
+    per patch) solved as one context per grid at once in FP64 (config 3) and in the mixed
+    form (config 4, objects against the FP64 solve of the same batch), with one patch per grid
+    solved alone in its own geometry beside it; and a wave of config-5 patches (N = 256, 150
+    KL modes, 2 x 100 frames) in FP32 with the adjoint gradient (its objects against FP64
+    solves are in --workload cfg5).  Each is one warm-up and one timed solve; the full-size
+    runs are --workload uneven / cfg5."""
+    from redux_amd import synth
+    from redux_amd.momfbd import Solver, solve_group, uneven_solvers
+    workers = min(16, os.cpu_count() or 1)
+    out = {}
+    groups = synth.uneven_batch(synth.UNEVEN_CLASSES, args.config3_patches, seed=args.seed, workers=workers)
+    groups = [g for g in groups if g["patches"]]
+    sets = [(g["patches"], g["shapes"]) for g in groups]
+    res = {}
+    for prec in ("fp64", "mixed"):
+        sv = uneven_solvers(sets, precision=prec)
+        solve_group(sv)
+        t0 = time.perf_counter()
+        res[prec] = solve_group(sv)
+        dt = time.perf_counter() - t0
+        for x in sv:
+            x.close()
+        n = sum(len(r) for r in res[prec])
+        conv = sum(1 for ra in res[prec] for r in ra if r["status"] == 0)
+        out[prec] = dict(patches=n, seconds=round(dt, 3), patches_per_s=round(n / dt, 3), converged=conv)
+        log(f"config {3 if prec == 'fp64' else 4}: {n} uneven patches {prec} in {dt:.2f}s")
+    errs = [float(np.linalg.norm(a["img"].astype(np.float64) - b["img"]) / np.linalg.norm(b["img"]))
+            for ra, rb in zip(res["mixed"], res["fp64"]) for a, b in zip(ra, rb)]
+    out["mixed"]["obj_rel_err_vs_fp64"] = {"median": float(np.median(errs)), "max": float(np.max(errs))}
+    plain, same = {}, True
+    for g, ra in zip(groups, res["fp64"]):
+        own = g["own"][0]
+        sv = Solver(own, precision="fp64", max_batch=1)
+        sv.load([own])
+        t0 = time.perf_counter()
+        r = sv.run()[0]
+        plain[f"N={g['setup']['N']}"] = round(time.perf_counter() - t0, 4)
+        sv.close()
+        sel = np.flatnonzero(g["shapes"][0]["imgOn"])
+        same &= bool(np.array_equal(r["img"], ra[0]["img"])
+                     and np.array_equal(r["alpha"], ra[0]["alpha"][sel][:, :g["shapes"][0]["nModes"]]))
+    est = sum(plain[f"N={g['setup']['N']}"] * len(g["patches"]) for g in groups)
+    config3 = dict(out["fp64"], patches_per_grid={f"N={g['setup']['N']}": len(g["patches"]) for g in groups},
+                   plain_seconds_per_patch=plain, plain_identical_to_batched=same,
+                   speedup_vs_plain=round(est / out["fp64"]["seconds"], 2))
+    # config 5: a wave of patches, FP32 with the adjoint gradient
+    _STATE.update(su5=synth.build_setup(synth.CFG5_PATCH), seed5=args.seed)
+    with ProcessPoolExecutor(max_workers=workers) as ex:
+        pbs = list(ex.map(_cfg5_patch, range(args.config5_patches), chunksize=1))
+    sv = make_solver(pbs, "fp32", len(pbs), "adjoint")
+    run_step(sv)
+    r5, dt5 = run_step(sv)
+    sv.close()
+    log(f"config 5: {len(pbs)} patches fp32 adjoint in {dt5:.2f}s")
+    return {
+        "config3_uneven_fp64": config3,
+        "config4_uneven_mixed": out["mixed"],
+        "config5_wave_fp32_adjoint": dict(patches=len(pbs), seconds=round(dt5, 3),
+                                          patches_per_s=round(len(pbs) / dt5, 4),
+                                          converged=sum(1 for r in r5 if r["status"] == 0),
+                                          evals_per_patch={"f": float(np.mean([r["nF"] for r in r5])),
+                                                           "grad": float(np.mean([r["nDF"] for r in r5]))}),
+        "note": "one timed solve each after a warm-up, on bounded samples of the stated sizes; full runs: "
+                "--workload uneven (1,024 patches), --workload cfg5 (64-patch waves)",
+    }
 
 
 def oracle_sample(pbs, n_sample, threads):
@@ -588,6 +662,8 @@ def headline(args):
         "fp64_roofline": roofline(st64, su["N"], nps, nh, "fp64"),
         "cpu_baseline": cpu,
     }
+    if args.config3_patches > 0:
+        line["other_configs"] = other_configs(args)
     print(json.dumps(line), flush=True)
 
 

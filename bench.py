@@ -90,9 +90,7 @@ def make_solver(pbs, precision, batch, gradient, timing=False):
     if timing:
         os.environ["MFBD_KERNEL_TIMING"] = "1"
     try:
-        # device line-search rounds launched directly: a graph replay of a round measured no
-        # faster, and rocprofv3's kernel trace of this command must see each kernel
-        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient, line_graph=False)
+        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient)
         sv.load(pbs)
     finally:
         os.environ.pop("MFBD_KERNEL_TIMING", None)

@@ -42,7 +42,7 @@ class Solver:
     """One context: geometry + structure + basis + constraints; patches are loaded in bulk."""
 
     def __init__(self, pb: dict, precision: str = "fp64", max_batch: int = 1, device: int = 0, settings=None,
-                 gradient: str = "reference", line_phases=None, line_rounds=None, line_graph=True):
+                 gradient: str = "reference", line_phases=None, line_rounds=None, line_graph=False):
         """line_phases: line-search probes from base + step * direction phases instead of the
         mode GEMM per probe (mfbd_params.linePhases).  By default on for the FP32 and mixed
         forms (their transform reads the line in place of the GEMM's output) and off for
@@ -56,7 +56,8 @@ class Solver:
         ~18 % faster, the host no longer between probes) and 0 for FP64, whose solve the
         device's transforms bound (lines that end early would wait for the read-back).
         line_graph: each round of probes as one captured HIP graph (mfbd_params.lineGraph),
-        else its kernels launched one by one."""
+        else (the default) its three kernels launched one by one: the two measured the same,
+        and rocprofv3's kernel trace cannot follow the graph launches."""
         self.pb = pb
         st = default_params(**pb.get("settings", {}))
         if settings:

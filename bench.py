@@ -373,6 +373,46 @@ def cfg5(args):
     print(json.dumps(line), flush=True)
 
 
+def trace_summary(trace_csv, bench_json):
+    """Markdown tables of a rocprofv3 kernel trace of this script's default run: per kernel,
+    the dispatches of the FP64 and the FP32 statistics steps (the windows from the first to
+    the n-th dispatch of each form's roofline kernel, n = the bench line's launch count), and
+    the roofline kernels' rocprofv3 mean beside bench.py's HIP-event mean."""
+    import csv
+    import collections
+    with open(bench_json) as f:
+        line = json.loads([x for x in f.read().splitlines() if x.startswith("{")][-1])
+    with open(trace_csv) as f:
+        ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in csv.DictReader(f))
+
+    def short(n):
+        return n.split("(")[0].replace("void ", "")
+
+    out = []
+    for title, key in (("FP64 statistics step", "fp64_roofline"), ("FP32 statistics step (the timed form)", "roofline")):
+        rl = line[key]
+        name, n = rl["kernel"], rl["launches"]
+        mine = [k for k in ks if short(k[2]).split("<")[0] == "mfbd::" + name][:n]
+        t0, t1 = mine[0][0], mine[-1][1]
+        tot = collections.defaultdict(lambda: [0, 0])
+        for s0, e0, k in ks:
+            if s0 >= t0 and e0 <= t1:
+                tot[short(k)][0] += e0 - s0
+                tot[short(k)][1] += 1
+        out.append(f"**{title}** (dispatches from the first `{name}` to its {n}th, "
+                   f"{(t1 - t0) / 1e6:.1f} ms):\n")
+        out.append("| kernel | dispatches | total ms | mean µs |\n|---|---:|---:|---:|")
+        for k, (t, c) in sorted(tot.items(), key=lambda kv: -kv[1][0]):
+            if t / 1e6 >= 0.05:
+                out.append(f"| `{k}` | {c} | {t / 1e6:.1f} | {t / c / 1e3:.2f} |")
+        mean = sum(e0 - s0 for s0, e0, _ in mine) / len(mine) / 1e6
+        out.append(f"\n`{name}`: rocprofv3 mean {mean:.4f} ms over {len(mine)} dispatches; bench.py's HIP-event "
+                   f"mean {rl['mean_launch_ms']:.4f} ms, {rl['images_per_launch']:.0f} images per launch, "
+                   f"{rl['achieved']:.0f} GB/s ({rl['frac']:.3f} of HBM peak), FFT {rl['fft_tflops']:.1f} TFLOP/s "
+                   f"({rl['fft_frac']:.3f} of the vector peak)\n")
+    print("\n".join(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -398,7 +438,13 @@ def main():
     ap.add_argument("--pmc-probe", choices=["fp32", "fp64", "mixed"], default=None,
                     help="run the short workload profiled for HBM traffic (FETCH_SIZE / WRITE_SIZE) in one "
                          "precision form and exit")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("TRACE_CSV", "BENCH_JSON"), default=None,
+                    help="print the statistics steps' kernel tables from a rocprofv3 kernel trace of this "
+                         "script's default run and its JSON line, and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(*args.trace_summary)
+        return
     if args.pmc_probe:
         pmc_probe(args.pmc_probe)
         return

@@ -82,3 +82,32 @@ def test_device_line_search_needs_line_phases():
         assert "host_line_search" not in sv.kernel_stats()
         sv.close()
     _same(out[4], out[0], "no line phases")
+
+
+def test_nonfinite_and_zero_weight_patches_as_on_the_host():
+    """A patch whose frames hold NaN (its lines run on NaN metrics until brent's iteration
+    limit ends them), beside good patches of the same batch, and a context whose objects
+    weigh 0 (a zero gradient, so no line starts) end with the statuses and results the
+    host's line search gives them, in FP32."""
+    from redux_amd import _lib as L
+    pb, _ = load()
+    good = dict(pb)
+    good["settings"] = dict(pb["settings"], MODE_START=pb["K"])
+    nan = dict(good)
+    nan["data"] = np.array(pb["data"], copy=True)
+    nan["data"][2, 10:20, 10:20] = np.nan
+    zero = dict(good)
+    zero["objects"] = [dict(o, weight=0.0) for o in pb["objects"]]
+    print("device line search: nan / good batch, zero weight alone", flush=True)
+    for pbs, want in (([nan, good, good], [L.PATCH_NONFINITE, L.PATCH_CONVERGED, L.PATCH_CONVERGED]),
+                      ([zero], [L.PATCH_NO_PROGRESS])):
+        (host,), _ = _solve(pbs, "fp32", 0)
+        (dev,), st = _solve(pbs, "fp32", 4, graph=False)
+        assert [r["status"] for r in dev] == want, [r["status"] for r in dev]
+        if len(pbs) > 1:
+            assert st["host_line_search"]["launches"] > 0          # the good patches' lines
+        for k, (x, y) in enumerate(zip(dev, host)):
+            assert x["status"] == y["status"] and x["nF"] == y["nF"] and x["iterations"] == y["iterations"], k
+            assert np.array_equal(x["alpha"], y["alpha"], equal_nan=True), k
+            if x["status"] != L.PATCH_NONFINITE:
+                assert np.array_equal(x["img"], y["img"]) and x["finalMetric"] == y["finalMetric"], k

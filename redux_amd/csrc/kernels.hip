@@ -3605,12 +3605,12 @@ private:
         }
         // one pinned block and one copy per evaluation: the caller's doubles xd, with line
         // rows each image's probe step (nQ), then the request tables (7 nQ + n ints), the
-        // caller's ints xi, and in rows mode the GEMM's row tables (5 per GEMM row), the line
-        // rows' (2) and each image's line row (nQ)
+        // caller's ints xi, the GEMM's row tables (5 per GEMM row), and in rows mode the
+        // line rows' (2) and each image's line row (nQ)
         const size_t nd = xd ? xd->size() : 0, nx = xi ? xi->size() : 0;
         const int nLR = nLine * nImg, nGemm = nQ - nLR + nStart * nImg;
         const size_t nd2 = nd + (nLine ? (size_t)nQ : 0);
-        const size_t nInts = 7 * (size_t)nQ + n + nx + (rowsMode ? 5 * (size_t)nGemm + 2 * (size_t)nLR + nQ : 0);
+        const size_t nInts = 7 * (size_t)nQ + n + nx + 5 * (size_t)nGemm + (rowsMode ? 2 * (size_t)nLR + nQ : 0);
         const size_t nBlk = nd2 + (nInts + 1) / 2;
         sc_->hTab.alloc(nBlk);
         if (nd) std::memcpy(sc_->hTab.p, xd->data(), nd * sizeof(double));
@@ -3631,11 +3631,14 @@ private:
         if (nx) std::memcpy(ints + 7 * (size_t)nQ + n, xi->data(), nx * sizeof(int));
         const size_t aRows = staged ? (size_t)(hi - lo + 1) * nImg : (size_t)nQ;   // alpha rows before the directions
         const int LS = maxSlots_ * nImg;
-        if (rowsMode) {
+        {
             // GEMM rows: phases of the requests without a line to the work array (row q), and
             // of a gradient request that starts a conjugate-gradient iteration also to its
             // slot's line base; for each line start the direction B dAlpha (no channel phase)
-            // to the slot's line direction.  Line rows: the probes' phases from the line.
+            // to the slot's line direction.  The rows go to the GEMM grouped by object (in
+            // order within an object), so a wave's 16 rows rarely straddle two objects (each
+            // object in a wave costs a pass over K); every row's sum is the same either way.
+            // Line rows: the probes' phases from the line.
             int* gA = ints + 7 * (size_t)nQ + n + nx;
             int* gP = gA + nGemm;
             int* gO = gP + nGemm;
@@ -3645,39 +3648,44 @@ private:
             int* lrow = lq + nLR;
             int* qLine = lrow + nLR;
             double* qS = sc_->hTab.p + nd;
-            int g = 0, d = 0, l = 0;
+            struct GRow { int a, p, o, out, out2; };
+            std::vector<GRow> rows;
+            rows.reserve(nGemm);
+            int d = 0, l = 0;
             for (int r = 0; r < n; ++r)
                 for (int j = 0; j < nImg; ++j) {
                     const int q = r * nImg + j, srow = rq[r].slot * nImg + j;
-                    if (rq[r].line == 0) {
-                        gA[g] = ints[2 * nQ + q];
-                        gP[g] = ints[4 * nQ + n + q];
-                        gO[g] = st.imgObj[j];
-                        gOut[g] = q;
-                        gOut2[g] = rq[r].saveBase ? srow : -1;
-                        ++g;
-                    } else if (rq[r].line == 1) {
-                        gA[g] = (int)aRows + d * nImg + j;
-                        gP[g] = -1;
-                        gO[g] = st.imgObj[j];
-                        gOut[g] = -1 - (LS + srow);
-                        gOut2[g] = -1;
-                        ++g;
-                    }
-                    qLine[q] = rq[r].line ? srow : -1;
-                    if (nLine) qS[q] = rq[r].line ? rq[r].lineS : 0.0;
-                    if (rq[r].line) {
-                        lq[l] = q;
-                        lrow[l] = srow;
-                        ++l;
+                    if (rq[r].line == 0)
+                        rows.push_back({ints[2 * nQ + q], ints[4 * nQ + n + q], st.imgObj[j], q,
+                                        rq[r].saveBase ? srow : -1});
+                    else if (rq[r].line == 1)
+                        rows.push_back({(int)aRows + d * nImg + j, -1, st.imgObj[j], -1 - (LS + srow), -1});
+                    if (rowsMode) {
+                        qLine[q] = rq[r].line ? srow : -1;
+                        if (nLine) qS[q] = rq[r].line ? rq[r].lineS : 0.0;
+                        if (rq[r].line) {
+                            lq[l] = q;
+                            lrow[l] = srow;
+                            ++l;
+                        }
                     }
                     if (rq[r].line == 1 && j == nImg - 1) ++d;
                 }
-            if (g != nGemm || l != nLR) throw std::runtime_error("forward: line tables");
-            sc_->hDir.alloc(std::max<size_t>((size_t)nStart * row, 1));
-            d = 0;
-            for (int r = 0; r < n; ++r)
-                if (rq[r].line == 1) std::memcpy(sc_->hDir.p + (size_t)d++ * row, rq[r].dAlpha, sizeof(double) * row);
+            if ((int)rows.size() != nGemm || l != nLR) throw std::runtime_error("forward: line tables");
+            std::stable_sort(rows.begin(), rows.end(), [](const GRow& x, const GRow& y) { return x.o < y.o; });
+            for (int g = 0; g < nGemm; ++g) {
+                gA[g] = rows[g].a;
+                gP[g] = rows[g].p;
+                gO[g] = rows[g].o;
+                gOut[g] = rows[g].out;
+                gOut2[g] = rows[g].out2;
+            }
+            if (nStart) {
+                sc_->hDir.alloc(std::max<size_t>((size_t)nStart * row, 1));
+                d = 0;
+                for (int r = 0; r < n; ++r)
+                    if (rq[r].line == 1) std::memcpy(sc_->hDir.p + (size_t)d++ * row, rq[r].dAlpha, sizeof(double) * row);
+            }
         }
         const size_t aAll = aRows + (size_t)nStart * nImg;
         sc_->alpha.alloc(std::max(aAll * K, sc_->alpha.n));
@@ -3698,22 +3706,20 @@ private:
         sc_->qm = {ti, ti + nQ, ti + 2 * nQ};
         sc_->qReq = ti + 3 * nQ;
         sc_->reqSlot = ti + 4 * nQ;
-        const int* qPhRow = ti + 4 * nQ + n;
-        const int* qObj = ti + 5 * nQ + n;
         const int* qScale = ti + 6 * nQ + n;
         sc_->tx = ti + 7 * nQ + n;
         sc_->td = sc_->tab.p;
         sc_->phi.alloc((size_t)nQ * nPS_);
         sc_->S.alloc((size_t)nQ * nOS_);
-        // the mode GEMM: every image of every request, or in rows mode the GEMM rows
-        const int* gA = rowsMode ? sc_->tx + nx : sc_->qm.arow;
-        const int* gP = rowsMode ? gA + nGemm : qPhRow;
-        const int* gO = rowsMode ? gP + nGemm : qObj;
-        const int* gOut = rowsMode ? gO + nGemm : nullptr;
-        const int* gOut2 = rowsMode ? gOut + nGemm : nullptr;
+        // the mode GEMM over its row tables (grouped by object)
+        const int* gA = sc_->tx + nx;
+        const int* gP = gA + nGemm;
+        const int* gO = gP + nGemm;
+        const int* gOut = gO + nGemm;
+        const int* gOut2 = gOut + nGemm;
         const int* lq = rowsMode ? gOut2 + nGemm : nullptr;
         const int* qLineD = rowsMode ? lq + 2 * nLR : nullptr;
-        const int rows = rowsMode ? nGemm : nQ;
+        const int rows = nGemm;
         T* lineBuf = rowsMode ? lineBuf_.p : nullptr;
         if (rows > 0) {
             if (bf16Phase_) {

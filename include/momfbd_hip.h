@@ -199,8 +199,8 @@ int mfbd_restore(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha,
  * patches' alphaOut rows, it gives what restorePatch saves with the solution. */
 int mfbd_restore_outputs(mfbd_ctx* ctx, int n, const int* patchIdx, const double* alpha, int saveMask, float* psf,
                          float* cobj, float* res);
-/* Diagnostics: copy a device array for patch p ("S" OTF on support, "D", "P", "Q",
- * "ftsum", "phi") or host state ("gamma_host", "noise_host", "otfnorm", "window",
+/* Diagnostics: copy a device array for patch p ("S" OTF on support, "D", "P", "Q" (as
+ * the last gradient evaluation or restoration left them), "ftsum", "phi") or host state ("gamma_host", "noise_host", "otfnorm", "window",
  * "shifts" [nImg][2] integer cut-out shifts (x, y), "tiltoff" [nImg][2], "sizes" = pupil
  * support, OTF support and stored (Hermitian half) OTF support point counts) in double;
  * returns the element count via *count. */

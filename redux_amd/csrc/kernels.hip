@@ -3579,8 +3579,9 @@ private:
     // phases, OTFs, P/Q and the metric for n requests (virtual images q = r*nImg + j).
     // Alpha rows come from the slot-indexed pinned staging when every request is
     // staged there (the solver's case), else they are packed per request.
+    // needPQ false: only the metric partials are kept (no request reads P and Q afterwards)
     void forward(EvalReq* rq, int n, int useGamma, const std::vector<int>* xi = nullptr,
-                 const std::vector<double>* xd = nullptr) {
+                 const std::vector<double>* xd = nullptr, bool needPQ = true) {
         const int K = g_.K, nImg = g_.nImg, nQ = n * nImg;
         const size_t row = (size_t)nImg * K;
         bool staged = true;
@@ -3756,7 +3757,8 @@ private:
         sc_->Qp.alloc((size_t)n * g_.nObj * nOS_);
         sc_->part.alloc((size_t)n * g_.nObj * nChunk);
         launch("k_pq", nQ, [&] {
-            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, sc_->Pp.p, sc_->Qp.p, sc_->part.p, sc_->S.p, D_.p,
+            hipLaunchKernelGGL(k_pq<T>, dim3(nChunk, g_.nObj, n), dim3(256), 0, st_, needPQ ? sc_->Pp.p : nullptr,
+                               needPQ ? sc_->Qp.p : nullptr, sc_->part.p, sc_->S.p, D_.p,
                                ftSum_.p, gamma_.p, sc_->reqSlot, chObj_.p, chFirst_.p, chCount_.p, wt_.p, imgScale_.p,
                                g_.nCh, g_.nObj, nImg, nOS_, useGamma);
         });
@@ -3815,7 +3817,9 @@ private:
         grad_tables(rq, n, nImg, gIdx, gEn, gs, nEn, extraPupil);
         const int nG = (int)gIdx.size();
         gIdx.insert(gIdx.end(), gEn.begin(), gEn.end());       // [0, nG): images, [nG, 2nG): counts
-        forward(rq, n, 1, &gIdx, &gs);
+        bool anyFd = false;
+        for (int r = 0; r < n; ++r) anyFd |= rq[r].wantGrad && rq[r].gradMethod == 1;
+        forward(rq, n, 1, &gIdx, &gs, nG > 0 || anyFd);        // P and Q for the gradients only
         const int nChunk = (nOS_ + 255) / 256;
         const size_t nOut = (size_t)n + (size_t)nG * K;
         sc_->out.alloc(nOut);
@@ -3824,8 +3828,6 @@ private:
                                nChunk, 1.0 / ((double)g_.s * g_.s));
         });
         if (nG) vogel_gradients(n, K, nImg, nG, nEn, extraPupil);
-        bool anyFd = false;
-        for (int r = 0; r < n; ++r) anyFd |= rq[r].wantGrad && rq[r].gradMethod == 1;
         if (anyFd) fd_gradient(rq, n, K, nImg);
         sc_->hOut.alloc(nOut);
         HIPCHK(hipMemcpyAsync(sc_->hOut.p, sc_->out.p, nOut * sizeof(double), hipMemcpyDeviceToHost, st_));

@@ -1496,17 +1496,41 @@ __device__ __noinline__ void otf_image(cx<float>* __restrict__ S, PhaseSet<(N / 
     // 3. all columns; the packed power spectrum goes to rows [0, N/2) of the tile
     spass_first_mid<float, N, RF, N, NT, 1>(t, 1, LD);
     spass_last_power<float, N, RL, N, NT, 1>(t, 1, LD, tw, t, 1, LD);
-    // 4. inverse row FFTs of the packed rows, then split into half spectra
+    // 4. inverse row FFTs of the packed rows
     fft_lines<float, N, +1, N / 2, NT, 1>(t, LD, 1, tw);
-    for (int e = threadIdx.x; e < (N / 2) * HC; e += NT) {
-        const int y = e / HC, k = e % HC;
-        const cx<float> z = t[y * LD + k], zm = t[y * LD + (N - k) % N];
-        hs[y * HC + k] = {0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)};
-        hs[(y + N / 2) * HC + k] = {0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)};
+    // 5. inverse column FFTs of the N/2+1 half-spectrum columns; the first pass forms its
+    //    inputs, the half spectra A_y(k) = (Z_y(k) + Z_y*(-k))/2 (rows y < N/2) and
+    //    B_y(k) = (Z_y(k) - Z_y*(-k))/2i (rows y + N/2), from the packed rows as it reads them
+    {
+        constexpr int NB = HC * (N / RF), BT = (NB + NT - 1) / NT;
+        cx<float> v[BT][RF];
+#pragma unroll
+        for (int i = 0; i < BT; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (NB % NT == 0 || b < NB) {
+                const int k = b % HC, j = b / HC;
+#pragma unroll
+                for (int r = 0; r < RF; ++r) {
+                    const int y = j + r * (N / RF), yy = y % (N / 2);
+                    const cx<float> z = t[yy * LD + k], zm = t[yy * LD + (N - k) % N];
+                    v[i][r] = y < N / 2 ? cx<float>{0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y)}
+                                        : cx<float>{0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x)};
+                }
+                dft_reg<float, RF, +1>(v[i]);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < BT; ++i) {
+            const int b = threadIdx.x + i * NT;
+            if (NB % NT == 0 || b < NB) {
+                const int k = b % HC, j = b / HC;
+#pragma unroll
+                for (int r = 0; r < RF; ++r) hs[(j * RF + r) * HC + k] = v[i][r];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    // 5. inverse column FFTs of the N/2+1 half-spectrum columns
-    fft_lines<float, N, +1, HC, NT, 1>(hs, 1, HC, tw);
+    spass<float, N, RL, RF, +1, HC, NT, 1>(hs, 1, HC, tw);
     constexpr float inv = 1.0f / (N * N);
     cx<float>* Sq = S + (size_t)q * nOS;
     for (int u = threadIdx.x; u < nOS; u += NT) {

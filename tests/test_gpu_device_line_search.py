@@ -18,9 +18,10 @@ from tests.golden_problem import load
 pytestmark = pytest.mark.gpu
 
 
-def _solve(pbs, prec, rounds, runs=1, graph=True):
+def _solve(pbs, prec, rounds, runs=1, graph=True, batch=None):
     from redux_amd.momfbd import Solver
-    sv = Solver(pbs[0], precision=prec, max_batch=len(pbs), line_phases=True, line_rounds=rounds, line_graph=graph)
+    sv = Solver(pbs[0], precision=prec, max_batch=batch or len(pbs), line_phases=True, line_rounds=rounds,
+                line_graph=graph)
     sv.load(pbs)
     out = [sv.run() for _ in range(runs)]
     st = sv.kernel_stats()
@@ -111,3 +112,20 @@ def test_nonfinite_and_zero_weight_patches_as_on_the_host():
             assert np.array_equal(x["alpha"], y["alpha"], equal_nan=True), k
             if x["status"] != L.PATCH_NONFINITE:
                 assert np.array_equal(x["img"], y["img"]) and x["finalMetric"] == y["finalMetric"], k
+
+
+def test_batches_smaller_than_the_load():
+    """Five patches solved two at a time (three batches on one context: the lanes' line
+    searches set up again for each) as with the host's line search."""
+    pb, _ = load()
+    rng = np.random.default_rng(23)
+    pbs = []
+    for k in range(5):
+        p = dict(pb)
+        p["data"] = rng.poisson(np.clip(pb["data"], 0, None)).astype(np.float32)
+        pbs.append(p)
+    print("device line search: 5 patches in batches of 2", flush=True)
+    (host,), _ = _solve(pbs, "fp32", 0, batch=2)
+    (dev,), st = _solve(pbs, "fp32", 4, graph=False, batch=2)
+    assert st["host_line_search"]["launches"] > 0
+    _same(dev, host, "batches of 2")

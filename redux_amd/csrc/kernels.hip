@@ -743,15 +743,27 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
             for (int b = 0; b < n; b += 30) {
                 const int e = min(b + 30, n);
                 double tr = 0, ti = 0, tq = 0;
-#pragma unroll 8
-                for (int j = f0 + b; j < f0 + e; ++j) {
-                    if (imgScale[(size_t)slot * nImg + j] == T(0)) continue;   // an image the patch lacks
-                    const cx<T> s = S[((size_t)r * nImg + j) * nOS + u];
-                    const cx<T> d = D[((size_t)slot * nImg + j) * nOS + u];
-                    const double sx = s.x, sy = s.y, dx = d.x, dy = d.y;
-                    tq += sx * sx + sy * sy;
-                    tr += dx * sx + dy * sy;      // conj(d) * s
-                    ti += dx * sy - dy * sx;
+                // eight images' S and D loaded together, then summed in image order; an
+                // image the patch lacks (scale 0) is neither read nor summed
+                constexpr int G = 8;
+                for (int j0 = f0 + b; j0 < f0 + e; j0 += G) {
+                    cx<T> sv[G], dv[G];
+                    bool on[G];
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        const int j = j0 + i;
+                        on[i] = j < f0 + e && imgScale[(size_t)slot * nImg + j] != T(0);
+                        sv[i] = on[i] ? S[((size_t)r * nImg + j) * nOS + u] : cx<T>{T(0), T(0)};
+                        dv[i] = on[i] ? D[((size_t)slot * nImg + j) * nOS + u] : cx<T>{T(0), T(0)};
+                    }
+#pragma unroll
+                    for (int i = 0; i < G; ++i) {
+                        if (!on[i]) continue;
+                        const double sx = sv[i].x, sy = sv[i].y, dx = dv[i].x, dy = dv[i].y;
+                        tq += sx * sx + sy * sy;
+                        tr += dx * sx + dy * sy;      // conj(d) * s
+                        ti += dx * sy - dy * sx;
+                    }
                 }
                 q += tq;
                 pr += tr;

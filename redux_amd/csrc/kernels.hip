@@ -2289,7 +2289,14 @@ __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const
         t[Tile<N>::at(pdeY[i], pdeX[i])] = {m * c, m * s};
     }
     __syncthreads();
-    tile_fft2<float, N, +1, NT>(t, tw, 0, N);
+    // the de-centred field is non-zero only in the rows and columns [0, N/4), [3N/4, N):
+    // row transforms of those rows only, column transforms whose first pass reads only them
+    constexpr int RF = N == 64 ? 8 : 16, RL = 8;
+    auto endRow = [](int l, int e) { return (l < N / 4 ? l : l + N / 2) * Tile<N>::LD + e; };
+    spass_at<float, N, RF, 1, +1, N / 2, NT>(t, endRow, tw);
+    spass_at<float, N, RL, RF, +1, N / 2, NT>(t, endRow, tw);
+    spass_first_pruned<float, N, RF, N, NT, 1, +1, 1>(t, 1, Tile<N>::LD);
+    spass<float, N, RL, RF, +1, N, NT, 1>(t, 1, Tile<N>::LD, tw);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int e = threadIdx.x + k * NT;
@@ -2301,7 +2308,11 @@ __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const
         }
     }
     __syncthreads();
-    tile_fft2<float, N, -1, NT>(t, tw, 0, N);
+    // forward: all rows, then only the columns of the pupil pixels
+    fft_lines<float, N, -1, N, NT, 1>(t, Tile<N>::LD, 1, tw);
+    auto endCol = [](int l, int e) { return e * Tile<N>::LD + (l < N / 4 ? l : l + N / 2); };
+    spass_at<float, N, RF, 1, -1, N / 2, NT>(t, endCol, tw);
+    spass_at<float, N, RL, RF, -1, N / 2, NT>(t, endCol, tw);
     float* vq = v + (size_t)blockIdx.x * nPS;
     for (int i = threadIdx.x; i < nPS; i += NT) {
         float s, c;

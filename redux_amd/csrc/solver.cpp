@@ -1932,10 +1932,11 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 lsInflight[e] = false;
             };
             // Restoration (Object::restorePatch) of the patches that have finished, while the
-            // others still iterate: their P0/Q0 sums go to the device on the synchronous lane behind the
-            // lanes' evaluations, in groups of RESTORE_MIN..RESTORE_MAX patches, and the
-            // filter and inverse transform run on a host thread.  A patch's restoration
-            // depends on its own final alpha only, so grouping leaves it unchanged.
+            // others still iterate: the sums, the filter, the inverse transform and the crop
+            // run on the device's synchronous lane beside the evaluation lanes, in groups of
+            // RESTORE_MIN..RESTORE_MAX patches, and the objects come back as floats.  A
+            // patch's restoration depends on its own final alpha only, so grouping leaves it
+            // unchanged.
             constexpr int RESTORE_MIN = 16, RESTORE_MAX = 32;
             const size_t objPer = (size_t)c->g.nObj * c->g.s * c->g.s;
             struct Restore {
@@ -2044,8 +2045,8 @@ int mfbd_solve_group(int n, mfbd_ctx* const* ctxs, const double* const* alpha0, 
         const int rc = check_ready(ctxs[k]);
         if (rc) return rc;
     }
-    // one host thread per context: each drives its own stream(s), batch and restoration
-    // threads, so the contexts' launches overlap on the device
+    // one host thread per context: each drives its own streams and batches, so the
+    // contexts' launches overlap on the device
     std::vector<int> rcs(n, MFBD_OK);
     std::vector<std::thread> th;
     for (int k = 1; k < n; ++k)

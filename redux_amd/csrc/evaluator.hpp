@@ -124,13 +124,16 @@ public:
     virtual void restore_objects_end(float* out) = 0;
     // Optional restoration outputs of n slots at their alphas (object.cpp:332-407), for the
     // images imgs[k] of slot k (their current shifts), each cropped to the centred s x s of
-    // the N x N grid, in double, rows in the order of the images over all slots: want bit
-    // 0 -> psf, the PSF (normalised inverse transform of the OTF, re-centred); bit 1 -> conv,
-    // the object of the image (objFull: [n][nObj][N][N], natural layout) convolved with its
-    // OTF; bit 2 -> win, the windowed, padded frame.  On the synchronous lane.
+    // the N x N grid and written as floats, zero for images a patch lacks: kOutPsf -> psf
+    // [n][nImg][s][s], the PSF (normalised inverse transform of the OTF, re-centred), or
+    // kOutPsfAvg -> psf [n][nObj][s][s], those PSFs averaged per object; kOutCobj -> cobj
+    // [n][nImg][s][s], the object of the image (objFull: [n][nObj][N][N], natural layout)
+    // convolved with its OTF; kOutRes -> res, that convolved object minus the windowed,
+    // padded frame.  On the synchronous lane.
+    static constexpr int kOutPsf = 1, kOutCobj = 2, kOutRes = 4, kOutPsfAvg = 8;
     virtual void restore_extra(int n, const int* slots, const double* const* alpha,
                                const std::vector<std::vector<ImgInit>>& imgs, const std::vector<float>& objFull,
-                               int want, double* psf, double* conv, double* win) = 0;
+                               int want, float* psf, float* cobj, float* res) = 0;
     virtual int nOS() const = 0;            // OTF support points (full)
     virtual int nOSStored() const = 0;      // points stored on the device (Hermitian half)
     // full support point u -> stored point h (value as stored), or -1-h (its conjugate)

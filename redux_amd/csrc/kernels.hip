@@ -1237,6 +1237,36 @@ __global__ void k_crop_re(double* __restrict__ out, const cx<T>* __restrict__ ti
     out[e] = (double)tiles[((size_t)q * N + y) * N + x].x * scale;
 }
 
+// Optional outputs as floats in the caller's layout, image q of patch r = tile[q] / nObj
+// at row r nImg + img[q]: the row's value, or with b the float value minus b (a residual)
+__global__ void k_out_rows(float* __restrict__ out, const double* __restrict__ a, const double* __restrict__ b,
+                           const int* __restrict__ tile, const int* __restrict__ img, int nI, int nObj, int nImg,
+                           int s2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nI * s2) return;
+    const int q = (int)(e / s2), k = (int)(e % s2), r = tile[q] / nObj;
+    const double v = a[e];
+    out[((size_t)r * nImg + img[q]) * s2 + k] = b ? (float)((double)(float)v - b[e]) : (float)v;
+}
+
+// The PSFs averaged per object (Object::restorePatch's PSF_AVG, object.cpp:347-364): the
+// patch's images of the object summed in image order in double, rounded to float, then
+// divided by their count; 0 for an object without images
+__global__ void k_psf_avg(float* __restrict__ out, const double* __restrict__ psf, const int* __restrict__ first,
+                          const int* __restrict__ img, const int* __restrict__ imgObj, int m, int nObj, int s2) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)m * nObj * s2) return;
+    const int k = (int)(e % s2), ro = (int)(e / s2), r = ro / nObj, o = ro % nObj;
+    double acc = 0;
+    int cnt = 0;
+    for (int q = first[r]; q < first[r + 1]; ++q)
+        if (imgObj[img[q]] == o) {
+            acc += psf[(size_t)q * s2 + k];
+            ++cnt;
+        }
+    out[e] = cnt ? (float)((double)(float)acc * (1.0 / cnt)) : 0.0f;
+}
+
 // D[slot][img][u] = tile[q][osNat[u]]
 template <typename T>
 __global__ void k_store_d(cx<T>* __restrict__ D, const cx<T>* __restrict__ tiles, const int* __restrict__ osNat,
@@ -3193,15 +3223,15 @@ public:
 
     void restore_extra(int n, const int* slots, const double* const* alpha,
                        const std::vector<std::vector<ImgInit>>& imgs, const std::vector<float>& objFull, int want,
-                       double* psf, double* conv, double* win) override {
+                       float* psf, float* cobj, float* res) override {
         use_lane(kSyncLane);
         if (n < 1 || !want) return;
-        const int nO = g_.nObj;
+        const int nO = g_.nObj, nImg = g_.nImg;
         const size_t s2 = (size_t)g_.s * g_.s;
-        if ((want & 2) && objFull.size() < (size_t)n * nO * N2)
-            throw std::runtime_error("restore_extra: objFull too small");
-        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / g_.nImg));
-        size_t outQ = 0;                                   // output row of the chunk's first image
+        const bool wPsf = want & (kOutPsf | kOutPsfAvg), wConv = want & (kOutCobj | kOutRes), wWin = want & kOutRes;
+        if (wConv && objFull.size() < (size_t)n * nO * N2) throw std::runtime_error("restore_extra: objFull too small");
+        const int perChunk = std::max(1, std::min(maxBatch_, maxTiles() / nImg));
+        const size_t psfPer = (size_t)((want & kOutPsfAvg) ? nO : nImg) * s2, imgPer = (size_t)nImg * s2;
         for (int b = 0; b < n; b += perChunk) {
             const int m = std::min(perChunk, n - b);
             std::vector<EvalReq> rq(m);
@@ -3211,75 +3241,109 @@ public:
                 rq[r].alpha = alpha[b + r];
                 nI += (int)imgs[b + r].size();
             }
-            if (!nI) continue;
-            forward(rq.data(), m, /*useGamma=*/0);        // S of request r: [r][nImg][nOS_]
-            // per image q: slot, image, shifts, object tile (request * nObj + object), S row
-            std::vector<int> ints(6 * (size_t)nI);
-            int q = 0;
-            for (int r = 0; r < m; ++r)
-                for (const ImgInit& im : imgs[b + r]) {
-                    ints[q] = im.slot;
-                    ints[nI + q] = im.img;
-                    ints[2 * nI + q] = im.shiftX;
-                    ints[3 * nI + q] = im.shiftY;
-                    ints[4 * nI + q] = r * nO + st.imgObj[im.img];
-                    ints[5 * nI + q] = r * g_.nImg + im.img;
-                    ++q;
+            // the chunk's float outputs, zero where a patch lacks an image
+            if (wPsf) xF_[0].alloc(std::max<size_t>((size_t)m * psfPer, 1));
+            if (want & kOutCobj) xF_[1].alloc(std::max<size_t>((size_t)m * imgPer, 1));
+            if (want & kOutRes) xF_[2].alloc(std::max<size_t>((size_t)m * imgPer, 1));
+            if (wPsf) HIPCHK(hipMemsetAsync(xF_[0].p, 0, (size_t)m * psfPer * sizeof(float), st_));
+            if (want & kOutCobj) HIPCHK(hipMemsetAsync(xF_[1].p, 0, (size_t)m * imgPer * sizeof(float), st_));
+            if (want & kOutRes) HIPCHK(hipMemsetAsync(xF_[2].p, 0, (size_t)m * imgPer * sizeof(float), st_));
+            if (nI) {
+                forward(rq.data(), m, /*useGamma=*/0);        // S of request r: [r][nImg][nOS_]
+                // per image q: slot, image, shifts, object tile (request * nObj + object), S row;
+                // then each request's first image row (m + 1)
+                std::vector<int> ints(6 * (size_t)nI + m + 1);
+                int q = 0;
+                for (int r = 0; r < m; ++r) {
+                    ints[6 * (size_t)nI + r] = q;
+                    for (const ImgInit& im : imgs[b + r]) {
+                        ints[q] = im.slot;
+                        ints[nI + q] = im.img;
+                        ints[2 * nI + q] = im.shiftX;
+                        ints[3 * nI + q] = im.shiftY;
+                        ints[4 * nI + q] = r * nO + st.imgObj[im.img];
+                        ints[5 * nI + q] = r * nImg + im.img;
+                        ++q;
+                    }
                 }
-            upload(xInts_, ints, st_);
-            const int* dSlot = xInts_.p;
-            const int* dImg = xInts_.p + nI;
-            const size_t nT = (size_t)nI * N2;
-            xT_.alloc(nT);
-            xOut_.alloc((size_t)nI * s2);
-            auto crop = [&](int sh, double scale) {
-                launch("k_crop_re", nI, [&] {
-                    hipLaunchKernelGGL(k_crop_re<T>, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, xOut_.p, xT_.p,
-                                       nI, N, g_.s, sh, scale);
+                ints[6 * (size_t)nI + m] = q;
+                upload(xInts_, ints, st_);
+                const int* dSlot = xInts_.p;
+                const int* dImg = xInts_.p + nI;
+                const int* dTile = xInts_.p + 4 * nI;
+                const int* dRow = xInts_.p + 5 * nI;
+                const int* dFirst = xInts_.p + 6 * nI;
+                const size_t nT = (size_t)nI * N2;
+                xT_.alloc(nT);
+                auto crop = [&](DBuf<double>& out, int sh, double scale) {
+                    out.alloc((size_t)nI * s2);
+                    launch("k_crop_re", nI, [&] {
+                        hipLaunchKernelGGL(k_crop_re<T>, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, out.p, xT_.p,
+                                           nI, N, g_.s, sh, scale);
+                    });
+                };
+                auto scatter = [&] {
+                    launch("k_otf_scatter", nI, [&] {
+                        hipLaunchKernelGGL(k_otf_scatter<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, sc_->S.p,
+                                           invOs_.p, dRow, nI, nOS_, N2);
+                    });
+                };
+                if (wPsf) {            // PSF: normalised inverse transform of the OTF, re-centred
+                    scatter();
+                    fft2(xT_.p, nI, +1);
+                    crop(xD_[0], N / 2, 1.0 / N2);
+                }
+                if (wConv) {           // convolved objects
+                    std::vector<C> h((size_t)m * nO * N2);
+                    const float* src = objFull.data() + (size_t)b * nO * N2;
+                    for (size_t k = 0; k < h.size(); ++k) h[k] = {(T)src[k], T(0)};
+                    upload(xO_, h, st_);
+                    fft2(xO_.p, m * nO, -1);
+                    scatter();
+                    launch("k_mul_obj", nI, [&] {
+                        hipLaunchKernelGGL(k_mul_obj<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, xO_.p, dTile, nI, N2,
+                                           (T)(1.0 / N2));
+                    });
+                    fft2(xT_.p, nI, +1);
+                    crop(xD_[1], 0, 1.0);
+                }
+                if (wWin) {            // windowed, padded frames (the inverse of imgFT)
+                    if (!winSet_) throw std::runtime_error("restore_extra: frames not initialised");
+                    launch("k_window", nI, [&] {
+                        hipLaunchKernelGGL(k_window<T>, dim3(nI), dim3(256), 0, st_, xT_.p, rawF_.p, win_.p, dSlot, dImg,
+                                           xInts_.p + 2 * nI, xInts_.p + 3 * nI, nImg, g_.dataY, g_.dataX, g_.startY,
+                                           g_.startX, g_.s, N);
+                    });
+                    crop(xD_[2], 0, 1.0);
+                }
+                // the float outputs in the caller's layout (SubImage::getPSF / addPSF,
+                // convolveImage, convolvedResidual): per image, or the PSFs averaged per object
+                const int ns2 = (int)s2;
+                launch("k_out_finish", nI, [&] {
+                    if (want & kOutPsf)
+                        hipLaunchKernelGGL(k_out_rows, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, xF_[0].p,
+                                           xD_[0].p, (const double*)nullptr, dTile, dImg, nI, nO, nImg, ns2);
+                    if (want & kOutPsfAvg)
+                        hipLaunchKernelGGL(k_psf_avg, dim3(nblk((size_t)m * nO * s2)), dim3(256), 0, st_, xF_[0].p,
+                                           xD_[0].p, dFirst, dImg, imgObj_.p, m, nO, ns2);
+                    if (want & kOutCobj)
+                        hipLaunchKernelGGL(k_out_rows, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, xF_[1].p,
+                                           xD_[1].p, (const double*)nullptr, dTile, dImg, nI, nO, nImg, ns2);
+                    if (want & kOutRes)
+                        hipLaunchKernelGGL(k_out_rows, dim3(nblk((size_t)nI * s2)), dim3(256), 0, st_, xF_[2].p,
+                                           xD_[1].p, xD_[2].p, dTile, dImg, nI, nO, nImg, ns2);
                 });
-            };
-            auto fetch = [&](double* dst) {
+            }
+            // through pinned staging into the caller's arrays
+            auto fetch = [&](DBuf<float>& d, float* dst, size_t per) {
+                hF_.alloc((size_t)m * per);
+                HIPCHK(hipMemcpyAsync(hF_.p, d.p, (size_t)m * per * sizeof(float), hipMemcpyDeviceToHost, st_));
                 HIPCHK(hipStreamSynchronize(st_));
-                HIPCHK(hipMemcpy(dst + outQ * s2, xOut_.p, (size_t)nI * s2 * sizeof(double), hipMemcpyDeviceToHost));
+                std::memcpy(dst + (size_t)b * per, hF_.p, (size_t)m * per * sizeof(float));
             };
-            auto scatter = [&] {
-                launch("k_otf_scatter", nI, [&] {
-                    hipLaunchKernelGGL(k_otf_scatter<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, sc_->S.p, invOs_.p,
-                                       xInts_.p + 5 * nI, nI, nOS_, N2);
-                });
-            };
-            if (want & 1) {            // PSF: normalised inverse transform of the OTF, re-centred
-                scatter();
-                fft2(xT_.p, nI, +1);
-                crop(N / 2, 1.0 / N2);
-                fetch(psf);
-            }
-            if (want & 2) {            // convolved objects
-                std::vector<C> h((size_t)m * nO * N2);
-                const float* src = objFull.data() + (size_t)b * nO * N2;
-                for (size_t k = 0; k < h.size(); ++k) h[k] = {(T)src[k], T(0)};
-                upload(xO_, h, st_);
-                fft2(xO_.p, m * nO, -1);
-                scatter();
-                launch("k_mul_obj", nI, [&] {
-                    hipLaunchKernelGGL(k_mul_obj<T>, dim3(nblk(nT)), dim3(256), 0, st_, xT_.p, xO_.p,
-                                       xInts_.p + 4 * nI, nI, N2, (T)(1.0 / N2));
-                });
-                fft2(xT_.p, nI, +1);
-                crop(0, 1.0);
-                fetch(conv);
-            }
-            if (want & 4) {            // windowed, padded frames (the inverse of imgFT)
-                if (!winSet_) throw std::runtime_error("restore_extra: frames not initialised");
-                launch("k_window", nI, [&] {
-                    hipLaunchKernelGGL(k_window<T>, dim3(nI), dim3(256), 0, st_, xT_.p, rawF_.p, win_.p, dSlot, dImg,
-                                       xInts_.p + 2 * nI, xInts_.p + 3 * nI, g_.nImg, g_.dataY, g_.dataX, g_.startY,
-                                       g_.startX, g_.s, N);
-                });
-                crop(0, 1.0);
-                fetch(win);
-            }
-            outQ += nI;
+            if (wPsf) fetch(xF_[0], psf, psfPer);
+            if (want & kOutCobj) fetch(xF_[1], cobj, imgPer);
+            if (want & kOutRes) fetch(xF_[2], res, imgPer);
         }
         collect(kSyncLane);
     }
@@ -4073,7 +4137,9 @@ private:
     // resident: basis, supports, per-patch frames, D, ftSum, gamma, channel phases
     DBuf<C> tw_, D_;
     DBuf<C> xT_, xO_;                  // optional restoration outputs: image and object tiles
-    DBuf<double> xOut_;
+    DBuf<double> xD_[3];               // PSFs, convolved objects, windowed frames [images][s][s]
+    DBuf<float> xF_[3];                // the float outputs: PSFs (or their averages), cobj, res
+    HBuf<float> hF_;
     DBuf<int> xInts_;
     DBuf<T> B_, phiCh_, pupil_, imgScale_;
     DBuf<T> lineBuf_;                 // [2][maxSlots][nImg][nPS]: each slot's line base phases, then directions

@@ -1662,19 +1662,17 @@ int mfbd_restore_outputs(mfbd_ctx* c, int n, const int* patchIdx, const double* 
     try {
         ensure_init(c);
         const int nImg = c->g.nImg, nObj = c->g.nObj;
-        const size_t s2 = (size_t)c->g.s * c->g.s, per = (size_t)nImg * c->g.K;
+        const size_t per = (size_t)nImg * c->g.K;
         const size_t N2 = (size_t)4 * c->g.P * c->g.P;
         std::vector<double> ae(per * n);
         std::vector<const double*> ap(n);
         std::vector<std::vector<ImgInit>> imgs(n);
-        std::vector<size_t> row0(n + 1, 0);           // first output row of each patch's images
         for (int k = 0; k < n; ++k) {
             const int p = patchIdx[k];
             effective_alpha(c, p, alpha + per * k, &ae[per * k]);
             ap[k] = &ae[per * k];
             for (int j = 0; j < nImg; ++j)
                 if (img_on(c, p, j)) imgs[k].push_back({p, j, c->shifts[p][2 * j], c->shifts[p][2 * j + 1]});
-            row0[k + 1] = row0[k] + imgs[k].size();
         }
         std::vector<float> objFull;
         if (mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) {
@@ -1684,52 +1682,14 @@ int mfbd_restore_outputs(mfbd_ctx* c, int n, const int* patchIdx, const double* 
             c->dev->restore_objects_begin(n, patchIdx, ap.data(), nf.data(), c2.data(), !c->prm.noFilter, true);
             c->dev->restore_objects_end(objFull.data());
         }
-        const int want = ((mask & (MFBD_SAVE_PSF | MFBD_SAVE_PSF_AVG)) ? 1 : 0) |
-                         ((mask & (MFBD_SAVE_COBJ | MFBD_SAVE_RESIDUAL)) ? 2 : 0) |
-                         ((mask & MFBD_SAVE_RESIDUAL) ? 4 : 0);
-        const size_t nQ = row0[n];
-        std::vector<double> dPsf((want & 1) ? nQ * s2 : 0), dConv((want & 2) ? nQ * s2 : 0),
-            dWin((want & 4) ? nQ * s2 : 0);
-        c->dev->restore_extra(n, patchIdx, ap.data(), imgs, objFull, want, dPsf.data(), dConv.data(), dWin.data());
-        for (int k = 0; k < n; ++k) {
-            const auto& im = imgs[k];
-            const size_t q0 = row0[k];
-            if (mask & MFBD_SAVE_PSF) {                 // one PSF per image (SubImage::getPSF)
-                float* out = psf + (size_t)k * nImg * s2;
-                std::fill(out, out + nImg * s2, 0.0f);
-                for (size_t q = 0; q < im.size(); ++q)
-                    for (size_t e = 0; e < s2; ++e) out[im[q].img * s2 + e] = (float)dPsf[(q0 + q) * s2 + e];
-            }
-            if (mask & MFBD_SAVE_PSF_AVG) {             // summed in double, rounded, then / imgCount
-                float* out = psf + (size_t)k * nObj * s2;
-                std::fill(out, out + nObj * s2, 0.0f);
-                for (int o = 0; o < nObj; ++o) {
-                    std::vector<double> acc(s2, 0.0);
-                    int cnt = 0;
-                    for (size_t q = 0; q < im.size(); ++q)
-                        if (c->st.imgObj[im[q].img] == o) {
-                            for (size_t e = 0; e < s2; ++e) acc[e] += dPsf[(q0 + q) * s2 + e];
-                            cnt++;
-                        }
-                    if (cnt)
-                        for (size_t e = 0; e < s2; ++e) out[o * s2 + e] = (float)((double)(float)acc[e] * (1.0 / cnt));
-                }
-            }
-            if (mask & MFBD_SAVE_COBJ) {
-                float* out = cobj + (size_t)k * nImg * s2;
-                std::fill(out, out + nImg * s2, 0.0f);
-                for (size_t q = 0; q < im.size(); ++q)
-                    for (size_t e = 0; e < s2; ++e) out[im[q].img * s2 + e] = (float)dConv[(q0 + q) * s2 + e];
-            }
-            if (mask & MFBD_SAVE_RESIDUAL) {            // float convolved object minus the frame
-                float* out = res + (size_t)k * nImg * s2;
-                std::fill(out, out + nImg * s2, 0.0f);
-                for (size_t q = 0; q < im.size(); ++q)
-                    for (size_t e = 0; e < s2; ++e)
-                        out[im[q].img * s2 + e] =
-                            (float)((double)(float)dConv[(q0 + q) * s2 + e] - dWin[(q0 + q) * s2 + e]);
-            }
-        }
+        // finished on the device in the caller's layout: PSF per image (SubImage::getPSF) or
+        // summed in double per object, rounded, then / imgCount; the convolved object; the
+        // float convolved object minus the frame; zero for images a patch lacks
+        const int want = ((mask & MFBD_SAVE_PSF) ? DeviceEvaluator::kOutPsf : 0) |
+                         ((mask & MFBD_SAVE_PSF_AVG) ? DeviceEvaluator::kOutPsfAvg : 0) |
+                         ((mask & MFBD_SAVE_COBJ) ? DeviceEvaluator::kOutCobj : 0) |
+                         ((mask & MFBD_SAVE_RESIDUAL) ? DeviceEvaluator::kOutRes : 0);
+        c->dev->restore_extra(n, patchIdx, ap.data(), imgs, objFull, want, psf, cobj, res);
     } catch (const std::exception& e) {
         return set_err(c, MFBD_EDEVICE, e.what());
     }

@@ -104,6 +104,16 @@ def test_after_a_ragged_solve_equal_each_patch_alone():
             np.testing.assert_array_equal(out[key][sel], want[key], err_msg=f"patch {k} {key}")
             assert np.all(out[key][off] == 0), (k, key)
         np.testing.assert_array_equal(avg, want_avg, err_msg=f"patch {k} psf avg")
+    # the three patches in one call, in another order, equal the calls one patch at a time
+    order = [2, 0, 1]
+    al3 = np.stack([got[k]["alpha"] for k in order])
+    many, many_avg = sv.restore_outputs(al3, ALL, patches=order), sv.restore_outputs(al3, 64, patches=order)["psf"]
+    for i, k in enumerate(order):
+        one = sv.restore_outputs(got[k]["alpha"], ALL | 64, patch=k)["psf"]
+        np.testing.assert_array_equal(many_avg[i], one, err_msg=f"patch {k} psf avg in one call")
+        one = sv.restore_outputs(got[k]["alpha"], ALL, patch=k)
+        for key in ("psf", "cobj", "res"):
+            np.testing.assert_array_equal(many[key][i], one[key], err_msg=f"patch {k} {key} in one call")
     sv.close()
 
 

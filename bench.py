@@ -129,7 +129,7 @@ def roofline(stats, N, nPS, nH, form):
     Q live on the stored (Hermitian) half of the OTF support, nH points; c and r are the
     complex and real element sizes of the form (16 and 8 for FP64, 8 and 4 for FP32).
     Algorithmic bytes per image:
-      k_otf_fused (FP32): read phi, write S                       r nPS + c nH
+      k_otf_fused, k_otf_wave (FP32): read phi, write S           r nPS + c nH
         (a line-search probe reads the line's base and direction:  2 r nPS + c nH)
       k_otf64_fused (FP64): read phi, write S                     r nPS + c nH
       k_pq: read S and D                                             2 c nH
@@ -147,7 +147,7 @@ def roofline(stats, N, nPS, nH, form):
     half = N * N // 2
     per_unit = {"fft_rows": 2 * c * N * N, "fft_cols": 2 * c * N * N, "k_field": r * nPS + c * N * N,
                 "k_power": 2 * c * N * N, "k_extract": 2 * c * nH, "k_phase_mfma": r * nPS, "k_pq": 2 * c * nH,
-                "k_otf_fused": r * nPS + c * nH, "k_otf64_fused": r * nPS + c * nH,
+                "k_otf_fused": r * nPS + c * nH, "k_otf_wave": r * nPS + c * nH, "k_otf64_fused": r * nPS + c * nH,
                 "k_vogel_fused": 2 * c * nH + 24 * nH + 2 * r * nPS,
                 "k_grad64_fused": 2 * c * nH + 24 * nH + 2 * r * nPS + 2 * r * N * N}
     kern = {k: v for k, v in stats.items() if not k.startswith("host") and not k.startswith("_")}
@@ -157,7 +157,7 @@ def roofline(stats, N, nPS, nH, form):
         mean_ms = s["ms"] / s["launches"]
         units_per_launch = s["units"] / s["launches"]
         # a line-search probe's transform reads the line's base and direction (2 r nPS) for phi
-        line = stats.get("_line_images", {}).get("units", 0) if name == "k_otf_fused" else 0
+        line = stats.get("_line_images", {}).get("units", 0) if name in ("k_otf_fused", "k_otf_wave") else 0
         bytes_per_launch = (per_unit[name] * s["units"] + r * nPS * line) / s["launches"]
         ach = bytes_per_launch / (mean_ms * 1e-3) / 1e9
         tr = measured_traffic(form).get(name)
@@ -167,7 +167,7 @@ def roofline(stats, N, nPS, nH, form):
                    mean_launch_ms=round(mean_ms, 4), images_per_launch=round(units_per_launch, 1),
                    bytes_per_image=round(bytes_per_launch / units_per_launch, 1), bytes_per_launch=int(bytes_per_launch),
                    line_probe_images=int(line))
-        if name in ("k_otf_fused", "k_otf64_fused"):
+        if name in ("k_otf_fused", "k_otf_wave", "k_otf64_fused"):
             flops = 2 * 5 * N * N * math.log2(N * N) * units_per_launch
             tf = flops / (mean_ms * 1e-3) / 1e12
             out.update(fft_tflops=round(tf, 2), fft_peak_tflops=VECTOR_PEAK_TFLOPS,
@@ -567,10 +567,10 @@ def stage_roofline(stats, N, P, form):
     stage bytes per image-evaluation of the path's roofline table: read phi on the P x P
     pupil grid and D_j on the N x N grid, r P^2 + c N^2 (c, r the complex and real sizes)."""
     c, r = (16, 8) if form == "fp64" else (8, 4)
-    names = [k for k in stats if k in ("k_otf64_fused", "k_otf_fused", "k_pq", "k_field_rows",
+    names = [k for k in stats if k in ("k_otf64_fused", "k_otf_fused", "k_otf_wave", "k_pq", "k_field_rows",
                                        "k_cols_power", "k_rows_inv_real", "k_cols_extract")]
     ms = sum(stats[k]["ms"] for k in names)
-    imgs = stats.get("k_otf64_fused", stats.get("k_otf_fused", {"units": 0}))["units"]
+    imgs = stats.get("k_otf64_fused", stats.get("k_otf_wave", stats.get("k_otf_fused", {"units": 0})))["units"]
     per = r * P * P + c * N * N
     ach = per * imgs / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     return dict(kernels=names, ms_per_step=round(ms, 3), image_evaluations=int(imgs), bytes_per_image=per,

@@ -105,7 +105,7 @@ def run_step(sv, restore=True):
     return res, time.perf_counter() - t0
 
 
-TRAFFIC_PROFILES = {"fp32": "k_otf_fused_cfg2_traffic.md", "fp64": "k_otf64_cfg2_traffic.md"}
+TRAFFIC_PROFILES = {"fp32": "k_otf_wave_cfg2_traffic.md", "fp64": "k_otf64_cfg2_traffic.md"}
 
 
 def measured_traffic(form):

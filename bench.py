@@ -83,14 +83,18 @@ def make_batch(n, seed, workers):
     return su, pbs
 
 
+LINE_GRAPH = True        # device line-search rounds replayed as captured HIP graphs (--line-graph)
+
+
 def make_solver(pbs, precision, batch, gradient, timing=False):
     """A context with the patches loaded (frames resident in HBM).  With `timing`,
-    every kernel launch is bracketed by HIP events on the pipeline's stream."""
+    every kernel launch is bracketed by HIP events on the pipeline's stream (and the
+    line-search rounds are launched kernel by kernel)."""
     from redux_amd.momfbd import Solver
     if timing:
         os.environ["MFBD_KERNEL_TIMING"] = "1"
     try:
-        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient)
+        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient, line_graph=LINE_GRAPH)
         sv.load(pbs)
     finally:
         os.environ.pop("MFBD_KERNEL_TIMING", None)
@@ -441,7 +445,13 @@ def main():
     ap.add_argument("--trace-summary", nargs=2, metavar=("TRACE_CSV", "BENCH_JSON"), default=None,
                     help="print the statistics steps' kernel tables from a rocprofv3 kernel trace of this "
                          "script's default run and its JSON line, and exit")
+    ap.add_argument("--line-graph", type=int, choices=[0, 1], default=1,
+                    help="1 (default): the FP32 and mixed forms' device line-search rounds replay captured HIP "
+                         "graphs; 0: their kernels launched one by one (the same kernels and results; for "
+                         "rocprofv3's kernel trace, which cannot follow graph launches)")
     args = ap.parse_args()
+    global LINE_GRAPH
+    LINE_GRAPH = bool(args.line_graph)
     if args.trace_summary:
         trace_summary(*args.trace_summary)
         return
@@ -450,7 +460,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: patches are independent and shard across GPUs with no "
-              "collective (see DESIGN.md); run with --gpus 1", file=sys.stderr)
+              "collective; run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.workload == "uneven":
         uneven(args)
@@ -674,8 +684,10 @@ def headline(args):
                                                        "MODE_START", "MODE_STEP", "GRADDIFF_STEP")}
                    | {"GETSTEP": "conjugate", "gradient": args.gradient,
                       "line_phases": "FP32 and mixed: probes from base + step * direction (metricAt); FP64: the GEMM per probe",
-                      "line_search": "FP32 and mixed: on the device (bracket + brent state machine per patch, rounds "
-                                     "of probes launched directly, read back every 4 rounds); FP64: stepped by the host"},
+                      "line_search": "FP32 and mixed: on the device (bracket + brent state machine per patch, each "
+                                     "round of probes " + ("replayed as a captured HIP graph" if LINE_GRAPH else
+                                                           "launched kernel by kernel") +
+                                     ", read back every 4 rounds); FP64: stepped by the host"},
                    "converged": "solve exits normally and restored objects within 1e-4 (relative RMS) of the "
                                 f"FP64 oracle's solve, on a sample of {len(idx)} patches; value = patches/s x "
                                 "sampled fraction",

@@ -88,8 +88,9 @@ typedef struct {        /* GlobalCfg solver knobs, defaults in config.cpp:629-63
                            host's line search.  0: the host steps the line search.  Forms
                            without a fused transform (N = 256) keep the host's.            */
     int lineGraph;      /* with lineRounds: 1 replays one captured HIP graph per round of
-                           probes, 0 launches the round's kernels one by one (the same
-                           kernels and results)                                             */
+                           probes (launched one by one anyway while kernel events time each
+                           launch, MFBD_KERNEL_TIMING), 0 launches the round's kernels one by
+                           one (the same kernels and results)                               */
 } mfbd_params;
 
 #define MFBD_GRAD_SCHEDULE_REFERENCE 0

@@ -2697,12 +2697,22 @@ __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const
 
 // ------------------------------------------------------------- evaluator ----
 
+// Stream capture (the device line search's graphs) and the calls HIP refuses while any
+// stream captures -- the device-wide synchronisation before a buffer is replaced, frees --
+// exclude each other across the contexts of one process (mfbd_solve_group runs one host
+// thread per context)
+std::mutex& capture_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
     size_t n = 0;
     void alloc(size_t count) {
         if (count <= n) return;
+        std::lock_guard<std::mutex> lk(capture_mutex());
         if (p) {
             HIPCHK(hipDeviceSynchronize());   // in-flight kernels may still read the old buffer
             HIPCHK(hipFree(p));
@@ -2711,7 +2721,10 @@ struct DBuf {
         n = count;
     }
     void free_() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::lock_guard<std::mutex> lk(capture_mutex());
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -2742,6 +2755,7 @@ struct HBuf {
     size_t n = 0;
     void alloc(size_t count) {
         if (count <= n) return;
+        std::lock_guard<std::mutex> lk(capture_mutex());
         if (p) {
             HIPCHK(hipDeviceSynchronize());
             HIPCHK(hipHostFree(p));
@@ -2750,7 +2764,10 @@ struct HBuf {
         n = count;
     }
     void free_() {
-        if (p) (void)hipHostFree(p);
+        if (p) {
+            std::lock_guard<std::mutex> lk(capture_mutex());
+            (void)hipHostFree(p);
+        }
         p = nullptr;
         n = 0;
     }
@@ -3278,7 +3295,7 @@ public:
             const int nL = (int)slots.size(), nImg = g_.nImg, nQ = nL * nImg;
             l.nL = nL;
             l.slots = slots;
-            l.useGraph = graph;
+            l.useGraph = graph && !timing_;     // kernel events time each launch: direct launches
             l.probes.assign(nL, 0);
             if (!nL) return true;
             if (!l.done) HIPCHK(hipEventCreateWithFlags(&l.done, hipEventDisableTiming));
@@ -3309,7 +3326,8 @@ public:
             std::copy(idle.begin(), idle.end(), l.hSt.p);
             HIPCHK(hipMemcpyAsync(l.st.p, l.hSt.p, nL * sizeof(LsState), hipMemcpyHostToDevice, st_));
             HIPCHK(hipStreamSynchronize(st_));
-            if (graph) {
+            if (l.useGraph) {
+                std::lock_guard<std::mutex> lk(capture_mutex());
                 HIPCHK(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal));
                 ls_round(l, false);
                 HIPCHK(hipStreamEndCapture(st_, &l.graph));

@@ -42,7 +42,7 @@ class Solver:
     """One context: geometry + structure + basis + constraints; patches are loaded in bulk."""
 
     def __init__(self, pb: dict, precision: str = "fp64", max_batch: int = 1, device: int = 0, settings=None,
-                 gradient: str = "reference", line_phases=None, line_rounds=None, line_graph=False):
+                 gradient: str = "reference", line_phases=None, line_rounds=None, line_graph=True):
         """line_phases: line-search probes from base + step * direction phases instead of the
         mode GEMM per probe (mfbd_params.linePhases).  By default on for the FP32 and mixed
         forms (their transform reads the line in place of the GEMM's output) and off for
@@ -55,9 +55,11 @@ class Solver:
         By default 4 where line phases are on (FP32 and mixed: 256 config-2 patches solve
         ~18 % faster, the host no longer between probes) and 0 for FP64, whose solve the
         device's transforms bound (lines that end early would wait for the read-back).
-        line_graph: each round of probes as one captured HIP graph (mfbd_params.lineGraph),
-        else (the default) its three kernels launched one by one: the two measured the same,
-        and rocprofv3's kernel trace cannot follow the graph launches."""
+        line_graph: each round of probes as one captured HIP graph (mfbd_params.lineGraph, the
+        default), else its three kernels launched one by one: the same kernels and results, and
+        the two measured the same (FP32 438 / 433-439 patches/s on 256 config-2 patches).  A
+        context with kernel events on (MFBD_KERNEL_TIMING) launches them one by one, so that
+        each kernel's time is its own; rocprofv3's kernel trace cannot follow graph launches."""
         self.pb = pb
         st = default_params(**pb.get("settings", {}))
         if settings:

@@ -1678,7 +1678,7 @@ __global__ void __launch_bounds__(NT) k_otf_fused(cx<float>* __restrict__ S, con
 namespace wl {
 constexpr int N = 128, LD = 130;
 constexpr int SIDE = N * LD;              // the split of the combined line of columns 0 and 64
-constexpr int TW = SIDE + N;              // twiddles
+constexpr int TW = SIDE + N;              // twiddles of pass 2, W^(r k1) at 16 r + k1 (r < 8, k1 < 16)
 constexpr int CUR = TW + N;               // the next image's phases on the P x P pupil grid (floats)
 constexpr int SIZE = CUR + N * N / 8;     // complex floats of LDS
 // slot of the pass-1 output k1 of lane l of a line.  LDS serves a wave's 64-bit accesses
@@ -1705,9 +1705,11 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 // pass 2 of lane m: o[p][k2] = X(k1 + 16 k2), k1 = m + 8 p, from the pass-1 outputs
-// Y_r(k1) at at(r, k1), r = 0..7 (spass with R = 8, NS = 16)
+// Y_r(k1) at at(r, k1), r = 0..7 (spass with R = 8, NS = 16); tw2 holds W^(r k1) at
+// 16 r + k1, so that lanes of consecutive k1 read consecutive twiddles (the N-entry table
+// W^e read at e = r k1 put up to 8 of them on one bank)
 template <int DIR, typename AT>
-__device__ __forceinline__ void pass2(cx<float> (&o)[2][8], int m, AT at, const cx<float>* tw) {
+__device__ __forceinline__ void pass2(cx<float> (&o)[2][8], int m, AT at, const cx<float>* tw2) {
 #pragma unroll
     for (int p = 0; p < 2; ++p) {
         const int k1 = m + 8 * p;
@@ -1715,7 +1717,7 @@ __device__ __forceinline__ void pass2(cx<float> (&o)[2][8], int m, AT at, const 
         for (int r = 0; r < 8; ++r) {
             cx<float> x = at(r, p, k1);
             if (r > 0) {
-                cx<float> w = tw[r * k1];
+                cx<float> w = tw2[16 * r + k1];
                 if (DIR > 0) w.y = -w.y;
                 x = cmul(x, w);
             }
@@ -1765,7 +1767,7 @@ __global__ void __launch_bounds__(NT) k_otf_wave(cx<float>* __restrict__ S, cons
     extern __shared__ cx<float> lds_dyn[];
     cx<float>* t = lds_dyn;
     cx<float>* tw = lds_dyn + wl::TW;
-    for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[e];
+    for (int e = threadIdx.x; e < N; e += NT) tw[e] = twg[(e >> 4) * (e & 15)];     // W^(r k1), e = 16 r + k1
     const bool rowWave = threadIdx.x < NT / 2;
     // images with a transform: probed (qLine != -2) and present (scale != 0)
     auto real = [&](int qq) { return !(qLine && qLine[qq] == -2) && imgScale[qImg[qq]] != 0.f; };

@@ -744,9 +744,10 @@ __global__ void __launch_bounds__(256) k_pq(double* __restrict__ Pout, double* _
             for (int b = 0; b < n; b += 30) {
                 const int e = min(b + 30, n);
                 double tr = 0, ti = 0, tq = 0;
-                // eight images' S and D loaded together, then summed in image order; an
-                // image the patch lacks (scale 0) is neither read nor summed
-                constexpr int G = 8;
+                // four images' S and D loaded together, then summed in image order; an
+                // image the patch lacks (scale 0) is neither read nor summed.  Four keeps the
+                // kernel at 58 registers, which fit beside a running k_otf_wave workgroup
+                constexpr int G = 4;
                 for (int j0 = f0 + b; j0 < f0 + e; j0 += G) {
                     cx<T> sv[G], dv[G];
                     bool on[G];
@@ -1753,8 +1754,12 @@ __device__ __forceinline__ void row_line(cx<float>* row, int rb, cx<float> (&v)[
 }
 }  // namespace wl
 
+// At most 112 registers per lane (amdgpu_num_vgpr counts half the unified VGPR/AGPR file on
+// gfx950, so 56 here): the 16 waves of the persistent workgroup then leave 64 registers per
+// lane of each SIMD, where a workgroup of k_pq (58) from another evaluation lane runs beside
+// the transforms; the unconstrained allocation (113) leaves 32 and k_pq waits for the grid
 template <int NT>
-__global__ void __launch_bounds__(NT) k_otf_wave(cx<float>* __restrict__ S, const float* __restrict__ phi,
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(56))) k_otf_wave(cx<float>* __restrict__ S, const float* __restrict__ phi,
                                                  const float* __restrict__ pupilG,
                                                  const int* __restrict__ qImg, const float* __restrict__ imgScale,
                                                  const int* __restrict__ hsrc, const int* __restrict__ psd,

@@ -2255,9 +2255,10 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
 //   rows y and y + N/2 packed as one complex row, forward row FFTs, the split into the
 //   half spectra of the two real rows folded into the first pass of the column FFTs
 //   (line layout `at` of k_otf64_fused: column 0 combined with column N/2), second pass,
-//   and the stored points through hread (unnormalised, as FourierTransform's forward FFT)
-template <int N, int NT>
-__global__ void __launch_bounds__(NT) k_init64_fused(cx<double>* __restrict__ D, const float* __restrict__ raw,
+//   and the stored points through hread (unnormalised, as FourierTransform's forward FFT),
+//   rounded once to TD (float for the FP32 forms, whose D is otherwise the same)
+template <int N, int NT, typename TD>
+__global__ void __launch_bounds__(NT) k_init64_fused(cx<TD>* __restrict__ D, const float* __restrict__ raw,
                                                      const double* __restrict__ window, const int* __restrict__ slot,
                                                      const int* __restrict__ img, const int* __restrict__ sx,
                                                      const int* __restrict__ sy, const int* __restrict__ hread,
@@ -2341,7 +2342,7 @@ __global__ void __launch_bounds__(NT) k_init64_fused(cx<double>* __restrict__ D,
         __syncthreads();
     }
     spass_at<double, N, RL, RF, -1, P, NT>(t, at, tw);
-    cx<double>* Dq = D + ((size_t)slot[q] * nImg + img[q]) * nOS;
+    cx<TD>* Dq = D + ((size_t)slot[q] * nImg + img[q]) * nOS;
     for (int u = threadIdx.x; u < nOS; u += NT) {
         const int code = hread[u];
         const int idx = code >= 0 ? code : -1 - code, y = idx / HC, k = idx % HC;
@@ -2353,7 +2354,7 @@ __global__ void __launch_bounds__(NT) k_init64_fused(cx<double>* __restrict__ D,
             v = k == 0 ? cx<double>{0.5 * (x.x + xm.x), 0.5 * (x.y - xm.y)}
                        : cx<double>{0.5 * (x.y + xm.y), -0.5 * (x.x - xm.x)};
         }
-        Dq[u] = {v.x, code >= 0 ? v.y : -v.y};
+        Dq[u] = {(TD)v.x, (TD)(code >= 0 ? v.y : -v.y)};
     }
 }
 
@@ -2804,6 +2805,8 @@ public:
     static constexpr size_t F64LDS = ((size_t)(N / 2) * (N + 1) + N) * sizeof(cx<double>);
     static constexpr size_t G64LDS = F64LDS + 2 * N * sizeof(cx<double>);     // plus two G'_H columns
     static constexpr size_t I64LDS = F64LDS + F64NT * sizeof(double);           // plus the block-sum scratch
+    // frame transforms by k_init64_fused in every precision (N <= 128)
+    static constexpr bool INIT64 = N <= 128;
 
     Pipeline(const Geometry& g, int maxSlots, int maxBatch, int device, bool bf16Phase = false)
         : g_(g), maxSlots_(maxSlots), maxBatch_(maxBatch), bf16Phase_(bf16Phase && std::is_same_v<T, float>) {
@@ -2836,6 +2839,14 @@ public:
             tw[k] = {(T)cosl(a), (T)sinl(a)};
         }
         upload(tw_, tw, st_);
+        if constexpr (INIT64 && !std::is_same_v<T, double>) {     // k_init64_fused's FP64 twiddles
+            std::vector<cx<double>> tw64(N);
+            for (int k = 0; k < N; ++k) {
+                long double a = -2.0L * 3.14159265358979323846264338327950288L * k / N;
+                tw64[k] = {(double)cosl(a), (double)sinl(a)};
+            }
+            upload(tw64_, tw64, st_);
+        }
         autotune_fft();
     }
 
@@ -3043,13 +3054,14 @@ public:
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)F64LDS));
             HIPCHK(hipFuncSetAttribute((const void*)k_grad64_fused<N, F64NT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)G64LDS));
-            HIPCHK(hipFuncSetAttribute((const void*)k_init64_fused<N, F64NT>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)I64LDS));
             int nCU = 0, dev = 0;
             HIPCHK(hipGetDevice(&dev));
             HIPCHK(hipDeviceGetAttribute(&nCU, hipDeviceAttributeMultiprocessorCount, dev));
             otfGrid_ = std::max(nCU, 1);         // persistent workgroups: one per CU (the LDS fits one)
         }
+        if constexpr (INIT64)
+            HIPCHK(hipFuncSetAttribute((const void*)k_init64_fused<N, F64NT, T>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)I64LDS));
         HIPCHK(hipFuncSetAttribute((const void*)k_noise, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)((128 * 128 + 6 * 128) * sizeof(double))));
         area_ = 0;
@@ -3218,11 +3230,14 @@ public:
         ints.insert(ints.end(), pairs.begin(), pairs.end());
         stage_async(sc_->initInts, sc_->hInit, ints);
         const int* base = sc_->initInts.p;
-        if constexpr (FUSED64) {
+        if constexpr (INIT64) {
+            const cx<double>* tw64 = nullptr;
+            if constexpr (std::is_same_v<T, double>) tw64 = tw_.p;
+            else tw64 = tw64_.p;
             launch("k_init64_fused", nQ, [&] {
-                hipLaunchKernelGGL((k_init64_fused<N, F64NT>), dim3(nQ), dim3(F64NT), I64LDS, st_, D_.p, rawF_.p, win_.p,
-                                   base, base + nQ, base + 2 * nQ, base + 3 * nQ, hread_.p, tw_.p, g_.nImg, g_.dataY,
-                                   g_.dataX, g_.startY, g_.startX, g_.s, nOS_);
+                hipLaunchKernelGGL((k_init64_fused<N, F64NT, T>), dim3(nQ), dim3(F64NT), I64LDS, st_, D_.p, rawF_.p,
+                                   win_.p, base, base + nQ, base + 2 * nQ, base + 3 * nQ, hread_.p, tw64, g_.nImg,
+                                   g_.dataY, g_.dataX, g_.startY, g_.startX, g_.s, nOS_);
             });
         } else {
             sc_->tiles.alloc((size_t)std::min(chunk, nQ) * N2);
@@ -4553,6 +4568,7 @@ private:
     std::vector<double> otfNorm_;
     // resident: basis, supports, per-patch frames, D, ftSum, gamma, channel phases
     DBuf<C> tw_, D_;
+    DBuf<cx<double>> tw64_;                  // FP64 twiddles of k_init64_fused in the FP32 forms
     DBuf<C> xT_, xO_;                  // optional restoration outputs: image and object tiles
     DBuf<double> xD_[3];               // PSFs, convolved objects, windowed frames [images][s][s]
     DBuf<float> xF_[3];                // the float outputs: PSFs (or their averages), cobj, res

@@ -2615,14 +2615,18 @@ __global__ void __launch_bounds__(NT) k_grad64_fused(double* __restrict__ v, dou
 // Vogel / adjoint weights (subimage.cpp:348-390) for one image:
 //   gl = IFFT(G')/N^2 with G' = (PQ D - PS S)/QS de-centred, hj = IFFT(PF')/N^2,
 //   t = FFT(hj * Re gl), v_i = Im(conj(PF_i) t(pde_i)) [* pupil_i]
+// Re gl is the transform of the Hermitian part G'_H = (G'(u) + G'(-u)*)/2, a real image:
+// G'_H on the columns [0, N/2] (the table gwt, as for k_grad64_fused), inverse transforms
+// of the N/2 lines columns 1 .. N/2-1 and G'_H(., 0) + i G'_H(., N/2) (real results), then
+// rows n and n + N/2 packed as one complex row (its columns above N/2 from C(n, N-k) =
+// C(n, k)*) and N/2 inverse row transforms: Re row n = N^2 gl(n, .), Im = N^2 gl(n + N/2, .)
 template <int N, int NT>
 __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const cx<float>* __restrict__ S,
                                                     const cx<float>* __restrict__ D, const double* __restrict__ P,
                                                     const double* __restrict__ Q, const float* __restrict__ phi,
                                                     const int* __restrict__ pdeY, const int* __restrict__ pdeX,
-                                                    const float* __restrict__ pupil, const int* __restrict__ osNatY,
-                                                    const int* __restrict__ osNatX, const int* __restrict__ osMirY,
-                                                    const int* __restrict__ osMirX, QMap qm, const int* __restrict__ qReq,
+                                                    const float* __restrict__ pupil, const int* __restrict__ gwt,
+                                                    QMap qm, const int* __restrict__ qReq,
                                                     const int* __restrict__ imgObj, const int* __restrict__ gIdx,
                                                     const cx<float>* __restrict__ tw,
                                                     int nPS, int nOS, int nImg, int nObj, int extraPupil) {
@@ -2643,19 +2647,43 @@ __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const
         const cx<float> d = Dq[u], s = Sq[u];
         const double gr = (pqr * d.x - pqi * d.y) - ps * s.x;
         const double gi = (pqr * d.y + pqi * d.x) - ps * s.y;
-        // de-centred position = natural index of the centred support point; G(-u) = G(u)*
+        // de-centred position = natural index of the centred support point
         const float g_r = (float)(gr / qs), g_i = (float)(gi / qs);
-        t[Tile<N>::at(osNatY[u], osNatX[u])] = {g_r, g_i};
-        if (osMirY[u] >= 0) t[Tile<N>::at(osMirY[u], osMirX[u])] = {g_r, -g_i};
+#pragma unroll
+        for (int w = 0; w < 2; ++w) {
+            const int c = gwt[w * nOS + u];
+            if (c < 0) continue;
+            const int f = c & 7;
+            const float sc = (f & 2) ? 0.5f : 1.0f;
+            t[c >> 3] = {sc * g_r, (f & 4) ? 0.f : ((f & 1) ? -sc * g_i : sc * g_i)};
+        }
     }
     __syncthreads();
-    tile_fft2<float, N, +1, NT>(t, tw, 0, N);
+    for (int y = threadIdx.x; y < N; y += NT) {
+        const cx<float> a = t[Tile<N>::at(y, 0)], b = t[Tile<N>::at(y, N / 2)];
+        t[Tile<N>::at(y, 0)] = {a.x - b.y, a.y + b.x};
+    }
+    __syncthreads();
+    fft_lines<float, N, +1, N / 2, NT, 1>(t, 1, Tile<N>::LD, tw);
+    for (int e = threadIdx.x; e < (N / 2) * (N / 2); e += NT) {
+        const int n = e / (N / 2), k = e % (N / 2);
+        const cx<float> a = t[Tile<N>::at(n, k)], b = t[Tile<N>::at(n + N / 2, k)];
+        if (k == 0) {            // a = C(n, 0) + i C(n, N/2), b the same of row n + N/2
+            t[Tile<N>::at(n, 0)] = {a.x, b.x};
+            t[Tile<N>::at(n, N / 2)] = {a.y, b.y};
+        } else {
+            t[Tile<N>::at(n, k)] = {a.x - b.y, a.y + b.x};
+            t[Tile<N>::at(n, N - k)] = {a.x + b.y, b.x - a.y};
+        }
+    }
+    __syncthreads();
+    fft_lines<float, N, +1, N / 2, NT, 1>(t, Tile<N>::LD, 1, tw);
     float greal[PER];
     constexpr float inv = 1.0f / (N * N);
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
-        const int e = threadIdx.x + k * NT;
-        greal[k] = (e < N * N) ? t[Tile<N>::at(e / N, e % N)].x * inv : 0.f;
+        const int e = threadIdx.x + k * NT, y = e / N, x = e % N;
+        greal[k] = (e < N * N) ? (y < N / 2 ? t[Tile<N>::at(y, x)].x : t[Tile<N>::at(y - N / 2, x)].y) * inv : 0.f;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) t[e] = {0.f, 0.f};
@@ -3019,15 +3047,28 @@ public:
         upload(pupil_, pup, st_);
         upload(osNat_, osNat, st_);
         upload(invOs_, invOs, st_);
-        {   // [Y, X of the representative, Y, X of its mirror or -1] in the natural layout
-            std::vector<int> yx(4 * (size_t)nOS_);
+        {   // where k_vogel_fused writes G'_H of stored point u in its N x (N+1) tile (natural
+            // layout, columns [0, N/2] only): two targets (-1 for none), target * 8 + flags
+            // (1: conjugate, 2: half, 4: real part), as the table of k_grad64_fused
+            std::vector<int> gwt(2 * (size_t)nOS_, -1);
+            auto target = [&](int nat) {
+                const int y = nat / N, x = nat % N;
+                return x > N / 2 ? -1 : y * (N + 1) + x;
+            };
+            auto enc = [](int t, int f) { return t < 0 ? -1 : t * 8 + f; };
             for (int u = 0; u < nOS_; ++u) {
-                yx[u] = osNat[u] / N;
-                yx[nOS_ + u] = osNat[u] % N;
-                yx[2 * nOS_ + u] = mirNat[u] >= 0 ? mirNat[u] / N : -1;
-                yx[3 * nOS_ + u] = mirNat[u] >= 0 ? mirNat[u] % N : -1;
+                const int pn = osNat[u], mn = ((N - pn / N) % N) * N + (N - pn % N) % N;
+                if (mirNat[u] >= 0) {
+                    gwt[u] = enc(target(pn), 0);
+                    gwt[nOS_ + u] = enc(target(mn), 1);
+                } else if (mn == pn) {
+                    gwt[u] = enc(target(pn), 4);
+                } else {
+                    gwt[u] = enc(target(pn), 2);
+                    gwt[nOS_ + u] = enc(target(mn), 3);
+                }
             }
-            upload(osyx_, yx, st_);
+            upload(gwt_, gwt, st_);
         }
         if constexpr (FUSED) {
             HIPCHK(hipFuncSetAttribute((const void*)k_otf_fused<N, FNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -4402,9 +4443,9 @@ private:
             if constexpr (FUSED) {
                 launch("k_vogel_fused", nG, [&] {
                     hipLaunchKernelGGL((k_vogel_fused<N, FNT>), dim3(nG), dim3(FNT), FLDS, st_, sc_->v.p, sc_->S.p, D_.p, sc_->Pp.p,
-                                       sc_->Qp.p, sc_->phi.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, osyx_.p,
-                                       osyx_.p + nOS_, osyx_.p + 2 * nOS_, osyx_.p + 3 * nOS_, sc_->qm, sc_->qReq, imgObj_.p,
-                                       gIdxD, tw_.p, nPS_, nOS_, nImg, g_.nObj, extraPupil);
+                                       sc_->Qp.p, sc_->phi.p, pyx_.p + 2 * nPS_, pyx_.p + 3 * nPS_, pupil_.p, gwt_.p,
+                                       sc_->qm, sc_->qReq, imgObj_.p, gIdxD, tw_.p, nPS_, nOS_, nImg, g_.nObj,
+                                       extraPupil);
                 });
             } else if constexpr (FUSED64) {
                 sc_->greal.alloc((size_t)nG * N2);
@@ -4594,7 +4635,7 @@ private:
     std::vector<double> hGamma_;             // host copy of gamma_ (slot, object)
     DBuf<int> gw_;
     DBuf<unsigned char> rowNZ_;             // natural rows of the OTF grid the pupil field touches
-    DBuf<int> invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_, pyx_, osyx_, hread_,
+    DBuf<int> invPio_, invPde_, osNat_, invOs_, pde_, chObj_, chFirst_, chCount_, imgObj_, imgCh_, pyx_, gwt_, hread_,
         invPs_, hsrc_, psd_;
     DBuf<float> rawF_, pupilG_;
     bool winSet_ = false;

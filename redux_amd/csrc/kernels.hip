@@ -2680,10 +2680,11 @@ __global__ void __launch_bounds__(NT) k_vogel_fused(float* __restrict__ v, const
     fft_lines<float, N, +1, N / 2, NT, 1>(t, Tile<N>::LD, 1, tw);
     float greal[PER];
     constexpr float inv = 1.0f / (N * N);
+    static_assert(PER * NT == N * N, "point e = threadIdx.x + k NT lies in the first half of the rows for k < PER/2");
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
         const int e = threadIdx.x + k * NT, y = e / N, x = e % N;
-        greal[k] = (e < N * N) ? (y < N / 2 ? t[Tile<N>::at(y, x)].x : t[Tile<N>::at(y - N / 2, x)].y) * inv : 0.f;
+        greal[k] = (k < PER / 2 ? t[Tile<N>::at(y, x)].x : t[Tile<N>::at(y - N / 2, x)].y) * inv;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < Tile<N>::SIZE; e += NT) t[e] = {0.f, 0.f};

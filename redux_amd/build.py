@@ -20,7 +20,11 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HIP_FLAGS = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=gfx950", "-Wall", "-Wno-unused-function",
              # packed FP32 instructions are kept out of every kernel
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             # matrix-core accumulators in ordinary VGPRs (gfx950's register file is unified):
+             # without it the mode GEMM copies its accumulators between VGPRs and AGPRs on
+             # every k-chunk (FP64 k_phase_mfma 113 -> 105 ms per 256-patch step, same results)
+             "-mllvm", "-amdgpu-mfma-vgpr-form"]
 CXX_FLAGS = ["-O3", "-std=c++20", "-fPIC", "-Wall", "-Wno-unused-function", f"-I{ROCM}/include",
              "-D__HIP_PLATFORM_AMD__",
              # complex products as (ac - bd, ad + bc) without the NaN/Inf recovery call

@@ -127,6 +127,24 @@ def measured_traffic(form):
     return {}
 
 
+SQ_PROFILE = "otf_cfg2_sq_counters.md"
+CLOCK_GHZ = 2.4          # MI355X peak engine clock
+SIMDS = 1024             # 256 CUs x 4 SIMDs
+
+
+def measured_valu(name):
+    """VALU wave-instructions per image of a transform kernel, counted with rocprofv3
+    (SQ_INSTS_VALU) and committed under profiles/."""
+    try:
+        with open(os.path.join(ROOT, "profiles", SQ_PROFILE)) as f:
+            for line in f:
+                if line.startswith("sq-json:"):
+                    return json.loads(line.split(":", 1)[1]).get(name)
+    except OSError:
+        pass
+    return None
+
+
 def roofline(stats, N, nPS, nH, form):
     """Dominant kernel by total time; achieved = algorithmic bytes per launch / mean
     launch time, over every launch of one step on a context with kernel events.  S, D, P and
@@ -176,6 +194,13 @@ def roofline(stats, N, nPS, nH, form):
             tf = flops / (mean_ms * 1e-3) / 1e12
             out.update(fft_tflops=round(tf, 2), fft_peak_tflops=VECTOR_PEAK_TFLOPS,
                        fft_frac=round(tf / VECTOR_PEAK_TFLOPS, 4))
+            sq = measured_valu(name)
+            if sq:
+                # share of the SIMDs' VALU issue cycles: a wave64 VALU instruction takes 4
+                # cycles of a 16-lane SIMD; instructions per image counted by rocprofv3
+                cyc = sq["valu_per_image"] * units_per_launch * 4
+                out.update(valu_issue_frac=round(cyc / (mean_ms * 1e-3 * CLOCK_GHZ * 1e9 * SIMDS), 3),
+                           valu_insts_per_image=sq["valu_per_image"])
     return out
 
 

@@ -1771,7 +1771,7 @@ __global__ void __launch_bounds__(NT) __attribute__((amdgpu_num_vgpr(56))) k_otf
                                                  const int* __restrict__ qLine = nullptr,
                                                  const double* __restrict__ qLineS = nullptr, size_t dirOff = 0) {
     static_assert(NT == 1024, "16 waves of 8 lines");
-    constexpr int N = wl::N, LD = wl::LD, P = N / 2, HC = N / 2 + 1;
+    constexpr int N = wl::N, LD = wl::LD, P = N / 2;
     extern __shared__ cx<float> lds_dyn[];
     cx<float>* t = lds_dyn;
     cx<float>* tw = lds_dyn + wl::TW;
@@ -2060,12 +2060,18 @@ __device__ __forceinline__ void spass_at(cx<T>* buf, F addr, const cx<T>* __rest
 //   6. the stored support points: hread codes name (y, k) of the N x (N/2+1) half
 //      array; columns 0 and N/2 separate as C0 = (X(y) + X*(-y))/2,
 //      C_{N/2} = (X(y) - X*(-y))/2i
-template <int N, int NT>
+//   LINE: line-search probes take their phases as base + s * direction from the line
+//   buffers (qLine[q] >= 0: the line's row, qLineS[q] = s; the directions dirOff further),
+//   the other images from phi, as k_line_phase would have written them
+template <int N, int NT, bool LINE>
 __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, const double* __restrict__ phi,
                                                     const int* __restrict__ invPs, const double* __restrict__ pupil,
                                                     const int* __restrict__ qImg, const double* __restrict__ imgScale,
                                                     const int* __restrict__ hread, const cx<double>* __restrict__ twg,
-                                                    int nPS, int nOS, int nQ, const int* __restrict__ qSkip = nullptr) {
+                                                    int nPS, int nOS, int nQ, const int* __restrict__ qSkip = nullptr,
+                                                    const double* __restrict__ line = nullptr,
+                                                    const int* __restrict__ qLine = nullptr,
+                                                    const double* __restrict__ qLineS = nullptr, size_t dirOff = 0) {
     constexpr int P = N / 2, LD = N + 1, HC = N / 2 + 1;
     constexpr int RF = N == 64 ? 8 : 16, RL = 8;             // radices of the two passes (N = RF RL)
     static_assert(RF * RL == N, "two-pass transforms only (N = 64, 128)");
@@ -2081,21 +2087,36 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
     // row butterfly (line, j) reads pupil row `line`, columns j + r N/RF, r < RF/2
     int kk[IR][RF / 2];
     double amp[IR][RF / 2], ph[IR][RF / 2];
+    double pd[LINE ? IR : 1][RF / 2];                        // LINE: the probe's direction
+    double ls = 0.0;                                         // LINE: its step (0: phases from phi)
 #pragma unroll
     for (int i = 0; i < IR; ++i) {
-        const int b = threadIdx.x + i * NT, line = b % P, j = b / P;
+        const int b = threadIdx.x + i * NT, row = b % P, j = b / P;
 #pragma unroll
         for (int r = 0; r < RF / 2; ++r) {
-            kk[i][r] = invPs[line * P + j + r * (N / RF)];
+            kk[i][r] = invPs[row * P + j + r * (N / RF)];
             amp[i][r] = kk[i][r] >= 0 ? pupil[kk[i][r]] : 0.0;
         }
     }
     auto fetch = [&](int qq) {
+        if constexpr (LINE) {
+            const int lr = qq < nQ ? qLine[qq] : -1;
+            const double* src = lr >= 0 ? line + (size_t)lr * nPS : phi + (size_t)qq * nPS;
+            ls = lr >= 0 ? qLineS[qq] : 0.0;
 #pragma unroll
-        for (int i = 0; i < IR; ++i)
+            for (int i = 0; i < IR; ++i)
 #pragma unroll
-            for (int r = 0; r < RF / 2; ++r)
-                ph[i][r] = (qq < nQ && kk[i][r] >= 0) ? phi[(size_t)qq * nPS + kk[i][r]] : 0.0;
+                for (int r = 0; r < RF / 2; ++r) {
+                    ph[i][r] = (qq < nQ && kk[i][r] >= 0) ? src[kk[i][r]] : 0.0;
+                    pd[i][r] = (lr >= 0 && kk[i][r] >= 0) ? src[dirOff + kk[i][r]] : 0.0;
+                }
+        } else {
+#pragma unroll
+            for (int i = 0; i < IR; ++i)
+#pragma unroll
+                for (int r = 0; r < RF / 2; ++r)
+                    ph[i][r] = (qq < nQ && kk[i][r] >= 0) ? phi[(size_t)qq * nPS + kk[i][r]] : 0.0;
+        }
     };
     fetch(blockIdx.x);
     auto at = [](int k, int y) { return y < P ? y * LD + k : (y - P) * LD + (k == 0 ? P : N - k); };
@@ -2120,7 +2141,10 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
 #pragma unroll
                 for (int r = 0; r < RF / 2; ++r) {
                     double sn, cs;
-                    sincos(ph[i][r], &sn, &cs);
+                    if constexpr (LINE)          // the expression of k_line_phase: base + s * dir
+                        sincos(ls != 0.0 ? ph[i][r] + ls * pd[i][r] : ph[i][r], &sn, &cs);
+                    else
+                        sincos(ph[i][r], &sn, &cs);
                     const double m = amp[i][r] * sc;
                     v[i][r + RF / 4] = {m * cs, m * sn};
                 }
@@ -3095,7 +3119,9 @@ public:
                                        (int)FLDS));
         }
         if constexpr (FUSED64) {
-            HIPCHK(hipFuncSetAttribute((const void*)k_otf64_fused<N, F64NT>,
+            HIPCHK(hipFuncSetAttribute((const void*)k_otf64_fused<N, F64NT, false>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)F64LDS));
+            HIPCHK(hipFuncSetAttribute((const void*)k_otf64_fused<N, F64NT, true>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)F64LDS));
             HIPCHK(hipFuncSetAttribute((const void*)k_grad64_fused<N, F64NT>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)G64LDS));
@@ -3414,8 +3440,6 @@ public:
         const int* tScale = l.tab.p + nL;
         int* tLine = l.tab.p + nL + nQ;
         int* tActive = l.tab.p + nL + 2 * nQ;
-        const int* tLq = l.tab.p + 2 * nL + 2 * nQ;
-        const int* tLrow = tLq + nQ;
         const size_t dirOff = (size_t)maxSlots_ * nImg * nPS_;
         auto go = [&](const char* name, auto&& f) {
             if (named)
@@ -3438,14 +3462,10 @@ public:
                                    lineBuf_.p, tLine, l.qS.p, dirOff);
             });
         } else if constexpr (FUSED64) {
-            go("k_line_phase", [&] {
-                hipLaunchKernelGGL(k_line_phase<T>, dim3(nblk((size_t)nQ * nPS_)), dim3(256), 0, st_, l.phi.p,
-                                   lineBuf_.p, tLq, tLrow, l.qS.p, nQ, nPS_, dirOff, tLine);
-            });
             go("k_otf64_fused", [&] {
-                hipLaunchKernelGGL((k_otf64_fused<N, F64NT>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT), F64LDS, st_,
-                                   l.S.p, l.phi.p, invPs_.p, pupil_.p, tScale, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_,
-                                   nQ, tLine);
+                hipLaunchKernelGGL((k_otf64_fused<N, F64NT, true>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT), F64LDS,
+                                   st_, l.S.p, l.phi.p, invPs_.p, pupil_.p, tScale, imgScale_.p, hread_.p, tw_.p, nPS_,
+                                   nOS_, nQ, tLine, lineBuf_.p, tLine, l.qS.p, dirOff);
             });
         }
         go("k_pq", [&] {
@@ -4143,8 +4163,14 @@ private:
         }
         if constexpr (FUSED64) {
             launch("k_otf64_fused", nQ, [&] {
-                hipLaunchKernelGGL((k_otf64_fused<N, F64NT>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT), F64LDS, st_,
-                                   Sout, phi, invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p, nPS_, nOS_, nQ);
+                if (lin.line)                  // line-search probes from their lines' phases
+                    hipLaunchKernelGGL((k_otf64_fused<N, F64NT, true>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT),
+                                       F64LDS, st_, Sout, phi, invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p,
+                                       nPS_, nOS_, nQ, nullptr, lin.line, lin.qLine, lin.qS, lin.dirOff);
+                else
+                    hipLaunchKernelGGL((k_otf64_fused<N, F64NT, false>), dim3(std::min(nQ, otfGrid_)), dim3(F64NT),
+                                       F64LDS, st_, Sout, phi, invPs_.p, pupil_.p, qImg, imgScale_.p, hread_.p, tw_.p,
+                                       nPS_, nOS_, nQ);
             });
             return;
         }
@@ -4329,7 +4355,7 @@ private:
         const double* qSD = nLine ? sc_->tab.p + nd : nullptr;
         LineIn lin{};
         if (nLine) {
-            if constexpr (FUSED) {          // the fused FP32 transform reads the line in place of phi
+            if constexpr (FUSED || FUSED64) {  // the fused transforms read the line in place of phi
                 lin = {lineBuf_.p, qLineD, qSD, (size_t)maxSlots_ * nImg * nPS_};
             } else {
                 launch("k_line_phase", nLR, [&] {

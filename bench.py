@@ -17,8 +17,12 @@ metric and CG).  A patch counts as converged when its solve exits normally and i
 restored objects are within 1e-4 (relative RMS) of the FP64 oracle's solve of the same
 patch; the oracle solves a fixed sample of the batch (its timing is the CPU baseline),
 and `value` is the FP32 rate times the sample's converged fraction.  Beside it: the
-plain FP64 drop-in (one patch at a time, on the sample patches), the batched FP64 and
-mixed (BF16 mode GEMM) forms, each with the same check against the oracle.
+plain FP64 drop-in (one patch at a time, on the sample patches, in the reference's
+arithmetic: the mode GEMM per line-search probe, the line search stepped by the host), the
+batched FP64 form (line-search probes from base + step * direction, the line search on
+the device, as the FP32 form), the reference's arithmetic batched (bit-identical to the
+plain drop-in) and the mixed (BF16 mode GEMM) form, each with the same check against the
+oracle.
 
 Each form gets a context with its patches loaded (frames resident in HBM); a step is
 one mfbd_solve of every loaded patch from zero alphas, so warmup and timed steps repeat
@@ -86,15 +90,19 @@ def make_batch(n, seed, workers):
 LINE_GRAPH = True        # device line-search rounds replayed as captured HIP graphs (--line-graph)
 
 
-def make_solver(pbs, precision, batch, gradient, timing=False):
+def make_solver(pbs, precision, batch, gradient, timing=False, line=None):
     """A context with the patches loaded (frames resident in HBM).  With `timing`,
     every kernel launch is bracketed by HIP events on the pipeline's stream (and the
-    line-search rounds are launched kernel by kernel)."""
+    line-search rounds are launched kernel by kernel).  `line`: True for line-search probes
+    from base + step * direction with the line search on the device, False for the mode
+    GEMM per probe stepped by the host (the reference's arithmetic), None for the Solver's
+    default of the precision form (FP32 / mixed: the former, FP64: the latter)."""
     from redux_amd.momfbd import Solver
     if timing:
         os.environ["MFBD_KERNEL_TIMING"] = "1"
+    lp = {} if line is None else dict(line_phases=bool(line), line_rounds=4 if line else 0)
     try:
-        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient, line_graph=LINE_GRAPH)
+        sv = Solver(pbs[0], precision=precision, max_batch=batch, gradient=gradient, line_graph=LINE_GRAPH, **lp)
         sv.load(pbs)
     finally:
         os.environ.pop("MFBD_KERNEL_TIMING", None)
@@ -154,6 +162,7 @@ def roofline(stats, N, nPS, nH, form):
       k_otf_fused, k_otf_wave (FP32): read phi, write S           r nPS + c nH
         (a line-search probe reads the line's base and direction:  2 r nPS + c nH)
       k_otf64_fused (FP64): read phi, write S                     r nPS + c nH
+        (a line-search probe reads the line's base and direction:  2 r nPS + c nH)
       k_pq: read S and D                                             2 c nH
       k_phase_mfma: write phi                                        r nPS
       k_vogel_fused: read S, D, P, Q (FP64), phi; write v            2 c nH + 24 nH + 2 r nPS
@@ -179,7 +188,8 @@ def roofline(stats, N, nPS, nH, form):
         mean_ms = s["ms"] / s["launches"]
         units_per_launch = s["units"] / s["launches"]
         # a line-search probe's transform reads the line's base and direction (2 r nPS) for phi
-        line = stats.get("_line_images", {}).get("units", 0) if name in ("k_otf_fused", "k_otf_wave") else 0
+        line = stats.get("_line_images", {}).get("units", 0) if name in ("k_otf_fused", "k_otf_wave",
+                                                                          "k_otf64_fused") else 0
         bytes_per_launch = (per_unit[name] * s["units"] + r * nPS * line) / s["launches"]
         ach = bytes_per_launch / (mean_ms * 1e-3) / 1e9
         tr = measured_traffic(form).get(name)
@@ -635,15 +645,15 @@ def headline(args):
 
     # Per-kernel statistics: one step of each form on a context that brackets every kernel
     # with HIP events and keeps its evaluation lanes on one stream
-    def stats_step(prec):
-        sv = make_solver(pbs, prec, len(pbs), args.gradient, timing=True)
+    def stats_step(prec, line=None):
+        sv = make_solver(pbs, prec, len(pbs), args.gradient, timing=True, line=line)
         res, _ = run_step(sv)
         st = sv.kernel_stats()
         sizes = [int(x) for x in sv.debug("sizes")]
         sv.close()
         return res, st, sizes
 
-    ref64, st64, (nps, nos, nh) = stats_step("fp64")
+    ref64, st64, (nps, nos, nh) = stats_step("fp64", line=True)
     ref32, st32, _ = stats_step("fp32")
 
     # Timed form: batched FP32 (warmup steps, then the timed steps) on a context without events
@@ -663,28 +673,36 @@ def headline(args):
     rate32 = len(pbs) / t32
     value = rate32 * chk32["fraction"]
 
-    def one_step(prec):
-        sv = make_solver(pbs, prec, len(pbs), args.gradient)
+    def one_step(prec, line=None):
+        sv = make_solver(pbs, prec, len(pbs), args.gradient, line=line)
+        run_step(sv)                    # warm-up (first-use costs: graph captures, workspaces)
         res, dt = run_step(sv)
         sv.close()
-        log(f"{prec} batched: {dt:.2f}s")
+        log(f"{prec} batched{'' if line is None else ' line phases' if line else ' GEMM per probe'}: {dt:.2f}s")
         return res, dt
 
-    res64, t64 = one_step("fp64")
+    # the batched FP64 form: line phases and the device line search; beside it the
+    # reference's arithmetic batched (the mode GEMM per probe, the host's line search)
+    res64, t64 = one_step("fp64", line=True)
+    res64r, t64r = one_step("fp64", line=False)
     resmx, tmx = one_step("mixed")
     chk64, chkmx = against_oracle(res64, idx, ores), against_oracle(resmx, idx, ores)
     same64 = all(np.array_equal(a["img"], b["img"]) and np.array_equal(a["alpha"], b["alpha"])
                  for a, b in zip(res64, ref64))
 
-    # plain FP64 drop-in: the sample patches one at a time; their results must equal the batch's
+    # plain FP64 drop-in: the sample patches one at a time in the reference's arithmetic (the
+    # mode GEMM per probe, the host's line search); their results must equal the same
+    # arithmetic batched
     plain = [pbs[k] for k in idx[: args.plain_patches]]
-    sv = make_solver(plain, "fp64", 1, args.gradient)
+    sv = make_solver(plain, "fp64", 1, args.gradient, line=False)
+    run_step(sv)                        # warm-up, as for the batched forms
     resp, tp = run_step(sv)
     sv.close()
     plain_rate = len(plain) / tp
-    plain_identical = all(np.array_equal(a["img"], res64[k]["img"]) and np.array_equal(a["alpha"], res64[k]["alpha"])
+    plain_identical = all(np.array_equal(a["img"], res64r[k]["img"]) and np.array_equal(a["alpha"], res64r[k]["alpha"])
                           for a, k in zip(resp, idx))
-    log(f"fp64 plain: {len(plain)} patches in {tp:.2f}s, identical to batched: {plain_identical}")
+    log(f"fp64 plain: {len(plain)} patches in {tp:.2f}s, identical to the GEMM-per-probe batch: {plain_identical}")
+    chk64r = against_oracle(res64r, idx, ores)
     chk_plain = against_oracle({k: r for k, r in zip(idx, resp)}, idx[: len(plain)], ores[: len(plain)])
 
     ev32 = {"f": float(np.mean([r["nF"] for r in res32])), "grad": float(np.mean([r["nDF"] for r in res32]))}
@@ -708,11 +726,12 @@ def headline(args):
                    "solver": {k: settings[k] for k in ("FTOL", "EPS", "MIN_ITER", "MAX_ITER", "N_DONE_ITER",
                                                        "MODE_START", "MODE_STEP", "GRADDIFF_STEP")}
                    | {"GETSTEP": "conjugate", "gradient": args.gradient,
-                      "line_phases": "FP32 and mixed: probes from base + step * direction (metricAt); FP64: the GEMM per probe",
-                      "line_search": "FP32 and mixed: on the device (bracket + brent state machine per patch, each "
+                      "line_phases": "batched forms: probes from base + step * direction (metricAt); plain "
+                                     "FP64 drop-in: the mode GEMM per probe (the reference's arithmetic)",
+                      "line_search": "batched forms: on the device (bracket + brent state machine per patch, each "
                                      "round of probes " + ("replayed as a captured HIP graph" if LINE_GRAPH else
                                                            "launched kernel by kernel") +
-                                     ", read back every 4 rounds); FP64: stepped by the host"},
+                                     ", read back every 4 rounds); plain FP64 drop-in: stepped by the host"},
                    "converged": "solve exits normally and restored objects within 1e-4 (relative RMS) of the "
                                 f"FP64 oracle's solve, on a sample of {len(idx)} patches; value = patches/s x "
                                 "sampled fraction",
@@ -721,11 +740,16 @@ def headline(args):
                              timed_steps_identical_to_stats_step=bool(same32)),
         "fp64_batched": dict(patches_per_s=round(len(pbs) / t64, 4), vs_oracle=chk64, evals_per_patch=ev64,
                              converged_patches_per_s=round(len(pbs) / t64 * chk64["fraction"], 4),
-                             identical_to_stats_step=bool(same64)),
+                             identical_to_stats_step=bool(same64),
+                             form="line phases and the device line search"),
+        "fp64_batched_gemm_per_probe": dict(patches_per_s=round(len(pbs) / t64r, 4), vs_oracle=chk64r,
+                                            converged_patches_per_s=round(len(pbs) / t64r * chk64r["fraction"], 4),
+                                            form="the reference's arithmetic (mode GEMM per probe, host line "
+                                                 "search), batched"),
         "mixed_bf16_batched": dict(patches_per_s=round(len(pbs) / tmx, 4), vs_oracle=chkmx,
                                    converged_patches_per_s=round(len(pbs) / tmx * chkmx["fraction"], 4)),
         "fp64_plain": dict(patches_per_s=round(plain_rate, 4), patches=len(plain),
-                           identical_to_fp64_batched=bool(plain_identical), vs_oracle=chk_plain),
+                           identical_to_fp64_batched_gemm_per_probe=bool(plain_identical), vs_oracle=chk_plain),
         "ratio_fp32_batched_vs_fp64_plain": {"raw": round(rate32 / plain_rate, 2),
                                              "converged": round(value / (plain_rate * chk_plain["fraction"]), 2)
                                              if chk_plain["fraction"] > 0 else None},

@@ -3576,7 +3576,7 @@ public:
             auto it = stats_.find(k);
             if (it != stats_.end() && (std::strcmp(k, "k_ls_round") == 0 || !l.useGraph)) it->second.units += img;
         }
-        if (timing_ && !l.useGraph && FUSED) {
+        if (timing_ && !l.useGraph && (FUSED || FUSED64)) {
             auto& li = stats_["_line_images"];
             li.name = "_line_images";
             li.units += img;

@@ -1333,11 +1333,14 @@ void shift_inits(mfbd_ctx* c, std::vector<PatchRun*>& rs, int lane) {
 // Evaluation lanes: each a contiguous share of the batch on a stream of its own; while the
 // device evaluates one share, the host resumes the others' patches.  MFBD_EVAL_LANES
 // (1..kMaxEvalLanes) overrides the default.
-// Evaluation lanes of a batch: MFBD_EVAL_LANES, else 2 for FP64 (its OTF kernel fills the
-// device; more lanes measured 7 % slower) and 4 for FP32 / MIXED (2 % faster than 2)
-int eval_lanes(int precision) {
+// Evaluation lanes of a batch: MFBD_EVAL_LANES, else 4 for FP32 / MIXED (2 % faster than 2)
+// and for FP64 2 with the host's line search (its OTF kernel fills the device; more lanes
+// measured 7 % slower), 3 with the device's (256 config-2 patches: 243 / 250 / 231
+// patches/s at 2 / 3 / 4 lanes)
+int eval_lanes(const mfbd_params& p) {
     const char* e = getenv("MFBD_EVAL_LANES");
-    const int v = e ? atoi(e) : (precision == MFBD_FP64 ? 2 : 4);
+    const bool deviceLS = p.lineRounds > 0 && p.linePhases && p.getstep == MFBD_GETSTEP_CONJUGATE;
+    const int v = e ? atoi(e) : (p.precision == MFBD_FP64 ? (deviceLS ? 3 : 2) : 4);
     return std::max(1, std::min(v, kMaxEvalLanes));
 }
 
@@ -1751,7 +1754,7 @@ int mfbd_solve(mfbd_ctx* c, const double* alpha0, mfbd_result* results, double* 
                 c->hostStats[3].launches++;
                 c->hostStats[3].units += nb;
             }
-            const int nLanes = std::max(1, std::min(eval_lanes(c->prm.precision), nb));
+            const int nLanes = std::max(1, std::min(eval_lanes(c->prm), nb));
             auto laneOf = [&](int k) { return (int)((long long)k * nLanes / nb); };
             // device line searches: per lane one captured round over the lane's patches
             std::vector<int> lsIdx(nb, -1);

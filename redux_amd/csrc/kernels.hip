@@ -2150,7 +2150,7 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
                 }
                 dft_reg<double, RF, -1>(v[i]);
             }
-            fetch(q + gridDim.x);
+            if constexpr (!LINE) fetch(q + gridDim.x);
 #pragma unroll
             for (int i = 0; i < IR; ++i) {
                 const int b = threadIdx.x + i * NT, line = b % P, j = b / P;
@@ -2255,6 +2255,12 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
             __syncthreads();
         }
         spass_at<double, N, RL, RF, +1, P, NT>(t, at, tw);
+        // LINE: the next image's base and direction are fetched only here, after the heavy
+        // passes, so that they do not hold registers through them (223 registers in place of
+        // 255: a k_pq workgroup of another lane then fits beside the two waves per SIMD;
+        // 258 against 249 patches/s on 256 config-2 patches). Fetched here without LINE the
+        // phases would cost the GEMM-per-probe path 1.5 %.
+        if constexpr (LINE) fetch(q + gridDim.x);
         // 6.
         constexpr double inv = 1.0 / ((double)N * N);
         cx<double>* Sq = S + (size_t)q * nOS;

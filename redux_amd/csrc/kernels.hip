@@ -2118,17 +2118,23 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
                     ph[i][r] = (qq < nQ && kk[i][r] >= 0) ? phi[(size_t)qq * nPS + kk[i][r]] : 0.0;
         }
     };
-    fetch(blockIdx.x);
+    // images not probed this round (device line search: qSkip -2) have no S and are passed
+    // over without a fetch of their phases
+    auto nextq = [&](int qq) {
+        do qq += gridDim.x;
+        while (qSkip && qq < nQ && qSkip[qq] == -2);
+        return qq;
+    };
+    int q0 = blockIdx.x;
+    if (qSkip && q0 < nQ && qSkip[q0] == -2) q0 = nextq(q0);
+    fetch(q0);
     auto at = [](int k, int y) { return y < P ? y * LD + k : (y - P) * LD + (k == 0 ? P : N - k); };
-    for (int q = blockIdx.x; q < nQ; q += gridDim.x) {
-        if (qSkip && qSkip[q] == -2) {          // not probed this round (device line search): no S
-            fetch(q + gridDim.x);
-            continue;
-        }
+    for (int q = q0; q < nQ; q = nextq(q)) {
+        const int qn = nextq(q);                // the image fetched during this one
         const double sc = imgScale[qImg[q]];
         if (sc == 0.0) {        // an image the patch lacks: its OTF is zero (t is not touched)
             for (int u = threadIdx.x; u < nOS; u += NT) S[(size_t)q * nOS + u] = {0.0, 0.0};
-            fetch(q + gridDim.x);
+            fetch(qn);
             continue;
         }
         // 1-2.
@@ -2150,7 +2156,7 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
                 }
                 dft_reg<double, RF, -1>(v[i]);
             }
-            if constexpr (!LINE) fetch(q + gridDim.x);
+            if constexpr (!LINE) fetch(qn);
 #pragma unroll
             for (int i = 0; i < IR; ++i) {
                 const int b = threadIdx.x + i * NT, line = b % P, j = b / P;
@@ -2260,7 +2266,7 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
         // 255: a k_pq workgroup of another lane then fits beside the two waves per SIMD;
         // 258 against 249 patches/s on 256 config-2 patches). Fetched here without LINE the
         // phases would cost the GEMM-per-probe path 1.5 %.
-        if constexpr (LINE) fetch(q + gridDim.x);
+        if constexpr (LINE) fetch(qn);
         // 6.
         constexpr double inv = 1.0 / ((double)N * N);
         cx<double>* Sq = S + (size_t)q * nOS;

@@ -2100,7 +2100,7 @@ __global__ void __launch_bounds__(NT) k_otf64_fused(cx<double>* __restrict__ S, 
     }
     auto fetch = [&](int qq) {
         if constexpr (LINE) {
-            const int lr = qq < nQ ? qLine[qq] : -1;
+            const int lr = (qLine && qq < nQ) ? qLine[qq] : -1;
             const double* src = lr >= 0 ? line + (size_t)lr * nPS : phi + (size_t)qq * nPS;
             ls = lr >= 0 ? qLineS[qq] : 0.0;
 #pragma unroll

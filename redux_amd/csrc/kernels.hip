@@ -424,12 +424,14 @@ struct MfmaVec<double> {
 };
 
 template <typename T>
-struct PhaseChunk { static constexpr int value = 7; };         // k-steps whose fragments are loaded together
+struct PhaseChunk { static constexpr int value = 2; };         // k-steps whose fragments are loaded together
 template <>
 struct PhaseChunk<double> { static constexpr int value = 1; };
 // Per 256-patch config-2 step (accumulators in VGPRs): FP64 98 ms at one k-step per chunk,
-// 100 at 2, 105 at 3, 111 at 5; FP32 45 ms at 7, 46 at 4, 48 at 13 (fewer registers, more
-// waves per SIMD to cover the fragment loads)
+// 100 at 2, 105 at 3, 111 at 5 (fewer registers, more waves per SIMD to cover the fragment
+// loads); FP32 45 ms at 7, 46 at 4, 48 at 13 on its own, but at 2 (62 registers) it fits in
+// the 64 per SIMD lane that k_otf_wave leaves, beside the other lanes' transforms: the FP32
+// step 473 -> 478 patches/s
 
 // Row q of a phase GEMM goes to phi + q nPS, or with an output table to phi + qOut[q] nPS
 // (qOut[q] >= 0) or phi1 + (-1 - qOut[q]) nPS; a channel row qPhRow[q] < 0 adds nothing.

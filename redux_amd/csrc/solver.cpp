@@ -1334,13 +1334,13 @@ void shift_inits(mfbd_ctx* c, std::vector<PatchRun*>& rs, int lane) {
 // device evaluates one share, the host resumes the others' patches.  MFBD_EVAL_LANES
 // (1..kMaxEvalLanes) overrides the default.
 // Evaluation lanes of a batch: MFBD_EVAL_LANES, else 4 for FP32 / MIXED (2 % faster than 2)
-// and for FP64 2 with the host's line search (its OTF kernel fills the device; more lanes
-// measured 7 % slower), 3 with the device's (256 config-2 patches: 243 / 250 / 231
-// patches/s at 2 / 3 / 4 lanes)
+// and for FP64 with the device's line search (256 config-2 patches: 263 / 268 patches/s at
+// 3 / 4 lanes), 2 for FP64 with the host's (its OTF kernel fills the device; more lanes
+// measured 7 % slower)
 int eval_lanes(const mfbd_params& p) {
     const char* e = getenv("MFBD_EVAL_LANES");
     const bool deviceLS = p.lineRounds > 0 && p.linePhases && p.getstep == MFBD_GETSTEP_CONJUGATE;
-    const int v = e ? atoi(e) : (p.precision == MFBD_FP64 ? (deviceLS ? 3 : 2) : 4);
+    const int v = e ? atoi(e) : (p.precision == MFBD_FP64 && !deviceLS ? 2 : 4);
     return std::max(1, std::min(v, kMaxEvalLanes));
 }
 

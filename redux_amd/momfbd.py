@@ -47,14 +47,14 @@ class Solver:
         mode GEMM per probe (mfbd_params.linePhases); the fused transforms read the line in
         place of the GEMM's output.  By default on for the FP32 and mixed forms and off for
         FP64, whose default is the reference's arithmetic (the plain drop-in); FP64 with
-        line phases and the device line search solves 256 config-2 patches at 250 patches/s
+        line phases and the device line search solves 256 config-2 patches at ~268 patches/s
         against 230, within 1e-4 of the oracle on the benchmark sample as the default.
 
         line_rounds > 0 (with line phases and the conjugate gradient): the line searches run
         on the device, the host reading their states every line_rounds rounds of probes
         (mfbd_params.lineRounds); the same probes and results as the host's line search.
         By default 4 where line phases are on (256 config-2 patches: FP32 ~18 % faster, FP64
-        250 against 238 patches/s, the host no longer between probes).
+        ~268 against 238 patches/s, the host no longer between probes).
         line_graph: each round of probes as one captured HIP graph (mfbd_params.lineGraph, the
         default), else its three kernels launched one by one: the same kernels and results, and
         the two measured the same (FP32 438 / 433-439 patches/s on 256 config-2 patches).  A

@@ -1,7 +1,8 @@
 """Full solves on the device against the oracle's solves of the same patches, at the
 restored-object criterion of the benchmark (relative RMS <= 1e-4 against the FP64 oracle).
 
-The batched FP64 form on four patches of the benchmark layout (config 2: 88 x 88 patches
+The batched FP64 form (the default, and the benchmark's form with line phases and the
+device line search) on four patches of the benchmark layout (config 2: 88 x 88 patches
 of the seeded 1024^2 field, 2 objects x 24 frames, 50 KL modes, FTOL 1e-3; the basis from
 the oracle's setup): converged, initial metrics within 5e-12, and most objects within 1e-4
 of the oracle's.  The paths of two FP64 implementations that round differently separate at
@@ -56,6 +57,25 @@ def test_fp64_batched_solve_within_1e4_of_oracle(batch):
     assert alp.max() <= 0.1
     for r, o in zip(res, ores):
         assert abs(r["initialMetric"] - o["initialMetric"]) <= 5e-12 * o["initialMetric"]   # measured 1.8e-12
+
+
+def test_fp64_line_phase_form_within_1e4_of_oracle(batch):
+    """The benchmark's batched FP64 form: line-search probes from base + step * direction
+    (read by the transform) and the line search on the device.  Its phases differ from the
+    GEMM's by rounding, so it takes its own path like the default, to the same bounds."""
+    from redux_amd.momfbd import Solver
+    pbs, ores = batch
+    sv = Solver(pbs[0], precision="fp64", max_batch=len(pbs), line_phases=True, line_rounds=4)
+    sv.load(pbs)
+    print("fp64 batched solve, line phases and device line search", flush=True)
+    res = sv.run()
+    sv.close()
+    obj, alp, fin = _errs(res, ores)
+    print("objects", obj, "alphas", alp, "final metric", fin, flush=True)
+    assert all(r["status"] == 0 for r in res)
+    assert (obj <= 1e-4).sum() >= len(obj) / 2 and obj.max() <= 2e-3
+    assert fin.max() <= 1e-4
+    assert alp.max() <= 0.1
 
 
 @pytest.mark.parametrize("prec", ["fp32", "mixed"])
